@@ -1,0 +1,205 @@
+/*
+ * wavey_fdtd.h -- C ABI of the MI355X-native FDTD wave solver.
+ *
+ * Drop-in for the wave-solver run behind `wavey simulate --fdtd <Hz>`:
+ * core/wavey-solver/src/run.rs:170-192 (run_fdtd), whose GPU implementation
+ * is Gpu::run_fdtd (core/wavey-solver/src/gpu_fdtd.rs:45-277) and whose CPU
+ * one is fdtd::run_cpu / run_cpu_ir (core/wavey-solver/src/fdtd.rs:616-641).
+ *
+ * Conventions (mirroring the reference):
+ *  - Cell indices are the reference's: (z*ny + y)*nx + x, x fastest
+ *    (fdtd.rs:227-229).
+ *  - Inputs are borrowed for the call; the caller owns every buffer. The
+ *    library copies inputs to the device and writes outputs into caller
+ *    buffers.
+ *  - Status: WV_OK = Ok(Some), WV_CANCELLED = Ok(None), negative = Err(String)
+ *    with the message in `err` (run.rs:184-187 falls back to the CPU on Err).
+ *  - `cancel` is polled between 128-step blocks (gpu_fdtd.rs:202-205);
+ *    `progress` gets done/total after each block, on the calling thread.
+ *  - One HIP device (the current one), one stream per plan, default priority.
+ *    Entry points are reentrant; calls on different plans may overlap.
+ */
+#ifndef WAVEY_FDTD_H
+#define WAVEY_FDTD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    WV_OK = 0,
+    WV_CANCELLED = 1,
+    WV_ERR_INVALID = -1, /* bad argument (the reference would panic or Err) */
+    WV_ERR_DEVICE = -2,  /* HIP error: the caller falls back to the CPU */
+    WV_ERR_LIMIT = -3,   /* grid/steps/boundary beyond what this build takes */
+    WV_ERR_NOMEM = -4    /* device memory exhausted */
+} wv_status;
+
+/* Output forms: the reference GPU path returns spectra only (gpu_fdtd.rs:262-276),
+ * run_cpu_ir returns the receiver time series (fdtd.rs:630-641). */
+enum { WV_MODE_SPECTRA = 1, WV_MODE_IR = 2, WV_MODE_BOTH = 3 };
+
+/* Arithmetic: FP32 fields and state (the reference GPU path), FP64 throughout
+ * (the reference CPU path), or FP32 fields with FP64 boundary-filter state. */
+enum { WV_FP32 = 0, WV_FP64 = 1, WV_FP32_STATE64 = 2 };
+
+/* fdtd.rs:178-187 (Slot): u+ = (1/2 - w)*inv_d*p+ + (cv*u- - k*g- + p/2 + w*p-)*inv_d. */
+typedef struct {
+    double inv_d, cv, k, w;
+} wv_slot;
+
+/* fdtd.rs:196-209 (FdtdGrid) with the boundary list in CSR form:
+ * node b sits at cell b_cell[b] and owns slots b_start[b] .. b_start[b+1]-1.
+ * flags: bit 6 = air, bits 0-5 = neighbour (-x,+x,-y,+y,-z,+z) is air. */
+typedef struct {
+    uint64_t n[3];
+    double dx, dt;
+    double origin[3];
+    const uint8_t* flags;     /* [n0*n1*n2] */
+    uint64_t n_boundary;
+    const uint32_t* b_cell;   /* [n_boundary] */
+    const uint32_t* b_start;  /* [n_boundary + 1] */
+    const wv_slot* slots;     /* [b_start[n_boundary]] */
+} wv_grid;
+
+/* One source tap (cell, weight): fdtd.rs:467-492, gpu_fdtd.rs:67-75. */
+typedef struct {
+    uint64_t cell;
+    double w;
+} wv_tap;
+
+typedef void (*wv_progress_fn)(double fraction, void* ctx);
+
+/* ------------------------------------------------------------------ run */
+
+/*
+ * One solve: replaces Gpu::run_fdtd (gpu_fdtd.rs:45-54) and, with
+ * WV_MODE_IR, run_cpu_ir (fdtd.rs:630-641). Steps = signal length. Records
+ * p^n before the update of step n (fdtd.rs:673-683): ir_out [n_recv][steps]
+ * (doubles), spectra_out [n_recv][nf] complex interleaved (re, im) with the
+ * DFT sum_n p^n e^{-j 2 pi fract(f dt n)} (fdtd.rs:679-682).
+ */
+wv_status wv_fdtd_run(const wv_grid* grid, const wv_tap* taps, size_t n_taps, const double* signal,
+                      size_t steps, const uint64_t* receivers, size_t n_recv, const double* freqs,
+                      size_t nf, int mode, int precision, const volatile int32_t* cancel,
+                      wv_progress_fn progress, void* ctx, double* spectra_out, double* ir_out,
+                      char* err, size_t err_len);
+
+/*
+ * Plan form of the same run: the grid is uploaded and its device layout built
+ * once, then many runs (source components, calibrations: run.rs:645-661,
+ * fdtd.rs:941-942) reuse it. A plan is used by one thread at a time.
+ */
+typedef struct wv_plan wv_plan;
+wv_status wv_plan_create(const wv_grid* grid, int precision, wv_plan** out, char* err, size_t err_len);
+wv_status wv_plan_run(wv_plan* plan, const wv_tap* taps, size_t n_taps, const double* signal,
+                      size_t steps, const uint64_t* receivers, size_t n_recv, const double* freqs,
+                      size_t nf, int mode, const volatile int32_t* cancel, wv_progress_fn progress,
+                      void* ctx, double* spectra_out, double* ir_out, char* err, size_t err_len);
+/*
+ * wv_plan_run in three parts, so a caller can advance a run in pieces
+ * (progress, cancellation, timing a window of steps). wv_run_begin zeroes
+ * the fields and state and builds the per-run device layout (taps join the
+ * special-cell list); wv_run_advance enqueues k more steps (k <= remaining)
+ * and returns when they are done; wv_run_finish reads the receivers back.
+ * The reference loop is one piece (gpu_fdtd.rs:200-276).
+ */
+wv_status wv_run_begin(wv_plan* plan, const wv_tap* taps, size_t n_taps, const double* signal,
+                       size_t steps, const uint64_t* receivers, size_t n_recv, char* err,
+                       size_t err_len);
+wv_status wv_run_advance(wv_plan* plan, size_t k, const volatile int32_t* cancel,
+                         wv_progress_fn progress, void* ctx, char* err, size_t err_len);
+wv_status wv_run_finish(wv_plan* plan, const double* freqs, size_t nf, int mode,
+                        double* spectra_out, double* ir_out, char* err, size_t err_len);
+/* Device time of the last wv_run_advance's step loop (HIP events on the
+ * plan's stream), its step count and kernel launches. No reference
+ * counterpart: measurement hook. */
+wv_status wv_plan_last_timing(const wv_plan* plan, double* step_loop_ms, uint64_t* steps,
+                              uint64_t* launches);
+/* Sizes of the plan's device layout (for roofline accounting). No reference
+ * counterpart. out: [cells, air_cells, boundary_nodes, slots, special_cells,
+ * pitched_cells, slots_with_k]. */
+wv_status wv_plan_stats(const wv_plan* plan, uint64_t out[7]);
+/* Step-kernel shape: z planes per work-group column (0 = default). No
+ * reference counterpart: the autotuner's knob. */
+wv_status wv_plan_set_tile(wv_plan* plan, uint32_t zchunk);
+void wv_plan_free(wv_plan* plan);
+
+/* ------------------------------------------------------------ grid build */
+
+/* impedance_fit.rs:27-44 (Branch, AdmittanceFit). */
+typedef struct {
+    double r, m, k;
+} wv_branch;
+typedef struct {
+    double beta0;
+    uint32_t n_branches;
+    const wv_branch* branches;
+} wv_admittance;
+
+/*
+ * A box room [0,size] with rectangular floor-to-ceiling pillars: the rooms
+ * FdtdGrid::build_dx (fdtd.rs:289-377) voxelizes for shoeboxes and sketches
+ * with rectangular holes. Surfaces: walls x=0, x=size0, y=0, y=size1, floor,
+ * ceiling, then the sides facing -x, +x, -y, +y of each pillar. Surface s is
+ * split into patch_gu[s] x patch_gv[s] patches over its (u, v) extent (x-walls
+ * u=y v=z, y-walls u=x v=z, floor/ceiling u=x v=y, pillar sides as the walls
+ * they parallel, over the pillar); patch (iu, iv) uses
+ * fits[patch_fit[patch_off[s] + iv*gu + iu]].
+ */
+typedef struct {
+    double size[3];
+    uint32_t n_pillars;
+    const double* pillars; /* [n_pillars][4] = x0, x1, y0, y1 */
+    uint32_t n_fits;
+    const wv_admittance* fits;
+    const uint32_t* patch_gu;
+    const uint32_t* patch_gv;
+    const uint32_t* patch_off;
+    const uint32_t* patch_fit;
+} wv_box_room;
+
+typedef struct wv_built_grid wv_built_grid;
+/* fdtd.rs:289-377 for box rooms: same flags, boundary order and slots. */
+wv_status wv_grid_build_box(const wv_box_room* room, double c, double dx, int threads,
+                            wv_built_grid** out, char* err, size_t err_len);
+const wv_grid* wv_built_grid_view(const wv_built_grid* g);
+void wv_grid_free(wv_built_grid* g);
+
+/* ---------------------------------------------------- source / receivers */
+
+/* fdtd.rs:601-610 */
+void wv_source_signal(double f_max, double dt, size_t steps, double* out);
+/* fdtd.rs:245-272; returns 0 and *cell, or WV_ERR_INVALID when none (None). */
+wv_status wv_nearest_air(const wv_grid* g, const double p[3], uint64_t* cell);
+/* fdtd.rs:467-492; *n_taps <= 8; WV_ERR_INVALID when None. */
+wv_status wv_taps_at(const wv_grid* g, const double p[3], wv_tap taps[8], size_t* n_taps);
+/* fdtd.rs:730-743: comps[4][2], comp_len[4], *n_comps = 1 or 4. None -> WV_ERR_INVALID. */
+wv_status wv_source_components(const wv_grid* g, uint64_t c, uint32_t order, wv_tap comps[8],
+                               uint32_t comp_len[4], uint32_t* n_comps);
+/* fdtd.rs:812-815; complex values interleaved (re, im): a0[2], a[6], out[8]. */
+void wv_component_weights(const double a0[2], const double a[6], double k, double dx, double out[8]);
+/* fdtd.rs:819-840; spectra/h0/out complex interleaved. */
+void wv_to_transfer(const double* spectra, size_t n_spec, const double* signal, size_t steps,
+                    double dt, double dx, const double* freqs, size_t nf, double c, double r_ref,
+                    const double* h0, double* out);
+
+/* ------------------------------------------------------------- materials */
+
+/* impedance_fit.rs:48-55; out complex (re, im). */
+void wv_admittance_eval(const wv_admittance* fit, double f, double out[2]);
+/* impedance_fit.rs:73-84 */
+double wv_paris(double beta_re, double beta_im, uint32_t n);
+/* material.rs:96-111 */
+double wv_fit_real_impedance(double alpha);
+
+/* Library version string. */
+const char* wv_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

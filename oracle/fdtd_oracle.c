@@ -1,0 +1,592 @@
+/*
+ * FP64 CPU oracle for the Wavey FDTD wave-solver path -- TEST INFRASTRUCTURE.
+ *
+ * A plain-C restatement of core/wavey-solver/src/fdtd.rs (and the pieces of
+ * core/wavey-materials it needs), used only by tests/, the smoke check and
+ * bench.py's CPU baseline, always as the checker. Arithmetic is written in
+ * the reference's evaluation order and compiled with -ffp-contract=off, so
+ * that no fused multiply-add changes a rounding the reference does not make.
+ * Threading follows the reference's rayon structure: the interior update is
+ * split over z-slabs (par_chunks_mut(nxy), fdtd.rs:699-708), the boundary
+ * nodes are updated in parallel (fdtd.rs:710-713), the scatter, injection
+ * and recording are serial. Every cell and node update is independent, so
+ * results do not depend on the thread count.
+ */
+#include "fdtd_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* fdtd.rs:137-140 */
+static const double COURANT = 0.5773502691896257;
+static const int64_t MAX_CELLS = 600000000;
+/* fdtd.rs:209, :224 */
+#define AIR ((uint8_t)(1u << 6))
+#define NO_BOUNDARY 0xFFFFFFFFu
+
+static const double PI = 3.141592653589793;
+
+/* ---------------------------------------------------------------- complex */
+typedef struct { double re, im; } cplx;
+static cplx cmk(double re, double im) { cplx z = {re, im}; return z; }
+static cplx cadd(cplx a, cplx b) { return cmk(a.re + b.re, a.im + b.im); }
+static cplx cmul(cplx a, cplx b) { return cmk(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re); }
+static cplx cscale(cplx a, double s) { return cmk(a.re * s, a.im * s); }
+/* num-complex 0.4 Div<Complex>: (a c + b d)/|z|^2, (b c - a d)/|z|^2 */
+static cplx cdiv(cplx a, cplx b) {
+    double n = b.re * b.re + b.im * b.im;
+    double re = a.re * b.re + a.im * b.im;
+    double im = a.im * b.re - a.re * b.im;
+    return cmk(re / n, im / n);
+}
+static double cnorm2(cplx a) { return a.re * a.re + a.im * a.im; }
+static cplx from_polar(double r, double th) { return cmk(r * cos(th), r * sin(th)); }
+/* Rust f64::fract: x - trunc(x) */
+static double fract(double x) { return x - trunc(x); }
+
+/* --------------------------------------------------- materials (L0 crate) */
+
+/* impedance_fit.rs:48-55 */
+void ob_admittance(const ob_fit* fit, double f, double out[2]) {
+    double w = 2.0 * PI * (f > 1e-6 ? f : 1e-6);
+    cplx jw = cmk(0.0, w);
+    cplx y = cmk(fit->beta0, 0.0);
+    for (int b = 0; b < fit->nb; ++b) {
+        /* b.r + jw*b.m + b.k/jw */
+        cplx den = cadd(cadd(cmk(fit->br[b].r, 0.0), cscale(jw, fit->br[b].m)),
+                        cdiv(cmk(fit->br[b].k, 0.0), jw));
+        y = cadd(y, cdiv(cmk(1.0, 0.0), den));
+    }
+    out[0] = y.re;
+    out[1] = y.im;
+}
+
+/* impedance_fit.rs:73-84 (paris): midpoint rule over n angles. */
+double ob_paris(double beta_re, double beta_im, int n) {
+    double h = PI / 2.0 / (double)n;
+    double a = 0.0;
+    cplx beta = cmk(beta_re, beta_im);
+    for (int i = 0; i < n; ++i) {
+        double th = ((double)i + 0.5) * h;
+        double c = cos(th);
+        cplx r = cdiv(cmk(c - beta.re, -beta.im), cmk(c + beta.re, beta.im));
+        a += (1.0 - cnorm2(r)) * sin(2.0 * th) * h;
+    }
+    return a;
+}
+
+/* material.rs:81-91 with R = (z c - 1)/(z c + 1) real. */
+static double stat_abs_real(double z) {
+    const int N = 256;
+    double h = (PI / 2.0) / (double)N;
+    double s = 0.0;
+    for (int i = 0; i < N; ++i) {
+        double th = ((double)i + 0.5) * h;
+        double c = cos(th);
+        double r = (z * c - 1.0) / (z * c + 1.0);
+        s += (1.0 - r * r) * sin(2.0 * th) * h;
+    }
+    return s;
+}
+
+/* material.rs:96-111 */
+double ob_fit_real_impedance(double alpha) {
+    double lo = 1.567, hi = 1e7;
+    if (alpha >= stat_abs_real(lo)) return lo;
+    if (alpha <= 0.0) return hi;
+    for (int i = 0; i < 80; ++i) {
+        double mid = sqrt(lo * hi);
+        if (stat_abs_real(mid) > alpha) lo = mid; else hi = mid;
+    }
+    return sqrt(lo * hi);
+}
+
+/* fdtd.rs:157-162 */
+double ob_phase_error_deg_per_m(double f, double c, double dx) {
+    double dt = COURANT * dx / c;
+    double w = 2.0 * PI * f;
+    double k_num = 2.0 / dx * asin(sin(w * dt / 2.0) / COURANT);
+    return fabs(k_num - w / c) * (180.0 / PI);
+}
+
+/* ------------------------------------------------------------ boundary */
+
+static int fit_eq(const ob_fit* a, const ob_fit* b) {
+    if (a->beta0 != b->beta0 || a->nb != b->nb) return 0;
+    for (int i = 0; i < a->nb; ++i)
+        if (a->br[i].r != b->br[i].r || a->br[i].m != b->br[i].m || a->br[i].k != b->br[i].k) return 0;
+    return 1;
+}
+
+/* fdtd.rs:539-571: faces with equal fits share slots (impedance / count). */
+int ob_boundary_slots(const int32_t* face_fits, int n_faces, const ob_fit* fits, double dt,
+                      ob_slot* out, int max_out) {
+    int grp[6], cnt[6], ng = 0, ns = 0;
+    for (int f = 0; f < n_faces; ++f) {
+        int s = face_fits[f], found = -1;
+        for (int g = 0; g < ng; ++g)
+            if (fit_eq(&fits[grp[g]], &fits[s])) { found = g; break; }
+        if (found >= 0) cnt[found]++;
+        else { grp[ng] = s; cnt[ng] = 1; ng++; }
+    }
+    for (int g = 0; g < ng; ++g) {
+        const ob_fit* fit = &fits[grp[g]];
+        double n = (double)cnt[g];
+        int total = (fit->beta0 > 0.0 ? 1 : 0) + fit->nb;
+        for (int b = -1; b < fit->nb; ++b) {
+            double r, m, k;
+            if (b < 0) {
+                if (!(fit->beta0 > 0.0)) continue;
+                r = 1.0 / fit->beta0; m = 0.0; k = 0.0;
+            } else {
+                r = fit->br[b].r; m = fit->br[b].m; k = fit->br[b].k;
+            }
+            r = r / n; m = m / n; k = k / n;
+            double d, cv, w;
+            if (m > 0.0) {
+                d = m / dt + r / 2.0 + k * dt / 4.0;
+                cv = m / dt - r / 2.0 - k * dt / 4.0;
+                w = 0.25;
+            } else {
+                d = r + k * dt / 2.0;
+                cv = -k * dt / 2.0;
+                w = 0.0;
+            }
+            if (ns >= max_out) return -1;
+            out[ns].inv_d = 1.0 / d;
+            out[ns].cv = cv;
+            out[ns].k = k;
+            out[ns].w = w;
+            ns++;
+        }
+        (void)total;
+    }
+    return ns;
+}
+
+/* fdtd.rs:581-598 */
+double ob_boundary_update(const ob_slot* slots, int n, double* st, double p, double p_prev,
+                          double lap, double lam, double dt) {
+    double s1 = 0.0, s2 = 0.0;
+    for (int j = 0; j < n; ++j) {
+        const ob_slot* sl = &slots[j];
+        double u = st[2 * j], g = st[2 * j + 1];
+        double e = (sl->cv * u - sl->k * g + 0.5 * p + sl->w * p_prev) * sl->inv_d;
+        s1 += (0.5 - sl->w) * sl->inv_d;
+        s2 += e - u;
+    }
+    double next = (2.0 * p - p_prev + lap - lam * s2) / (1.0 + lam * s1);
+    for (int j = 0; j < n; ++j) {
+        const ob_slot* sl = &slots[j];
+        double u0 = st[2 * j], g = st[2 * j + 1];
+        double e = (sl->cv * u0 - sl->k * g + 0.5 * p + sl->w * p_prev) * sl->inv_d;
+        double u = (0.5 - sl->w) * sl->inv_d * next + e;
+        st[2 * j + 1] = g + 0.5 * dt * (u + u0);
+        st[2 * j] = u;
+    }
+    return next;
+}
+
+/* ---------------------------------------------------------------- grid */
+
+/* Surface patch lookup for the box-room special case of the BVH face search
+ * (fdtd.rs:351-363): the ray from an air node toward a missing neighbour
+ * crosses the surface halfway between the two nodes. */
+static int patch_fit(const ob_room* room, int s, double u, double v, double u0, double u1,
+                     double v0, double v1) {
+    int gu = room->gu[s], gv = room->gv[s];
+    int iu = (int)floor((u - u0) / (u1 - u0) * gu);
+    int iv = (int)floor((v - v0) / (v1 - v0) * gv);
+    if (iu < 0) iu = 0;
+    if (iu >= gu) iu = gu - 1;
+    if (iv < 0) iv = 0;
+    if (iv >= gv) iv = gv - 1;
+    return room->patch_fit[room->patch_off[s] + iv * gu + iu];
+}
+
+static int in_pillar(const ob_room* room, double x, double y) {
+    for (int p = 0; p < room->n_pillars; ++p) {
+        const double* q = room->pillars + 4 * p;
+        if (x > q[0] && x < q[1] && y > q[2] && y < q[3]) return p;
+    }
+    return -1;
+}
+
+/* Room::contains for an extruded box with pillar holes
+ * (core/wavey-geom/src/room.rs:151-157). */
+static int room_contains(const ob_room* room, double x, double y, double z) {
+    return z > 0.0 && z < room->L[2] && x > 0.0 && x < room->L[0] && y > 0.0 && y < room->L[1] &&
+           in_pillar(room, x, y) < 0;
+}
+
+/* The fit behind the missing face of direction d of the node at (x,y,z). */
+static int face_fit(const ob_room* room, double x, double y, double z, int d, double dx) {
+    static const int DX[6] = {-1, 1, 0, 0, 0, 0}, DY[6] = {0, 0, -1, 1, 0, 0}, DZ[6] = {0, 0, 0, 0, -1, 1};
+    double hx = x + DX[d] * dx * 0.5, hy = y + DY[d] * dx * 0.5, hz = z + DZ[d] * dx * 0.5;
+    double nx = x + DX[d] * dx, ny = y + DY[d] * dx;
+    const double* L = room->L;
+    int p = (d < 4) ? in_pillar(room, nx, ny) : -1;
+    if (p >= 0) {
+        const double* q = room->pillars + 4 * p;
+        /* The side whose outward normal faces the node: -dir. */
+        int side = d ^ 1;
+        int s = 6 + 4 * p + side;
+        if (d < 2) return patch_fit(room, s, hy, hz, q[2], q[3], 0.0, L[2]);
+        return patch_fit(room, s, hx, hz, q[0], q[1], 0.0, L[2]);
+    }
+    switch (d) {
+    case 0: return patch_fit(room, 0, hy, hz, 0.0, L[1], 0.0, L[2]);
+    case 1: return patch_fit(room, 1, hy, hz, 0.0, L[1], 0.0, L[2]);
+    case 2: return patch_fit(room, 2, hx, hz, 0.0, L[0], 0.0, L[2]);
+    case 3: return patch_fit(room, 3, hx, hz, 0.0, L[0], 0.0, L[2]);
+    case 4: return patch_fit(room, 4, hx, hy, 0.0, L[0], 0.0, L[1]);
+    default: return patch_fit(room, 5, hx, hy, 0.0, L[0], 0.0, L[1]);
+    }
+}
+
+void ob_grid_free(ob_grid* g) {
+    free(g->flags);
+    free(g->bidx);
+    free(g->b_cell);
+    free(g->b_start);
+    free(g->slots);
+    memset(g, 0, sizeof(*g));
+}
+
+/* fdtd.rs:289-377 (build_dx) for a box room with pillars and no cabinets. */
+int ob_build_box(const ob_room* room, double c, double dx, ob_grid* g) {
+    memset(g, 0, sizeof(*g));
+    double dt = COURANT * dx / c;
+    int64_t n[3];
+    for (int k = 0; k < 3; ++k) n[k] = (int64_t)ceil(room->L[k] / dx) + 2;
+    int64_t cells = n[0] * n[1] * n[2];
+    if (cells > MAX_CELLS) return -1;
+    g->dx = dx;
+    g->dt = dt;
+    for (int k = 0; k < 3; ++k) {
+        g->n[k] = n[k];
+        g->origin[k] = 0.0 - dx * 0.5;
+    }
+    uint8_t* air = (uint8_t*)calloc((size_t)cells, 1);
+    g->flags = (uint8_t*)calloc((size_t)cells, 1);
+    g->bidx = (uint32_t*)malloc((size_t)cells * 4);
+    if (!air || !g->flags || !g->bidx) return -2;
+    const int64_t nx = n[0], nxy = n[0] * n[1];
+#pragma omp parallel for schedule(static)
+    for (int64_t z = 0; z < n[2]; ++z)
+        for (int64_t y = 0; y < n[1]; ++y)
+            for (int64_t x = 0; x < nx; ++x) {
+                int64_t i = z * nxy + y * nx + x;
+                double px = g->origin[0] + (double)x * dx, py = g->origin[1] + (double)y * dx,
+                       pz = g->origin[2] + (double)z * dx;
+                air[i] = x > 0 && y > 0 && z > 0 && x < n[0] - 1 && y < n[1] - 1 && z < n[2] - 1 &&
+                         room_contains(room, px, py, pz);
+            }
+    /* Serial, in cell order, as fdtd.rs:368-375 builds the boundary list. */
+    int64_t cap_b = 1024, cap_s = 4096;
+    g->b_cell = (uint32_t*)malloc(cap_b * 4);
+    g->b_start = (uint32_t*)malloc((cap_b + 1) * 4);
+    g->slots = (ob_slot*)malloc(cap_s * sizeof(ob_slot));
+    static const int64_t DXI[6] = {-1, 1, 0, 0, 0, 0}, DYI[6] = {0, 0, -1, 1, 0, 0}, DZI[6] = {0, 0, 0, 0, -1, 1};
+    for (int64_t i = 0; i < cells; ++i) {
+        g->bidx[i] = NO_BOUNDARY;
+        if (!air[i]) continue;
+        int64_t z = i / nxy, r = i % nxy, y = r / nx, x = r % nx;
+        double px = g->origin[0] + (double)x * dx, py = g->origin[1] + (double)y * dx,
+               pz = g->origin[2] + (double)z * dx;
+        uint8_t fl = AIR;
+        int32_t faces[6];
+        int nf = 0;
+        for (int k = 0; k < 6; ++k) {
+            int64_t j = (z + DZI[k]) * nxy + (y + DYI[k]) * nx + (x + DXI[k]);
+            if (air[j]) fl |= (uint8_t)(1u << k);
+            else faces[nf++] = face_fit(room, px, py, pz, k, dx);
+        }
+        g->flags[i] = fl;
+        ob_slot sl[6 * (OB_MAX_BRANCHES + 1)];
+        int ns = ob_boundary_slots(faces, nf, room->fits, dt, sl, 6 * (OB_MAX_BRANCHES + 1));
+        if (ns < 0) return -3;
+        if (ns > 0) {
+            if (g->nb + 1 >= cap_b) {
+                cap_b *= 2;
+                g->b_cell = (uint32_t*)realloc(g->b_cell, cap_b * 4);
+                g->b_start = (uint32_t*)realloc(g->b_start, (cap_b + 1) * 4);
+            }
+            while (g->ns + ns > cap_s) {
+                cap_s *= 2;
+                g->slots = (ob_slot*)realloc(g->slots, cap_s * sizeof(ob_slot));
+            }
+            g->bidx[i] = (uint32_t)g->nb;
+            g->b_cell[g->nb] = (uint32_t)i;
+            g->b_start[g->nb] = (uint32_t)g->ns;
+            memcpy(g->slots + g->ns, sl, ns * sizeof(ob_slot));
+            g->ns += ns;
+            g->nb++;
+        }
+    }
+    g->b_start[g->nb] = (uint32_t)g->ns;
+    free(air);
+    return 0;
+}
+
+/* ---------------------------------------------------------------- sources */
+
+/* fdtd.rs:601-610 */
+void ob_source_signal(double f_max, double dt, int64_t steps, double* out) {
+    double sigma = 0.342 / f_max;
+    double t0 = 6.0 * sigma;
+    for (int64_t n = 0; n < steps; ++n) {
+        double t = (double)n * dt - t0;
+        out[n] = -t / (sigma * sigma) * exp(-(t * t) / (2.0 * sigma * sigma)) * dt;
+    }
+}
+
+static int64_t gidx(const ob_grid* g, int64_t x, int64_t y, int64_t z) {
+    return (z * g->n[1] + y) * g->n[0] + x;
+}
+
+/* fdtd.rs:245-272 */
+int64_t ob_nearest_air(const ob_grid* g, const double p[3]) {
+    double q[3];
+    int64_t c[3];
+    for (int k = 0; k < 3; ++k) {
+        q[k] = (p[k] - g->origin[k]) / g->dx;
+        c[k] = (int64_t)round(q[k]);
+    }
+    for (int64_t r = 0; r < 17; ++r) {
+        int have = 0;
+        double bd = 0.0;
+        int64_t bi = -1;
+        for (int64_t dz = -r; dz <= r; ++dz)
+            for (int64_t dy = -r; dy <= r; ++dy)
+                for (int64_t ddx = -r; ddx <= r; ++ddx) {
+                    int64_t x = c[0] + ddx, y = c[1] + dy, z = c[2] + dz;
+                    if (x < 0 || y < 0 || z < 0 || x >= g->n[0] || y >= g->n[1] || z >= g->n[2]) continue;
+                    int64_t i = gidx(g, x, y, z);
+                    if (g->flags[i] & AIR) {
+                        double ex = g->origin[0] + (double)x * g->dx - p[0];
+                        double ey = g->origin[1] + (double)y * g->dx - p[1];
+                        double ez = g->origin[2] + (double)z * g->dx - p[2];
+                        double d = ex * ex + ey * ey + ez * ez;
+                        if (!have || d < bd) { have = 1; bd = d; bi = i; }
+                    }
+                }
+        if (have) return bi;
+    }
+    return -1;
+}
+
+/* fdtd.rs:467-492 */
+int ob_taps_at(const ob_grid* g, const double p[3], int64_t* cells, double* weights) {
+    double q[3], f[3], b[3];
+    for (int k = 0; k < 3; ++k) {
+        q[k] = (p[k] - g->origin[k]) / g->dx;
+        b[k] = floor(q[k]);
+        f[k] = q[k] - b[k];
+    }
+    int nt = 0;
+    for (int ix = 0; ix < 2; ++ix) {
+        double wx = ix ? f[0] : 1.0 - f[0];
+        for (int iy = 0; iy < 2; ++iy) {
+            double wy = iy ? f[1] : 1.0 - f[1];
+            for (int iz = 0; iz < 2; ++iz) {
+                double wz = iz ? f[2] : 1.0 - f[2];
+                int64_t x = (int64_t)b[0] + ix, y = (int64_t)b[1] + iy, z = (int64_t)b[2] + iz;
+                if (x < 0 || y < 0 || z < 0 || x >= g->n[0] || y >= g->n[1] || z >= g->n[2]) continue;
+                int64_t i = gidx(g, x, y, z);
+                double w = wx * wy * wz;
+                if ((g->flags[i] & AIR) && w > 1e-9) {
+                    cells[nt] = i;
+                    weights[nt] = w;
+                    nt++;
+                }
+            }
+        }
+    }
+    double sum = 0.0;
+    for (int t = 0; t < nt; ++t) sum += weights[t];
+    if (sum < 0.25) {
+        int64_t i = ob_nearest_air(g, p);
+        if (i < 0) return 0;
+        cells[0] = i;
+        weights[0] = 1.0;
+        return 1;
+    }
+    for (int t = 0; t < nt; ++t) weights[t] = weights[t] / sum;
+    return nt;
+}
+
+/* fdtd.rs:730-743 */
+int ob_source_components(const ob_grid* g, int64_t c, int order, int64_t* cc, double* cw, int* len) {
+    int64_t N = g->n[0] * g->n[1] * g->n[2];
+    cc[0] = c;
+    cw[0] = 1.0;
+    len[0] = 1;
+    if (order < 1) return 1;
+    int64_t steps[3] = {1, g->n[0], g->n[0] * g->n[1]};
+    for (int s = 0; s < 3; ++s) {
+        int64_t plus = c + steps[s];
+        if (c < steps[s]) return 0;
+        int64_t minus = c - steps[s];
+        if (plus >= N || !(g->flags[plus] & AIR) || !(g->flags[minus] & AIR)) return 0;
+        cc[2 * (s + 1)] = plus;
+        cw[2 * (s + 1)] = 1.0;
+        cc[2 * (s + 1) + 1] = minus;
+        cw[2 * (s + 1) + 1] = -1.0;
+        len[s + 1] = 2;
+    }
+    return 4;
+}
+
+/* fdtd.rs:812-815 */
+void ob_component_weights(const double a0[2], const double a[6], double k, double dx, double out[8]) {
+    cplx dip = cmk(0.0, 2.0 * sin(k * dx));
+    out[0] = a0[0];
+    out[1] = a0[1];
+    for (int i = 0; i < 3; ++i) {
+        cplx v = cdiv(cmk(a[2 * i], a[2 * i + 1]), dip);
+        out[2 + 2 * i] = v.re;
+        out[3 + 2 * i] = v.im;
+    }
+}
+
+/* Σ_n from_polar(x_n, -2π fract(f dt n)), summed in n order (fdtd.rs:680-681, :824-828). */
+static cplx dft_at(const double* x, int64_t steps, double f, double dt) {
+    cplx acc = cmk(0.0, 0.0);
+    for (int64_t n = 0; n < steps; ++n) {
+        double ph = -2.0 * PI * fract(f * dt * (double)n);
+        acc = cadd(acc, from_polar(x[n], ph));
+    }
+    return acc;
+}
+
+/* fdtd.rs:819-840 */
+void ob_to_transfer(const double* spectra, int64_t n_spec, const double* signal, int64_t steps,
+                    double dt, double dx, const double* freqs, int64_t nf, double c, double r_ref,
+                    const double* h0, double* out) {
+    cplx* s = (cplx*)malloc((size_t)nf * sizeof(cplx));
+    for (int64_t fi = 0; fi < nf; ++fi) s[fi] = dft_at(signal, steps, freqs[fi], dt);
+    double scale = dt * dt / (dx * dx * dx) * 4.0 * PI * c * c * r_ref;
+    for (int64_t i = 0; i < n_spec; ++i) {
+        int64_t fi = i % nf;
+        cplx v = cmk(spectra[2 * i], spectra[2 * i + 1]);
+        cplx t = cmul(cscale(cdiv(v, s[fi]), scale), cmk(h0[2 * fi], h0[2 * fi + 1]));
+        out[2 * i] = t.re;
+        out[2 * i + 1] = t.im;
+    }
+    free(s);
+}
+
+/* ---------------------------------------------------------------- time loop */
+
+static int run_loop(const ob_grid_view* g, const int64_t* tap_cells, const double* tap_w,
+                    int64_t n_taps, const double* signal, int64_t steps, const int64_t* recv,
+                    int64_t n_recv, double* rec, int threads, const volatile int32_t* cancel,
+                    double* p_out, double* prev_out, double* state_out) {
+    const int64_t nx = g->n[0], nxy = g->n[0] * g->n[1], cells = nxy * g->n[2];
+    const double l2 = COURANT * COURANT, lam = COURANT;
+    const int64_t offs[6] = {-1, 1, -nx, nx, -nxy, nxy};
+    double* p = (double*)calloc((size_t)cells, sizeof(double));
+    double* prev = (double*)calloc((size_t)cells, sizeof(double));
+    uint32_t* bidx = (uint32_t*)malloc((size_t)cells * 4);
+    int64_t ns = g->nb ? g->b_start[g->nb] : 0;
+    double* st = (double*)calloc((size_t)(2 * ns + 2), sizeof(double));
+    double* bnext = (double*)calloc((size_t)(g->nb + 1), sizeof(double));
+    if (!p || !prev || !bidx || !st || !bnext) return -2;
+    for (int64_t i = 0; i < cells; ++i) bidx[i] = NO_BOUNDARY;
+    for (int64_t b = 0; b < g->nb; ++b) bidx[g->b_cell[b]] = (uint32_t)b;
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#else
+    (void)threads;
+#endif
+    const uint8_t* flags = g->flags;
+    int status = 0;
+    for (int64_t n = 0; n < steps; ++n) {
+        if (n % 64 == 0 && cancel && *cancel) { status = 1; break; }
+        /* Record p^n (fdtd.rs:673-683). */
+        if (rec)
+            for (int64_t ri = 0; ri < n_recv; ++ri) rec[ri * steps + n] = p[recv[ri]];
+        /* Interior (fdtd.rs:699-708). */
+#pragma omp parallel for schedule(static)
+        for (int64_t z = 0; z < g->n[2]; ++z) {
+            int64_t base = z * nxy;
+            for (int64_t k = 0; k < nxy; ++k) {
+                int64_t i = base + k;
+                if (!(flags[i] & AIR) || bidx[i] != NO_BOUNDARY) continue;
+                uint8_t f = flags[i];
+                double sum = 0.0, kair = 0.0;
+                for (int d = 0; d < 6; ++d)
+                    if (f & (1u << d)) { sum += p[i + offs[d]]; kair += 1.0; }
+                double lap = l2 * (sum - kair * p[i]);
+                prev[i] = 2.0 * p[i] - prev[i] + lap;
+            }
+        }
+        /* Boundary nodes (fdtd.rs:710-713). */
+#pragma omp parallel for schedule(static)
+        for (int64_t b = 0; b < g->nb; ++b) {
+            int64_t i = g->b_cell[b];
+            uint8_t f = flags[i];
+            double sum = 0.0, kair = 0.0;
+            for (int d = 0; d < 6; ++d)
+                if (f & (1u << d)) { sum += p[i + offs[d]]; kair += 1.0; }
+            double lap = l2 * (sum - kair * p[i]);
+            int64_t j0 = g->b_start[b], j1 = g->b_start[b + 1];
+            bnext[b] = ob_boundary_update(g->slots + j0, (int)(j1 - j0), st + 2 * j0, p[i], prev[i],
+                                          lap, lam, g->dt);
+        }
+        /* Scatter, then inject (fdtd.rs:714-720). */
+        for (int64_t b = 0; b < g->nb; ++b) prev[g->b_cell[b]] = bnext[b];
+        for (int64_t t = 0; t < n_taps; ++t) prev[tap_cells[t]] += tap_w[t] * signal[n];
+        double* tmp = p;
+        p = prev;
+        prev = tmp;
+    }
+    if (p_out) memcpy(p_out, p, (size_t)cells * sizeof(double));
+    if (prev_out) memcpy(prev_out, prev, (size_t)cells * sizeof(double));
+    if (state_out) memcpy(state_out, st, (size_t)(2 * ns) * sizeof(double));
+    free(p);
+    free(prev);
+    free(bidx);
+    free(st);
+    free(bnext);
+    return status;
+}
+
+int ob_run(const ob_grid_view* g, const int64_t* tap_cells, const double* tap_w, int64_t n_taps,
+           const double* signal, int64_t steps, const int64_t* recv, int64_t n_recv,
+           const double* freqs, int64_t nf, double* ir_out, double* spec_out, int threads,
+           const volatile int32_t* cancel) {
+    double* rec = ir_out;
+    int own = 0;
+    if (!rec && spec_out && n_recv > 0) {
+        rec = (double*)malloc((size_t)(n_recv * steps) * sizeof(double));
+        own = 1;
+    }
+    int st = run_loop(g, tap_cells, tap_w, n_taps, signal, steps, recv, n_recv, rec, threads, cancel,
+                      NULL, NULL, NULL);
+    if (st == 0 && spec_out) {
+        /* The per-step DFT of fdtd.rs:679-682, evaluated from the recorded
+         * series in the same n order, so the sums are identical. */
+#pragma omp parallel for schedule(dynamic)
+        for (int64_t k = 0; k < n_recv * nf; ++k) {
+            int64_t ri = k / nf, fi = k % nf;
+            cplx a = dft_at(rec + ri * steps, steps, freqs[fi], g->dt);
+            spec_out[2 * k] = a.re;
+            spec_out[2 * k + 1] = a.im;
+        }
+    }
+    if (own) free(rec);
+    return st;
+}
+
+int ob_run_fields(const ob_grid_view* g, const int64_t* tap_cells, const double* tap_w,
+                  int64_t n_taps, const double* signal, int64_t steps, int threads, double* p_out,
+                  double* prev_out, double* state_out) {
+    return run_loop(g, tap_cells, tap_w, n_taps, signal, steps, NULL, 0, NULL, threads, NULL, p_out,
+                    prev_out, state_out);
+}

@@ -1,0 +1,130 @@
+/*
+ * FP64 CPU oracle for the Wavey FDTD wave-solver path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This is a plain-C restatement of the reference
+ * scheme (core/wavey-solver/src/fdtd.rs). It is used by tests/, by the smoke
+ * check and as the CPU baseline leg of bench.py, always as the checker and
+ * never as the thing measured or shipped. The product (wavey_amd/) never
+ * links or calls it.
+ *
+ * Each function cites the reference file:line it restates. Where the
+ * reference reaches a function through geometry this restatement does not
+ * have (BVH ray casts, Room::contains on sketches), the box-room special
+ * case is restated and the semantics are named at the function.
+ */
+#ifndef WAVEY_FDTD_ORACLE_H
+#define WAVEY_FDTD_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OB_MAX_BRANCHES 8
+
+/* core/wavey-materials/src/impedance_fit.rs:27-44 (Branch, AdmittanceFit). */
+typedef struct { double r, m, k; } ob_branch;
+typedef struct { double beta0; int32_t nb; ob_branch br[OB_MAX_BRANCHES]; } ob_fit;
+
+/* core/wavey-solver/src/fdtd.rs:178-187 (Slot). */
+typedef struct { double inv_d, cv, k, w; } ob_slot;
+
+/*
+ * Box room [0,L0]x[0,L1]x[0,L2] with rectangular floor-to-ceiling pillars.
+ * Surfaces: 0..5 are the walls x=0, x=L0, y=0, y=L1, floor z=0, ceiling
+ * z=L2; pillar p owns surfaces 6+4p+{0,1,2,3} = its sides facing -x, +x,
+ * -y, +y. Surface s is cut into a gu[s] x gv[s] patch grid over its (u,v)
+ * extent (walls x: u=y v=z; walls y: u=x v=z; floor/ceiling: u=x v=y;
+ * pillar sides: the same as the wall they are parallel to, over the pillar
+ * extent), and patch (iu,iv) uses fit patch_fit[patch_off[s] + iv*gu[s] + iu].
+ */
+typedef struct {
+    double L[3];
+    int32_t n_pillars;
+    const double* pillars; /* [n_pillars][4] = x0, x1, y0, y1 */
+    int32_t n_fits;
+    const ob_fit* fits;
+    const int32_t* gu;
+    const int32_t* gv;
+    const int32_t* patch_off;
+    const int32_t* patch_fit;
+} ob_room;
+
+/* core/wavey-solver/src/fdtd.rs:196-209 (FdtdGrid), boundary in CSR form. */
+typedef struct {
+    double origin[3], dx, dt;
+    int64_t n[3];
+    uint8_t* flags;   /* [N] */
+    uint32_t* bidx;   /* [N] */
+    int64_t nb;
+    uint32_t* b_cell; /* [nb] */
+    uint32_t* b_start;/* [nb+1] */
+    int64_t ns;
+    ob_slot* slots;   /* [ns] */
+} ob_grid;
+
+int ob_build_box(const ob_room* room, double c, double dx, ob_grid* out);
+void ob_grid_free(ob_grid* g);
+
+/* fdtd.rs:539-571 */
+int ob_boundary_slots(const int32_t* face_fits, int n_faces, const ob_fit* fits, double dt,
+                      ob_slot* out, int max_out);
+/* fdtd.rs:581-598; states are interleaved (u, g) pairs. */
+double ob_boundary_update(const ob_slot* slots, int n, double* states, double p, double p_prev,
+                          double lap, double lam, double dt);
+/* fdtd.rs:601-610 */
+void ob_source_signal(double f_max, double dt, int64_t steps, double* out);
+/* fdtd.rs:245-272; returns -1 when no air node is within 16 cells. */
+int64_t ob_nearest_air(const ob_grid* g, const double p[3]);
+/* fdtd.rs:467-492; returns the number of taps (0 = None). */
+int ob_taps_at(const ob_grid* g, const double p[3], int64_t* cells, double* weights);
+/* fdtd.rs:730-743; returns the number of components (0 = None).
+ * comp_cells/comp_w are [4][2]; comp_len[4]. */
+int ob_source_components(const ob_grid* g, int64_t c, int order, int64_t* comp_cells,
+                         double* comp_w, int* comp_len);
+/* fdtd.rs:812-815; a0, a[3] and out[4] are complex (re, im). */
+void ob_component_weights(const double a0[2], const double a[6], double k, double dx, double out[8]);
+/* fdtd.rs:819-840; spectra/h0/out complex interleaved. */
+void ob_to_transfer(const double* spectra, int64_t n_spec, const double* signal, int64_t steps,
+                    double dt, double dx, const double* freqs, int64_t nf, double c, double r_ref,
+                    const double* h0, double* out);
+/* fdtd.rs:157-162 */
+double ob_phase_error_deg_per_m(double f, double c, double dx);
+/* impedance_fit.rs:48-55, :73-84; material.rs:96-111 */
+void ob_admittance(const ob_fit* fit, double f, double out[2]);
+double ob_paris(double beta_re, double beta_im, int n);
+double ob_fit_real_impedance(double alpha);
+
+/*
+ * fdtd.rs:644-725 (run_cpu_with). Records the IR [n_recv][steps] when ir_out
+ * is non-null and the DFT spectra [n_recv][nf] (complex) when spec_out is
+ * non-null. The grid is passed as plain arrays (the same view the GPU library
+ * takes) so both sides can run on one grid. Returns 0, or 1 when the cancel
+ * flag was seen set at a 64-step check.
+ */
+typedef struct {
+    int64_t n[3];
+    double dx, dt;
+    const uint8_t* flags;
+    int64_t nb;
+    const uint32_t* b_cell;
+    const uint32_t* b_start;
+    const ob_slot* slots;
+} ob_grid_view;
+
+int ob_run(const ob_grid_view* g, const int64_t* tap_cells, const double* tap_w, int64_t n_taps,
+           const double* signal, int64_t steps, const int64_t* recv, int64_t n_recv,
+           const double* freqs, int64_t nf, double* ir_out, double* spec_out, int threads,
+           const volatile int32_t* cancel);
+
+/* Same loop, stopped after `steps` steps; returns both field arrays for tests. */
+int ob_run_fields(const ob_grid_view* g, const int64_t* tap_cells, const double* tap_w,
+                  int64_t n_taps, const double* signal, int64_t steps, int threads, double* p_out,
+                  double* prev_out, double* state_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,89 @@
+"""The C ABI: every function include/*.h declares is exported, and argument
+errors come back as status codes with messages (no GPU needed)."""
+import ctypes as C
+import glob
+import os
+import re
+import subprocess
+
+import numpy as np
+
+from wavey_amd import configs, fdtd
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(wv_\w+)\s*\(", text):
+            names.add(m.group(1))
+    names.discard("wv_progress_fn")
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    L = fdtd.lib()
+    names = declared_functions()
+    assert len(names) >= 20
+    out = subprocess.run(["nm", "-D", "--defined-only", fdtd.LIB_PATH], capture_output=True, text=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    missing = names - exported
+    assert not missing, missing
+    for n in names:
+        getattr(L, n)
+    assert set(fdtd.EXPORTS) <= exported
+
+
+def test_library_is_gfx950():
+    # The library built here carries a gfx950 code object.
+    with open(fdtd.LIB_PATH, "rb") as f:
+        assert b"gfx950" in f.read()
+
+
+def test_invalid_grid_is_an_error_not_a_crash():
+    L = fdtd.lib()
+    g = fdtd.WvGrid()  # null arrays
+    h = C.c_void_p()
+    err = C.create_string_buffer(256)
+    st = L.wv_plan_create(C.byref(g), 0, C.byref(h), err, 256)
+    assert st == -1 and b"invalid grid" in err.value
+    cfg = configs.shoebox(fdtd.fit_real_impedance)
+    grid = fdtd.FdtdGrid.build_box(cfg["room"], 343.0, 0.1)
+    st = L.wv_plan_create(C.byref(grid.view()), 7, C.byref(h), err, 256)
+    assert st == -1
+    # Boundary list pointing outside the grid.
+    bad = fdtd.FdtdGrid(n=grid.n, origin=grid.origin, dx=grid.dx, dt=grid.dt, flags=grid.flags,
+                        b_cell=np.array([grid.cells + 5], np.uint32), b_start=np.array([0, 1], np.uint32),
+                        slots=np.zeros((1, 4)))
+    st = L.wv_plan_create(C.byref(bad.view()), 0, C.byref(h), err, 256)
+    assert st == -1 and b"out of range" in err.value
+
+
+def test_grid_limit_is_reported():
+    room = {"L": (60.0, 60.0, 30.0), "pillars": [], "fits": [(0.1, [])], "patches": [(1, 1, [0])] * 6}
+    try:
+        fdtd.FdtdGrid.build_box(room, 343.0, 0.04)
+    except fdtd.FdtdError as e:
+        assert e.status == -3 and "M cells" in str(e)
+    else:
+        raise AssertionError("a 1.7G-cell grid must be refused")
+
+
+def test_no_device_is_a_device_error():
+    # Without a GPU (this container) a valid grid fails at the first HIP call
+    # with WV_ERR_DEVICE: the caller's CPU fallback path (run.rs:186-191).
+    from conftest import gpu_available
+
+    if gpu_available():
+        return
+    cfg = configs.shoebox(fdtd.fit_real_impedance)
+    grid = fdtd.FdtdGrid.build_box(cfg["room"], 343.0, 0.1)
+    try:
+        fdtd.Plan(grid)
+    except fdtd.FdtdError as e:
+        assert e.status in (-2, -4) and str(e)
+    else:
+        raise AssertionError("expected a device error without a GPU")

@@ -1,0 +1,92 @@
+"""GPU parity: the HIP solver against the FP64 oracle on the same seeded inputs.
+
+Tolerances (relative L2 over all receiver samples):
+* FP64 device run vs FP64 oracle: 1e-9 (same scheme; only FMA contraction and
+  summation order differ).
+* FP32 and FP32-with-FP64-state device runs vs FP64 oracle: 1e-4 (the accuracy
+  target for receiver impulse responses).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from wavey_amd import configs, fdtd, workload
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"fp64": 1e-9, "fp32": 1e-4, "fp32_state64": 1e-4}
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def oracle_case(cfg, steps):
+    og = oracle.build_box(cfg["room"], cfg["c"], cfg["dx"])
+    sig = oracle.source_signal(cfg["f_src"], og.dt, steps)
+    rx = [oracle.nearest_air(og, p) for p in cfg["receivers"]]
+    runs = []
+    for s in cfg["sources"]:
+        if s["kind"] == "omni":
+            runs.append(oracle.taps_at(og, s["pos"]))
+        else:
+            node = oracle.nearest_air(og, s["pos"])
+            runs.extend(oracle.source_components(og, node, 1))
+    return og, sig, rx, runs
+
+
+@pytest.fixture(scope="module")
+def shoebox():
+    cfg = configs.shoebox(fdtd.fit_real_impedance)
+    case = workload.build_case(cfg)
+    og, sig, rx, runs = oracle_case(cfg, cfg["steps"])
+    assert case.receivers == rx and case.runs[0][2] == runs[0]
+    ref, spec, _ = oracle.run(og, runs[0], sig, rx, freqs=[31.25, 62.5, 125.0, 250.0], threads=8)
+    return case, ref, spec
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "fp32_state64"])
+def test_shoebox_ir_matches_oracle(shoebox, prec):
+    case, ref, _ = shoebox
+    print(f"case: shoebox 42x32x27 2000 steps {prec}", flush=True)
+    ir = fdtd.Gpu(prec).run_ir(case.grid, case.runs[0][2], case.signal, case.receivers)
+    err = rel_l2(ir, ref)
+    print(f"relative L2 {err:.3e}")
+    assert ir.shape == ref.shape
+    assert err < TOL[prec]
+    for r in range(len(ref)):
+        assert rel_l2(ir[r], ref[r]) < 10 * TOL[prec]
+
+
+def test_shoebox_spectra_match_oracle(shoebox):
+    case, _, spec = shoebox
+    freqs = [31.25, 62.5, 125.0, 250.0]
+    out = fdtd.Gpu("fp64").run_fdtd(case.grid, case.runs[0][2], case.signal, case.receivers, freqs)
+    assert out.shape == spec.shape
+    assert np.max(np.abs(out - spec)) / np.max(np.abs(spec)) < 1e-9
+
+
+@pytest.fixture(scope="module")
+def pillared_mini():
+    # The pillared room's materials, pillars, cardioid and receivers on a
+    # coarser grid (dx 0.08 m: 152 x 115 x 52) and 1500 steps.
+    cfg = configs.pillared_room()
+    cfg = dict(cfg, dx=0.08, steps=1500)
+    case = workload.build_case(cfg)
+    og, sig, rx, runs = oracle_case(cfg, cfg["steps"])
+    assert case.receivers == rx
+    assert [r[2] for r in case.runs] == runs
+    refs = [oracle.run(og, taps, sig, rx, threads=8)[0] for taps in runs]
+    return case, refs
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "fp32_state64"])
+def test_pillared_room_components_match_oracle(pillared_mini, prec):
+    case, refs = pillared_mini
+    print(f"case: pillared room dx 0.08 1500 steps, 4 cardioid components, {prec}", flush=True)
+    with fdtd.Plan(case.grid, prec) as plan:
+        for (_, ci, taps), ref in zip(case.runs, refs):
+            ir, _ = plan.run(taps, case.signal, case.receivers, mode=fdtd.MODE_IR)
+            err = rel_l2(ir, ref)
+            print(f"component {ci}: relative L2 {err:.3e}")
+            assert err < TOL[prec]

@@ -1,0 +1,968 @@
+// MI355X (gfx950) FDTD wave-solver run: device layout, kernels, run loop.
+//
+// The scheme is Wavey's (core/wavey-solver/src/fdtd.rs:1-30): the 3D
+// leapfrog p+ = (2 - K l2) p + l2 sum_air p_nb - p-, with locally reacting
+// frequency-dependent boundaries solved per node in closed form
+// (fdtd.rs:581-598). The layout is this build's own:
+//
+//  * Fields p, prev live in a pitched grid: row pitch = nx rounded up to 64
+//    cells, so every 64-lane wave reads one aligned 256-byte row segment.
+//  * An interior bitmap (one bit per pitched cell) marks plain air cells:
+//    six air neighbours, no boundary slots, no source tap. They are updated
+//    by work-groups that march through z with the z-neighbours held in
+//    registers (the column's p at z-1, z, z+1), reading p, prev and the
+//    bitmap once per cell and writing prev once: 12 bytes per FP32 cell.
+//  * Every other air cell (and any source tap) is a "special" cell, found by
+//    a wave-ballot / prefix-sum compaction over the grid and stored in cell
+//    order as structure-of-arrays. Special cells carry their neighbour mask,
+//    their slot range (structure-of-arrays slot constants and state) and
+//    their merged tap weight, so injection needs no launch of its own.
+//  * One fused launch per time step: interior tiles, special cells and
+//    receiver recording are disjoint block ranges of the same grid. Within
+//    a step p is only read and each cell of prev is written by one thread,
+//    so the parts need no ordering between them.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "wavey_fdtd.h"
+#include "wv_internal.h"
+
+using namespace wv;
+
+namespace {
+
+struct DevErr {
+    std::string msg;
+    wv_status code;
+};
+
+#define WV_HIP(expr)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            throw DevErr{std::string(#expr) + ": " + hipGetErrorString(e_),                      \
+                         e_ == hipErrorOutOfMemory ? WV_ERR_NOMEM : WV_ERR_DEVICE};               \
+    } while (0)
+
+// Owned device allocation.
+struct DBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void alloc(size_t n) {
+        if (n <= bytes && p) return;
+        release();
+        size_t b = std::max<size_t>(n, 256);
+        WV_HIP(hipMalloc(&p, b));
+        bytes = b;
+    }
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+constexpr uint32_t kBlock = 256;
+constexpr uint32_t kTileX = 64;
+constexpr uint32_t kTileY = 4;
+constexpr uint32_t kStepsPerBlock = 128;  // cancel/progress granularity, as gpu_fdtd.rs:35
+constexpr uint32_t kMetaSolid = 1u << 7;  // a tap on a non-air cell (no update, injection only)
+
+// ------------------------------------------------------------------ kernels
+
+template <typename T, typename S>
+struct StepParams {
+    const T* __restrict__ p;
+    T* __restrict__ prev;
+    const uint32_t* __restrict__ interior;
+    uint32_t nx, ny, nz, pitch, sz, wpr;
+    uint32_t tiles_x, tiles_y, zchunk;
+    uint32_t main_blocks, main_remap, main_per_xcd;
+    uint32_t n_special, special_blocks;
+    const uint32_t* __restrict__ sp_cell;
+    const uint32_t* __restrict__ sp_meta;
+    const uint32_t* __restrict__ sp_slot0;
+    const S* __restrict__ sp_tapw;
+    const S* __restrict__ sl_inv_d;
+    const S* __restrict__ sl_cv;
+    const S* __restrict__ sl_k;
+    const S* __restrict__ sl_w;
+    S* __restrict__ st_u;
+    S* __restrict__ st_g;
+    uint32_t has_k;
+    uint32_t n_recv;
+    const uint32_t* __restrict__ recv;
+    T* __restrict__ ir;  // this step's receiver row
+    T l2, c2;            // interior: p+ = c2 p + l2 sum - p-
+    S l2s, lam, half_dt, s_n;
+};
+
+// Interior: one thread per (x, y) column of a 64 x 4 tile, marching z over
+// [z0, z1). p at z-1 and z stay in registers; p at z+1 is the one new load
+// of the column per plane. x and y neighbours are loads of rows the same
+// work-group (or its neighbours) reads in the same plane, served by cache.
+template <typename T, typename S>
+__device__ __forceinline__ void interior_part(const StepParams<T, S>& a, uint32_t t) {
+    const uint32_t tx = t % a.tiles_x;
+    const uint32_t r = t / a.tiles_x;
+    const uint32_t ty = r % a.tiles_y;
+    const uint32_t zc = r / a.tiles_y;
+    const uint32_t x = tx * kTileX + (threadIdx.x & (kTileX - 1));
+    const uint32_t y = ty * kTileY + (threadIdx.x / kTileX);
+    if (x >= a.nx || y >= a.ny) return;
+    const uint32_t z0 = 1 + zc * a.zchunk;
+    const uint32_t z1 = min(z0 + a.zchunk, a.nz - 1);
+    uint32_t i = z0 * a.sz + y * a.pitch + x;
+    const uint32_t* bm = a.interior + (z0 * a.ny + y) * a.wpr + (x >> 5);
+    const uint32_t bstride = a.ny * a.wpr;
+    const uint32_t bit = x & 31u;
+    T zm = a.p[i - a.sz];
+    T zc_ = a.p[i];
+    for (uint32_t z = z0; z < z1; ++z) {
+        const T zp = a.p[i + a.sz];
+        const uint32_t w = *bm;
+        if ((w >> bit) & 1u) {
+            const T xm = a.p[i - 1], xp = a.p[i + 1], ym = a.p[i - a.pitch], yp = a.p[i + a.pitch];
+            const T q = a.prev[i];
+            if constexpr (sizeof(T) == 8) {
+                // FP64: the reference's form, neighbour order -x, +x, -y, +y,
+                // -z, +z (fdtd.rs:685-706).
+                T s = xm;
+                s += xp;
+                s += ym;
+                s += yp;
+                s += zm;
+                s += zp;
+                a.prev[i] = a.c2 * zc_ + (a.l2 * s - q);
+            } else {
+                // FP32: the same update as p + (v + l2 * sum (p_nb - p)) with
+                // v = p - p-. Neighbour differences and v are exact for close
+                // values (Sterbenz), so a near-uniform field rounds only in
+                // the final add instead of in l2 * sum ~ 2p: no correlated
+                // rounding drives the room's DC mode.
+                const T d = ((xm - zc_) + (xp - zc_)) + ((ym - zc_) + (yp - zc_)) + ((zm - zc_) + (zp - zc_));
+                a.prev[i] = zc_ + ((zc_ - q) + a.l2 * d);
+            }
+        }
+        zm = zc_;
+        zc_ = zp;
+        i += a.sz;
+        bm += bstride;
+    }
+}
+
+// Special cells: general neighbour mask, boundary slots, source taps.
+template <typename T, typename S>
+__device__ __forceinline__ void special_part(const StepParams<T, S>& a, uint32_t b) {
+    const uint32_t s = b * kBlock + threadIdx.x;
+    if (s >= a.n_special) return;
+    const uint32_t i = a.sp_cell[s];
+    const uint32_t meta = a.sp_meta[s];
+    const S pp = (S)a.prev[i];
+    S next;
+    if (meta & kMetaSolid) {
+        next = pp;  // a tap on a solid cell: the reference only injects there
+    } else {
+        const S pc = (S)a.p[i];
+        const uint32_t ns = meta >> 8;
+        S lap;
+        if constexpr (sizeof(S) == 8) {
+            // fdtd.rs:685-696: l2 (sum_air - k p), neighbours in -x..+z order.
+            S sum = 0, k = 0;
+            if (meta & 1u) { sum += (S)a.p[i - 1]; k += 1; }
+            if (meta & 2u) { sum += (S)a.p[i + 1]; k += 1; }
+            if (meta & 4u) { sum += (S)a.p[i - a.pitch]; k += 1; }
+            if (meta & 8u) { sum += (S)a.p[i + a.pitch]; k += 1; }
+            if (meta & 16u) { sum += (S)a.p[i - a.sz]; k += 1; }
+            if (meta & 32u) { sum += (S)a.p[i + a.sz]; k += 1; }
+            lap = a.l2s * (sum - k * pc);
+        } else {
+            // FP32: sum of neighbour differences (see the interior update).
+            S d = 0;
+            if (meta & 1u) d += (S)a.p[i - 1] - pc;
+            if (meta & 2u) d += (S)a.p[i + 1] - pc;
+            if (meta & 4u) d += (S)a.p[i - a.pitch] - pc;
+            if (meta & 8u) d += (S)a.p[i + a.pitch] - pc;
+            if (meta & 16u) d += (S)a.p[i - a.sz] - pc;
+            if (meta & 32u) d += (S)a.p[i + a.sz] - pc;
+            lap = a.l2s * d;
+        }
+        if (ns == 0) {
+            if constexpr (sizeof(S) == 8) next = S(2) * pc - pp + lap;
+            else next = pc + ((pc - pp) + lap);
+        } else {
+            // fdtd.rs:581-598: each slot's u+ is linear in p+, so p+ is solved
+            // first from the summed face flux, then the states advance.
+            const uint32_t j0 = a.sp_slot0[s], j1 = j0 + ns;
+            S s1 = 0, s2 = 0;
+            for (uint32_t j = j0; j < j1; ++j) {
+                const S inv_d = a.sl_inv_d[j], w = a.sl_w[j];
+                const S u = a.st_u[j];
+                S e = a.sl_cv[j] * u + S(0.5) * pc + w * pp;
+                if (a.has_k) e -= a.sl_k[j] * a.st_g[j];
+                e *= inv_d;
+                s1 += (S(0.5) - w) * inv_d;
+                s2 += e - u;
+            }
+            if constexpr (sizeof(S) == 8) {
+                next = (S(2) * pc - pp + lap - a.lam * s2) / (S(1) + a.lam * s1);
+            } else {
+                // The same solve written as pc + (small terms): pc + ((pc - pp)
+                // + lap - lam (s2 + s1 pc)) / (1 + lam s1).
+                next = pc + ((pc - pp) + lap - a.lam * (s2 + s1 * pc)) / (S(1) + a.lam * s1);
+            }
+            for (uint32_t j = j0; j < j1; ++j) {
+                const S inv_d = a.sl_inv_d[j], w = a.sl_w[j];
+                const S u = a.st_u[j];
+                S e = a.sl_cv[j] * u + S(0.5) * pc + w * pp;
+                S g = 0;
+                if (a.has_k) {
+                    g = a.st_g[j];
+                    e -= a.sl_k[j] * g;
+                }
+                e *= inv_d;
+                const S un = (S(0.5) - w) * inv_d * next + e;
+                if (a.has_k) a.st_g[j] = g + a.half_dt * (un + u);
+                a.st_u[j] = un;
+            }
+        }
+    }
+    next += a.sp_tapw[s] * a.s_n;  // injection after the update (fdtd.rs:718-720)
+    a.prev[i] = (T)next;
+}
+
+template <typename T, typename S>
+__global__ __launch_bounds__(kBlock) void k_step(StepParams<T, S> a) {
+    uint32_t b = blockIdx.x;
+    if (b < a.main_blocks) {
+        // Blocks go round-robin to the 8 XCDs; give each XCD a contiguous
+        // run of tiles so y-neighbour rows are shared within one L2.
+        uint32_t t = b;
+        if (b < a.main_remap) t = (b & 7u) * a.main_per_xcd + (b >> 3);
+        interior_part(a, t);
+        return;
+    }
+    b -= a.main_blocks;
+    if (b < a.special_blocks) {
+        special_part(a, b);
+        return;
+    }
+    b -= a.special_blocks;
+    const uint32_t r = b * kBlock + threadIdx.x;
+    if (r < a.n_recv) a.ir[r] = a.p[a.recv[r]];  // p^n before its update (fdtd.rs:673-683)
+}
+
+// Marks cells that must not take the interior path (boundary nodes, taps).
+__global__ void k_mark(uint32_t* __restrict__ force, const uint32_t* __restrict__ cells, uint32_t n,
+                       uint32_t nx, uint32_t pitch) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t c = cells[t];
+    const uint32_t q = (c / nx) * pitch + c % nx;
+    atomicOr(&force[q >> 5], 1u << (q & 31u));
+}
+
+struct Cls {
+    bool interior, special;
+};
+
+// Neighbour bits of cell (x, y, z) that stay inside the grid: flags from a
+// caller can never send a read outside the field arrays.
+__device__ __forceinline__ uint32_t inside_mask(uint32_t x, uint32_t y, uint32_t z, uint32_t nx, uint32_t ny,
+                                                uint32_t nz) {
+    return (x > 0 ? 1u : 0u) | (x + 1 < nx ? 2u : 0u) | (y > 0 ? 4u : 0u) | (y + 1 < ny ? 8u : 0u) |
+           (z > 0 ? 16u : 0u) | (z + 1 < nz ? 32u : 0u);
+}
+
+__device__ __forceinline__ Cls classify(const uint8_t* __restrict__ flags, const uint32_t* __restrict__ force,
+                                        uint64_t q, uint64_t npitched, uint32_t pitch, uint32_t nx, uint32_t ny,
+                                        uint32_t nz) {
+    Cls c{false, false};
+    if (q >= npitched) return c;
+    const uint32_t x = (uint32_t)(q % pitch);
+    if (x >= nx) return c;
+    const uint64_t row = q / pitch;
+    const uint8_t f = flags[row * nx + x];
+    const bool forced = (force[q >> 5] >> (q & 31u)) & 1u;
+    const bool air = f & 64u;
+    const uint32_t m = f & 63u & inside_mask(x, (uint32_t)(row % ny), (uint32_t)(row / ny), nx, ny, nz);
+    c.interior = air && m == 63u && !forced;
+    c.special = (air && !c.interior) || (forced && !air);
+    return c;
+}
+
+// Pass 1 of the compaction: interior bitmap words straight from the wave
+// ballot, and the number of special cells per 256-cell block.
+__global__ __launch_bounds__(kBlock) void k_classify_count(const uint8_t* __restrict__ flags,
+                                                           const uint32_t* __restrict__ force,
+                                                           uint32_t* __restrict__ interior,
+                                                           uint32_t* __restrict__ counts, uint64_t npitched,
+                                                           uint32_t pitch, uint32_t nx, uint32_t ny, uint32_t nz) {
+    __shared__ uint32_t wc[kBlock / 64];
+    const uint64_t q = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    const Cls c = classify(flags, force, q, npitched, pitch, nx, ny, nz);
+    const uint64_t bin = __ballot(c.interior);
+    const uint64_t bsp = __ballot(c.special);
+    const uint32_t lane = threadIdx.x & 63u;
+    if (q < npitched) {
+        if (lane == 0) interior[q >> 5] = (uint32_t)bin;
+        if (lane == 32) interior[q >> 5] = (uint32_t)(bin >> 32);
+    }
+    if (lane == 0) wc[threadIdx.x >> 6] = (uint32_t)__popcll(bsp);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t s = 0;
+        for (uint32_t w = 0; w < kBlock / 64; ++w) s += wc[w];
+        counts[blockIdx.x] = s;
+    }
+}
+
+// Exclusive scan of the block counts in one work-group.
+__global__ __launch_bounds__(1024) void k_scan(const uint32_t* __restrict__ counts, uint32_t* __restrict__ offs,
+                                               uint32_t n, uint32_t* __restrict__ total) {
+    __shared__ uint32_t part[1024];
+    const uint32_t per = (n + 1023u) / 1024u;
+    const uint32_t lo = min(threadIdx.x * per, n), hi = min(lo + per, n);
+    uint32_t s = 0;
+    for (uint32_t i = lo; i < hi; ++i) s += counts[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        const uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t base = threadIdx.x ? part[threadIdx.x - 1] : 0u;
+    for (uint32_t i = lo; i < hi; ++i) {
+        offs[i] = base;
+        base += counts[i];
+    }
+    if (threadIdx.x == 1023) *total = part[1023];
+}
+
+__device__ __forceinline__ int64_t find_sorted(const uint32_t* __restrict__ a, uint32_t n, uint32_t key) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return (lo < n && a[lo] == key) ? (int64_t)lo : -1;
+}
+
+// Pass 2: each special cell's rank = block offset + earlier waves + earlier
+// lanes (mbcnt-style popcount of the ballot below the lane).
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_classify_write(
+    const uint8_t* __restrict__ flags, const uint32_t* __restrict__ force, const uint32_t* __restrict__ offs,
+    uint64_t npitched, uint32_t pitch, uint32_t nx, uint32_t ny, uint32_t nz,
+    const uint32_t* __restrict__ b_cell_sorted,
+    const uint32_t* __restrict__ b_node_sorted, uint32_t nb, const uint32_t* __restrict__ b_start,
+    const uint32_t* __restrict__ tap_cell, const S* __restrict__ tap_w, uint32_t n_tap,
+    uint32_t* __restrict__ sp_cell, uint32_t* __restrict__ sp_meta, uint32_t* __restrict__ sp_slot0,
+    S* __restrict__ sp_tapw) {
+    __shared__ uint32_t wc[kBlock / 64];
+    const uint64_t q = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    const Cls c = classify(flags, force, q, npitched, pitch, nx, ny, nz);
+    const uint64_t bsp = __ballot(c.special);
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    if (lane == 0) wc[wave] = (uint32_t)__popcll(bsp);
+    __syncthreads();
+    if (!c.special) return;
+    uint32_t o = offs[blockIdx.x];
+    for (uint32_t w = 0; w < wave; ++w) o += wc[w];
+    o += (uint32_t)__popcll(bsp & ((1ull << lane) - 1ull));
+    const uint32_t x = (uint32_t)(q % pitch);
+    const uint64_t row = q / pitch;
+    const uint32_t ref = (uint32_t)(row * nx + x);
+    const uint8_t f = flags[ref];
+    const uint32_t m = f & 63u & inside_mask(x, (uint32_t)(row % ny), (uint32_t)(row / ny), nx, ny, nz);
+    uint32_t meta = (f & 64u) ? m : kMetaSolid;
+    uint32_t slot0 = 0;
+    if (f & 64u) {
+        const int64_t b = find_sorted(b_cell_sorted, nb, ref);
+        if (b >= 0) {
+            const uint32_t node = b_node_sorted[b];
+            slot0 = b_start[node];
+            meta |= (b_start[node + 1] - slot0) << 8;
+        }
+    }
+    const int64_t t = find_sorted(tap_cell, n_tap, ref);
+    sp_cell[o] = (uint32_t)q;
+    sp_meta[o] = meta;
+    sp_slot0[o] = slot0;
+    sp_tapw[o] = t >= 0 ? tap_w[t] : S(0);
+}
+
+// Receiver spectra from the recorded series, FP64: sum_n p^n e^{-j 2 pi fract(f dt n)}
+// in n order (fdtd.rs:679-682).
+template <typename T>
+__global__ void k_dft(const T* __restrict__ ir, uint32_t steps, uint32_t n_recv, const double* __restrict__ freqs,
+                      uint32_t nf, double dt, double* __restrict__ out) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_recv * nf) return;
+    const uint32_t r = t / nf, fi = t % nf;
+    const double fdt = freqs[fi] * dt;
+    double re = 0.0, im = 0.0;
+    for (uint32_t n = 0; n < steps; ++n) {
+        const double x = fdt * (double)n;
+        const double ph = -2.0 * 3.141592653589793 * (x - trunc(x));
+        double sv, cv;
+        sincos(ph, &sv, &cv);
+        const double v = (double)ir[(size_t)n * n_recv + r];
+        re += v * cv;
+        im += v * sv;
+    }
+    out[2 * t] = re;
+    out[2 * t + 1] = im;
+}
+
+template <typename T>
+__global__ void k_fill(T* __restrict__ p, size_t n, T v) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+}  // namespace
+
+// ------------------------------------------------------------------ plan
+
+struct wv_plan {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int prec = WV_FP32;
+    uint32_t nx = 0, ny = 0, nz = 0, pitch = 0, wpr = 0, sz = 0;
+    uint64_t npitched = 0, cells = 0, air = 0, nb = 0, ns = 0, ns_k = 0;
+    double dx = 0, dt = 0;
+    uint32_t zchunk = 0;
+    uint32_t n_blocks_cls = 0;
+
+    DBuf flags, force_base, force, interior, counts, offs, total;
+    DBuf b_cell_sorted, b_node_sorted, b_start;
+    DBuf sl_inv_d, sl_cv, sl_k, sl_w, st_u, st_g;
+    DBuf fa, fb;
+    DBuf sp_cell, sp_meta, sp_slot0, sp_tapw;
+    DBuf tap_cell, tap_w, recv, ir, freqs, spec;
+    uint32_t n_special = 0;
+
+    // Current run.
+    bool active = false;
+    size_t steps = 0, done = 0;
+    uint32_t n_recv = 0;
+    std::vector<double> signal;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    double last_ms = 0.0;
+    uint64_t last_steps = 0, last_launches = 0;
+
+    size_t tsize() const { return prec == WV_FP64 ? 8 : 4; }
+    size_t ssize() const { return prec == WV_FP32 ? 4 : 8; }
+    ~wv_plan() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto e : {ev[0], ev[1], t0, t1})
+            if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+// Host -> device copy of a vector (synchronous on the plan's stream).
+template <class V>
+void upload(wv_plan* pl, DBuf& d, const V& v) {
+    d.alloc(v.size() * sizeof(v[0]));
+    if (!v.empty())
+        WV_HIP(hipMemcpyAsync(d.p, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice, pl->stream));
+}
+
+template <class S>
+std::vector<S> cast_vec(const std::vector<double>& v) {
+    return std::vector<S>(v.begin(), v.end());
+}
+
+void plan_setup(wv_plan* pl, const wv_grid* g) {
+    WV_HIP(hipGetDevice(&pl->device));
+    WV_HIP(hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking));
+    WV_HIP(hipEventCreate(&pl->ev[0]));
+    WV_HIP(hipEventCreate(&pl->ev[1]));
+    WV_HIP(hipEventCreate(&pl->t0));
+    WV_HIP(hipEventCreate(&pl->t1));
+    pl->nx = (uint32_t)g->n[0];
+    pl->ny = (uint32_t)g->n[1];
+    pl->nz = (uint32_t)g->n[2];
+    pl->cells = g->n[0] * g->n[1] * g->n[2];
+    pl->pitch = (pl->nx + 63u) & ~63u;
+    pl->wpr = pl->pitch / 32u;
+    pl->sz = pl->pitch * pl->ny;
+    pl->npitched = (uint64_t)pl->sz * pl->nz;
+    pl->dx = g->dx;
+    pl->dt = g->dt;
+    const uint64_t cells = pl->cells;
+    WV_HIP(hipSetDevice(pl->device));
+
+    // Flags as given; the device layout is derived from them per run.
+    pl->flags.alloc(cells);
+    WV_HIP(hipMemcpyAsync(pl->flags.p, g->flags, cells, hipMemcpyHostToDevice, pl->stream));
+    uint64_t air = 0;
+    for (uint64_t i = 0; i < cells; ++i) air += (g->flags[i] >> 6) & 1u;
+    pl->air = air;
+
+    // Boundary nodes sorted by cell (the reference builds them in cell order;
+    // any order is accepted), with their CSR slot ranges.
+    const uint64_t nb = g->n_boundary;
+    pl->nb = nb;
+    std::vector<uint32_t> order(nb);
+    std::iota(order.begin(), order.end(), 0u);
+    bool sorted = true;
+    for (uint64_t b = 1; b < nb; ++b)
+        if (g->b_cell[b] <= g->b_cell[b - 1]) { sorted = false; break; }
+    if (!sorted)
+        std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return g->b_cell[a] < g->b_cell[b]; });
+    std::vector<uint32_t> bcell(nb), bstart(g->b_start, g->b_start + nb + 1);
+    for (uint64_t b = 0; b < nb; ++b) bcell[b] = g->b_cell[order[b]];
+    upload(pl, pl->b_cell_sorted, bcell);
+    upload(pl, pl->b_node_sorted, order);
+    upload(pl, pl->b_start, bstart);
+    const uint64_t ns = nb ? g->b_start[nb] : 0;
+    pl->ns = ns;
+    std::vector<double> inv_d(ns), cv(ns), kk(ns), w(ns);
+    uint64_t ns_k = 0;
+    for (uint64_t j = 0; j < ns; ++j) {
+        inv_d[j] = g->slots[j].inv_d;
+        cv[j] = g->slots[j].cv;
+        kk[j] = g->slots[j].k;
+        w[j] = g->slots[j].w;
+        ns_k += g->slots[j].k != 0.0;
+    }
+    pl->ns_k = ns_k;
+    if (pl->prec == WV_FP32) {
+        upload(pl, pl->sl_inv_d, cast_vec<float>(inv_d));
+        upload(pl, pl->sl_cv, cast_vec<float>(cv));
+        upload(pl, pl->sl_k, cast_vec<float>(kk));
+        upload(pl, pl->sl_w, cast_vec<float>(w));
+    } else {
+        upload(pl, pl->sl_inv_d, inv_d);
+        upload(pl, pl->sl_cv, cv);
+        upload(pl, pl->sl_k, kk);
+        upload(pl, pl->sl_w, w);
+    }
+    pl->st_u.alloc(ns * pl->ssize());
+    pl->st_g.alloc(ns * pl->ssize());
+
+    // Boundary marks, reused by every run.
+    const uint64_t words = pl->npitched / 32;
+    pl->force_base.alloc(words * 4);
+    pl->force.alloc(words * 4);
+    pl->interior.alloc(words * 4);
+    WV_HIP(hipMemsetAsync(pl->force_base.p, 0, words * 4, pl->stream));
+    if (nb)
+        hipLaunchKernelGGL(k_mark, dim3(div_up(nb, 256)), dim3(256), 0, pl->stream, pl->force_base.as<uint32_t>(),
+                           pl->b_cell_sorted.as<uint32_t>(), (uint32_t)nb, pl->nx, pl->pitch);
+    WV_HIP(hipGetLastError());
+    pl->n_blocks_cls = div_up(pl->npitched, kBlock);
+    pl->counts.alloc((size_t)pl->n_blocks_cls * 4);
+    pl->offs.alloc((size_t)pl->n_blocks_cls * 4);
+    pl->total.alloc(4);
+    pl->fa.alloc(pl->npitched * pl->tsize());
+    pl->fb.alloc(pl->npitched * pl->tsize());
+    WV_HIP(hipStreamSynchronize(pl->stream));
+}
+
+template <class T>
+void fill_async(wv_plan* pl, DBuf& d, size_t n) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_fill<T>, dim3(div_up(n, 256)), dim3(256), 0, pl->stream, d.as<T>(), n, T(0));
+    WV_HIP(hipGetLastError());
+}
+
+template <class S>
+void run_begin_impl(wv_plan* pl, const wv_tap* taps, size_t n_taps, const double* signal, size_t steps,
+                    const uint64_t* receivers, size_t n_recv) {
+    // Taps: one per cell, weights of a cell given twice added (gpu_fdtd.rs:67-75), sorted by cell.
+    std::vector<std::pair<uint32_t, double>> merged;
+    for (size_t t = 0; t < n_taps; ++t) {
+        auto it = std::find_if(merged.begin(), merged.end(),
+                               [&](const std::pair<uint32_t, double>& m) { return m.first == taps[t].cell; });
+        if (it != merged.end()) it->second += taps[t].w;
+        else merged.emplace_back((uint32_t)taps[t].cell, taps[t].w);
+    }
+    std::sort(merged.begin(), merged.end());
+    std::vector<uint32_t> tc(merged.size());
+    std::vector<S> tw(merged.size());
+    for (size_t t = 0; t < merged.size(); ++t) {
+        tc[t] = merged[t].first;
+        tw[t] = (S)merged[t].second;
+    }
+    upload(pl, pl->tap_cell, tc);
+    upload(pl, pl->tap_w, tw);
+    const uint64_t words = pl->npitched / 32;
+    WV_HIP(hipMemcpyAsync(pl->force.p, pl->force_base.p, words * 4, hipMemcpyDeviceToDevice, pl->stream));
+    if (!tc.empty())
+        hipLaunchKernelGGL(k_mark, dim3(div_up(tc.size(), 256)), dim3(256), 0, pl->stream, pl->force.as<uint32_t>(),
+                           pl->tap_cell.as<uint32_t>(), (uint32_t)tc.size(), pl->nx, pl->pitch);
+    hipLaunchKernelGGL(k_classify_count, dim3(pl->n_blocks_cls), dim3(kBlock), 0, pl->stream,
+                       pl->flags.as<uint8_t>(), pl->force.as<uint32_t>(), pl->interior.as<uint32_t>(),
+                       pl->counts.as<uint32_t>(), pl->npitched, pl->pitch, pl->nx, pl->ny, pl->nz);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, pl->stream, pl->counts.as<uint32_t>(), pl->offs.as<uint32_t>(),
+                       pl->n_blocks_cls, pl->total.as<uint32_t>());
+    WV_HIP(hipGetLastError());
+    uint32_t n_special = 0;
+    WV_HIP(hipMemcpyAsync(&n_special, pl->total.p, 4, hipMemcpyDeviceToHost, pl->stream));
+    WV_HIP(hipStreamSynchronize(pl->stream));
+    pl->n_special = n_special;
+    pl->sp_cell.alloc((size_t)n_special * 4);
+    pl->sp_meta.alloc((size_t)n_special * 4);
+    pl->sp_slot0.alloc((size_t)n_special * 4);
+    pl->sp_tapw.alloc((size_t)n_special * sizeof(S));
+    hipLaunchKernelGGL(k_classify_write<S>, dim3(pl->n_blocks_cls), dim3(kBlock), 0, pl->stream,
+                       pl->flags.as<uint8_t>(), pl->force.as<uint32_t>(), pl->offs.as<uint32_t>(), pl->npitched,
+                       pl->pitch, pl->nx, pl->ny, pl->nz, pl->b_cell_sorted.as<uint32_t>(), pl->b_node_sorted.as<uint32_t>(),
+                       (uint32_t)pl->nb, pl->b_start.as<uint32_t>(), pl->tap_cell.as<uint32_t>(), pl->tap_w.as<S>(),
+                       (uint32_t)tc.size(), pl->sp_cell.as<uint32_t>(), pl->sp_meta.as<uint32_t>(),
+                       pl->sp_slot0.as<uint32_t>(), pl->sp_tapw.as<S>());
+    WV_HIP(hipGetLastError());
+    // Receivers: pitched cell indices.
+    std::vector<uint32_t> rc(n_recv);
+    for (size_t r = 0; r < n_recv; ++r) {
+        const uint64_t c = receivers[r];
+        rc[r] = (uint32_t)((c / pl->nx) * pl->pitch + c % pl->nx);
+    }
+    upload(pl, pl->recv, rc);
+    pl->ir.alloc(std::max<size_t>(1, steps * n_recv) * pl->tsize());
+    // Zeroed fields, state and records.
+    if (pl->prec == WV_FP64) {
+        fill_async<double>(pl, pl->fa, pl->npitched);
+        fill_async<double>(pl, pl->fb, pl->npitched);
+        fill_async<double>(pl, pl->ir, steps * n_recv);
+    } else {
+        fill_async<float>(pl, pl->fa, pl->npitched);
+        fill_async<float>(pl, pl->fb, pl->npitched);
+        fill_async<float>(pl, pl->ir, steps * n_recv);
+    }
+    fill_async<S>(pl, pl->st_u, pl->ns);
+    fill_async<S>(pl, pl->st_g, pl->ns);
+    WV_HIP(hipStreamSynchronize(pl->stream));
+    pl->signal.assign(signal, signal + steps);
+    pl->steps = steps;
+    pl->done = 0;
+    pl->n_recv = (uint32_t)n_recv;
+    pl->active = true;
+}
+
+template <class T, class S>
+StepParams<T, S> make_params(wv_plan* pl) {
+    StepParams<T, S> a{};
+    a.interior = pl->interior.as<uint32_t>();
+    a.nx = pl->nx;
+    a.ny = pl->ny;
+    a.nz = pl->nz;
+    a.pitch = pl->pitch;
+    a.sz = pl->sz;
+    a.wpr = pl->wpr;
+    a.tiles_x = div_up(pl->nx, kTileX);
+    a.tiles_y = div_up(pl->ny, kTileY);
+    uint32_t zc = pl->zchunk;
+    if (zc == 0) zc = 32;
+    const uint32_t zint = pl->nz > 2 ? pl->nz - 2 : 0;
+    a.zchunk = zc;
+    const uint32_t nzc = zint ? div_up(zint, zc) : 0;
+    a.main_blocks = a.tiles_x * a.tiles_y * nzc;
+    a.main_remap = a.main_blocks & ~7u;
+    a.main_per_xcd = a.main_remap / 8;
+    a.n_special = pl->n_special;
+    a.special_blocks = div_up(pl->n_special, kBlock);
+    a.sp_cell = pl->sp_cell.as<uint32_t>();
+    a.sp_meta = pl->sp_meta.as<uint32_t>();
+    a.sp_slot0 = pl->sp_slot0.as<uint32_t>();
+    a.sp_tapw = pl->sp_tapw.as<S>();
+    a.sl_inv_d = pl->sl_inv_d.as<S>();
+    a.sl_cv = pl->sl_cv.as<S>();
+    a.sl_k = pl->sl_k.as<S>();
+    a.sl_w = pl->sl_w.as<S>();
+    a.st_u = pl->st_u.as<S>();
+    a.st_g = pl->st_g.as<S>();
+    a.has_k = pl->ns_k > 0;
+    a.n_recv = pl->n_recv;
+    a.recv = pl->recv.as<uint32_t>();
+    const double l2 = kCourant * kCourant;
+    if (sizeof(T) == 4) {
+        // FP32 lambda^2: 1/3 rounded toward zero, so 6 l2 <= 2 (the scheme stays
+        // within its stability limit), and c2 = 2 - 6 l2 exactly, so a constant
+        // field is still preserved.
+        float l2f = (float)l2;
+        if ((double)l2f > l2) l2f = std::nextafter(l2f, 0.0f);
+        a.l2 = (T)l2f;
+        a.c2 = (T)(2.0 - 6.0 * (double)l2f);
+        a.l2s = (S)(sizeof(S) == 4 ? (double)l2f : l2);
+    } else {
+        a.l2 = (T)l2;
+        a.c2 = (T)(2.0 - 6.0 * l2);
+        a.l2s = (S)l2;
+    }
+    a.lam = (S)kCourant;
+    a.half_dt = (S)(0.5 * pl->dt);
+    return a;
+}
+
+template <class T, class S>
+wv_status advance_impl(wv_plan* pl, size_t k, const volatile int32_t* cancel, wv_progress_fn progress, void* ctx) {
+    StepParams<T, S> a = make_params<T, S>(pl);
+    const uint32_t grid = a.main_blocks + a.special_blocks + div_up(pl->n_recv, kBlock);
+    const size_t end = pl->done + k;
+    T* A = pl->fa.as<T>();
+    T* B = pl->fb.as<T>();
+    T* ir = pl->ir.as<T>();
+    uint64_t launches = 0;
+    int blk = 0;
+    bool pending = false;
+    wv_status st = WV_OK;
+    WV_HIP(hipEventRecord(pl->t0, pl->stream));
+    while (pl->done < end) {
+        if (cancel && *cancel) { st = WV_CANCELLED; break; }
+        const size_t stop = std::min(end, pl->done + kStepsPerBlock);
+        for (size_t n = pl->done; n < stop; ++n) {
+            // Even steps: p = A, prev = B; odd steps swap (gpu_fdtd.rs:242-243).
+            a.p = (n & 1) ? B : A;
+            a.prev = (n & 1) ? A : B;
+            a.s_n = (S)(sizeof(S) == 4 ? (double)(float)pl->signal[n] : pl->signal[n]);
+            a.ir = ir + n * pl->n_recv;
+            if (grid) hipLaunchKernelGGL((k_step<T, S>), dim3(grid), dim3(kBlock), 0, pl->stream, a);
+            ++launches;
+        }
+        WV_HIP(hipGetLastError());
+        WV_HIP(hipEventRecord(pl->ev[blk], pl->stream));
+        // Wait for the previous block while this one runs, so the device
+        // always has a block queued.
+        if (pending) {
+            WV_HIP(hipEventSynchronize(pl->ev[blk ^ 1]));
+            if (progress) progress((double)(pl->done) / (double)pl->steps, ctx);
+        }
+        pending = true;
+        pl->done = stop;
+        blk ^= 1;
+    }
+    WV_HIP(hipEventRecord(pl->t1, pl->stream));
+    WV_HIP(hipEventSynchronize(pl->t1));
+    if (st == WV_OK && progress) progress((double)pl->done / (double)pl->steps, ctx);
+    float ms = 0.f;
+    WV_HIP(hipEventElapsedTime(&ms, pl->t0, pl->t1));
+    pl->last_ms = ms;
+    pl->last_steps = launches;
+    pl->last_launches = launches;
+    return st;
+}
+
+template <class T>
+void finish_impl(wv_plan* pl, const double* freqs, size_t nf, int mode, double* spectra_out, double* ir_out) {
+    const size_t R = pl->n_recv, Tn = pl->done;
+    if ((mode & WV_MODE_SPECTRA) && spectra_out && R && nf) {
+        pl->freqs.alloc(nf * 8);
+        WV_HIP(hipMemcpyAsync(pl->freqs.p, freqs, nf * 8, hipMemcpyHostToDevice, pl->stream));
+        pl->spec.alloc(R * nf * 16);
+        hipLaunchKernelGGL(k_dft<T>, dim3(div_up(R * nf, 64)), dim3(64), 0, pl->stream, pl->ir.as<T>(),
+                           (uint32_t)Tn, (uint32_t)R, pl->freqs.as<double>(), (uint32_t)nf, pl->dt,
+                           pl->spec.as<double>());
+        WV_HIP(hipGetLastError());
+        WV_HIP(hipMemcpyAsync(spectra_out, pl->spec.p, R * nf * 16, hipMemcpyDeviceToHost, pl->stream));
+    }
+    if ((mode & WV_MODE_IR) && ir_out && R) {
+        std::vector<T> h(Tn * R);
+        if (Tn) WV_HIP(hipMemcpyAsync(h.data(), pl->ir.p, Tn * R * sizeof(T), hipMemcpyDeviceToHost, pl->stream));
+        WV_HIP(hipStreamSynchronize(pl->stream));
+        for (size_t r = 0; r < R; ++r)
+            for (size_t n = 0; n < Tn; ++n) ir_out[r * pl->steps + n] = (double)h[n * R + r];
+    }
+    WV_HIP(hipStreamSynchronize(pl->stream));
+}
+
+wv_status check_grid(const wv_grid* g, char* err, size_t err_len) {
+    if (!g || !g->flags || (g->n_boundary && (!g->b_cell || !g->b_start || !g->slots)) || !g->b_start) {
+        set_err(err, err_len, "invalid grid: null arrays");
+        return WV_ERR_INVALID;
+    }
+    for (int k = 0; k < 3; ++k)
+        if (g->n[k] < 3) {
+            set_err(err, err_len, "invalid grid: every axis needs at least 3 cells");
+            return WV_ERR_INVALID;
+        }
+    const uint64_t cells = g->n[0] * g->n[1] * g->n[2];
+    const uint64_t pitched = ((g->n[0] + 63) & ~63ull) * g->n[1] * g->n[2];
+    if (cells > kMaxCells || pitched >= (1ull << 32)) {
+        set_err(err, err_len, "grid of " + std::to_string(cells / 1000000) + " M cells exceeds the GPU layout limit");
+        return WV_ERR_LIMIT;
+    }
+    if (g->n_boundary >= (1ull << 31)) {
+        set_err(err, err_len, "too many boundary nodes for the GPU layout");
+        return WV_ERR_LIMIT;
+    }
+    for (uint64_t b = 0; b < g->n_boundary; ++b)
+        if (g->b_cell[b] >= cells || g->b_start[b + 1] < g->b_start[b]) {
+            set_err(err, err_len, "invalid grid: boundary list out of range");
+            return WV_ERR_INVALID;
+        }
+    return WV_OK;
+}
+
+template <class F>
+wv_status guarded(char* err, size_t err_len, F f) {
+    try {
+        return f();
+    } catch (const DevErr& e) {
+        set_err(err, err_len, e.msg);
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_err(err, err_len, "host allocation failed");
+        return WV_ERR_NOMEM;
+    }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ C ABI
+
+extern "C" wv_status wv_plan_create(const wv_grid* grid, int precision, wv_plan** out, char* err, size_t err_len) {
+    if (!out || (precision != WV_FP32 && precision != WV_FP64 && precision != WV_FP32_STATE64)) {
+        set_err(err, err_len, "wv_plan_create: invalid argument");
+        return WV_ERR_INVALID;
+    }
+    wv_status s = check_grid(grid, err, err_len);
+    if (s != WV_OK) return s;
+    return guarded(err, err_len, [&]() -> wv_status {
+        auto pl = std::make_unique<wv_plan>();
+        pl->prec = precision;
+        plan_setup(pl.get(), grid);
+        *out = pl.release();
+        return WV_OK;
+    });
+}
+
+extern "C" void wv_plan_free(wv_plan* plan) { delete plan; }
+
+extern "C" wv_status wv_plan_set_tile(wv_plan* plan, uint32_t zchunk) {
+    if (!plan || zchunk > 4096) return WV_ERR_INVALID;
+    plan->zchunk = zchunk;
+    return WV_OK;
+}
+
+extern "C" wv_status wv_run_begin(wv_plan* pl, const wv_tap* taps, size_t n_taps, const double* signal, size_t steps,
+                                  const uint64_t* receivers, size_t n_recv, char* err, size_t err_len) {
+    if (!pl || (n_taps && !taps) || (steps && !signal) || (n_recv && !receivers)) {
+        set_err(err, err_len, "wv_run_begin: invalid argument");
+        return WV_ERR_INVALID;
+    }
+    for (size_t t = 0; t < n_taps; ++t)
+        if (taps[t].cell >= pl->cells) {
+            set_err(err, err_len, "source tap outside the grid");
+            return WV_ERR_INVALID;
+        }
+    for (size_t r = 0; r < n_recv; ++r)
+        if (receivers[r] >= pl->cells) {
+            set_err(err, err_len, "receiver outside the grid");
+            return WV_ERR_INVALID;
+        }
+    if ((uint64_t)steps * n_recv >= (1ull << 34)) {
+        set_err(err, err_len, "receiver record too large");
+        return WV_ERR_LIMIT;
+    }
+    return guarded(err, err_len, [&]() -> wv_status {
+        WV_HIP(hipSetDevice(pl->device));
+        if (pl->prec == WV_FP32) run_begin_impl<float>(pl, taps, n_taps, signal, steps, receivers, n_recv);
+        else run_begin_impl<double>(pl, taps, n_taps, signal, steps, receivers, n_recv);
+        return WV_OK;
+    });
+}
+
+extern "C" wv_status wv_run_advance(wv_plan* pl, size_t k, const volatile int32_t* cancel, wv_progress_fn progress,
+                                    void* ctx, char* err, size_t err_len) {
+    if (!pl || !pl->active || pl->done + k > pl->steps) {
+        set_err(err, err_len, "wv_run_advance: no run, or more steps than the signal has");
+        return WV_ERR_INVALID;
+    }
+    return guarded(err, err_len, [&]() -> wv_status {
+        WV_HIP(hipSetDevice(pl->device));
+        switch (pl->prec) {
+            case WV_FP32: return advance_impl<float, float>(pl, k, cancel, progress, ctx);
+            case WV_FP64: return advance_impl<double, double>(pl, k, cancel, progress, ctx);
+            default: return advance_impl<float, double>(pl, k, cancel, progress, ctx);
+        }
+    });
+}
+
+extern "C" wv_status wv_run_finish(wv_plan* pl, const double* freqs, size_t nf, int mode, double* spectra_out,
+                                   double* ir_out, char* err, size_t err_len) {
+    if (!pl || !pl->active || (nf && !freqs)) {
+        set_err(err, err_len, "wv_run_finish: invalid argument");
+        return WV_ERR_INVALID;
+    }
+    return guarded(err, err_len, [&]() -> wv_status {
+        WV_HIP(hipSetDevice(pl->device));
+        if (pl->prec == WV_FP64) finish_impl<double>(pl, freqs, nf, mode, spectra_out, ir_out);
+        else finish_impl<float>(pl, freqs, nf, mode, spectra_out, ir_out);
+        pl->active = false;
+        return WV_OK;
+    });
+}
+
+extern "C" wv_status wv_plan_run(wv_plan* plan, const wv_tap* taps, size_t n_taps, const double* signal, size_t steps,
+                                 const uint64_t* receivers, size_t n_recv, const double* freqs, size_t nf, int mode,
+                                 const volatile int32_t* cancel, wv_progress_fn progress, void* ctx,
+                                 double* spectra_out, double* ir_out, char* err, size_t err_len) {
+    wv_status s = wv_run_begin(plan, taps, n_taps, signal, steps, receivers, n_recv, err, err_len);
+    if (s != WV_OK) return s;
+    s = wv_run_advance(plan, steps, cancel, progress, ctx, err, err_len);
+    if (s != WV_OK) {
+        plan->active = false;
+        return s;
+    }
+    return wv_run_finish(plan, freqs, nf, mode, spectra_out, ir_out, err, err_len);
+}
+
+extern "C" wv_status wv_fdtd_run(const wv_grid* grid, const wv_tap* taps, size_t n_taps, const double* signal,
+                                 size_t steps, const uint64_t* receivers, size_t n_recv, const double* freqs,
+                                 size_t nf, int mode, int precision, const volatile int32_t* cancel,
+                                 wv_progress_fn progress, void* ctx, double* spectra_out, double* ir_out, char* err,
+                                 size_t err_len) {
+    wv_plan* plan = nullptr;
+    wv_status s = wv_plan_create(grid, precision, &plan, err, err_len);
+    if (s != WV_OK) return s;
+    s = wv_plan_run(plan, taps, n_taps, signal, steps, receivers, n_recv, freqs, nf, mode, cancel, progress, ctx,
+                    spectra_out, ir_out, err, err_len);
+    wv_plan_free(plan);
+    return s;
+}
+
+extern "C" wv_status wv_plan_last_timing(const wv_plan* plan, double* ms, uint64_t* steps, uint64_t* launches) {
+    if (!plan) return WV_ERR_INVALID;
+    if (ms) *ms = plan->last_ms;
+    if (steps) *steps = plan->last_steps;
+    if (launches) *launches = plan->last_launches;
+    return WV_OK;
+}
+
+extern "C" wv_status wv_plan_stats(const wv_plan* plan, uint64_t out[7]) {
+    if (!plan || !out) return WV_ERR_INVALID;
+    out[0] = plan->cells;
+    out[1] = plan->air;
+    out[2] = plan->nb;
+    out[3] = plan->ns;
+    out[4] = plan->n_special;
+    out[5] = plan->npitched;
+    out[6] = plan->ns_k;
+    return WV_OK;
+}

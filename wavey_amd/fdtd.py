@@ -1,0 +1,474 @@
+"""Python view of the C ABI in include/wavey_fdtd.h (libwavey_fdtd.so).
+
+Names and argument meanings follow the reference's FDTD interface in
+core/wavey-solver/src/fdtd.rs and gpu_fdtd.rs:
+
+* ``FdtdGrid``: the voxel grid (fdtd.rs:196-209) with ``build_box`` (the
+  box-room case of ``build_dx``, fdtd.rs:289-377), ``index``, ``center``,
+  ``nearest_air`` (:245-272) and ``taps_at`` (:467-492).
+* ``source_signal`` (:601-610), ``source_components`` (:730-743),
+  ``component_weights`` (:812-815), ``to_transfer`` (:819-840).
+* ``Gpu.run_fdtd`` (gpu_fdtd.rs:45-54): ``Ok(Some(spectra))`` is the returned
+  array, ``Ok(None)`` (cancelled) is ``None`` and ``Err(String)`` raises
+  ``FdtdError`` with the message. ``Gpu.run_ir`` is ``run_cpu_ir``
+  (fdtd.rs:630-641) on the device.
+
+There is no CPU fallback here: every run goes through the HIP library, and a
+missing library raises at import of the first call.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libwavey_fdtd.so")
+
+WV_OK, WV_CANCELLED = 0, 1
+MODE_SPECTRA, MODE_IR, MODE_BOTH = 1, 2, 3
+FP32, FP64, FP32_STATE64 = 0, 1, 2
+PRECISIONS = {"fp32": FP32, "fp64": FP64, "fp32_state64": FP32_STATE64}
+COURANT = 0.5773502691896257
+AIR = 1 << 6
+
+
+class FdtdError(RuntimeError):
+    """Err(String) of the reference run (gpu_fdtd.rs:54); carries the status."""
+
+    def __init__(self, status, msg):
+        super().__init__(msg)
+        self.status = status
+
+
+class WvSlot(C.Structure):
+    _fields_ = [("inv_d", C.c_double), ("cv", C.c_double), ("k", C.c_double), ("w", C.c_double)]
+
+
+class WvGrid(C.Structure):
+    _fields_ = [("n", C.c_uint64 * 3), ("dx", C.c_double), ("dt", C.c_double), ("origin", C.c_double * 3),
+                ("flags", C.POINTER(C.c_uint8)), ("n_boundary", C.c_uint64), ("b_cell", C.POINTER(C.c_uint32)),
+                ("b_start", C.POINTER(C.c_uint32)), ("slots", C.POINTER(WvSlot))]
+
+
+class WvTap(C.Structure):
+    _fields_ = [("cell", C.c_uint64), ("w", C.c_double)]
+
+
+class WvBranch(C.Structure):
+    _fields_ = [("r", C.c_double), ("m", C.c_double), ("k", C.c_double)]
+
+
+class WvAdmittance(C.Structure):
+    _fields_ = [("beta0", C.c_double), ("n_branches", C.c_uint32), ("branches", C.POINTER(WvBranch))]
+
+
+class WvBoxRoom(C.Structure):
+    _fields_ = [("size", C.c_double * 3), ("n_pillars", C.c_uint32), ("pillars", C.POINTER(C.c_double)),
+                ("n_fits", C.c_uint32), ("fits", C.POINTER(WvAdmittance)), ("patch_gu", C.POINTER(C.c_uint32)),
+                ("patch_gv", C.POINTER(C.c_uint32)), ("patch_off", C.POINTER(C.c_uint32)),
+                ("patch_fit", C.POINTER(C.c_uint32))]
+
+
+PROGRESS_FN = C.CFUNCTYPE(None, C.c_double, C.c_void_p)
+
+EXPORTS = [
+    "wv_fdtd_run", "wv_plan_create", "wv_plan_run", "wv_run_begin", "wv_run_advance", "wv_run_finish",
+    "wv_plan_last_timing", "wv_plan_stats", "wv_plan_set_tile", "wv_plan_free", "wv_grid_build_box",
+    "wv_built_grid_view", "wv_grid_free", "wv_source_signal", "wv_nearest_air", "wv_taps_at",
+    "wv_source_components", "wv_component_weights", "wv_to_transfer", "wv_admittance_eval", "wv_paris",
+    "wv_fit_real_impedance", "wv_version",
+]
+
+_lib = None
+_lock = threading.Lock()
+
+
+def lib():
+    """The loaded HIP library; raises when it has not been built."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise FdtdError(-2, f"libwavey_fdtd.so is not built ({LIB_PATH}); run __graft_entry__.build()")
+            L = C.CDLL(LIB_PATH)
+            P = C.POINTER
+            st, sz, dp = C.c_int, C.c_size_t, P(C.c_double)
+            errbuf = [C.c_char_p, sz]
+            L.wv_fdtd_run.argtypes = [P(WvGrid), P(WvTap), sz, dp, sz, P(C.c_uint64), sz, dp, sz, C.c_int, C.c_int,
+                                      P(C.c_int32), PROGRESS_FN, C.c_void_p, dp, dp] + errbuf
+            L.wv_fdtd_run.restype = st
+            L.wv_plan_create.argtypes = [P(WvGrid), C.c_int, P(C.c_void_p)] + errbuf
+            L.wv_plan_create.restype = st
+            L.wv_plan_run.argtypes = [C.c_void_p, P(WvTap), sz, dp, sz, P(C.c_uint64), sz, dp, sz, C.c_int,
+                                      P(C.c_int32), PROGRESS_FN, C.c_void_p, dp, dp] + errbuf
+            L.wv_plan_run.restype = st
+            L.wv_run_begin.argtypes = [C.c_void_p, P(WvTap), sz, dp, sz, P(C.c_uint64), sz] + errbuf
+            L.wv_run_begin.restype = st
+            L.wv_run_advance.argtypes = [C.c_void_p, sz, P(C.c_int32), PROGRESS_FN, C.c_void_p] + errbuf
+            L.wv_run_advance.restype = st
+            L.wv_run_finish.argtypes = [C.c_void_p, dp, sz, C.c_int, dp, dp] + errbuf
+            L.wv_run_finish.restype = st
+            L.wv_plan_last_timing.argtypes = [C.c_void_p, dp, P(C.c_uint64), P(C.c_uint64)]
+            L.wv_plan_last_timing.restype = st
+            L.wv_plan_stats.argtypes = [C.c_void_p, P(C.c_uint64)]
+            L.wv_plan_stats.restype = st
+            L.wv_plan_set_tile.argtypes = [C.c_void_p, C.c_uint32]
+            L.wv_plan_set_tile.restype = st
+            L.wv_plan_free.argtypes = [C.c_void_p]
+            L.wv_plan_free.restype = None
+            L.wv_grid_build_box.argtypes = [P(WvBoxRoom), C.c_double, C.c_double, C.c_int, P(C.c_void_p)] + errbuf
+            L.wv_grid_build_box.restype = st
+            L.wv_built_grid_view.argtypes = [C.c_void_p]
+            L.wv_built_grid_view.restype = P(WvGrid)
+            L.wv_grid_free.argtypes = [C.c_void_p]
+            L.wv_grid_free.restype = None
+            L.wv_source_signal.argtypes = [C.c_double, C.c_double, sz, dp]
+            L.wv_source_signal.restype = None
+            L.wv_nearest_air.argtypes = [P(WvGrid), dp, P(C.c_uint64)]
+            L.wv_nearest_air.restype = st
+            L.wv_taps_at.argtypes = [P(WvGrid), dp, P(WvTap), P(C.c_size_t)]
+            L.wv_taps_at.restype = st
+            L.wv_source_components.argtypes = [P(WvGrid), C.c_uint64, C.c_uint32, P(WvTap), P(C.c_uint32),
+                                               P(C.c_uint32)]
+            L.wv_source_components.restype = st
+            L.wv_component_weights.argtypes = [dp, dp, C.c_double, C.c_double, dp]
+            L.wv_component_weights.restype = None
+            L.wv_to_transfer.argtypes = [dp, sz, dp, sz, C.c_double, C.c_double, dp, sz, C.c_double, C.c_double,
+                                         dp, dp]
+            L.wv_to_transfer.restype = None
+            L.wv_admittance_eval.argtypes = [P(WvAdmittance), C.c_double, dp]
+            L.wv_admittance_eval.restype = None
+            L.wv_paris.argtypes = [C.c_double, C.c_double, C.c_uint32]
+            L.wv_paris.restype = C.c_double
+            L.wv_fit_real_impedance.argtypes = [C.c_double]
+            L.wv_fit_real_impedance.restype = C.c_double
+            L.wv_version.argtypes = []
+            L.wv_version.restype = C.c_char_p
+            _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _check(status, err):
+    if status < 0:
+        raise FdtdError(status, err.value.decode(errors="replace"))
+    return status
+
+
+# ------------------------------------------------------------------- grid
+
+def _admittances(fits):
+    keep = []
+    arr = (WvAdmittance * len(fits))()
+    for i, (beta0, branches) in enumerate(fits):
+        br = (WvBranch * max(1, len(branches)))()
+        for j, (r, m, k) in enumerate(branches):
+            br[j].r, br[j].m, br[j].k = float(r), float(m), float(k)
+        keep.append(br)
+        arr[i].beta0 = float(beta0)
+        arr[i].n_branches = len(branches)
+        arr[i].branches = C.cast(br, C.POINTER(WvBranch))
+    return arr, keep
+
+
+@dataclass
+class FdtdGrid:
+    """Voxelized room (fdtd.rs:196-209). Arrays are numpy; ``view()`` is the C struct."""
+    n: tuple
+    origin: tuple
+    dx: float
+    dt: float
+    flags: np.ndarray
+    b_cell: np.ndarray
+    b_start: np.ndarray
+    slots: np.ndarray  # (S, 4): inv_d, cv, k, w
+    _keep: list = field(default_factory=list, repr=False)
+
+    @classmethod
+    def build_box(cls, room, c, dx, threads=0):
+        """Box room with pillars (see ``wv_box_room``): fdtd.rs:289-377."""
+        L = lib()
+        r = WvBoxRoom()
+        for k in range(3):
+            r.size[k] = float(room["L"][k])
+        pil = np.ascontiguousarray(np.array(room.get("pillars", []), dtype=np.float64).reshape(-1, 4))
+        fits, keep = _admittances(room["fits"])
+        patches = room["patches"]
+        if len(patches) != 6 + 4 * len(pil):
+            raise FdtdError(-1, "one patch entry per surface is needed")
+        gu = np.array([p[0] for p in patches], dtype=np.uint32)
+        gv = np.array([p[1] for p in patches], dtype=np.uint32)
+        off = np.zeros(len(patches), dtype=np.uint32)
+        off[1:] = np.cumsum(gu * gv)[:-1]
+        pf = np.concatenate([np.asarray(p[2], dtype=np.uint32).ravel() for p in patches]).astype(np.uint32)
+        r.n_pillars = len(pil)
+        r.pillars = _dp(pil)
+        r.n_fits = len(fits)
+        r.fits = C.cast(fits, C.POINTER(WvAdmittance))
+        u32 = lambda a: a.ctypes.data_as(C.POINTER(C.c_uint32))  # noqa: E731
+        r.patch_gu, r.patch_gv, r.patch_off, r.patch_fit = u32(gu), u32(gv), u32(off), u32(pf)
+        h = C.c_void_p()
+        err = C.create_string_buffer(512)
+        _check(L.wv_grid_build_box(C.byref(r), c, dx, threads, C.byref(h), err, 512), err)
+        try:
+            v = L.wv_built_grid_view(h).contents
+            n = tuple(int(v.n[k]) for k in range(3))
+            N = n[0] * n[1] * n[2]
+            nb = int(v.n_boundary)
+            flags = np.ctypeslib.as_array(v.flags, shape=(N,)).copy()
+            b_cell = np.ctypeslib.as_array(v.b_cell, shape=(nb,)).copy() if nb else np.zeros(0, np.uint32)
+            b_start = np.ctypeslib.as_array(v.b_start, shape=(nb + 1,)).copy()
+            ns = int(b_start[-1])
+            if ns:
+                raw = np.ctypeslib.as_array(C.cast(v.slots, C.POINTER(C.c_double)), shape=(4 * ns,)).copy()
+                slots = raw.reshape(ns, 4)
+            else:
+                slots = np.zeros((0, 4))
+            return cls(n=n, origin=tuple(v.origin[k] for k in range(3)), dx=v.dx, dt=v.dt, flags=flags,
+                       b_cell=b_cell, b_start=b_start, slots=np.ascontiguousarray(slots))
+        finally:
+            L.wv_grid_free(h)
+
+    @property
+    def cells(self):
+        return self.n[0] * self.n[1] * self.n[2]
+
+    def air_cells(self):
+        return int(np.count_nonzero(self.flags & AIR))
+
+    def index(self, x, y, z):
+        return (z * self.n[1] + y) * self.n[0] + x
+
+    def center(self, i):
+        nx, ny = self.n[0], self.n[1]
+        z, r = divmod(int(i), nx * ny)
+        y, x = divmod(r, nx)
+        return np.array([self.origin[0] + x * self.dx, self.origin[1] + y * self.dx, self.origin[2] + z * self.dx])
+
+    def view(self):
+        g = WvGrid()
+        for k in range(3):
+            g.n[k] = self.n[k]
+            g.origin[k] = self.origin[k]
+        g.dx, g.dt = self.dx, self.dt
+        self.flags = np.ascontiguousarray(self.flags, dtype=np.uint8)
+        self.b_cell = np.ascontiguousarray(self.b_cell, dtype=np.uint32)
+        self.b_start = np.ascontiguousarray(self.b_start, dtype=np.uint32)
+        self.slots = np.ascontiguousarray(self.slots, dtype=np.float64).reshape(-1, 4)
+        g.flags = self.flags.ctypes.data_as(C.POINTER(C.c_uint8))
+        g.n_boundary = len(self.b_cell)
+        g.b_cell = self.b_cell.ctypes.data_as(C.POINTER(C.c_uint32))
+        g.b_start = self.b_start.ctypes.data_as(C.POINTER(C.c_uint32))
+        g.slots = self.slots.ctypes.data_as(C.POINTER(WvSlot))
+        return g
+
+    def nearest_air(self, p):
+        out = C.c_uint64()
+        pp = np.ascontiguousarray(p, dtype=np.float64)
+        st = lib().wv_nearest_air(C.byref(self.view()), _dp(pp), C.byref(out))
+        return None if st != WV_OK else int(out.value)
+
+    def taps_at(self, p):
+        taps = (WvTap * 8)()
+        n = C.c_size_t()
+        pp = np.ascontiguousarray(p, dtype=np.float64)
+        st = lib().wv_taps_at(C.byref(self.view()), _dp(pp), taps, C.byref(n))
+        if st != WV_OK:
+            return None
+        return [(int(taps[t].cell), float(taps[t].w)) for t in range(n.value)]
+
+
+# ------------------------------------------------------------ sources etc.
+
+def source_signal(f_max, dt, steps):
+    out = np.zeros(int(steps))
+    lib().wv_source_signal(f_max, dt, int(steps), _dp(out))
+    return out
+
+
+def source_components(grid, c, order):
+    comps = (WvTap * 8)()
+    ln = (C.c_uint32 * 4)()
+    nc = C.c_uint32()
+    st = lib().wv_source_components(C.byref(grid.view()), int(c), int(order), comps, ln, C.byref(nc))
+    if st != WV_OK:
+        return None
+    return [[(int(comps[2 * k + t].cell), float(comps[2 * k + t].w)) for t in range(ln[k])] for k in range(nc.value)]
+
+
+def component_weights(a0, a, k, dx):
+    a0v = np.array([a0.real, a0.imag])
+    av = np.array([[v.real, v.imag] for v in a]).ravel()
+    out = np.zeros(8)
+    lib().wv_component_weights(_dp(a0v), _dp(av), k, dx, _dp(out))
+    return out[0::2] + 1j * out[1::2]
+
+
+def to_transfer(spectra, signal, grid, freqs, c, r_ref, h0):
+    spec = np.asarray(spectra, dtype=np.complex128).ravel()
+    sr = np.ascontiguousarray(np.stack([spec.real, spec.imag], axis=1).ravel())
+    sig = np.ascontiguousarray(signal, dtype=np.float64)
+    fr = np.ascontiguousarray(freqs, dtype=np.float64)
+    h = np.asarray(h0, dtype=np.complex128)
+    hr = np.ascontiguousarray(np.stack([h.real, h.imag], axis=1).ravel())
+    out = np.zeros(2 * len(spec))
+    lib().wv_to_transfer(_dp(sr), len(spec), _dp(sig), len(sig), grid.dt, grid.dx, _dp(fr), len(fr), c, r_ref,
+                         _dp(hr), _dp(out))
+    return out[0::2] + 1j * out[1::2]
+
+
+def admittance(fit, f):
+    arr, keep = _admittances([fit])
+    out = np.zeros(2)
+    lib().wv_admittance_eval(C.cast(arr, C.POINTER(WvAdmittance)), f, _dp(out))
+    return complex(out[0], out[1])
+
+
+def paris(beta, n=256):
+    return lib().wv_paris(beta.real, beta.imag, n)
+
+
+def fit_real_impedance(alpha):
+    return lib().wv_fit_real_impedance(alpha)
+
+
+# -------------------------------------------------------------------- runs
+
+def _taps_array(taps):
+    arr = (WvTap * max(1, len(taps)))()
+    for t, (cell, w) in enumerate(taps):
+        arr[t].cell = int(cell)
+        arr[t].w = float(w)
+    return arr
+
+
+class Plan:
+    """A grid resident on the device (wv_plan): many runs, one upload."""
+
+    def __init__(self, grid: FdtdGrid, precision="fp32"):
+        self.grid = grid
+        self.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        self._h = C.c_void_p()
+        err = C.create_string_buffer(1024)
+        _check(lib().wv_plan_create(C.byref(grid.view()), self.precision, C.byref(self._h), err, 1024), err)
+        self._steps = 0
+        self._nrecv = 0
+
+    def close(self):
+        if self._h:
+            lib().wv_plan_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_tile(self, zchunk):
+        _check(lib().wv_plan_set_tile(self._h, int(zchunk)), C.create_string_buffer(8))
+
+    def stats(self):
+        out = (C.c_uint64 * 7)()
+        lib().wv_plan_stats(self._h, out)
+        keys = ["cells", "air_cells", "boundary_nodes", "slots", "special_cells", "pitched_cells", "slots_with_k"]
+        return dict(zip(keys, [int(v) for v in out]))
+
+    def begin(self, taps, signal, receivers):
+        sig = np.ascontiguousarray(signal, dtype=np.float64)
+        rc = np.ascontiguousarray(receivers, dtype=np.uint64)
+        tp = _taps_array(taps)
+        err = C.create_string_buffer(1024)
+        _check(lib().wv_run_begin(self._h, tp, len(taps), _dp(sig), len(sig),
+                                  rc.ctypes.data_as(C.POINTER(C.c_uint64)), len(rc), err, 1024), err)
+        self._steps = len(sig)
+        self._nrecv = len(rc)
+
+    def advance(self, k, cancel=None, progress=None):
+        err = C.create_string_buffer(1024)
+        cflag = None if cancel is None else cancel
+        cb = PROGRESS_FN(lambda f, _ctx: progress(f)) if progress else PROGRESS_FN()
+        st = lib().wv_run_advance(self._h, int(k), cflag.ctypes.data_as(C.POINTER(C.c_int32)) if cflag is not None
+                                  else None, cb, None, err, 1024)
+        return _check(st, err)
+
+    def finish(self, freqs=(), mode=MODE_BOTH):
+        fr = np.ascontiguousarray(freqs, dtype=np.float64)
+        R, T, F = self._nrecv, self._steps, len(fr)
+        ir = np.zeros((R, T)) if mode & MODE_IR else None
+        spec = np.zeros(2 * R * F) if (mode & MODE_SPECTRA and F) else None
+        err = C.create_string_buffer(1024)
+        _check(lib().wv_run_finish(self._h, _dp(fr) if F else None, F, mode,
+                                   _dp(spec) if spec is not None else None, _dp(ir) if ir is not None else None,
+                                   err, 1024), err)
+        sp = (spec[0::2] + 1j * spec[1::2]).reshape(R, F) if spec is not None else None
+        return ir, sp
+
+    def last_timing(self):
+        ms = C.c_double()
+        steps = C.c_uint64()
+        launches = C.c_uint64()
+        lib().wv_plan_last_timing(self._h, C.byref(ms), C.byref(steps), C.byref(launches))
+        return ms.value, steps.value, launches.value
+
+    def run(self, taps, signal, receivers, freqs=(), mode=MODE_BOTH, cancel=None, progress=None):
+        """Returns (ir [R][T], spectra [R][F]) or None when cancelled."""
+        self.begin(taps, signal, receivers)
+        st = self.advance(len(signal), cancel=cancel, progress=progress)
+        if st == WV_CANCELLED:
+            return None
+        return self.finish(freqs, mode)
+
+
+class Gpu:
+    """The device context, as the reference's ``Gpu`` (gpu.rs:104-107)."""
+
+    def __init__(self, precision="fp32"):
+        self.precision = precision
+        lib()
+
+    def run_fdtd(self, g, source, signal, receivers, freqs, cancel=None, progress=None):
+        """gpu_fdtd.rs:45-54: spectra [R][F] complex, None when cancelled, FdtdError on Err."""
+        with Plan(g, self.precision) as p:
+            out = p.run(source, signal, receivers, freqs, MODE_SPECTRA, cancel, progress)
+        return None if out is None else out[1]
+
+    def run_ir(self, g, source, signal, receivers, cancel=None, progress=None):
+        """run_cpu_ir (fdtd.rs:630-641) on the device: [R][T] pressure series."""
+        with Plan(g, self.precision) as p:
+            out = p.run(source, signal, receivers, (), MODE_IR, cancel, progress)
+        return None if out is None else out[0]
+
+
+def fdtd_run(g, taps, signal, receivers, freqs=(), mode=MODE_BOTH, precision="fp32", cancel=None, progress=None):
+    """The one-call C entry point wv_fdtd_run."""
+    sig = np.ascontiguousarray(signal, dtype=np.float64)
+    rc = np.ascontiguousarray(receivers, dtype=np.uint64)
+    fr = np.ascontiguousarray(freqs, dtype=np.float64)
+    R, T, F = len(rc), len(sig), len(fr)
+    ir = np.zeros((R, T))
+    spec = np.zeros(2 * R * max(F, 1))
+    err = C.create_string_buffer(1024)
+    cb = PROGRESS_FN(lambda f, _ctx: progress(f)) if progress else PROGRESS_FN()
+    prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+    st = lib().wv_fdtd_run(C.byref(g.view()), _taps_array(taps), len(taps), _dp(sig), T,
+                           rc.ctypes.data_as(C.POINTER(C.c_uint64)), R, _dp(fr) if F else None, F, mode, prec,
+                           cancel.ctypes.data_as(C.POINTER(C.c_int32)) if cancel is not None else None, cb, None,
+                           _dp(spec), _dp(ir), err, 1024)
+    _check(st, err)
+    if st == WV_CANCELLED:
+        return None
+    return ir, (spec[0::2] + 1j * spec[1::2])[: R * F].reshape(R, F)

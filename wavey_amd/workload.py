@@ -1,0 +1,51 @@
+"""Turns a seeded room config into solver inputs with the library's host side.
+
+The wave job of the reference does the same per source
+(core/wavey-solver/src/run.rs:593-661): receivers snap to the nearest air
+node, an omni source spreads over trilinear taps (taps_at, falling back to
+the nearest node), a directional source runs as its monopole and three
+dipole components (source_components) combined in frequency.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import fdtd
+
+
+@dataclass
+class Case:
+    cfg: dict
+    grid: fdtd.FdtdGrid
+    signal: np.ndarray
+    receivers: list
+    runs: list      # [(source index, component index, taps)]
+
+
+def build_case(cfg, threads=0, steps=None):
+    g = fdtd.FdtdGrid.build_box(cfg["room"], cfg["c"], cfg["dx"], threads)
+    T = cfg["steps"] if steps is None else steps
+    signal = fdtd.source_signal(cfg["f_src"], g.dt, T)
+    receivers = []
+    for p in cfg["receivers"]:
+        r = g.nearest_air(p)
+        if r is None:
+            raise fdtd.FdtdError(-1, f"receiver {p} has no air node nearby")
+        receivers.append(r)
+    runs = []
+    for si, s in enumerate(cfg["sources"]):
+        if s["kind"] == "omni":
+            taps = g.taps_at(s["pos"])
+            if taps is None:
+                raise fdtd.FdtdError(-1, f"source {si} is not inside the grid")
+            runs.append((si, 0, taps))
+        else:
+            node = g.nearest_air(s["pos"])
+            comps = fdtd.source_components(g, node, 1)
+            if comps is None:
+                comps = [[(node, 1.0)]]
+            for ci, taps in enumerate(comps):
+                runs.append((si, ci, taps))
+    return Case(cfg=cfg, grid=g, signal=signal, receivers=receivers, runs=runs)
