@@ -69,6 +69,7 @@ def main():
     ap.add_argument("--workload", default="hall", choices=["hall", "pillared_room", "shoebox"])
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64", "fp32_state64"])
     ap.add_argument("--zchunk", type=int, default=0)
+    ap.add_argument("--variant", type=int, default=-1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -91,9 +92,13 @@ def main():
     print(f"bench: {cfg['name']} grid {case.grid.n}, built in {t_build:.1f} s", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     plan = fdtd.Plan(case.grid, args.precision)
-    if args.zchunk:
-        plan.set_tile(args.zchunk)
+    if args.zchunk or args.variant >= 0:
+        plan.set_tile(args.zchunk, args.variant)
     taps = case.runs[0][2]
+    # A plain copy of one field buffer: the attainable-bandwidth reference and
+    # the calibration dispatch for HBM traffic counters.
+    copy_bytes = case.grid.cells * (8 if args.precision == "fp64" else 4)
+    copy_ms = plan.copy_probe(copy_bytes, reps=5)
     plan.begin(taps, case.signal, case.receivers)
     t_setup = time.perf_counter() - t0
     st = plan.stats()
@@ -134,7 +139,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "k_step", "bytes_per_launch": int(nbytes),
-                     "launch_us": round(launch_s * 1e6, 2)},
+                     "launch_us": round(launch_s * 1e6, 2),
+                     "copy_GBs": round(2 * copy_bytes / (copy_ms * 1e-3) / 1e9, 1), "copy_bytes": 2 * copy_bytes},
         "setup_s": {"grid_build": round(t_build, 2), "upload_and_layout": round(t_setup, 2)},
     }
     if not args.no_cpu:

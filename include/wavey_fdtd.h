@@ -123,9 +123,17 @@ wv_status wv_plan_last_timing(const wv_plan* plan, double* step_loop_ms, uint64_
  * counterpart. out: [cells, air_cells, boundary_nodes, slots, special_cells,
  * pitched_cells, slots_with_k]. */
 wv_status wv_plan_stats(const wv_plan* plan, uint64_t out[7]);
-/* Step-kernel shape: z planes per work-group column (0 = default). No
- * reference counterpart: the autotuner's knob. */
-wv_status wv_plan_set_tile(wv_plan* plan, uint32_t zchunk);
+/* Step-kernel shape: z planes per work-group column (0 = default, else a
+ * power of two up to 32) and the
+ * interior-kernel variant (-1 = default; 0..3 = tiles of 4, 8, 16, 2 rows,
+ * each row 256 FP32 or 128 FP64 cells). No reference counterpart: the
+ * autotuner's knobs. */
+wv_status wv_plan_set_tile(wv_plan* plan, uint32_t zchunk, int32_t variant);
+/* Copies `bytes` (at most one field buffer) between the plan's field
+ * buffers `reps` times with a plain streaming kernel and returns the mean
+ * time per copy. It overwrites the fields: call it outside a run. No
+ * reference counterpart: the bandwidth and HBM-counter calibration. */
+wv_status wv_plan_copy_probe(wv_plan* plan, uint64_t bytes, uint32_t reps, double* ms);
 void wv_plan_free(wv_plan* plan);
 
 /* ------------------------------------------------------------ grid build */

@@ -90,3 +90,17 @@ def test_pillared_room_components_match_oracle(pillared_mini, prec):
             err = rel_l2(ir, ref)
             print(f"component {ci}: relative L2 {err:.3e}")
             assert err < TOL[prec]
+
+
+@pytest.mark.parametrize("variant", range(4))
+def test_interior_variants_agree(pillared_mini, variant):
+    # Every interior-kernel variant (tile rows 4, 8, 16, 2) computes the
+    # same update: FP64 against the oracle at 1e-9, FP32 at 1e-4.
+    case, refs = pillared_mini
+    _, ci, taps = case.runs[1]
+    for prec in ("fp64", "fp32"):
+        print(f"case: pillared room mini, variant {variant}, {prec}", flush=True)
+        with fdtd.Plan(case.grid, prec) as plan:
+            plan.set_tile(zchunk=16, variant=variant)
+            ir, _ = plan.run(taps, case.signal, case.receivers, mode=fdtd.MODE_IR)
+        assert rel_l2(ir, refs[1]) < TOL[prec]

@@ -7,11 +7,13 @@
 //
 //  * Fields p, prev live in a pitched grid: row pitch = nx rounded up to 64
 //    cells, so every 64-lane wave reads one aligned 256-byte row segment.
-//  * An interior bitmap (one bit per pitched cell) marks plain air cells:
-//    six air neighbours, no boundary slots, no source tap. They are updated
-//    by work-groups that march through z with the z-neighbours held in
-//    registers (the column's p at z-1, z, z+1), reading p, prev and the
-//    bitmap once per cell and writing prev once: 12 bytes per FP32 cell.
+//  * An interior bitmap marks plain air cells: six air neighbours, no
+//    boundary slots, no source tap. It is packed along z: one 32-bit word per
+//    (x, y) column holds the bits of 32 consecutive planes, so a lane reads
+//    its cells' bits once per z-chunk. Interior cells are updated by
+//    work-groups that march through z with the z-neighbours held in
+//    registers (the column's p at z-1, z, z+1), reading p and prev once per
+//    cell and writing prev once: 12 bytes per FP32 cell.
 //  * Every other air cell (and any source tap) is a "special" cell, found by
 //    a wave-ballot / prefix-sum compaction over the grid and stored in cell
 //    order as structure-of-arrays. Special cells carry their neighbour mask,
@@ -75,9 +77,8 @@ struct DBuf {
     T* as() const { return static_cast<T*>(p); }
 };
 
-constexpr uint32_t kBlock = 256;
-constexpr uint32_t kTileX = 64;
-constexpr uint32_t kTileY = 4;
+constexpr uint32_t kBlock = 256;  // setup kernels
+constexpr uint32_t kTileX = 64;   // interior tile: one 64-lane wave per row
 constexpr uint32_t kStepsPerBlock = 128;  // cancel/progress granularity, as gpu_fdtd.rs:35
 constexpr uint32_t kMetaSolid = 1u << 7;  // a tap on a non-air cell (no update, injection only)
 
@@ -87,11 +88,12 @@ template <typename T, typename S>
 struct StepParams {
     const T* __restrict__ p;
     T* __restrict__ prev;
-    const uint32_t* __restrict__ interior;
+    const uint32_t* __restrict__ zbits;  // [(nz - 2 + 31) / 32][ny][pitch]
     uint32_t nx, ny, nz, pitch, sz, wpr;
     uint32_t tiles_x, tiles_y, zchunk;
-    uint32_t main_blocks, main_remap, main_per_xcd;
+    uint32_t main_blocks;
     uint32_t n_special, special_blocks;
+    uint32_t main_groups, special_groups;  // groups of 8 blocks (one per XCD)
     const uint32_t* __restrict__ sp_cell;
     const uint32_t* __restrict__ sp_meta;
     const uint32_t* __restrict__ sp_slot0;
@@ -110,64 +112,174 @@ struct StepParams {
     S l2s, lam, half_dt, s_n;
 };
 
-// Interior: one thread per (x, y) column of a 64 x 4 tile, marching z over
-// [z0, z1). p at z-1 and z stay in registers; p at z+1 is the one new load
-// of the column per plane. x and y neighbours are loads of rows the same
-// work-group (or its neighbours) reads in the same plane, served by cache.
-template <typename T, typename S>
+template <typename T>
+__device__ __forceinline__ T interior_update(T c, T q, T xm, T xp, T ym, T yp, T zm, T zp, T l2, T c2) {
+    if constexpr (sizeof(T) == 8) {
+        // FP64: the reference's form, neighbour order -x, +x, -y, +y, -z, +z
+        // (fdtd.rs:685-706).
+        T s = xm;
+        s += xp;
+        s += ym;
+        s += yp;
+        s += zm;
+        s += zp;
+        return c2 * c + (l2 * s - q);
+    } else {
+        // FP32: the same update as p + (v + l2 * sum (p_nb - p)) with
+        // v = p - p-. Neighbour differences and v are exact for close values
+        // (Sterbenz), so a near-uniform field rounds only in the final add
+        // instead of in l2 * sum ~ 2p: no correlated rounding drives the
+        // room's DC mode.
+        const T d = ((xm - c) + (xp - c)) + ((ym - c) + (yp - c)) + ((zm - c) + (zp - c));
+        return c + ((c - q) + l2 * d);
+    }
+}
+
+// Lane i takes v from lane i-1 (wave_shr:1); lane 0 keeps `fill`.
+__device__ __forceinline__ int dpp_from_lower(int v, int fill) {
+    return __builtin_amdgcn_update_dpp(fill, v, 0x138, 0xF, 0xF, false);
+}
+// Lane i takes v from lane i+1 (wave_shl:1); lane 63 keeps `fill`.
+__device__ __forceinline__ int dpp_from_upper(int v, int fill) {
+    return __builtin_amdgcn_update_dpp(fill, v, 0x130, 0xF, 0xF, false);
+}
+template <typename T>
+__device__ __forceinline__ T from_lower_lane(T v, T fill) {
+    if constexpr (sizeof(T) == 4) {
+        return __int_as_float(dpp_from_lower(__float_as_int(v), __float_as_int(fill)));
+    } else {
+        const long long a = __double_as_longlong(v), f = __double_as_longlong(fill);
+        const int lo = dpp_from_lower((int)a, (int)f), hi = dpp_from_lower((int)(a >> 32), (int)(f >> 32));
+        return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+    }
+}
+template <typename T>
+__device__ __forceinline__ T from_upper_lane(T v, T fill) {
+    if constexpr (sizeof(T) == 4) {
+        return __int_as_float(dpp_from_upper(__float_as_int(v), __float_as_int(fill)));
+    } else {
+        const long long a = __double_as_longlong(v), f = __double_as_longlong(fill);
+        const int lo = dpp_from_upper((int)a, (int)f), hi = dpp_from_upper((int)(a >> 32), (int)(f >> 32));
+        return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+    }
+}
+
+// W consecutive cells of a row per lane, moved as one 16-byte access.
+template <typename T>
+struct Cells {
+    static constexpr int W = 16 / sizeof(T);
+    T v[W];
+    __device__ __forceinline__ void load(const T* __restrict__ p) {
+        if constexpr (W == 4) {
+            const float4 q = *reinterpret_cast<const float4*>(p);
+            v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+        } else {
+            const double2 q = *reinterpret_cast<const double2*>(p);
+            v[0] = q.x; v[1] = q.y;
+        }
+    }
+    __device__ __forceinline__ void store(T* __restrict__ p) const {
+        if constexpr (W == 4) *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+        else *reinterpret_cast<double2*>(p) = make_double2(v[0], v[1]);
+    }
+};
+
+// Interior: a work-group owns a tile of 64*W cells (x) by TY rows (y), one
+// 64-lane wave per row, each lane W consecutive cells (16 bytes: 4 FP32 or
+// 2 FP64), and marches z over [z0, z1). The column's p at z-1 and z stay in
+// registers, so per plane a lane loads p at z+1, prev, the two y-neighbour
+// rows and the bitmap word, all as 16-byte (or 4-byte) accesses of whole
+// 1 KiB row segments per wave. x-neighbours inside a lane are in its
+// registers; across lanes they move by wave shifts, and lanes 0 and 63 take
+// the row-end cells of the neighbour tiles. The loop keeps few registers
+// live, so many waves per SIMD hide the memory latency. Only cells whose
+// interior bit is set (six air neighbours) are computed and stored.
+template <typename T, typename S, int TY>
 __device__ __forceinline__ void interior_part(const StepParams<T, S>& a, uint32_t t) {
+    constexpr int W = Cells<T>::W;
+    constexpr uint32_t BX = kTileX * W;
     const uint32_t tx = t % a.tiles_x;
     const uint32_t r = t / a.tiles_x;
     const uint32_t ty = r % a.tiles_y;
     const uint32_t zc = r / a.tiles_y;
-    const uint32_t x = tx * kTileX + (threadIdx.x & (kTileX - 1));
-    const uint32_t y = ty * kTileY + (threadIdx.x / kTileX);
-    if (x >= a.nx || y >= a.ny) return;
+    const uint32_t lane = threadIdx.x & (kTileX - 1);
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kTileX);
+    const uint32_t xl = tx * BX + lane * W;  // first cell of this lane
+    const uint32_t y = ty * TY + wave;
+    if (y >= a.ny) return;
+    // Lanes past the row pitch (the last tile of a row) re-read the row's
+    // last 16 bytes and update nothing: no traffic beyond the row.
+    const bool live = xl < a.pitch;
+    const uint32_t x = live ? xl : a.pitch - W;
     const uint32_t z0 = 1 + zc * a.zchunk;
     const uint32_t z1 = min(z0 + a.zchunk, a.nz - 1);
-    uint32_t i = z0 * a.sz + y * a.pitch + x;
-    const uint32_t* bm = a.interior + (z0 * a.ny + y) * a.wpr + (x >> 5);
-    const uint32_t bstride = a.ny * a.wpr;
-    const uint32_t bit = x & 31u;
-    T zm = a.p[i - a.sz];
-    T zc_ = a.p[i];
-    for (uint32_t z = z0; z < z1; ++z) {
-        const T zp = a.p[i + a.sz];
-        const uint32_t w = *bm;
-        if ((w >> bit) & 1u) {
-            const T xm = a.p[i - 1], xp = a.p[i + 1], ym = a.p[i - a.pitch], yp = a.p[i + a.pitch];
-            const T q = a.prev[i];
-            if constexpr (sizeof(T) == 8) {
-                // FP64: the reference's form, neighbour order -x, +x, -y, +y,
-                // -z, +z (fdtd.rs:685-706).
-                T s = xm;
-                s += xp;
-                s += ym;
-                s += yp;
-                s += zm;
-                s += zp;
-                a.prev[i] = a.c2 * zc_ + (a.l2 * s - q);
-            } else {
-                // FP32: the same update as p + (v + l2 * sum (p_nb - p)) with
-                // v = p - p-. Neighbour differences and v are exact for close
-                // values (Sterbenz), so a near-uniform field rounds only in
-                // the final add instead of in l2 * sum ~ 2p: no correlated
-                // rounding drives the room's DC mode.
-                const T d = ((xm - zc_) + (xp - zc_)) + ((ym - zc_) + (yp - zc_)) + ((zm - zc_) + (zp - zc_));
-                a.prev[i] = zc_ + ((zc_ - q) + a.l2 * d);
-            }
+    const uint32_t col = y * a.pitch + x;
+    const uint32_t ym_off = y > 0 ? a.pitch : 0u;
+    const uint32_t yp_off = y + 1 < a.ny ? a.pitch : 0u;
+    const uint32_t hl_off = x > 0 ? 1u : 0u;                     // lane 0: x - 1
+    const uint32_t hr_off = x + W < a.pitch ? (uint32_t)W : 0u;  // lane 63: x + W
+    const uint32_t full = (1u << W) - 1u;
+    const uint32_t lane_mask = live ? full : 0u;
+    // The chunk lies inside one 32-plane block of the z-packed bitmap
+    // (z-chunks divide 32 and start at plane 1 + a multiple of the chunk).
+    const uint32_t zb = (z0 - 1) >> 5;
+    uint32_t zwv[W];
+    {
+        const uint32_t* zp0 = a.zbits + (zb * a.ny + y) * a.pitch + x;
+        if constexpr (W == 4) {
+            const uint4 q = *reinterpret_cast<const uint4*>(zp0);
+            zwv[0] = q.x; zwv[1] = q.y; zwv[2] = q.z; zwv[3] = q.w;
+        } else {
+            const uint2 q = *reinterpret_cast<const uint2*>(zp0);
+            zwv[0] = q.x; zwv[1] = q.y;
         }
-        zm = zc_;
-        zc_ = zp;
-        i += a.sz;
-        bm += bstride;
+    }
+
+    Cells<T> zm, zcur;
+    zm.load(a.p + (z0 - 1) * a.sz + col);
+    zcur.load(a.p + z0 * a.sz + col);
+    for (uint32_t z = z0; z < z1; ++z) {
+        const uint32_t i = z * a.sz + col;
+        // All of the plane's loads are issued before any of them is used.
+        T hl = 0, hr = 0;
+        if (lane == 0) hl = a.p[i - hl_off];
+        if (lane == kTileX - 1) hr = a.p[i + hr_off];
+        Cells<T> zp, q, ym, yp;
+        zp.load(a.p + i + a.sz);
+        q.load(a.prev + i);
+        ym.load(a.p + i - ym_off);
+        yp.load(a.p + i + yp_off);
+        const T lo = from_lower_lane(zcur.v[W - 1], hl);
+        const T hi = from_upper_lane(zcur.v[0], hr);
+        Cells<T> out;
+#pragma unroll
+        for (int e = 0; e < W; ++e) {
+            const T xm = e == 0 ? lo : zcur.v[e - 1];
+            const T xp = e == W - 1 ? hi : zcur.v[e + 1];
+            out.v[e] = interior_update(zcur.v[e], q.v[e], xm, xp, ym.v[e], yp.v[e], zm.v[e], zp.v[e], a.l2, a.c2);
+        }
+        const uint32_t sh = (z - 1) & 31u;
+        uint32_t bits = 0;
+#pragma unroll
+        for (int e = 0; e < W; ++e) bits |= ((zwv[e] >> sh) & 1u) << e;
+        bits &= lane_mask;
+        if (bits == full) {
+            out.store(a.prev + i);
+        } else if (bits) {
+            // Row ends next to walls: only the interior cells.
+#pragma unroll
+            for (int e = 0; e < W; ++e)
+                if ((bits >> e) & 1u) a.prev[i + e] = out.v[e];
+        }
+        zm = zcur;
+        zcur = zp;
     }
 }
 
 // Special cells: general neighbour mask, boundary slots, source taps.
 template <typename T, typename S>
 __device__ __forceinline__ void special_part(const StepParams<T, S>& a, uint32_t b) {
-    const uint32_t s = b * kBlock + threadIdx.x;
+    const uint32_t s = b * blockDim.x + threadIdx.x;
     if (s >= a.n_special) return;
     const uint32_t i = a.sp_cell[s];
     const uint32_t meta = a.sp_meta[s];
@@ -178,26 +290,33 @@ __device__ __forceinline__ void special_part(const StepParams<T, S>& a, uint32_t
     } else {
         const S pc = (S)a.p[i];
         const uint32_t ns = meta >> 8;
+        // Neighbour loads are unconditional (masked-out ones read the cell
+        // itself; the mask never points outside the grid), so they issue
+        // together; the mask then selects the air neighbours.
+        const uint32_t off[6] = {1u, 1u, a.pitch, a.pitch, a.sz, a.sz};
+        S nb[6];
+#pragma unroll
+        for (int d = 0; d < 6; ++d) {
+            const uint32_t o = (meta >> d) & 1u ? off[d] : 0u;
+            nb[d] = (S)a.p[(d & 1) ? i + o : i - o];
+        }
         S lap;
         if constexpr (sizeof(S) == 8) {
             // fdtd.rs:685-696: l2 (sum_air - k p), neighbours in -x..+z order.
             S sum = 0, k = 0;
-            if (meta & 1u) { sum += (S)a.p[i - 1]; k += 1; }
-            if (meta & 2u) { sum += (S)a.p[i + 1]; k += 1; }
-            if (meta & 4u) { sum += (S)a.p[i - a.pitch]; k += 1; }
-            if (meta & 8u) { sum += (S)a.p[i + a.pitch]; k += 1; }
-            if (meta & 16u) { sum += (S)a.p[i - a.sz]; k += 1; }
-            if (meta & 32u) { sum += (S)a.p[i + a.sz]; k += 1; }
+#pragma unroll
+            for (int d = 0; d < 6; ++d)
+                if ((meta >> d) & 1u) {
+                    sum += nb[d];
+                    k += 1;
+                }
             lap = a.l2s * (sum - k * pc);
         } else {
             // FP32: sum of neighbour differences (see the interior update).
             S d = 0;
-            if (meta & 1u) d += (S)a.p[i - 1] - pc;
-            if (meta & 2u) d += (S)a.p[i + 1] - pc;
-            if (meta & 4u) d += (S)a.p[i - a.pitch] - pc;
-            if (meta & 8u) d += (S)a.p[i + a.pitch] - pc;
-            if (meta & 16u) d += (S)a.p[i - a.sz] - pc;
-            if (meta & 32u) d += (S)a.p[i + a.sz] - pc;
+#pragma unroll
+            for (int e = 0; e < 6; ++e)
+                if ((meta >> e) & 1u) d += nb[e] - pc;
             lap = a.l2s * d;
         }
         if (ns == 0) {
@@ -205,17 +324,37 @@ __device__ __forceinline__ void special_part(const StepParams<T, S>& a, uint32_t
             else next = pc + ((pc - pp) + lap);
         } else {
             // fdtd.rs:581-598: each slot's u+ is linear in p+, so p+ is solved
-            // first from the summed face flux, then the states advance.
-            const uint32_t j0 = a.sp_slot0[s], j1 = j0 + ns;
+            // first from the summed face flux, then the states advance. Slots
+            // are read four at a time (indices clamped to the node's last
+            // slot, extra lanes masked), so a node costs one memory round
+            // trip per four slots and pass.
+            constexpr uint32_t G = 4;
+            const uint32_t j0 = a.sp_slot0[s], jl = j0 + ns - 1;
             S s1 = 0, s2 = 0;
-            for (uint32_t j = j0; j < j1; ++j) {
-                const S inv_d = a.sl_inv_d[j], w = a.sl_w[j];
-                const S u = a.st_u[j];
-                S e = a.sl_cv[j] * u + S(0.5) * pc + w * pp;
-                if (a.has_k) e -= a.sl_k[j] * a.st_g[j];
-                e *= inv_d;
-                s1 += (S(0.5) - w) * inv_d;
-                s2 += e - u;
+            for (uint32_t jb = 0; jb < ns; jb += G) {
+                S inv_d[G], w[G], cv[G], u[G], kk[G], g[G];
+#pragma unroll
+                for (uint32_t k = 0; k < G; ++k) {
+                    const uint32_t j = min(j0 + jb + k, jl);
+                    inv_d[k] = a.sl_inv_d[j];
+                    w[k] = a.sl_w[j];
+                    cv[k] = a.sl_cv[j];
+                    u[k] = a.st_u[j];
+                    if (a.has_k) {
+                        kk[k] = a.sl_k[j];
+                        g[k] = a.st_g[j];
+                    }
+                }
+#pragma unroll
+                for (uint32_t k = 0; k < G; ++k) {
+                    if (jb + k < ns) {
+                        S e = cv[k] * u[k] + S(0.5) * pc + w[k] * pp;
+                        if (a.has_k) e -= kk[k] * g[k];
+                        e *= inv_d[k];
+                        s1 += (S(0.5) - w[k]) * inv_d[k];
+                        s2 += e - u[k];
+                    }
+                }
             }
             if constexpr (sizeof(S) == 8) {
                 next = (S(2) * pc - pp + lap - a.lam * s2) / (S(1) + a.lam * s1);
@@ -224,19 +363,32 @@ __device__ __forceinline__ void special_part(const StepParams<T, S>& a, uint32_t
                 // + lap - lam (s2 + s1 pc)) / (1 + lam s1).
                 next = pc + ((pc - pp) + lap - a.lam * (s2 + s1 * pc)) / (S(1) + a.lam * s1);
             }
-            for (uint32_t j = j0; j < j1; ++j) {
-                const S inv_d = a.sl_inv_d[j], w = a.sl_w[j];
-                const S u = a.st_u[j];
-                S e = a.sl_cv[j] * u + S(0.5) * pc + w * pp;
-                S g = 0;
-                if (a.has_k) {
-                    g = a.st_g[j];
-                    e -= a.sl_k[j] * g;
+            for (uint32_t jb = 0; jb < ns; jb += G) {
+                S inv_d[G], w[G], cv[G], u[G], kk[G], g[G];
+#pragma unroll
+                for (uint32_t k = 0; k < G; ++k) {
+                    const uint32_t j = min(j0 + jb + k, jl);
+                    inv_d[k] = a.sl_inv_d[j];
+                    w[k] = a.sl_w[j];
+                    cv[k] = a.sl_cv[j];
+                    u[k] = a.st_u[j];
+                    if (a.has_k) {
+                        kk[k] = a.sl_k[j];
+                        g[k] = a.st_g[j];
+                    }
                 }
-                e *= inv_d;
-                const S un = (S(0.5) - w) * inv_d * next + e;
-                if (a.has_k) a.st_g[j] = g + a.half_dt * (un + u);
-                a.st_u[j] = un;
+#pragma unroll
+                for (uint32_t k = 0; k < G; ++k) {
+                    if (jb + k < ns) {
+                        const uint32_t j = j0 + jb + k;
+                        S e = cv[k] * u[k] + S(0.5) * pc + w[k] * pp;
+                        if (a.has_k) e -= kk[k] * g[k];
+                        e *= inv_d[k];
+                        const S un = (S(0.5) - w[k]) * inv_d[k] * next + e;
+                        if (a.has_k) a.st_g[j] = g[k] + a.half_dt * (un + u[k]);
+                        a.st_u[j] = un;
+                    }
+                }
             }
         }
     }
@@ -244,24 +396,30 @@ __device__ __forceinline__ void special_part(const StepParams<T, S>& a, uint32_t
     a.prev[i] = (T)next;
 }
 
-template <typename T, typename S>
-__global__ __launch_bounds__(kBlock) void k_step(StepParams<T, S> a) {
-    uint32_t b = blockIdx.x;
-    if (b < a.main_blocks) {
-        // Blocks go round-robin to the 8 XCDs; give each XCD a contiguous
-        // run of tiles so y-neighbour rows are shared within one L2.
-        uint32_t t = b;
-        if (b < a.main_remap) t = (b & 7u) * a.main_per_xcd + (b >> 3);
-        interior_part(a, t);
+// One time step. Blocks come in groups of 8 (consecutive blocks go to the 8
+// XCDs round-robin); groups of interior tiles and groups of special cells
+// are interleaved evenly, so the latency-bound special cells run alongside
+// the bandwidth-bound tiles instead of as a tail. Receiver blocks come last.
+template <typename T, typename S, int TY>
+__global__ __launch_bounds__(kTileX* TY) void k_step(StepParams<T, S> a) {
+    const uint32_t g = blockIdx.x >> 3, k = blockIdx.x & 7u;
+    const uint32_t groups = a.main_groups + a.special_groups;
+    if (g < groups) {
+        const uint64_t sg = a.special_groups;
+        const uint32_t before = (uint32_t)((uint64_t)g * sg / groups);
+        const uint32_t after = (uint32_t)((uint64_t)(g + 1) * sg / groups);
+        if (after > before) {
+            special_part(a, before * 8 + k);
+        } else {
+            // Consecutive tiles go to the 8 XCDs round-robin: neighbouring
+            // tiles run at the same time and spread over all HBM stacks.
+            const uint32_t m = (g - before) * 8 + k;
+            if (m >= a.main_blocks) return;
+            interior_part<T, S, TY>(a, m);
+        }
         return;
     }
-    b -= a.main_blocks;
-    if (b < a.special_blocks) {
-        special_part(a, b);
-        return;
-    }
-    b -= a.special_blocks;
-    const uint32_t r = b * kBlock + threadIdx.x;
+    const uint32_t r = (blockIdx.x - groups * 8) * blockDim.x + threadIdx.x;
     if (r < a.n_recv) a.ir[r] = a.p[a.recv[r]];  // p^n before its update (fdtd.rs:673-683)
 }
 
@@ -304,23 +462,36 @@ __device__ __forceinline__ Cls classify(const uint8_t* __restrict__ flags, const
     return c;
 }
 
-// Pass 1 of the compaction: interior bitmap words straight from the wave
-// ballot, and the number of special cells per 256-cell block.
+// The z-packed interior bitmap: word (kb, y, x) holds the interior bits of
+// planes 1 + 32 kb .. 32 + 32 kb of column (x, y), bit k for plane 1 + 32 kb + k.
+__global__ __launch_bounds__(kBlock) void k_zbits(const uint8_t* __restrict__ flags, const uint32_t* __restrict__ force,
+                                                  uint32_t* __restrict__ zbits, uint64_t npitched, uint32_t pitch,
+                                                  uint32_t nx, uint32_t ny, uint32_t nz, uint32_t nzb) {
+    const uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= (uint64_t)nzb * ny * pitch) return;
+    const uint32_t x = (uint32_t)(t % pitch);
+    const uint64_t r = t / pitch;
+    const uint32_t y = (uint32_t)(r % ny), kb = (uint32_t)(r / ny);
+    uint32_t w = 0;
+    for (uint32_t k = 0; k < 32; ++k) {
+        const uint32_t z = 1 + kb * 32 + k;
+        if (z + 1 >= nz) break;
+        const uint64_t q = ((uint64_t)z * ny + y) * pitch + x;
+        if (classify(flags, force, q, npitched, pitch, nx, ny, nz).interior) w |= 1u << k;
+    }
+    zbits[t] = w;
+}
+
+// Pass 1 of the compaction: the number of special cells per 256-cell block.
 __global__ __launch_bounds__(kBlock) void k_classify_count(const uint8_t* __restrict__ flags,
                                                            const uint32_t* __restrict__ force,
-                                                           uint32_t* __restrict__ interior,
                                                            uint32_t* __restrict__ counts, uint64_t npitched,
                                                            uint32_t pitch, uint32_t nx, uint32_t ny, uint32_t nz) {
     __shared__ uint32_t wc[kBlock / 64];
     const uint64_t q = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     const Cls c = classify(flags, force, q, npitched, pitch, nx, ny, nz);
-    const uint64_t bin = __ballot(c.interior);
     const uint64_t bsp = __ballot(c.special);
     const uint32_t lane = threadIdx.x & 63u;
-    if (q < npitched) {
-        if (lane == 0) interior[q >> 5] = (uint32_t)bin;
-        if (lane == 32) interior[q >> 5] = (uint32_t)(bin >> 32);
-    }
     if (lane == 0) wc[threadIdx.x >> 6] = (uint32_t)__popcll(bsp);
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -431,6 +602,13 @@ __global__ void k_dft(const T* __restrict__ ir, uint32_t steps, uint32_t n_recv,
     out[2 * t + 1] = im;
 }
 
+// Plain streaming copy: the calibration kernel for bandwidth and HBM
+// traffic counters (reads n4 * 16 bytes, writes the same).
+__global__ __launch_bounds__(kBlock) void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, size_t n4) {
+    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n4) dst[i] = src[i];
+}
+
 template <typename T>
 __global__ void k_fill(T* __restrict__ p, size_t n, T v) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -438,6 +616,11 @@ __global__ void k_fill(T* __restrict__ p, size_t n, T v) {
 }
 
 inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+// Number of interior-kernel variants (the table is with launch_step) and
+// the default one.
+constexpr int kVariants = 4;
+constexpr int kDefaultVariant = 0;
 
 }  // namespace
 
@@ -451,9 +634,11 @@ struct wv_plan {
     uint64_t npitched = 0, cells = 0, air = 0, nb = 0, ns = 0, ns_k = 0;
     double dx = 0, dt = 0;
     uint32_t zchunk = 0;
+    uint32_t nzb = 0;  // 32-plane blocks of the z-packed bitmap
+    int variant = kDefaultVariant;
     uint32_t n_blocks_cls = 0;
 
-    DBuf flags, force_base, force, interior, counts, offs, total;
+    DBuf flags, force_base, force, zbits, counts, offs, total;
     DBuf b_cell_sorted, b_node_sorted, b_start;
     DBuf sl_inv_d, sl_cv, sl_k, sl_w, st_u, st_g;
     DBuf fa, fb;
@@ -569,7 +754,8 @@ void plan_setup(wv_plan* pl, const wv_grid* g) {
     const uint64_t words = pl->npitched / 32;
     pl->force_base.alloc(words * 4);
     pl->force.alloc(words * 4);
-    pl->interior.alloc(words * 4);
+    pl->nzb = (pl->nz - 2 + 31) / 32;
+    pl->zbits.alloc((size_t)pl->nzb * pl->ny * pl->pitch * 4);
     WV_HIP(hipMemsetAsync(pl->force_base.p, 0, words * 4, pl->stream));
     if (nb)
         hipLaunchKernelGGL(k_mark, dim3(div_up(nb, 256)), dim3(256), 0, pl->stream, pl->force_base.as<uint32_t>(),
@@ -616,9 +802,12 @@ void run_begin_impl(wv_plan* pl, const wv_tap* taps, size_t n_taps, const double
     if (!tc.empty())
         hipLaunchKernelGGL(k_mark, dim3(div_up(tc.size(), 256)), dim3(256), 0, pl->stream, pl->force.as<uint32_t>(),
                            pl->tap_cell.as<uint32_t>(), (uint32_t)tc.size(), pl->nx, pl->pitch);
+    hipLaunchKernelGGL(k_zbits, dim3(div_up((uint64_t)pl->nzb * pl->ny * pl->pitch, kBlock)), dim3(kBlock), 0,
+                       pl->stream, pl->flags.as<uint8_t>(), pl->force.as<uint32_t>(), pl->zbits.as<uint32_t>(),
+                       pl->npitched, pl->pitch, pl->nx, pl->ny, pl->nz, pl->nzb);
     hipLaunchKernelGGL(k_classify_count, dim3(pl->n_blocks_cls), dim3(kBlock), 0, pl->stream,
-                       pl->flags.as<uint8_t>(), pl->force.as<uint32_t>(), pl->interior.as<uint32_t>(),
-                       pl->counts.as<uint32_t>(), pl->npitched, pl->pitch, pl->nx, pl->ny, pl->nz);
+                       pl->flags.as<uint8_t>(), pl->force.as<uint32_t>(), pl->counts.as<uint32_t>(), pl->npitched,
+                       pl->pitch, pl->nx, pl->ny, pl->nz);
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, pl->stream, pl->counts.as<uint32_t>(), pl->offs.as<uint32_t>(),
                        pl->n_blocks_cls, pl->total.as<uint32_t>());
     WV_HIP(hipGetLastError());
@@ -666,27 +855,27 @@ void run_begin_impl(wv_plan* pl, const wv_tap* taps, size_t n_taps, const double
 }
 
 template <class T, class S>
-StepParams<T, S> make_params(wv_plan* pl) {
+StepParams<T, S> make_params(wv_plan* pl, uint32_t ty_rows) {
     StepParams<T, S> a{};
-    a.interior = pl->interior.as<uint32_t>();
+    a.zbits = pl->zbits.as<uint32_t>();
     a.nx = pl->nx;
     a.ny = pl->ny;
     a.nz = pl->nz;
     a.pitch = pl->pitch;
     a.sz = pl->sz;
     a.wpr = pl->wpr;
-    a.tiles_x = div_up(pl->nx, kTileX);
-    a.tiles_y = div_up(pl->ny, kTileY);
+    a.tiles_x = div_up(pl->nx, kTileX * (16 / sizeof(T)));
+    a.tiles_y = div_up(pl->ny, ty_rows);
     uint32_t zc = pl->zchunk;
-    if (zc == 0) zc = 32;
+    if (zc == 0) zc = 16;
     const uint32_t zint = pl->nz > 2 ? pl->nz - 2 : 0;
     a.zchunk = zc;
     const uint32_t nzc = zint ? div_up(zint, zc) : 0;
     a.main_blocks = a.tiles_x * a.tiles_y * nzc;
-    a.main_remap = a.main_blocks & ~7u;
-    a.main_per_xcd = a.main_remap / 8;
     a.n_special = pl->n_special;
-    a.special_blocks = div_up(pl->n_special, kBlock);
+    a.special_blocks = div_up(pl->n_special, kTileX * ty_rows);
+    a.main_groups = div_up(a.main_blocks, 8);
+    a.special_groups = div_up(a.special_blocks, 8);
     a.sp_cell = pl->sp_cell.as<uint32_t>();
     a.sp_meta = pl->sp_meta.as<uint32_t>();
     a.sp_slot0 = pl->sp_slot0.as<uint32_t>();
@@ -720,10 +909,24 @@ StepParams<T, S> make_params(wv_plan* pl) {
     return a;
 }
 
+// Interior-kernel variants: TY rows per tile (block = 64 x TY threads).
+constexpr uint32_t kVariantRows[] = {4, 8, 16, 2};
+
+template <class T, class S>
+void launch_step(int variant, uint32_t grid, hipStream_t st, const StepParams<T, S>& a) {
+    switch (variant) {
+        case 0: hipLaunchKernelGGL((k_step<T, S, 4>), dim3(grid), dim3(64 * 4), 0, st, a); break;
+        case 1: hipLaunchKernelGGL((k_step<T, S, 8>), dim3(grid), dim3(64 * 8), 0, st, a); break;
+        case 2: hipLaunchKernelGGL((k_step<T, S, 16>), dim3(grid), dim3(64 * 16), 0, st, a); break;
+        default: hipLaunchKernelGGL((k_step<T, S, 2>), dim3(grid), dim3(64 * 2), 0, st, a); break;
+    }
+}
+
 template <class T, class S>
 wv_status advance_impl(wv_plan* pl, size_t k, const volatile int32_t* cancel, wv_progress_fn progress, void* ctx) {
-    StepParams<T, S> a = make_params<T, S>(pl);
-    const uint32_t grid = a.main_blocks + a.special_blocks + div_up(pl->n_recv, kBlock);
+    const uint32_t ty_rows = kVariantRows[pl->variant];
+    StepParams<T, S> a = make_params<T, S>(pl, ty_rows);
+    const uint32_t grid = (a.main_groups + a.special_groups) * 8 + div_up(pl->n_recv, kTileX * ty_rows);
     const size_t end = pl->done + k;
     T* A = pl->fa.as<T>();
     T* B = pl->fb.as<T>();
@@ -742,7 +945,7 @@ wv_status advance_impl(wv_plan* pl, size_t k, const volatile int32_t* cancel, wv
             a.prev = (n & 1) ? A : B;
             a.s_n = (S)(sizeof(S) == 4 ? (double)(float)pl->signal[n] : pl->signal[n]);
             a.ir = ir + n * pl->n_recv;
-            if (grid) hipLaunchKernelGGL((k_step<T, S>), dim3(grid), dim3(kBlock), 0, pl->stream, a);
+            if (grid) launch_step<T, S>(pl->variant, grid, pl->stream, a);
             ++launches;
         }
         WV_HIP(hipGetLastError());
@@ -854,9 +1057,11 @@ extern "C" wv_status wv_plan_create(const wv_grid* grid, int precision, wv_plan*
 
 extern "C" void wv_plan_free(wv_plan* plan) { delete plan; }
 
-extern "C" wv_status wv_plan_set_tile(wv_plan* plan, uint32_t zchunk) {
-    if (!plan || zchunk > 4096) return WV_ERR_INVALID;
+extern "C" wv_status wv_plan_set_tile(wv_plan* plan, uint32_t zchunk, int32_t variant) {
+    // z-chunks must divide the 32-plane blocks of the interior bitmap.
+    if (!plan || zchunk > 32 || (zchunk & (zchunk - 1)) || variant >= kVariants) return WV_ERR_INVALID;
     plan->zchunk = zchunk;
+    plan->variant = variant < 0 ? kDefaultVariant : variant;
     return WV_OK;
 }
 
@@ -953,6 +1158,27 @@ extern "C" wv_status wv_plan_last_timing(const wv_plan* plan, double* ms, uint64
     if (steps) *steps = plan->last_steps;
     if (launches) *launches = plan->last_launches;
     return WV_OK;
+}
+
+extern "C" wv_status wv_plan_copy_probe(wv_plan* plan, uint64_t bytes, uint32_t reps, double* ms) {
+    if (!plan || !ms || reps == 0) return WV_ERR_INVALID;
+    char err[8];
+    return guarded(err, sizeof err, [&]() -> wv_status {
+        WV_HIP(hipSetDevice(plan->device));
+        const size_t n4 = std::min<uint64_t>(bytes, plan->fa.bytes) / 16;
+        if (n4 == 0) return WV_ERR_INVALID;
+        WV_HIP(hipEventRecord(plan->t0, plan->stream));
+        for (uint32_t r = 0; r < reps; ++r)
+            hipLaunchKernelGGL(k_copy, dim3(div_up(n4, kBlock)), dim3(kBlock), 0, plan->stream,
+                               plan->fa.as<float4>(), plan->fb.as<float4>(), n4);
+        WV_HIP(hipGetLastError());
+        WV_HIP(hipEventRecord(plan->t1, plan->stream));
+        WV_HIP(hipEventSynchronize(plan->t1));
+        float t = 0.f;
+        WV_HIP(hipEventElapsedTime(&t, plan->t0, plan->t1));
+        *ms = t / reps;
+        return WV_OK;
+    });
 }
 
 extern "C" wv_status wv_plan_stats(const wv_plan* plan, uint64_t out[7]) {

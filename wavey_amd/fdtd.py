@@ -77,7 +77,7 @@ PROGRESS_FN = C.CFUNCTYPE(None, C.c_double, C.c_void_p)
 
 EXPORTS = [
     "wv_fdtd_run", "wv_plan_create", "wv_plan_run", "wv_run_begin", "wv_run_advance", "wv_run_finish",
-    "wv_plan_last_timing", "wv_plan_stats", "wv_plan_set_tile", "wv_plan_free", "wv_grid_build_box",
+    "wv_plan_last_timing", "wv_plan_stats", "wv_plan_set_tile", "wv_plan_copy_probe", "wv_plan_free", "wv_grid_build_box",
     "wv_built_grid_view", "wv_grid_free", "wv_source_signal", "wv_nearest_air", "wv_taps_at",
     "wv_source_components", "wv_component_weights", "wv_to_transfer", "wv_admittance_eval", "wv_paris",
     "wv_fit_real_impedance", "wv_version",
@@ -116,8 +116,10 @@ def lib():
             L.wv_plan_last_timing.restype = st
             L.wv_plan_stats.argtypes = [C.c_void_p, P(C.c_uint64)]
             L.wv_plan_stats.restype = st
-            L.wv_plan_set_tile.argtypes = [C.c_void_p, C.c_uint32]
+            L.wv_plan_set_tile.argtypes = [C.c_void_p, C.c_uint32, C.c_int32]
             L.wv_plan_set_tile.restype = st
+            L.wv_plan_copy_probe.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, dp]
+            L.wv_plan_copy_probe.restype = st
             L.wv_plan_free.argtypes = [C.c_void_p]
             L.wv_plan_free.restype = None
             L.wv_grid_build_box.argtypes = [P(WvBoxRoom), C.c_double, C.c_double, C.c_int, P(C.c_void_p)] + errbuf
@@ -378,8 +380,18 @@ class Plan:
     def __exit__(self, *a):
         self.close()
 
-    def set_tile(self, zchunk):
-        _check(lib().wv_plan_set_tile(self._h, int(zchunk)), C.create_string_buffer(8))
+    def set_tile(self, zchunk=0, variant=-1):
+        st = lib().wv_plan_set_tile(self._h, int(zchunk), int(variant))
+        if st != WV_OK:
+            raise FdtdError(st, "invalid tile settings")
+
+    def copy_probe(self, nbytes, reps=10):
+        """Mean ms of a plain copy of nbytes between the field buffers (clobbers them)."""
+        ms = C.c_double()
+        st = lib().wv_plan_copy_probe(self._h, int(nbytes), int(reps), C.byref(ms))
+        if st != WV_OK:
+            raise FdtdError(st, "copy probe failed")
+        return ms.value
 
     def stats(self):
         out = (C.c_uint64 * 7)()
