@@ -867,7 +867,7 @@ StepParams<T, S> make_params(wv_plan* pl, uint32_t ty_rows) {
     a.tiles_x = div_up(pl->nx, kTileX * (16 / sizeof(T)));
     a.tiles_y = div_up(pl->ny, ty_rows);
     uint32_t zc = pl->zchunk;
-    if (zc == 0) zc = 16;
+    if (zc == 0) zc = 8;
     const uint32_t zint = pl->nz > 2 ? pl->nz - 2 : 0;
     a.zchunk = zc;
     const uint32_t nzc = zint ? div_up(zint, zc) : 0;
