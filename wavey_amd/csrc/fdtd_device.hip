@@ -6,23 +6,24 @@
 // (fdtd.rs:581-598). The layout is this build's own:
 //
 //  * Fields p, prev live in a pitched grid: row pitch = nx rounded up to 64
-//    cells, so every 64-lane wave reads one aligned 256-byte row segment.
-//  * An interior bitmap marks plain air cells: six air neighbours, no
-//    boundary slots, no source tap. It is packed along z: one 32-bit word per
-//    (x, y) column holds the bits of 32 consecutive planes, so a lane reads
-//    its cells' bits once per z-chunk. Interior cells are updated by
-//    work-groups that march through z with the z-neighbours held in
-//    registers (the column's p at z-1, z, z+1), reading p and prev once per
-//    cell and writing prev once: 12 bytes per FP32 cell.
-//  * Every other air cell (and any source tap) is a "special" cell, found by
-//    a wave-ballot / prefix-sum compaction over the grid and stored in cell
-//    order as structure-of-arrays. Special cells carry their neighbour mask,
-//    their slot range (structure-of-arrays slot constants and state) and
-//    their merged tap weight, so injection needs no launch of its own.
-//  * One fused launch per time step: interior tiles, special cells and
-//    receiver recording are disjoint block ranges of the same grid. Within
-//    a step p is only read and each cell of prev is written by one thread,
-//    so the parts need no ordering between them.
+//    cells, so a wave's 16-byte-per-lane accesses cover aligned row segments.
+//  * One launch per time step. A work-group owns a tile of the xy plane
+//    (256 FP32 or 128 FP64 cells by TY rows, one 64-lane wave per row, 16
+//    bytes per lane) and marches a z-chunk; the column's p at z-1 and z stay
+//    in registers. Per plane a lane reads p at z+1, prev and the two
+//    y-neighbour rows, and writes prev: 12 bytes per FP32 cell.
+//  * Two bitmaps, packed along z (one 32-bit word per column per 32 planes,
+//    read once per chunk), say which cells are plain interior air (six air
+//    neighbours, no boundary slots, no source tap) and which are "special"
+//    (every other air cell, and taps). Special cells are updated by the same
+//    lane from the neighbour values it already holds: their metadata sit in
+//    a list compacted (wave ballot + prefix sums) in the order the waves
+//    consume them, so a lane finds its cell's entry from the bitmap alone.
+//  * Boundary slot constants are deduplicated into a small table of slot
+//    sets; slot state is stored slot-major, state[j][rank], so the reads of a
+//    wave's special cells are contiguous.
+//  * Receivers are recorded by extra blocks of the same launch. Within a step
+//    p is only read and each prev cell is written by one thread.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -31,6 +32,7 @@
 #include <memory>
 #include <numeric>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "wavey_fdtd.h"
@@ -77,10 +79,18 @@ struct DBuf {
     T* as() const { return static_cast<T*>(p); }
 };
 
-constexpr uint32_t kBlock = 256;  // setup kernels
-constexpr uint32_t kTileX = 64;   // interior tile: one 64-lane wave per row
+constexpr uint32_t kBlock = 256;          // setup kernels and compaction blocks
+constexpr uint32_t kTileX = 64;           // one 64-lane wave per tile row
 constexpr uint32_t kStepsPerBlock = 128;  // cancel/progress granularity, as gpu_fdtd.rs:35
-constexpr uint32_t kMetaSolid = 1u << 7;  // a tap on a non-air cell (no update, injection only)
+
+// Special-cell record: {meta, first constant of its slot set}. Meta bits:
+// 0-5 air neighbours (-x, +x, -y, +y, -z, +z), 6 solid, 7 tap, 8-13 slot
+// count, 14-26 position in its wave's chunk (dz * tile width + x in tile).
+constexpr uint32_t kMetaSolid = 1u << 6;  // a tap on a non-air cell: injection only
+constexpr uint32_t kMetaTap = 1u << 7;    // has a tap weight
+constexpr uint32_t kMetaSlotShift = 8;
+constexpr uint32_t kMetaPosShift = 14;
+constexpr uint32_t kMaxSlots = 63;
 
 // ------------------------------------------------------------------ kernels
 
@@ -88,21 +98,21 @@ template <typename T, typename S>
 struct StepParams {
     const T* __restrict__ p;
     T* __restrict__ prev;
-    const uint32_t* __restrict__ zbits;  // [(nz - 2 + 31) / 32][ny][pitch]
-    uint32_t nx, ny, nz, pitch, sz, wpr;
-    uint32_t tiles_x, tiles_y, zchunk;
-    uint32_t main_blocks;
-    uint32_t n_special, special_blocks;
-    uint32_t main_groups, special_groups;  // groups of 8 blocks (one per XCD)
-    const uint32_t* __restrict__ sp_cell;
-    const uint32_t* __restrict__ sp_meta;
-    const uint32_t* __restrict__ sp_slot0;
+    const uint32_t* __restrict__ zbits;  // interior bits [nzb][ny][pitch]
+    uint32_t nx, ny, nz, pitch, sz;
+    uint32_t tiles_x, tiles_y, zchunk, main_blocks;
+    // Special cells, in wave order: wave w owns [seg_base[w * seg_stride],
+    // seg_base[(w + 1) * seg_stride]).
+    const uint32_t* __restrict__ seg_base;
+    uint32_t seg_stride;
+    const uint2* __restrict__ sp_rec;
     const S* __restrict__ sp_tapw;
-    const S* __restrict__ sl_inv_d;
-    const S* __restrict__ sl_cv;
-    const S* __restrict__ sl_k;
-    const S* __restrict__ sl_w;
-    S* __restrict__ st_u;
+    uint32_t sp_pad;  // row length of the slot-major state arrays
+    const S* __restrict__ set_inv_d;
+    const S* __restrict__ set_cv;
+    const S* __restrict__ set_k;
+    const S* __restrict__ set_w;
+    S* __restrict__ st_u;  // [slot][rank]
     S* __restrict__ st_g;
     uint32_t has_k;
     uint32_t n_recv;
@@ -184,18 +194,155 @@ struct Cells {
     }
 };
 
-// Interior: a work-group owns a tile of 64*W cells (x) by TY rows (y), one
-// 64-lane wave per row, each lane W consecutive cells (16 bytes: 4 FP32 or
-// 2 FP64), and marches z over [z0, z1). The column's p at z-1 and z stay in
-// registers, so per plane a lane loads p at z+1, prev, the two y-neighbour
-// rows and the bitmap word, all as 16-byte (or 4-byte) accesses of whole
-// 1 KiB row segments per wave. x-neighbours inside a lane are in its
-// registers; across lanes they move by wave shifts, and lanes 0 and 63 take
-// the row-end cells of the neighbour tiles. The loop keeps few registers
-// live, so many waves per SIMD hide the memory latency. Only cells whose
-// interior bit is set (six air neighbours) are computed and stored.
+template <int W>
+__device__ __forceinline__ void load_words(const uint32_t* __restrict__ p, uint32_t (&w)[W]) {
+    if constexpr (W == 4) {
+        const uint4 q = *reinterpret_cast<const uint4*>(p);
+        w[0] = q.x; w[1] = q.y; w[2] = q.z; w[3] = q.w;
+    } else {
+        const uint2 q = *reinterpret_cast<const uint2*>(p);
+        w[0] = q.x; w[1] = q.y;
+    }
+}
+
+template <int W>
+__device__ __forceinline__ uint32_t plane_bits(const uint32_t (&w)[W], uint32_t sh) {
+    uint32_t b = 0;
+#pragma unroll
+    for (int e = 0; e < W; ++e) b |= ((w[e] >> sh) & 1u) << e;
+    return b;
+}
+
+// Constants and state of slots jb .. jb+G-1 of a node (indices clamped to
+// the last slot; the caller masks the extras).
+template <typename S, uint32_t G>
+struct SlotGroup {
+    S inv_d[G], w[G], cv[G], u[G], kk[G], g[G];
+    template <typename T>
+    __device__ __forceinline__ void load(const StepParams<T, S>& a, uint32_t c0, uint32_t rank, uint32_t jb,
+                                         uint32_t ns) {
+#pragma unroll
+        for (uint32_t k = 0; k < G; ++k) {
+            const uint32_t j = min(jb + k, ns - 1);
+            inv_d[k] = a.set_inv_d[c0 + j];
+            w[k] = a.set_w[c0 + j];
+            cv[k] = a.set_cv[c0 + j];
+            u[k] = a.st_u[j * a.sp_pad + rank];
+            kk[k] = 0;
+            g[k] = 0;
+            if (a.has_k) {
+                kk[k] = a.set_k[c0 + j];
+                g[k] = a.st_g[j * a.sp_pad + rank];
+            }
+        }
+    }
+    // The part of slot k's u+ that does not depend on p+ (fdtd.rs:589-592).
+    __device__ __forceinline__ S rest(uint32_t k, S pc, S pp) const {
+        return (cv[k] * u[k] + S(0.5) * pc + w[k] * pp - kk[k] * g[k]) * inv_d[k];
+    }
+    __device__ __forceinline__ void sums(uint32_t jb, uint32_t ns, S pc, S pp, S& s1, S& s2) const {
+#pragma unroll
+        for (uint32_t k = 0; k < G; ++k)
+            if (jb + k < ns) {
+                s1 += (S(0.5) - w[k]) * inv_d[k];
+                s2 += rest(k, pc, pp) - u[k];
+            }
+    }
+    template <typename T>
+    __device__ __forceinline__ void advance(const StepParams<T, S>& a, uint32_t rank, uint32_t jb, uint32_t ns, S pc,
+                                            S pp, S next) const {
+#pragma unroll
+        for (uint32_t k = 0; k < G; ++k)
+            if (jb + k < ns) {
+                const uint32_t j = jb + k;
+                const S un = (S(0.5) - w[k]) * inv_d[k] * next + rest(k, pc, pp);
+                if (a.has_k) a.st_g[j * a.sp_pad + rank] = g[k] + a.half_dt * (un + u[k]);
+                a.st_u[j * a.sp_pad + rank] = un;
+            }
+    }
+};
+
+// One special cell: the air-neighbour Laplacian, the boundary solve of
+// fdtd.rs:581-598 when it has slots, and its tap (injection after the
+// update, fdtd.rs:718-720). nb = -x, +x, -y, +y, -z, +z; c0 is the first
+// constant of the node's slot set and rank its index in the state arrays.
+template <typename T, typename S>
+__device__ __forceinline__ S special_update(const StepParams<T, S>& a, uint32_t meta, uint32_t c0, uint32_t rank,
+                                            S pc, S pp, const S (&nb)[6]) {
+    S next;
+    if (meta & kMetaSolid) {
+        next = pp;  // a tap on a solid cell: the reference only injects there
+    } else {
+        S lap;
+        if constexpr (sizeof(S) == 8) {
+            // fdtd.rs:685-696: l2 (sum_air - k p), neighbours in -x..+z order.
+            S sum = 0, k = 0;
+#pragma unroll
+            for (int d = 0; d < 6; ++d)
+                if ((meta >> d) & 1u) {
+                    sum += nb[d];
+                    k += 1;
+                }
+            lap = a.l2s * (sum - k * pc);
+        } else {
+            // FP32: sum of neighbour differences (see interior_update).
+            S d = 0;
+#pragma unroll
+            for (int e = 0; e < 6; ++e)
+                if ((meta >> e) & 1u) d += nb[e] - pc;
+            lap = a.l2s * d;
+        }
+        const uint32_t ns = (meta >> kMetaSlotShift) & kMaxSlots;
+        if (ns == 0) {
+            if constexpr (sizeof(S) == 8) next = S(2) * pc - pp + lap;
+            else next = pc + ((pc - pp) + lap);
+        } else {
+            // Each slot's u+ is linear in p+, so p+ is solved first from the
+            // summed face flux, then the states advance. The first G slots
+            // stay in registers (one pass for nodes with up to G slots);
+            // further groups are read again for the second pass. G = 3 in
+            // FP32 (a wall face with a three-branch admittance) keeps the
+            // kernel within 64 VGPRs, 8 waves per SIMD.
+            constexpr uint32_t G = sizeof(S) == 4 ? 3 : 2;
+            SlotGroup<S, G> g0;
+            g0.load(a, c0, rank, 0, ns);
+            S s1 = 0, s2 = 0;
+            g0.sums(0, ns, pc, pp, s1, s2);
+            for (uint32_t jb = G; jb < ns; jb += G) {
+                SlotGroup<S, G> gj;
+                gj.load(a, c0, rank, jb, ns);
+                gj.sums(jb, ns, pc, pp, s1, s2);
+            }
+            if constexpr (sizeof(S) == 8) {
+                next = (S(2) * pc - pp + lap - a.lam * s2) / (S(1) + a.lam * s1);
+            } else {
+                // The same solve written as pc + (small terms).
+                next = pc + ((pc - pp) + lap - a.lam * (s2 + s1 * pc)) / (S(1) + a.lam * s1);
+            }
+            g0.advance(a, rank, 0, ns, pc, pp, next);
+            for (uint32_t jb = G; jb < ns; jb += G) {
+                SlotGroup<S, G> gj;
+                gj.load(a, c0, rank, jb, ns);
+                gj.advance(a, rank, jb, ns, pc, pp, next);
+            }
+        }
+    }
+    if (meta & kMetaTap) next += a.sp_tapw[rank] * a.s_n;
+    return next;
+}
+
+// A tile of 64*W cells (x) by TY rows (y), one 64-lane wave per row, each
+// lane W consecutive cells (16 bytes: 4 FP32 or 2 FP64), marching z over
+// [z0, z1). Per plane a lane loads p at z+1, prev, the two y-neighbour rows
+// and, on lanes 0 and 63, the row-end cells of the neighbour tiles; x
+// neighbours inside a lane are in its registers and move across lanes by
+// wave shifts. All of a plane's loads are issued before any is used and the
+// loop keeps few registers live, so many waves per SIMD hide the memory
+// latency. Interior cells are written here; the wave then updates the
+// special cells of its chunk, 64 at a time, while the rows they read are
+// still in cache.
 template <typename T, typename S, int TY>
-__device__ __forceinline__ void interior_part(const StepParams<T, S>& a, uint32_t t) {
+__device__ __forceinline__ void tile_part(const StepParams<T, S>& a, uint32_t t) {
     constexpr int W = Cells<T>::W;
     constexpr uint32_t BX = kTileX * W;
     const uint32_t tx = t % a.tiles_x;
@@ -207,6 +354,9 @@ __device__ __forceinline__ void interior_part(const StepParams<T, S>& a, uint32_
     const uint32_t xl = tx * BX + lane * W;  // first cell of this lane
     const uint32_t y = ty * TY + wave;
     if (y >= a.ny) return;
+    const uint32_t w_id = (zc * a.tiles_x + tx) * a.ny + y;
+    const uint32_t s_begin = a.seg_base[w_id * a.seg_stride];
+    const uint32_t s_end = a.seg_base[(w_id + 1) * a.seg_stride];
     // Lanes past the row pitch (the last tile of a row) re-read the row's
     // last 16 bytes and update nothing: no traffic beyond the row.
     const bool live = xl < a.pitch;
@@ -222,25 +372,18 @@ __device__ __forceinline__ void interior_part(const StepParams<T, S>& a, uint32_
     const uint32_t lane_mask = live ? full : 0u;
     // The chunk lies inside one 32-plane block of the z-packed bitmap
     // (z-chunks divide 32 and start at plane 1 + a multiple of the chunk).
-    const uint32_t zb = (z0 - 1) >> 5;
-    uint32_t zwv[W];
-    {
-        const uint32_t* zp0 = a.zbits + (zb * a.ny + y) * a.pitch + x;
-        if constexpr (W == 4) {
-            const uint4 q = *reinterpret_cast<const uint4*>(zp0);
-            zwv[0] = q.x; zwv[1] = q.y; zwv[2] = q.z; zwv[3] = q.w;
-        } else {
-            const uint2 q = *reinterpret_cast<const uint2*>(zp0);
-            zwv[0] = q.x; zwv[1] = q.y;
-        }
-    }
+    uint32_t ib[W];
+    load_words<W>(a.zbits + (((z0 - 1) >> 5) * a.ny + y) * a.pitch + x, ib);
 
     Cells<T> zm, zcur;
     zm.load(a.p + (z0 - 1) * a.sz + col);
     zcur.load(a.p + z0 * a.sz + col);
+    // The first special records are read now, so they have arrived when
+    // the z-march ends.
+    uint2 rec = make_uint2(0u, 0u);
+    if (s_begin + lane < s_end) rec = a.sp_rec[s_begin + lane];
     for (uint32_t z = z0; z < z1; ++z) {
         const uint32_t i = z * a.sz + col;
-        // All of the plane's loads are issued before any of them is used.
         T hl = 0, hr = 0;
         if (lane == 0) hl = a.p[i - hl_off];
         if (lane == kTileX - 1) hr = a.p[i + hr_off];
@@ -258,11 +401,7 @@ __device__ __forceinline__ void interior_part(const StepParams<T, S>& a, uint32_
             const T xp = e == W - 1 ? hi : zcur.v[e + 1];
             out.v[e] = interior_update(zcur.v[e], q.v[e], xm, xp, ym.v[e], yp.v[e], zm.v[e], zp.v[e], a.l2, a.c2);
         }
-        const uint32_t sh = (z - 1) & 31u;
-        uint32_t bits = 0;
-#pragma unroll
-        for (int e = 0; e < W; ++e) bits |= ((zwv[e] >> sh) & 1u) << e;
-        bits &= lane_mask;
+        const uint32_t bits = plane_bits<W>(ib, (z - 1) & 31u) & lane_mask;
         if (bits == full) {
             out.store(a.prev + i);
         } else if (bits) {
@@ -274,153 +413,38 @@ __device__ __forceinline__ void interior_part(const StepParams<T, S>& a, uint32_
         zm = zcur;
         zcur = zp;
     }
-}
 
-// Special cells: general neighbour mask, boundary slots, source taps.
-template <typename T, typename S>
-__device__ __forceinline__ void special_part(const StepParams<T, S>& a, uint32_t b) {
-    const uint32_t s = b * blockDim.x + threadIdx.x;
-    if (s >= a.n_special) return;
-    const uint32_t i = a.sp_cell[s];
-    const uint32_t meta = a.sp_meta[s];
-    const S pp = (S)a.prev[i];
-    S next;
-    if (meta & kMetaSolid) {
-        next = pp;  // a tap on a solid cell: the reference only injects there
-    } else {
-        const S pc = (S)a.p[i];
-        const uint32_t ns = meta >> 8;
-        // Neighbour loads are unconditional (masked-out ones read the cell
-        // itself; the mask never points outside the grid), so they issue
-        // together; the mask then selects the air neighbours.
-        const uint32_t off[6] = {1u, 1u, a.pitch, a.pitch, a.sz, a.sz};
-        S nb[6];
-#pragma unroll
-        for (int d = 0; d < 6; ++d) {
-            const uint32_t o = (meta >> d) & 1u ? off[d] : 0u;
-            nb[d] = (S)a.p[(d & 1) ? i + o : i - o];
-        }
-        S lap;
-        if constexpr (sizeof(S) == 8) {
-            // fdtd.rs:685-696: l2 (sum_air - k p), neighbours in -x..+z order.
-            S sum = 0, k = 0;
-#pragma unroll
-            for (int d = 0; d < 6; ++d)
-                if ((meta >> d) & 1u) {
-                    sum += nb[d];
-                    k += 1;
-                }
-            lap = a.l2s * (sum - k * pc);
-        } else {
-            // FP32: sum of neighbour differences (see the interior update).
-            S d = 0;
-#pragma unroll
-            for (int e = 0; e < 6; ++e)
-                if ((meta >> e) & 1u) d += nb[e] - pc;
-            lap = a.l2s * d;
-        }
-        if (ns == 0) {
-            if constexpr (sizeof(S) == 8) next = S(2) * pc - pp + lap;
-            else next = pc + ((pc - pp) + lap);
-        } else {
-            // fdtd.rs:581-598: each slot's u+ is linear in p+, so p+ is solved
-            // first from the summed face flux, then the states advance. Slots
-            // are read four at a time (indices clamped to the node's last
-            // slot, extra lanes masked), so a node costs one memory round
-            // trip per four slots and pass.
-            constexpr uint32_t G = 4;
-            const uint32_t j0 = a.sp_slot0[s], jl = j0 + ns - 1;
-            S s1 = 0, s2 = 0;
-            for (uint32_t jb = 0; jb < ns; jb += G) {
-                S inv_d[G], w[G], cv[G], u[G], kk[G], g[G];
-#pragma unroll
-                for (uint32_t k = 0; k < G; ++k) {
-                    const uint32_t j = min(j0 + jb + k, jl);
-                    inv_d[k] = a.sl_inv_d[j];
-                    w[k] = a.sl_w[j];
-                    cv[k] = a.sl_cv[j];
-                    u[k] = a.st_u[j];
-                    if (a.has_k) {
-                        kk[k] = a.sl_k[j];
-                        g[k] = a.st_g[j];
-                    }
-                }
-#pragma unroll
-                for (uint32_t k = 0; k < G; ++k) {
-                    if (jb + k < ns) {
-                        S e = cv[k] * u[k] + S(0.5) * pc + w[k] * pp;
-                        if (a.has_k) e -= kk[k] * g[k];
-                        e *= inv_d[k];
-                        s1 += (S(0.5) - w[k]) * inv_d[k];
-                        s2 += e - u[k];
-                    }
-                }
-            }
-            if constexpr (sizeof(S) == 8) {
-                next = (S(2) * pc - pp + lap - a.lam * s2) / (S(1) + a.lam * s1);
-            } else {
-                // The same solve written as pc + (small terms): pc + ((pc - pp)
-                // + lap - lam (s2 + s1 pc)) / (1 + lam s1).
-                next = pc + ((pc - pp) + lap - a.lam * (s2 + s1 * pc)) / (S(1) + a.lam * s1);
-            }
-            for (uint32_t jb = 0; jb < ns; jb += G) {
-                S inv_d[G], w[G], cv[G], u[G], kk[G], g[G];
-#pragma unroll
-                for (uint32_t k = 0; k < G; ++k) {
-                    const uint32_t j = min(j0 + jb + k, jl);
-                    inv_d[k] = a.sl_inv_d[j];
-                    w[k] = a.sl_w[j];
-                    cv[k] = a.sl_cv[j];
-                    u[k] = a.st_u[j];
-                    if (a.has_k) {
-                        kk[k] = a.sl_k[j];
-                        g[k] = a.st_g[j];
-                    }
-                }
-#pragma unroll
-                for (uint32_t k = 0; k < G; ++k) {
-                    if (jb + k < ns) {
-                        const uint32_t j = j0 + jb + k;
-                        S e = cv[k] * u[k] + S(0.5) * pc + w[k] * pp;
-                        if (a.has_k) e -= kk[k] * g[k];
-                        e *= inv_d[k];
-                        const S un = (S(0.5) - w[k]) * inv_d[k] * next + e;
-                        if (a.has_k) a.st_g[j] = g[k] + a.half_dt * (un + u[k]);
-                        a.st_u[j] = un;
-                    }
-                }
-            }
-        }
+    // The chunk's special cells. A record's position gives the cell; its
+    // neighbour loads are unconditional (a masked-out neighbour reads the
+    // cell itself, and the mask never points outside the grid).
+    const uint32_t row0 = z0 * a.sz + y * a.pitch + tx * BX;
+    for (uint32_t s = s_begin + lane; s < s_end; s += kTileX) {
+        uint2 next_rec = make_uint2(0u, 0u);
+        if (s + kTileX < s_end) next_rec = a.sp_rec[s + kTileX];
+        const uint32_t meta = rec.x;
+        const uint32_t pos = meta >> kMetaPosShift;
+        const uint32_t i = row0 + (pos / BX) * a.sz + pos % BX;
+        const T c = a.p[i];
+        const T q = a.prev[i];
+        const S nb[6] = {(S)a.p[i - ((meta & 1u) ? 1u : 0u)], (S)a.p[i + ((meta & 2u) ? 1u : 0u)],
+                         (S)a.p[i - ((meta & 4u) ? a.pitch : 0u)], (S)a.p[i + ((meta & 8u) ? a.pitch : 0u)],
+                         (S)a.p[i - ((meta & 16u) ? a.sz : 0u)], (S)a.p[i + ((meta & 32u) ? a.sz : 0u)]};
+        a.prev[i] = (T)special_update(a, meta, rec.y, s, (S)c, (S)q, nb);
+        rec = next_rec;
     }
-    next += a.sp_tapw[s] * a.s_n;  // injection after the update (fdtd.rs:718-720)
-    a.prev[i] = (T)next;
 }
 
-// One time step. Blocks come in groups of 8 (consecutive blocks go to the 8
-// XCDs round-robin); groups of interior tiles and groups of special cells
-// are interleaved evenly, so the latency-bound special cells run alongside
-// the bandwidth-bound tiles instead of as a tail. Receiver blocks come last.
+// One time step: tile blocks, then receiver blocks (p^n before its update,
+// fdtd.rs:673-683). Consecutive tiles go to the 8 XCDs round-robin, so
+// neighbouring tiles run at the same time and spread over all HBM stacks.
 template <typename T, typename S, int TY>
 __global__ __launch_bounds__(kTileX* TY) void k_step(StepParams<T, S> a) {
-    const uint32_t g = blockIdx.x >> 3, k = blockIdx.x & 7u;
-    const uint32_t groups = a.main_groups + a.special_groups;
-    if (g < groups) {
-        const uint64_t sg = a.special_groups;
-        const uint32_t before = (uint32_t)((uint64_t)g * sg / groups);
-        const uint32_t after = (uint32_t)((uint64_t)(g + 1) * sg / groups);
-        if (after > before) {
-            special_part(a, before * 8 + k);
-        } else {
-            // Consecutive tiles go to the 8 XCDs round-robin: neighbouring
-            // tiles run at the same time and spread over all HBM stacks.
-            const uint32_t m = (g - before) * 8 + k;
-            if (m >= a.main_blocks) return;
-            interior_part<T, S, TY>(a, m);
-        }
+    if (blockIdx.x < a.main_blocks) {
+        tile_part<T, S, TY>(a, blockIdx.x);
         return;
     }
-    const uint32_t r = (blockIdx.x - groups * 8) * blockDim.x + threadIdx.x;
-    if (r < a.n_recv) a.ir[r] = a.p[a.recv[r]];  // p^n before its update (fdtd.rs:673-683)
+    const uint32_t r = (blockIdx.x - a.main_blocks) * blockDim.x + threadIdx.x;
+    if (r < a.n_recv) a.ir[r] = a.p[a.recv[r]];
 }
 
 // Marks cells that must not take the interior path (boundary nodes, taps).
@@ -433,10 +457,6 @@ __global__ void k_mark(uint32_t* __restrict__ force, const uint32_t* __restrict_
     atomicOr(&force[q >> 5], 1u << (q & 31u));
 }
 
-struct Cls {
-    bool interior, special;
-};
-
 // Neighbour bits of cell (x, y, z) that stay inside the grid: flags from a
 // caller can never send a read outside the field arrays.
 __device__ __forceinline__ uint32_t inside_mask(uint32_t x, uint32_t y, uint32_t z, uint32_t nx, uint32_t ny,
@@ -445,54 +465,83 @@ __device__ __forceinline__ uint32_t inside_mask(uint32_t x, uint32_t y, uint32_t
            (z > 0 ? 16u : 0u) | (z + 1 < nz ? 32u : 0u);
 }
 
+struct Geo {
+    uint64_t npitched;
+    uint32_t pitch, nx, ny, nz;
+};
+
+struct Cls {
+    bool interior, special;
+    uint32_t mask;  // air neighbours inside the grid
+    bool air;
+};
+
 __device__ __forceinline__ Cls classify(const uint8_t* __restrict__ flags, const uint32_t* __restrict__ force,
-                                        uint64_t q, uint64_t npitched, uint32_t pitch, uint32_t nx, uint32_t ny,
-                                        uint32_t nz) {
-    Cls c{false, false};
-    if (q >= npitched) return c;
-    const uint32_t x = (uint32_t)(q % pitch);
-    if (x >= nx) return c;
-    const uint64_t row = q / pitch;
-    const uint8_t f = flags[row * nx + x];
+                                        uint32_t x, uint32_t y, uint32_t z, const Geo& g) {
+    Cls c{false, false, 0u, false};
+    if (x >= g.nx || y >= g.ny || z >= g.nz) return c;
+    const uint64_t row = (uint64_t)z * g.ny + y;
+    const uint64_t q = row * g.pitch + x;
+    const uint8_t f = flags[row * g.nx + x];
     const bool forced = (force[q >> 5] >> (q & 31u)) & 1u;
-    const bool air = f & 64u;
-    const uint32_t m = f & 63u & inside_mask(x, (uint32_t)(row % ny), (uint32_t)(row / ny), nx, ny, nz);
-    c.interior = air && m == 63u && !forced;
-    c.special = (air && !c.interior) || (forced && !air);
+    c.air = f & 64u;
+    c.mask = f & 63u & inside_mask(x, y, z, g.nx, g.ny, g.nz);
+    c.interior = c.air && c.mask == 63u && !forced;
+    c.special = (c.air && !c.interior) || (forced && !c.air);
     return c;
 }
 
-// The z-packed interior bitmap: word (kb, y, x) holds the interior bits of
-// planes 1 + 32 kb .. 32 + 32 kb of column (x, y), bit k for plane 1 + 32 kb + k.
+// The z-packed interior bitmap: word (kb, y, x) holds the bits of planes
+// 1 + 32 kb .. 32 + 32 kb of column (x, y), bit k for plane 1 + 32 kb + k.
 __global__ __launch_bounds__(kBlock) void k_zbits(const uint8_t* __restrict__ flags, const uint32_t* __restrict__ force,
-                                                  uint32_t* __restrict__ zbits, uint64_t npitched, uint32_t pitch,
-                                                  uint32_t nx, uint32_t ny, uint32_t nz, uint32_t nzb) {
+                                                  uint32_t* __restrict__ zbits, Geo g, uint32_t nzb) {
     const uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (t >= (uint64_t)nzb * ny * pitch) return;
-    const uint32_t x = (uint32_t)(t % pitch);
-    const uint64_t r = t / pitch;
-    const uint32_t y = (uint32_t)(r % ny), kb = (uint32_t)(r / ny);
-    uint32_t w = 0;
+    if (t >= (uint64_t)nzb * g.ny * g.pitch) return;
+    const uint32_t x = (uint32_t)(t % g.pitch);
+    const uint64_t r = t / g.pitch;
+    const uint32_t y = (uint32_t)(r % g.ny), kb = (uint32_t)(r / g.ny);
+    uint32_t wi = 0;
     for (uint32_t k = 0; k < 32; ++k) {
         const uint32_t z = 1 + kb * 32 + k;
-        if (z + 1 >= nz) break;
-        const uint64_t q = ((uint64_t)z * ny + y) * pitch + x;
-        if (classify(flags, force, q, npitched, pitch, nx, ny, nz).interior) w |= 1u << k;
+        if (z + 1 >= g.nz) break;
+        wi |= (uint32_t)classify(flags, force, x, y, z, g).interior << k;
     }
-    zbits[t] = w;
+    zbits[t] = wi;
 }
 
-// Pass 1 of the compaction: the number of special cells per 256-cell block.
-__global__ __launch_bounds__(kBlock) void k_classify_count(const uint8_t* __restrict__ flags,
-                                                           const uint32_t* __restrict__ force,
-                                                           uint32_t* __restrict__ counts, uint64_t npitched,
-                                                           uint32_t pitch, uint32_t nx, uint32_t ny, uint32_t nz) {
+// The wave order of special cells: wave w = (zc * tiles_x + tx) * ny + y owns
+// positions [w * seg, (w + 1) * seg); position (w, dz, xi) is cell
+// (tx * BX + xi, y, 1 + zc * zchunk + dz). Every wave segment is a whole
+// number of compaction blocks.
+struct WaveOrder {
+    uint32_t bx, zchunk, seg, tiles_x;
+};
+
+__device__ __forceinline__ bool wave_cell(uint64_t qw, const WaveOrder& o, const Geo& g, uint32_t* x, uint32_t* y,
+                                          uint32_t* z) {
+    const uint64_t w = qw / o.seg;
+    const uint32_t rr = (uint32_t)(qw % o.seg);
+    const uint32_t dz = rr / o.bx, xi = rr % o.bx;
+    if (dz >= o.zchunk) return false;
+    *y = (uint32_t)(w % g.ny);
+    const uint64_t t2 = w / g.ny;
+    const uint32_t tx = (uint32_t)(t2 % o.tiles_x), zc = (uint32_t)(t2 / o.tiles_x);
+    *x = tx * o.bx + xi;
+    *z = 1 + zc * o.zchunk + dz;
+    return *x < g.nx && *z + 1 < g.nz;
+}
+
+// Pass 1 of the compaction: special cells per 256 positions of the wave order.
+__global__ __launch_bounds__(kBlock) void k_sp_count(const uint8_t* __restrict__ flags,
+                                                     const uint32_t* __restrict__ force, uint32_t* __restrict__ counts,
+                                                     Geo g, WaveOrder o) {
     __shared__ uint32_t wc[kBlock / 64];
-    const uint64_t q = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    const Cls c = classify(flags, force, q, npitched, pitch, nx, ny, nz);
-    const uint64_t bsp = __ballot(c.special);
-    const uint32_t lane = threadIdx.x & 63u;
-    if (lane == 0) wc[threadIdx.x >> 6] = (uint32_t)__popcll(bsp);
+    const uint64_t qw = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    uint32_t x, y, z;
+    bool sp = false;
+    if (wave_cell(qw, o, g, &x, &y, &z)) sp = classify(flags, force, x, y, z, g).special;
+    const uint64_t b = __ballot(sp);
+    if ((threadIdx.x & 63u) == 0) wc[threadIdx.x >> 6] = (uint32_t)__popcll(b);
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t s = 0;
@@ -501,7 +550,7 @@ __global__ __launch_bounds__(kBlock) void k_classify_count(const uint8_t* __rest
     }
 }
 
-// Exclusive scan of the block counts in one work-group.
+// Exclusive scan in one work-group.
 __global__ __launch_bounds__(1024) void k_scan(const uint32_t* __restrict__ counts, uint32_t* __restrict__ offs,
                                                uint32_t n, uint32_t* __restrict__ total) {
     __shared__ uint32_t part[1024];
@@ -535,48 +584,53 @@ __device__ __forceinline__ int64_t find_sorted(const uint32_t* __restrict__ a, u
     return (lo < n && a[lo] == key) ? (int64_t)lo : -1;
 }
 
-// Pass 2: each special cell's rank = block offset + earlier waves + earlier
-// lanes (mbcnt-style popcount of the ballot below the lane).
+// Pass 2: each special cell's rank is its block's offset + earlier waves +
+// earlier lanes (popcount of the ballot below the lane). Writes the
+// record (metadata word with the cell's position in its wave's chunk, and
+// the first constant of its slot set) and tap weight.
 template <typename S>
-__global__ __launch_bounds__(kBlock) void k_classify_write(
-    const uint8_t* __restrict__ flags, const uint32_t* __restrict__ force, const uint32_t* __restrict__ offs,
-    uint64_t npitched, uint32_t pitch, uint32_t nx, uint32_t ny, uint32_t nz,
-    const uint32_t* __restrict__ b_cell_sorted,
-    const uint32_t* __restrict__ b_node_sorted, uint32_t nb, const uint32_t* __restrict__ b_start,
-    const uint32_t* __restrict__ tap_cell, const S* __restrict__ tap_w, uint32_t n_tap,
-    uint32_t* __restrict__ sp_cell, uint32_t* __restrict__ sp_meta, uint32_t* __restrict__ sp_slot0,
-    S* __restrict__ sp_tapw) {
+__global__ __launch_bounds__(kBlock) void k_sp_write(const uint8_t* __restrict__ flags,
+                                                     const uint32_t* __restrict__ force,
+                                                     const uint32_t* __restrict__ offs, Geo g, WaveOrder o,
+                                                     const uint32_t* __restrict__ b_cell_sorted,
+                                                     const uint32_t* __restrict__ b_set_sorted, uint32_t nb,
+                                                     const uint32_t* __restrict__ set_n,
+                                                     const uint32_t* __restrict__ set_off,
+                                                     const uint32_t* __restrict__ tap_cell, const S* __restrict__ tap_w,
+                                                     uint32_t n_tap, uint2* __restrict__ sp_rec,
+                                                     S* __restrict__ sp_tapw) {
     __shared__ uint32_t wc[kBlock / 64];
-    const uint64_t q = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    const Cls c = classify(flags, force, q, npitched, pitch, nx, ny, nz);
-    const uint64_t bsp = __ballot(c.special);
+    const uint64_t qw = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    uint32_t x = 0, y = 0, z = 0;
+    Cls c{false, false, 0u, false};
+    if (wave_cell(qw, o, g, &x, &y, &z)) c = classify(flags, force, x, y, z, g);
+    const uint64_t b = __ballot(c.special);
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    if (lane == 0) wc[wave] = (uint32_t)__popcll(bsp);
+    if (lane == 0) wc[wave] = (uint32_t)__popcll(b);
     __syncthreads();
     if (!c.special) return;
-    uint32_t o = offs[blockIdx.x];
-    for (uint32_t w = 0; w < wave; ++w) o += wc[w];
-    o += (uint32_t)__popcll(bsp & ((1ull << lane) - 1ull));
-    const uint32_t x = (uint32_t)(q % pitch);
-    const uint64_t row = q / pitch;
-    const uint32_t ref = (uint32_t)(row * nx + x);
-    const uint8_t f = flags[ref];
-    const uint32_t m = f & 63u & inside_mask(x, (uint32_t)(row % ny), (uint32_t)(row / ny), nx, ny, nz);
-    uint32_t meta = (f & 64u) ? m : kMetaSolid;
-    uint32_t slot0 = 0;
-    if (f & 64u) {
-        const int64_t b = find_sorted(b_cell_sorted, nb, ref);
-        if (b >= 0) {
-            const uint32_t node = b_node_sorted[b];
-            slot0 = b_start[node];
-            meta |= (b_start[node + 1] - slot0) << 8;
+    uint32_t rank = offs[blockIdx.x];
+    for (uint32_t w = 0; w < wave; ++w) rank += wc[w];
+    rank += (uint32_t)__popcll(b & ((1ull << lane) - 1ull));
+    const uint32_t ref = (uint32_t)(((uint64_t)z * g.ny + y) * g.nx + x);
+    uint32_t meta = (c.air ? c.mask : kMetaSolid) | ((uint32_t)(qw % o.seg) << kMetaPosShift);
+    uint32_t c0 = 0;
+    if (c.air) {
+        const int64_t bi = find_sorted(b_cell_sorted, nb, ref);
+        if (bi >= 0) {
+            const uint32_t set = b_set_sorted[bi];
+            meta |= set_n[set] << kMetaSlotShift;
+            c0 = set_off[set];
         }
     }
     const int64_t t = find_sorted(tap_cell, n_tap, ref);
-    sp_cell[o] = (uint32_t)q;
-    sp_meta[o] = meta;
-    sp_slot0[o] = slot0;
-    sp_tapw[o] = t >= 0 ? tap_w[t] : S(0);
+    S w = 0;
+    if (t >= 0) {
+        meta |= kMetaTap;
+        w = tap_w[t];
+    }
+    sp_rec[rank] = make_uint2(meta, c0);
+    sp_tapw[rank] = w;
 }
 
 // Receiver spectra from the recorded series, FP64: sum_n p^n e^{-j 2 pi fract(f dt n)}
@@ -617,10 +671,11 @@ __global__ void k_fill(T* __restrict__ p, size_t n, T v) {
 
 inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
-// Number of interior-kernel variants (the table is with launch_step) and
-// the default one.
+// Number of tile-kernel variants (rows per tile: see launch_step), the
+// default one, and the default z-chunk.
 constexpr int kVariants = 4;
 constexpr int kDefaultVariant = 0;
+constexpr uint32_t kDefaultZChunk = 4;
 
 }  // namespace
 
@@ -630,21 +685,22 @@ struct wv_plan {
     int device = 0;
     hipStream_t stream = nullptr;
     int prec = WV_FP32;
-    uint32_t nx = 0, ny = 0, nz = 0, pitch = 0, wpr = 0, sz = 0;
+    uint32_t nx = 0, ny = 0, nz = 0, pitch = 0, sz = 0;
     uint64_t npitched = 0, cells = 0, air = 0, nb = 0, ns = 0, ns_k = 0;
     double dx = 0, dt = 0;
-    uint32_t zchunk = 0;
-    uint32_t nzb = 0;  // 32-plane blocks of the z-packed bitmap
+    uint32_t zchunk = 0;      // requested (0 = default)
+    uint32_t run_zchunk = 0;  // the current run's layout
+    uint32_t nzb = 0;         // 32-plane blocks of the z-packed bitmaps
     int variant = kDefaultVariant;
-    uint32_t n_blocks_cls = 0;
+    uint32_t max_slots = 0, n_sets = 0;
 
-    DBuf flags, force_base, force, zbits, counts, offs, total;
-    DBuf b_cell_sorted, b_node_sorted, b_start;
-    DBuf sl_inv_d, sl_cv, sl_k, sl_w, st_u, st_g;
+    DBuf flags, force_base, force, zbits, counts, offs;
+    DBuf b_cell_sorted, b_set_sorted;
+    DBuf set_n, set_off, set_inv_d, set_cv, set_k, set_w, st_u, st_g;
     DBuf fa, fb;
-    DBuf sp_cell, sp_meta, sp_slot0, sp_tapw;
+    DBuf sp_rec, sp_tapw;
     DBuf tap_cell, tap_w, recv, ir, freqs, spec;
-    uint32_t n_special = 0;
+    uint32_t n_special = 0, sp_pad = 0, seg_stride = 0;
 
     // Current run.
     bool active = false;
@@ -658,6 +714,7 @@ struct wv_plan {
 
     size_t tsize() const { return prec == WV_FP64 ? 8 : 4; }
     size_t ssize() const { return prec == WV_FP32 ? 4 : 8; }
+    uint32_t lane_cells() const { return (uint32_t)(16 / tsize()); }
     ~wv_plan() {
         if (stream) (void)hipStreamSynchronize(stream);
         for (auto e : {ev[0], ev[1], t0, t1})
@@ -668,7 +725,8 @@ struct wv_plan {
 
 namespace {
 
-// Host -> device copy of a vector (synchronous on the plan's stream).
+// Host -> device copy of a vector (asynchronous on the plan's stream; the
+// vector must outlive the next synchronize).
 template <class V>
 void upload(wv_plan* pl, DBuf& d, const V& v) {
     d.alloc(v.size() * sizeof(v[0]));
@@ -680,6 +738,8 @@ template <class S>
 std::vector<S> cast_vec(const std::vector<double>& v) {
     return std::vector<S>(v.begin(), v.end());
 }
+
+Geo geo_of(const wv_plan* pl) { return Geo{pl->npitched, pl->pitch, pl->nx, pl->ny, pl->nz}; }
 
 void plan_setup(wv_plan* pl, const wv_grid* g) {
     WV_HIP(hipGetDevice(&pl->device));
@@ -693,13 +753,11 @@ void plan_setup(wv_plan* pl, const wv_grid* g) {
     pl->nz = (uint32_t)g->n[2];
     pl->cells = g->n[0] * g->n[1] * g->n[2];
     pl->pitch = (pl->nx + 63u) & ~63u;
-    pl->wpr = pl->pitch / 32u;
     pl->sz = pl->pitch * pl->ny;
     pl->npitched = (uint64_t)pl->sz * pl->nz;
     pl->dx = g->dx;
     pl->dt = g->dt;
     const uint64_t cells = pl->cells;
-    WV_HIP(hipSetDevice(pl->device));
 
     // Flags as given; the device layout is derived from them per run.
     pl->flags.alloc(cells);
@@ -709,7 +767,8 @@ void plan_setup(wv_plan* pl, const wv_grid* g) {
     pl->air = air;
 
     // Boundary nodes sorted by cell (the reference builds them in cell order;
-    // any order is accepted), with their CSR slot ranges.
+    // any order is accepted), each with its slot set: nodes whose slot lists
+    // are equal (same faces and materials) share one set of constants.
     const uint64_t nb = g->n_boundary;
     pl->nb = nb;
     std::vector<uint32_t> order(nb);
@@ -719,52 +778,73 @@ void plan_setup(wv_plan* pl, const wv_grid* g) {
         if (g->b_cell[b] <= g->b_cell[b - 1]) { sorted = false; break; }
     if (!sorted)
         std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return g->b_cell[a] < g->b_cell[b]; });
-    std::vector<uint32_t> bcell(nb), bstart(g->b_start, g->b_start + nb + 1);
-    for (uint64_t b = 0; b < nb; ++b) bcell[b] = g->b_cell[order[b]];
-    upload(pl, pl->b_cell_sorted, bcell);
-    upload(pl, pl->b_node_sorted, order);
-    upload(pl, pl->b_start, bstart);
-    const uint64_t ns = nb ? g->b_start[nb] : 0;
-    pl->ns = ns;
-    std::vector<double> inv_d(ns), cv(ns), kk(ns), w(ns);
+    std::vector<uint32_t> bcell(nb), bset(nb);
+    std::unordered_map<std::string, uint32_t> sets;
+    std::vector<uint32_t> set_n, set_off;
+    std::vector<double> inv_d, cv, kk, w;
     uint64_t ns_k = 0;
-    for (uint64_t j = 0; j < ns; ++j) {
-        inv_d[j] = g->slots[j].inv_d;
-        cv[j] = g->slots[j].cv;
-        kk[j] = g->slots[j].k;
-        w[j] = g->slots[j].w;
-        ns_k += g->slots[j].k != 0.0;
+    // Set 0 is the empty set (a node listed without slots).
+    set_n.push_back(0);
+    set_off.push_back(0);
+    sets.emplace(std::string(), 0u);
+    for (uint64_t k = 0; k < nb; ++k) {
+        const uint32_t node = order[k];
+        bcell[k] = g->b_cell[node];
+        const uint32_t j0 = g->b_start[node], j1 = g->b_start[node + 1];
+        if (j1 - j0 > kMaxSlots) throw DevErr{"a boundary node has more than 63 slots", WV_ERR_LIMIT};
+        std::string key(reinterpret_cast<const char*>(g->slots + j0), (j1 - j0) * sizeof(wv_slot));
+        auto it = sets.find(key);
+        if (it == sets.end()) {
+            const uint32_t id = (uint32_t)set_n.size();
+            set_n.push_back(j1 - j0);
+            set_off.push_back((uint32_t)inv_d.size());
+            for (uint32_t j = j0; j < j1; ++j) {
+                inv_d.push_back(g->slots[j].inv_d);
+                cv.push_back(g->slots[j].cv);
+                kk.push_back(g->slots[j].k);
+                w.push_back(g->slots[j].w);
+            }
+            it = sets.emplace(std::move(key), id).first;
+        }
+        bset[k] = it->second;
+        pl->max_slots = std::max(pl->max_slots, j1 - j0);
+        for (uint32_t j = j0; j < j1; ++j) ns_k += g->slots[j].k != 0.0;
     }
+    pl->ns = nb ? g->b_start[nb] : 0;
     pl->ns_k = ns_k;
+    pl->n_sets = (uint32_t)set_n.size();
+    upload(pl, pl->b_cell_sorted, bcell);
+    upload(pl, pl->b_set_sorted, bset);
+    upload(pl, pl->set_n, set_n);
+    upload(pl, pl->set_off, set_off);
+    std::vector<float> f_inv, f_cv, f_k, f_w;
     if (pl->prec == WV_FP32) {
-        upload(pl, pl->sl_inv_d, cast_vec<float>(inv_d));
-        upload(pl, pl->sl_cv, cast_vec<float>(cv));
-        upload(pl, pl->sl_k, cast_vec<float>(kk));
-        upload(pl, pl->sl_w, cast_vec<float>(w));
+        f_inv = cast_vec<float>(inv_d);
+        f_cv = cast_vec<float>(cv);
+        f_k = cast_vec<float>(kk);
+        f_w = cast_vec<float>(w);
+        upload(pl, pl->set_inv_d, f_inv);
+        upload(pl, pl->set_cv, f_cv);
+        upload(pl, pl->set_k, f_k);
+        upload(pl, pl->set_w, f_w);
     } else {
-        upload(pl, pl->sl_inv_d, inv_d);
-        upload(pl, pl->sl_cv, cv);
-        upload(pl, pl->sl_k, kk);
-        upload(pl, pl->sl_w, w);
+        upload(pl, pl->set_inv_d, inv_d);
+        upload(pl, pl->set_cv, cv);
+        upload(pl, pl->set_k, kk);
+        upload(pl, pl->set_w, w);
     }
-    pl->st_u.alloc(ns * pl->ssize());
-    pl->st_g.alloc(ns * pl->ssize());
 
     // Boundary marks, reused by every run.
     const uint64_t words = pl->npitched / 32;
     pl->force_base.alloc(words * 4);
     pl->force.alloc(words * 4);
-    pl->nzb = (pl->nz - 2 + 31) / 32;
-    pl->zbits.alloc((size_t)pl->nzb * pl->ny * pl->pitch * 4);
     WV_HIP(hipMemsetAsync(pl->force_base.p, 0, words * 4, pl->stream));
     if (nb)
         hipLaunchKernelGGL(k_mark, dim3(div_up(nb, 256)), dim3(256), 0, pl->stream, pl->force_base.as<uint32_t>(),
                            pl->b_cell_sorted.as<uint32_t>(), (uint32_t)nb, pl->nx, pl->pitch);
     WV_HIP(hipGetLastError());
-    pl->n_blocks_cls = div_up(pl->npitched, kBlock);
-    pl->counts.alloc((size_t)pl->n_blocks_cls * 4);
-    pl->offs.alloc((size_t)pl->n_blocks_cls * 4);
-    pl->total.alloc(4);
+    pl->nzb = (pl->nz - 2 + 31) / 32;
+    pl->zbits.alloc((size_t)pl->nzb * pl->ny * pl->pitch * 4);
     pl->fa.alloc(pl->npitched * pl->tsize());
     pl->fb.alloc(pl->npitched * pl->tsize());
     WV_HIP(hipStreamSynchronize(pl->stream));
@@ -775,6 +855,23 @@ void fill_async(wv_plan* pl, DBuf& d, size_t n) {
     if (!n) return;
     hipLaunchKernelGGL(k_fill<T>, dim3(div_up(n, 256)), dim3(256), 0, pl->stream, d.as<T>(), n, T(0));
     WV_HIP(hipGetLastError());
+}
+
+struct Layout {
+    uint32_t zchunk, bx, tiles_x, nzc, seg;
+    uint64_t nwaves, positions;
+};
+
+Layout layout_of(const wv_plan* pl) {
+    Layout L;
+    L.zchunk = pl->run_zchunk;
+    L.bx = kTileX * pl->lane_cells();
+    L.tiles_x = div_up(pl->nx, L.bx);
+    L.nzc = div_up(pl->nz - 2, L.zchunk);
+    L.seg = (L.zchunk * L.bx + kBlock - 1) / kBlock * kBlock;
+    L.nwaves = (uint64_t)L.nzc * L.tiles_x * pl->ny;
+    L.positions = L.nwaves * L.seg;
+    return L;
 }
 
 template <class S>
@@ -802,30 +899,42 @@ void run_begin_impl(wv_plan* pl, const wv_tap* taps, size_t n_taps, const double
     if (!tc.empty())
         hipLaunchKernelGGL(k_mark, dim3(div_up(tc.size(), 256)), dim3(256), 0, pl->stream, pl->force.as<uint32_t>(),
                            pl->tap_cell.as<uint32_t>(), (uint32_t)tc.size(), pl->nx, pl->pitch);
+    const Geo geo = geo_of(pl);
     hipLaunchKernelGGL(k_zbits, dim3(div_up((uint64_t)pl->nzb * pl->ny * pl->pitch, kBlock)), dim3(kBlock), 0,
                        pl->stream, pl->flags.as<uint8_t>(), pl->force.as<uint32_t>(), pl->zbits.as<uint32_t>(),
-                       pl->npitched, pl->pitch, pl->nx, pl->ny, pl->nz, pl->nzb);
-    hipLaunchKernelGGL(k_classify_count, dim3(pl->n_blocks_cls), dim3(kBlock), 0, pl->stream,
-                       pl->flags.as<uint8_t>(), pl->force.as<uint32_t>(), pl->counts.as<uint32_t>(), pl->npitched,
-                       pl->pitch, pl->nx, pl->ny, pl->nz);
+                       geo, pl->nzb);
+    // Special cells in wave order.
+    pl->run_zchunk = pl->zchunk ? pl->zchunk : kDefaultZChunk;
+    const Layout L = layout_of(pl);
+    const WaveOrder wo{L.bx, L.zchunk, L.seg, L.tiles_x};
+    const uint64_t nblocks = L.positions / kBlock;
+    if (nblocks >= (1ull << 31)) throw DevErr{"grid too large for the special-cell layout", WV_ERR_LIMIT};
+    // offs[b] = special cells before compaction block b; offs[nblocks] = total.
+    pl->counts.alloc(nblocks * 4);
+    pl->offs.alloc((nblocks + 1) * 4);
+    pl->seg_stride = L.seg / kBlock;
+    hipLaunchKernelGGL(k_sp_count, dim3((uint32_t)nblocks), dim3(kBlock), 0, pl->stream, pl->flags.as<uint8_t>(),
+                       pl->force.as<uint32_t>(), pl->counts.as<uint32_t>(), geo, wo);
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, pl->stream, pl->counts.as<uint32_t>(), pl->offs.as<uint32_t>(),
-                       pl->n_blocks_cls, pl->total.as<uint32_t>());
+                       (uint32_t)nblocks, pl->offs.as<uint32_t>() + nblocks);
     WV_HIP(hipGetLastError());
     uint32_t n_special = 0;
-    WV_HIP(hipMemcpyAsync(&n_special, pl->total.p, 4, hipMemcpyDeviceToHost, pl->stream));
+    WV_HIP(hipMemcpyAsync(&n_special, pl->offs.as<uint32_t>() + nblocks, 4, hipMemcpyDeviceToHost, pl->stream));
     WV_HIP(hipStreamSynchronize(pl->stream));
     pl->n_special = n_special;
-    pl->sp_cell.alloc((size_t)n_special * 4);
-    pl->sp_meta.alloc((size_t)n_special * 4);
-    pl->sp_slot0.alloc((size_t)n_special * 4);
+    pl->sp_pad = (n_special + 63u) & ~63u;
+    pl->sp_rec.alloc((size_t)n_special * 8);
     pl->sp_tapw.alloc((size_t)n_special * sizeof(S));
-    hipLaunchKernelGGL(k_classify_write<S>, dim3(pl->n_blocks_cls), dim3(kBlock), 0, pl->stream,
-                       pl->flags.as<uint8_t>(), pl->force.as<uint32_t>(), pl->offs.as<uint32_t>(), pl->npitched,
-                       pl->pitch, pl->nx, pl->ny, pl->nz, pl->b_cell_sorted.as<uint32_t>(), pl->b_node_sorted.as<uint32_t>(),
-                       (uint32_t)pl->nb, pl->b_start.as<uint32_t>(), pl->tap_cell.as<uint32_t>(), pl->tap_w.as<S>(),
-                       (uint32_t)tc.size(), pl->sp_cell.as<uint32_t>(), pl->sp_meta.as<uint32_t>(),
-                       pl->sp_slot0.as<uint32_t>(), pl->sp_tapw.as<S>());
+    hipLaunchKernelGGL(k_sp_write<S>, dim3((uint32_t)nblocks), dim3(kBlock), 0, pl->stream, pl->flags.as<uint8_t>(),
+                       pl->force.as<uint32_t>(), pl->offs.as<uint32_t>(), geo, wo, pl->b_cell_sorted.as<uint32_t>(),
+                       pl->b_set_sorted.as<uint32_t>(), (uint32_t)pl->nb, pl->set_n.as<uint32_t>(),
+                       pl->set_off.as<uint32_t>(), pl->tap_cell.as<uint32_t>(), pl->tap_w.as<S>(), (uint32_t)tc.size(),
+                       pl->sp_rec.as<uint2>(), pl->sp_tapw.as<S>());
     WV_HIP(hipGetLastError());
+    // Slot state, slot-major: state[j][rank].
+    const size_t nstate = (size_t)pl->max_slots * pl->sp_pad;
+    pl->st_u.alloc(nstate * pl->ssize());
+    pl->st_g.alloc(nstate * pl->ssize());
     // Receivers: pitched cell indices.
     std::vector<uint32_t> rc(n_recv);
     for (size_t r = 0; r < n_recv; ++r) {
@@ -844,8 +953,8 @@ void run_begin_impl(wv_plan* pl, const wv_tap* taps, size_t n_taps, const double
         fill_async<float>(pl, pl->fb, pl->npitched);
         fill_async<float>(pl, pl->ir, steps * n_recv);
     }
-    fill_async<S>(pl, pl->st_u, pl->ns);
-    fill_async<S>(pl, pl->st_g, pl->ns);
+    fill_async<S>(pl, pl->st_u, nstate);
+    fill_async<S>(pl, pl->st_g, nstate);
     WV_HIP(hipStreamSynchronize(pl->stream));
     pl->signal.assign(signal, signal + steps);
     pl->steps = steps;
@@ -857,33 +966,26 @@ void run_begin_impl(wv_plan* pl, const wv_tap* taps, size_t n_taps, const double
 template <class T, class S>
 StepParams<T, S> make_params(wv_plan* pl, uint32_t ty_rows) {
     StepParams<T, S> a{};
+    const Layout L = layout_of(pl);
     a.zbits = pl->zbits.as<uint32_t>();
     a.nx = pl->nx;
     a.ny = pl->ny;
     a.nz = pl->nz;
     a.pitch = pl->pitch;
     a.sz = pl->sz;
-    a.wpr = pl->wpr;
-    a.tiles_x = div_up(pl->nx, kTileX * (16 / sizeof(T)));
+    a.tiles_x = L.tiles_x;
     a.tiles_y = div_up(pl->ny, ty_rows);
-    uint32_t zc = pl->zchunk;
-    if (zc == 0) zc = 8;
-    const uint32_t zint = pl->nz > 2 ? pl->nz - 2 : 0;
-    a.zchunk = zc;
-    const uint32_t nzc = zint ? div_up(zint, zc) : 0;
-    a.main_blocks = a.tiles_x * a.tiles_y * nzc;
-    a.n_special = pl->n_special;
-    a.special_blocks = div_up(pl->n_special, kTileX * ty_rows);
-    a.main_groups = div_up(a.main_blocks, 8);
-    a.special_groups = div_up(a.special_blocks, 8);
-    a.sp_cell = pl->sp_cell.as<uint32_t>();
-    a.sp_meta = pl->sp_meta.as<uint32_t>();
-    a.sp_slot0 = pl->sp_slot0.as<uint32_t>();
+    a.zchunk = L.zchunk;
+    a.main_blocks = a.tiles_x * a.tiles_y * L.nzc;
+    a.seg_base = pl->offs.as<uint32_t>();
+    a.seg_stride = pl->seg_stride;
+    a.sp_rec = pl->sp_rec.as<uint2>();
     a.sp_tapw = pl->sp_tapw.as<S>();
-    a.sl_inv_d = pl->sl_inv_d.as<S>();
-    a.sl_cv = pl->sl_cv.as<S>();
-    a.sl_k = pl->sl_k.as<S>();
-    a.sl_w = pl->sl_w.as<S>();
+    a.sp_pad = pl->sp_pad;
+    a.set_inv_d = pl->set_inv_d.as<S>();
+    a.set_cv = pl->set_cv.as<S>();
+    a.set_k = pl->set_k.as<S>();
+    a.set_w = pl->set_w.as<S>();
     a.st_u = pl->st_u.as<S>();
     a.st_g = pl->st_g.as<S>();
     a.has_k = pl->ns_k > 0;
@@ -909,7 +1011,7 @@ StepParams<T, S> make_params(wv_plan* pl, uint32_t ty_rows) {
     return a;
 }
 
-// Interior-kernel variants: TY rows per tile (block = 64 x TY threads).
+// Tile-kernel variants: TY rows per tile (block = 64 x TY threads).
 constexpr uint32_t kVariantRows[] = {4, 8, 16, 2};
 
 template <class T, class S>
@@ -926,7 +1028,7 @@ template <class T, class S>
 wv_status advance_impl(wv_plan* pl, size_t k, const volatile int32_t* cancel, wv_progress_fn progress, void* ctx) {
     const uint32_t ty_rows = kVariantRows[pl->variant];
     StepParams<T, S> a = make_params<T, S>(pl, ty_rows);
-    const uint32_t grid = (a.main_groups + a.special_groups) * 8 + div_up(pl->n_recv, kTileX * ty_rows);
+    const uint32_t grid = a.main_blocks + div_up(pl->n_recv, kTileX * ty_rows);
     const size_t end = pl->done + k;
     T* A = pl->fa.as<T>();
     T* B = pl->fb.as<T>();
@@ -1071,9 +1173,13 @@ extern "C" wv_status wv_run_begin(wv_plan* pl, const wv_tap* taps, size_t n_taps
         set_err(err, err_len, "wv_run_begin: invalid argument");
         return WV_ERR_INVALID;
     }
+    // Taps must lie in the planes the step kernel updates (1 .. nz-2); every
+    // air cell does. The reference injects anywhere (fdtd.rs:718-720), but a
+    // tap on the outer padding planes could only reach a receiver there.
+    const uint64_t plane = (uint64_t)pl->nx * pl->ny;
     for (size_t t = 0; t < n_taps; ++t)
-        if (taps[t].cell >= pl->cells) {
-            set_err(err, err_len, "source tap outside the grid");
+        if (taps[t].cell >= pl->cells || taps[t].cell / plane == 0 || taps[t].cell / plane + 1 >= pl->nz) {
+            set_err(err, err_len, "source tap outside the updated planes of the grid");
             return WV_ERR_INVALID;
         }
     for (size_t r = 0; r < n_recv; ++r)
