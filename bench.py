@@ -11,13 +11,19 @@ synchronize on both sides).
 
 Prints one JSON line with the metric, the roofline of the step kernel
 (algorithmic bytes per launch / measured launch time, HIP events on the
-kernel's stream) and a CPU baseline (the FP64 oracle, threaded, on a bounded
-sample of the same grid).
+kernel's stream; HBM traffic per launch from two rocprofv3 counter passes of
+a short run, when rocprofv3 is installed) and a CPU baseline (the FP64
+oracle, threaded, on a bounded sample of the same grid).
 """
 import argparse
+import glob
 import json
 import os
+import shutil
+import sqlite3
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -41,7 +47,9 @@ def algorithmic_bytes(st, n_recv, n_taps, fp64=False):
 
 def cpu_baseline(cfg, case, budget_s=15.0, threads=None):
     """The FP64 oracle (restatement of fdtd.rs:644-725, z-slab threaded) on the
-    same grid, timed over a bounded number of steps from rest."""
+    same grid, from rest. Runs of n1 and n2 steps pay the same set-up (field
+    allocation and first touch), so the step rate is the difference quotient;
+    n2 is sized to about budget_s of stepping."""
     import oracle
 
     threads = threads or min(16, os.cpu_count() or 1)
@@ -49,16 +57,74 @@ def cpu_baseline(cfg, case, budget_s=15.0, threads=None):
     taps = case.runs[0][2]
     rx = case.receivers
     sig = case.signal
-    t0 = time.perf_counter()
-    oracle.run(og, taps, sig[:2], rx, threads=threads)
-    t_probe = (time.perf_counter() - t0) / 2
-    n = int(max(3, min(len(sig), budget_s / max(t_probe, 1e-6))))
-    t0 = time.perf_counter()
-    oracle.run(og, taps, sig[:n], rx, threads=threads)
-    dt = time.perf_counter() - t0
+
+    def timed(n):
+        t0 = time.perf_counter()
+        oracle.run(og, taps, sig[:n], rx, threads=threads)
+        return time.perf_counter() - t0
+
+    n1 = 2
+    t1 = timed(n1)
+    t_probe = max(timed(2 * n1) - t1, 1e-6) / n1
+    n2 = int(min(len(sig), max(2 * n1 + 1, n1 + budget_s / t_probe)))
+    t2 = timed(n2)
+    dt = max(t2 - t1, 1e-9)
     air = int(np.count_nonzero(og.flags & 64))
-    return {"value": air * n / dt / 1e9, "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
-            "sample": f"{cfg['name']} grid, FP64 oracle, {n} of {cfg['steps']} steps from rest, {dt:.1f} s"}
+    return {"value": round(air * (n2 - n1) / dt / 1e9, 4), "unit": "Gcell-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{cfg['name']} grid, FP64 oracle, {threads} threads: runs of {n2} and {n1} steps from rest, "
+                      f"{n2 - n1} steps in {dt:.1f} s (difference of the two runs)"}
+
+
+def _counter_kib(db_dir, counter, kernel):
+    """Mean value per dispatch of one counter for kernels whose (demangled)
+    name contains `kernel`, from a rocprofv3 SQLite output."""
+    dbs = glob.glob(os.path.join(db_dir, "**", "*.db"), recursive=True)
+    if not dbs:
+        raise RuntimeError(f"no rocprofv3 database under {db_dir}")
+    vals = []
+    for db in dbs:
+        con = sqlite3.connect(db)
+        vals += [r[0] for r in con.execute(
+            "select value from counters_collection where counter_name = ? and kernel_name like ?",
+            (counter, "%" + kernel + "%"))]
+        con.close()
+    if not vals:
+        raise RuntimeError(f"{counter} not recorded for {kernel}")
+    return sum(vals) / len(vals), len(vals)
+
+
+def measure_traffic(args, copy_bytes):
+    """HBM bytes per step-kernel launch from two rocprofv3 counter passes
+    (FETCH_SIZE, then WRITE_SIZE) over a short run of this workload. Each
+    counter is scaled by what it reads for the plain copy kernel of the same
+    run, which moves copy_bytes in each direction."""
+    exe = shutil.which("rocprofv3")
+    if exe is None:
+        return None, "rocprofv3 not found"
+    out = {}
+    for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix=f"wv_{ctr.lower()}_", dir="/tmp")
+        cmd = [exe, "--pmc", ctr, "-d", d, "-o", "run", "--", sys.executable, os.path.abspath(__file__),
+               "--no-cpu", "--no-traffic", "--steps", "10", "--warmup", "2", "--workload", args.workload,
+               "--precision", args.precision, "--zchunk", str(args.zchunk), "--variant", str(args.variant)]
+        print(f"bench: counter pass {ctr}", file=sys.stderr, flush=True)
+        env = dict(os.environ, TMPDIR="/tmp")
+        try:
+            r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE,
+                               timeout=180)
+            if r.returncode != 0:
+                return None, f"rocprofv3 {ctr} pass exited with {r.returncode}"
+            step_kib, n = _counter_kib(d, ctr, "k_step")
+            copy_kib, _ = _counter_kib(d, ctr, "k_copy")
+        except (subprocess.TimeoutExpired, RuntimeError, sqlite3.Error) as e:
+            return None, f"{ctr} pass: {e}"
+        finally:
+            shutil.rmtree(d, ignore_errors=True)
+        scale = copy_bytes / (copy_kib * 1024.0)
+        out[ctr] = {"kib_per_dispatch": round(step_kib, 1), "dispatches": n, "scale": round(scale, 4),
+                    "bytes": int(step_kib * 1024.0 * scale)}
+    return out["FETCH_SIZE"]["bytes"] + out["WRITE_SIZE"]["bytes"], out
 
 
 def main():
@@ -71,6 +137,7 @@ def main():
     ap.add_argument("--zchunk", type=int, default=0)
     ap.add_argument("--variant", type=int, default=-1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the HBM counter passes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -143,6 +210,10 @@ def main():
                      "copy_GBs": round(2 * copy_bytes / (copy_ms * 1e-3) / 1e9, 1), "copy_bytes": 2 * copy_bytes},
         "setup_s": {"grid_build": round(t_build, 2), "upload_and_layout": round(t_setup, 2)},
     }
+    if not args.no_traffic:
+        traffic, detail = measure_traffic(args, copy_bytes)
+        line["roofline"]["traffic"] = traffic
+        line["roofline"]["traffic_detail"] = detail
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(cfg, case)
     print(json.dumps(line), flush=True)

@@ -82,6 +82,7 @@ struct DBuf {
 constexpr uint32_t kBlock = 256;          // setup kernels and compaction blocks
 constexpr uint32_t kTileX = 64;           // one 64-lane wave per tile row
 constexpr uint32_t kStepsPerBlock = 128;  // cancel/progress granularity, as gpu_fdtd.rs:35
+constexpr uint32_t kSubPlanes = 4;        // planes per special-cell pass of a z-chunk
 
 // Special-cell record: {meta, first constant of its slot set}. Meta bits:
 // 0-5 air neighbours (-x, +x, -y, +y, -z, +z), 6 solid, 7 tap, 8-13 slot
@@ -354,9 +355,7 @@ __device__ __forceinline__ void tile_part(const StepParams<T, S>& a, uint32_t t)
     const uint32_t xl = tx * BX + lane * W;  // first cell of this lane
     const uint32_t y = ty * TY + wave;
     if (y >= a.ny) return;
-    const uint32_t w_id = (zc * a.tiles_x + tx) * a.ny + y;
-    const uint32_t s_begin = a.seg_base[w_id * a.seg_stride];
-    const uint32_t s_end = a.seg_base[(w_id + 1) * a.seg_stride];
+    const uint32_t w_seg = ((zc * a.tiles_x + tx) * a.ny + y) * a.seg_stride;
     // Lanes past the row pitch (the last tile of a row) re-read the row's
     // last 16 bytes and update nothing: no traffic beyond the row.
     const bool live = xl < a.pitch;
@@ -378,59 +377,72 @@ __device__ __forceinline__ void tile_part(const StepParams<T, S>& a, uint32_t t)
     Cells<T> zm, zcur;
     zm.load(a.p + (z0 - 1) * a.sz + col);
     zcur.load(a.p + z0 * a.sz + col);
-    // The first special records are read now, so they have arrived when
-    // the z-march ends.
-    uint2 rec = make_uint2(0u, 0u);
-    if (s_begin + lane < s_end) rec = a.sp_rec[s_begin + lane];
-    for (uint32_t z = z0; z < z1; ++z) {
-        const uint32_t i = z * a.sz + col;
-        T hl = 0, hr = 0;
-        if (lane == 0) hl = a.p[i - hl_off];
-        if (lane == kTileX - 1) hr = a.p[i + hr_off];
-        Cells<T> zp, q, ym, yp;
-        zp.load(a.p + i + a.sz);
-        q.load(a.prev + i);
-        ym.load(a.p + i - ym_off);
-        yp.load(a.p + i + yp_off);
-        const T lo = from_lower_lane(zcur.v[W - 1], hl);
-        const T hi = from_upper_lane(zcur.v[0], hr);
-        Cells<T> out;
-#pragma unroll
-        for (int e = 0; e < W; ++e) {
-            const T xm = e == 0 ? lo : zcur.v[e - 1];
-            const T xp = e == W - 1 ? hi : zcur.v[e + 1];
-            out.v[e] = interior_update(zcur.v[e], q.v[e], xm, xp, ym.v[e], yp.v[e], zm.v[e], zp.v[e], a.l2, a.c2);
-        }
-        const uint32_t bits = plane_bits<W>(ib, (z - 1) & 31u) & lane_mask;
-        if (bits == full) {
-            out.store(a.prev + i);
-        } else if (bits) {
-            // Row ends next to walls: only the interior cells.
-#pragma unroll
-            for (int e = 0; e < W; ++e)
-                if ((bits >> e) & 1u) a.prev[i + e] = out.v[e];
-        }
-        zm = zcur;
-        zcur = zp;
-    }
-
-    // The chunk's special cells. A record's position gives the cell; its
-    // neighbour loads are unconditional (a masked-out neighbour reads the
-    // cell itself, and the mask never points outside the grid).
+    // The chunk is marched in sub-chunks of kSubPlanes planes; after each,
+    // the wave updates that sub-chunk's special cells while the rows they
+    // read are still in cache. Records are sorted by plane, so a sub-chunk's
+    // records start at the compaction block of its first plane.
     const uint32_t row0 = z0 * a.sz + y * a.pitch + tx * BX;
-    for (uint32_t s = s_begin + lane; s < s_end; s += kTileX) {
-        uint2 next_rec = make_uint2(0u, 0u);
-        if (s + kTileX < s_end) next_rec = a.sp_rec[s + kTileX];
-        const uint32_t meta = rec.x;
-        const uint32_t pos = meta >> kMetaPosShift;
-        const uint32_t i = row0 + (pos / BX) * a.sz + pos % BX;
-        const T c = a.p[i];
-        const T q = a.prev[i];
-        const S nb[6] = {(S)a.p[i - ((meta & 1u) ? 1u : 0u)], (S)a.p[i + ((meta & 2u) ? 1u : 0u)],
-                         (S)a.p[i - ((meta & 4u) ? a.pitch : 0u)], (S)a.p[i + ((meta & 8u) ? a.pitch : 0u)],
-                         (S)a.p[i - ((meta & 16u) ? a.sz : 0u)], (S)a.p[i + ((meta & 32u) ? a.sz : 0u)]};
-        a.prev[i] = (T)special_update(a, meta, rec.y, s, (S)c, (S)q, nb);
-        rec = next_rec;
+    uint32_t s_begin = a.seg_base[w_seg];
+    for (uint32_t zs = z0; zs < z1; zs += kSubPlanes) {
+        const uint32_t ze = min(zs + kSubPlanes, z1);
+        const uint32_t s_end =
+            ze == z1 ? a.seg_base[w_seg + a.seg_stride] : a.seg_base[w_seg + (ze - z0) * BX / kBlock];
+        // The first special records are requested now, so they have arrived
+        // when the sub-chunk's z-march ends.
+        uint2 rec = make_uint2(0u, 0u);
+        if (s_begin + lane < s_end) rec = a.sp_rec[s_begin + lane];
+        for (uint32_t z = zs; z < ze; ++z) {
+            const uint32_t i = z * a.sz + col;
+            T hl = 0, hr = 0;
+            if (lane == 0) hl = a.p[i - hl_off];
+            if (lane == kTileX - 1) hr = a.p[i + hr_off];
+            Cells<T> zp, q, ym, yp;
+            zp.load(a.p + i + a.sz);
+            q.load(a.prev + i);
+            ym.load(a.p + i - ym_off);
+            yp.load(a.p + i + yp_off);
+            const T lo = from_lower_lane(zcur.v[W - 1], hl);
+            const T hi = from_upper_lane(zcur.v[0], hr);
+            Cells<T> out;
+#pragma unroll
+            for (int e = 0; e < W; ++e) {
+                const T xm = e == 0 ? lo : zcur.v[e - 1];
+                const T xp = e == W - 1 ? hi : zcur.v[e + 1];
+                out.v[e] =
+                    interior_update(zcur.v[e], q.v[e], xm, xp, ym.v[e], yp.v[e], zm.v[e], zp.v[e], a.l2, a.c2);
+            }
+            const uint32_t bits = plane_bits<W>(ib, (z - 1) & 31u) & lane_mask;
+            if (bits == full) {
+                out.store(a.prev + i);
+            } else if (bits) {
+                // Row ends next to walls: only the interior cells.
+#pragma unroll
+                for (int e = 0; e < W; ++e)
+                    if ((bits >> e) & 1u) a.prev[i + e] = out.v[e];
+            }
+            zm = zcur;
+            zcur = zp;
+        }
+
+        // The sub-chunk's special cells, 64 at a time. A record's position
+        // gives the cell; its neighbour loads are unconditional (a masked-out
+        // neighbour reads the cell itself, and the mask never points outside
+        // the grid).
+        for (uint32_t s = s_begin + lane; s < s_end; s += kTileX) {
+            uint2 next_rec = make_uint2(0u, 0u);
+            if (s + kTileX < s_end) next_rec = a.sp_rec[s + kTileX];
+            const uint32_t meta = rec.x;
+            const uint32_t pos = meta >> kMetaPosShift;
+            const uint32_t i = row0 + (pos / BX) * a.sz + pos % BX;
+            const T c = a.p[i];
+            const T q = a.prev[i];
+            const S nb[6] = {(S)a.p[i - ((meta & 1u) ? 1u : 0u)], (S)a.p[i + ((meta & 2u) ? 1u : 0u)],
+                             (S)a.p[i - ((meta & 4u) ? a.pitch : 0u)], (S)a.p[i + ((meta & 8u) ? a.pitch : 0u)],
+                             (S)a.p[i - ((meta & 16u) ? a.sz : 0u)], (S)a.p[i + ((meta & 32u) ? a.sz : 0u)]};
+            a.prev[i] = (T)special_update(a, meta, rec.y, s, (S)c, (S)q, nb);
+            rec = next_rec;
+        }
+        s_begin = s_end;
     }
 }
 
