@@ -136,6 +136,80 @@ wv_status wv_plan_set_tile(wv_plan* plan, uint32_t zchunk, int32_t variant);
 wv_status wv_plan_copy_probe(wv_plan* plan, uint64_t bytes, uint32_t reps, double* ms);
 void wv_plan_free(wv_plan* plan);
 
+/* Device bytes a run of this grid will hold (fields, layout, slot state,
+ * records), estimated before any allocation; the server admits requests by
+ * it. No reference counterpart. */
+wv_status wv_footprint_estimate(const wv_grid* grid, int precision, size_t steps, size_t n_recv,
+                                size_t n_taps, uint64_t* bytes);
+
+/* ----------------------------------------------------------------- batch
+ *
+ * Many runs (rooms, the per-source and per-component runs of a wave job,
+ * run.rs:645-661) of different grid sizes advanced together: one launch per
+ * step covers every instance's tiles and 128-step blocks are replayed as HIP
+ * graphs. Each instance runs the same arithmetic as wv_plan_run, so its
+ * results equal that run's. The reference runs them one after another. A
+ * batch is used by one thread at a time; inputs are copied at wv_batch_add.
+ */
+typedef struct wv_batch wv_batch;
+wv_status wv_batch_create(int precision, wv_batch** out, char* err, size_t err_len);
+/* Uploads a grid and starts its run from rest; *id numbers instances 0, 1, ... */
+wv_status wv_batch_add(wv_batch* batch, const wv_grid* grid, const wv_tap* taps, size_t n_taps,
+                       const double* signal, size_t steps, const uint64_t* receivers, size_t n_recv,
+                       int32_t* id, char* err, size_t err_len);
+/* Times the whole batch over trial_steps steps per candidate: rows per tile
+ * for the launch, then the z-chunk of each size class (floor(log2(air
+ * cells)), largest first). The choices are kept for later batches and servers
+ * of the process; every instance restarts from rest. */
+wv_status wv_batch_autotune(wv_batch* batch, uint32_t trial_steps, char* err, size_t err_len);
+/* Sets one shape for the whole batch (variant as wv_plan_set_tile; zchunk 0 =
+ * default); instances restart from rest. */
+wv_status wv_batch_set_tile(wv_batch* batch, int32_t variant, uint32_t zchunk, char* err,
+                            size_t err_len);
+/* Runs every instance to its last step. use_graphs = 0 issues the same
+ * launches one by one. Cancel is polled between step blocks. */
+wv_status wv_batch_run(wv_batch* batch, int use_graphs, const volatile int32_t* cancel,
+                       wv_progress_fn progress, void* ctx, char* err, size_t err_len);
+/* An instance's results, as wv_run_finish (ir_out [R][T], spectra [R][F][2]). */
+wv_status wv_batch_result(wv_batch* batch, int32_t id, const double* freqs, size_t nf, int mode,
+                          double* spectra_out, double* ir_out, char* err, size_t err_len);
+/* out: [cells, air_cells, steps, zchunk, size_class, device_bytes]. */
+wv_status wv_batch_info(const wv_batch* batch, int32_t id, uint64_t out[6]);
+/* Last wv_batch_run: device ms (HIP events on the batch stream); out: [step
+ * launches per instance block sequence, cell updates, graph launches, graph
+ * captures, rows-per-tile variant]. */
+wv_status wv_batch_timing(const wv_batch* batch, double* ms, uint64_t out[5]);
+void wv_batch_free(wv_batch* batch);
+
+/* ----------------------------------------------------------------- serving
+ *
+ * A request loop on a worker thread. Submitted runs wait in a queue; each is
+ * admitted when its estimated device footprint fits under hbm_cap_bytes
+ * beside the running ones, joins the running batch at the next 128-step
+ * boundary (continuous batching) and leaves it when done, its results copied
+ * to host memory and its device memory freed. The grid passed to
+ * wv_serve_submit is borrowed until wv_serve_wait returns for that request;
+ * taps, signal, receivers and frequencies are copied. No reference
+ * counterpart: the reference runs one solve at a time.
+ */
+typedef struct wv_server wv_server;
+wv_status wv_serve_create(int precision, uint64_t hbm_cap_bytes, wv_server** out, char* err,
+                          size_t err_len);
+wv_status wv_serve_submit(wv_server* server, const wv_grid* grid, const wv_tap* taps,
+                          size_t n_taps, const double* signal, size_t steps,
+                          const uint64_t* receivers, size_t n_recv, const double* freqs, size_t nf,
+                          int mode, uint64_t* id, char* err, size_t err_len);
+/* Blocks until the request is done; results as wv_run_finish; *latency_s =
+ * completion - submission. Each id can be waited for once. */
+wv_status wv_serve_wait(wv_server* server, uint64_t id, double* spectra_out, double* ir_out,
+                        double* latency_s, char* err, size_t err_len);
+/* out: [completed, p50, p95, p99 latency s, first submission to last
+ * completion s, cell updates, most requests running at once, peak device
+ * bytes, failed, graph captures]. */
+wv_status wv_serve_stats(wv_server* server, double out[10]);
+/* Finishes every submitted request, then stops the worker. */
+void wv_serve_free(wv_server* server);
+
 /* ------------------------------------------------------------ grid build */
 
 /* impedance_fit.rs:27-44 (Branch, AdmittanceFit). */

@@ -87,3 +87,28 @@ def test_no_device_is_a_device_error():
         assert e.status in (-2, -4) and str(e)
     else:
         raise AssertionError("expected a device error without a GPU")
+
+
+def test_batch_and_server_without_device_are_errors():
+    from conftest import gpu_available
+
+    if gpu_available():
+        return
+    for make in (lambda: fdtd.Batch("fp32"), lambda: fdtd.Server("fp32")):
+        try:
+            make()
+        except fdtd.FdtdError as e:
+            assert e.status == -2 and str(e)
+        else:
+            raise AssertionError("expected a device error without a GPU")
+
+
+def test_footprint_estimate_covers_fields():
+    cfg = configs.shoebox(fdtd.fit_real_impedance)
+    grid = fdtd.FdtdGrid.build_box(cfg["room"], 343.0, 0.1)
+    n = grid.n
+    pitched = ((n[0] + 63) // 64 * 64) * n[1] * n[2]
+    e32 = fdtd.footprint_estimate(grid, "fp32", 2000, 4, 8)
+    e64 = fdtd.footprint_estimate(grid, "fp64", 2000, 4, 8)
+    assert e32 > 2 * 4 * pitched + 2000 * 4 * 4
+    assert e64 > 2 * 8 * pitched and e64 > e32

@@ -1,0 +1,149 @@
+"""GPU: the batched launch and the request server against plain runs.
+
+Each instance of a batch, and each served request, runs the same kernels and
+arithmetic per cell as its plain run (``Plan.run``), whatever the tile shape
+and whichever instances share the launch, so the receiver series must be
+bit-identical. The plain FP64 path is itself checked against the FP64 oracle
+in test_gpu_parity.py, and one batched FP64 instance is checked against the
+oracle here (1e-9 relative L2).
+"""
+import time
+
+import numpy as np
+import pytest
+
+import oracle
+from wavey_amd import configs, fdtd, workload
+
+pytestmark = pytest.mark.gpu
+
+
+def small_rooms(n, seed=7, steps=(260, 700)):
+    """Mix-recipe rooms at dx 0.1 m with different sizes and step counts (not
+    multiples of 128, so instances finish inside a graph block)."""
+    rng = np.random.default_rng(seed)
+    lib = configs.material_library(seed)
+    out = []
+    for i in range(n):
+        dims = (rng.uniform(2.5, 6.0), rng.uniform(2.5, 5.0), rng.uniform(2.4, 3.2))
+        cfg = configs.mix_room(rng, lib, dx=0.1, dims=dims, n_rx=6)
+        cfg["steps"] = int(rng.integers(steps[0], steps[1]))
+        case = workload.build_case(cfg)
+        out.append(case)
+    return out
+
+
+@pytest.fixture(scope="module")
+def rooms():
+    cases = small_rooms(5)
+    plain = {}
+    for prec in ("fp32", "fp64"):
+        for i, c in enumerate(cases):
+            with fdtd.Plan(c.grid, prec) as p:
+                plain[(prec, i)] = p.run(c.runs[0][2], c.signal, c.receivers, mode=fdtd.MODE_IR)[0]
+    return cases, plain
+
+
+@pytest.mark.parametrize("graphs", [True, False])
+def test_batch_equals_plain_runs(rooms, graphs):
+    cases, plain = rooms
+    print(f"case: batch of {len(cases)} rooms, fp32, graphs={graphs}", flush=True)
+    with fdtd.Batch("fp32") as b:
+        ids = [b.add(c.grid, c.runs[0][2], c.signal, c.receivers) for c in cases]
+        assert b.run(graphs=graphs) == fdtd.WV_OK
+        t = b.timing()
+        for i, iid in enumerate(ids):
+            ir, _ = b.result(iid)
+            assert ir.shape == plain[("fp32", i)].shape
+            assert np.array_equal(ir, plain[("fp32", i)]), f"instance {i}"
+    if graphs:
+        assert t["graph_launches"] > 0 and t["graph_captures"] > 0
+    else:
+        assert t["graph_launches"] == 0
+
+
+def test_batch_autotuned_shapes_keep_results(rooms):
+    cases, plain = rooms
+    print("case: batch autotune, fp32", flush=True)
+    with fdtd.Batch("fp32") as b:
+        ids = [b.add(c.grid, c.runs[0][2], c.signal, c.receivers) for c in cases]
+        b.autotune(trial_steps=4)
+        zchunks = {b.info(i)["zchunk"] for i in ids}
+        assert zchunks <= {2, 4, 8}
+        b.run()
+        for i, iid in enumerate(ids):
+            assert np.array_equal(b.result(iid)[0], plain[("fp32", i)]), f"instance {i}"
+
+
+def test_batch_fp64_matches_plain_and_oracle(rooms):
+    cases, plain = rooms
+    print("case: batch fp64 vs plain fp64 and the oracle", flush=True)
+    with fdtd.Batch("fp64") as b:
+        ids = [b.add(c.grid, c.runs[0][2], c.signal, c.receivers) for c in cases[:3]]
+        b.set_tile(variant=1, zchunk=2)
+        b.run()
+        irs = [b.result(i)[0] for i in ids]
+    for i in range(3):
+        assert np.array_equal(irs[i], plain[("fp64", i)])
+    c = cases[0]
+    og = oracle.build_box(c.cfg["room"], c.cfg["c"], c.cfg["dx"])
+    ref = oracle.run(og, c.runs[0][2], c.signal, c.receivers, threads=8)[0]
+    err = np.linalg.norm(irs[0] - ref) / np.linalg.norm(ref)
+    assert err < 1e-9
+
+
+def test_batch_spectra_equal_plain(rooms):
+    cases, _ = rooms
+    freqs = [40.0, 80.0, 160.0]
+    c = cases[1]
+    with fdtd.Plan(c.grid, "fp32") as p:
+        _, ref = p.run(c.runs[0][2], c.signal, c.receivers, freqs, mode=fdtd.MODE_SPECTRA)
+    with fdtd.Batch("fp32") as b:
+        iid = b.add(c.grid, c.runs[0][2], c.signal, c.receivers)
+        b.run()
+        _, sp = b.result(iid, freqs, mode=fdtd.MODE_SPECTRA)
+    assert np.array_equal(sp, ref)
+
+
+def test_server_results_same_alone_and_in_mix(rooms):
+    """Requests submitted while others run join the running batch; each
+    result equals the request run alone."""
+    cases, plain = rooms
+    print("case: server, 5 requests in two waves, fp32", flush=True)
+    with fdtd.Server("fp32") as sv:
+        first = [sv.submit(c.grid, c.runs[0][2], c.signal, c.receivers) for c in cases[:2]]
+        time.sleep(0.01)
+        second = [sv.submit(c.grid, c.runs[0][2], c.signal, c.receivers) for c in cases[2:]]
+        out = [sv.wait(r) for r in first + second]
+        st = sv.stats()
+    for i, (ir, _, lat) in enumerate(out):
+        assert np.array_equal(ir, plain[("fp32", i)]), f"request {i}"
+        assert lat > 0
+    assert st["completed"] == 5 and st["failed"] == 0
+    assert st["max_running"] >= 2
+    assert 0 < st["p50_s"] <= st["p95_s"] <= st["p99_s"]
+
+
+def test_server_admission_cap_serializes(rooms):
+    """A cap that holds one request at a time: requests run one after another
+    and still give the same results."""
+    cases, plain = rooms
+    need = max(fdtd.footprint_estimate(c.grid, "fp32", len(c.signal), len(c.receivers), len(c.runs[0][2]))
+               for c in cases[:3])
+    print(f"case: server with cap {need} bytes", flush=True)
+    with fdtd.Server("fp32", hbm_cap_bytes=need + 1) as sv:
+        ids = [sv.submit(c.grid, c.runs[0][2], c.signal, c.receivers) for c in cases[:3]]
+        out = [sv.wait(r) for r in ids]
+        st = sv.stats()
+    for i, (ir, _, _) in enumerate(out):
+        assert np.array_equal(ir, plain[("fp32", i)])
+    assert st["max_running"] == 1
+
+
+def test_server_rejects_oversized_request(rooms):
+    cases, _ = rooms
+    c = cases[0]
+    with fdtd.Server("fp32", hbm_cap_bytes=1 << 20) as sv:
+        rid = sv.submit(c.grid, c.runs[0][2], c.signal, c.receivers)
+        with pytest.raises(fdtd.FdtdError):
+            sv.wait(rid)

@@ -1,0 +1,887 @@
+// Batched and served FDTD runs on one MI355X.
+//
+// Many instances (rooms, per-speaker solves) of different sizes advance
+// together: one launch per time step covers the tiles of every active
+// instance (a block table maps launch blocks to instances), and each
+// instance's step counter lives in device memory, so a block of 128 steps is
+// captured once as a HIP graph (128 step launches and one counter advance)
+// and replayed until the set of instances changes. Instances share nothing,
+// and each runs the plain run's kernels and arithmetic, so its receiver
+// series is bit-identical alone and in a batch.
+//
+// Tile shapes come from an autotuner that times the whole batch: first the
+// rows per tile (one template for the launch), then the z-chunk of each size
+// class of instances, coordinate by coordinate.
+//
+// The server admits queued requests by HBM footprint into a running batch at
+// the next 128-step boundary, so requests that arrive while others run share
+// the device (continuous batching); finished instances leave at the same
+// boundaries and their memory is freed.
+//
+// The reference runs a wave job's sources and components one after another
+// (core/wavey-solver/src/run.rs:645-661); each instance here is one such run.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "plan.h"
+
+using namespace wv;
+
+namespace {
+
+constexpr uint32_t kGraphSteps = kStepsPerBlock;  // steps per captured block
+constexpr uint32_t kTuneZChunks[] = {2, 4, 8};
+constexpr int kTuneVariants[] = {0, 1, 3};
+
+template <typename T, typename S>
+struct BatchEntry {
+    StepParams<T, S> a;  // even-step buffers (p = fa, prev = fb) and the record base
+    const S* sig;        // the signal in the slot arithmetic's type
+    uint32_t steps;
+    uint32_t n0;      // first step of the current 128-step block
+    uint32_t block0;  // first launch block of this instance
+    uint32_t pad;
+};
+
+// One time step of every instance in the table: block b runs local block
+// b - block0 of instance block_entry[b] (its tiles, then its receivers) at
+// step n0 + k. Instances past their last step return at once.
+template <typename T, typename S, int TY>
+__global__ __launch_bounds__(kTileX* TY) void k_batch_step(const BatchEntry<T, S>* __restrict__ entries,
+                                                           const uint32_t* __restrict__ block_entry, uint32_t k) {
+    const BatchEntry<T, S>& be = entries[block_entry[blockIdx.x]];
+    const uint32_t n = be.n0 + k;
+    if (n >= be.steps) return;
+    StepParams<T, S> a = be.a;
+    if (n & 1u) {
+        // Odd steps swap the two buffers (gpu_fdtd.rs:242-243).
+        a.p = be.a.prev;
+        a.prev = const_cast<T*>(be.a.p);
+    }
+    a.s_n = be.sig[n];
+    a.ir = be.a.ir + (size_t)n * a.n_recv;
+    const uint32_t lb = blockIdx.x - be.block0;
+    if (lb < a.main_blocks) {
+        tile_part<T, S, TY>(a, lb);
+        return;
+    }
+    // p^n before its update (fdtd.rs:673-683).
+    const uint32_t r = (lb - a.main_blocks) * blockDim.x + threadIdx.x;
+    if (r < a.n_recv) a.ir[r] = a.p[a.recv[r]];
+}
+
+template <typename T, typename S>
+__global__ void k_batch_tick(BatchEntry<T, S>* __restrict__ entries, uint32_t n, uint32_t add) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) entries[i].n0 += add;
+}
+
+template <typename T, typename S>
+void launch_batch_step(int variant, uint32_t grid, hipStream_t st, const BatchEntry<T, S>* e, const uint32_t* be,
+                       uint32_t k) {
+    switch (variant) {
+        case 0: hipLaunchKernelGGL((k_batch_step<T, S, 4>), dim3(grid), dim3(64 * 4), 0, st, e, be, k); break;
+        case 1: hipLaunchKernelGGL((k_batch_step<T, S, 8>), dim3(grid), dim3(64 * 8), 0, st, e, be, k); break;
+        case 2: hipLaunchKernelGGL((k_batch_step<T, S, 16>), dim3(grid), dim3(64 * 16), 0, st, e, be, k); break;
+        default: hipLaunchKernelGGL((k_batch_step<T, S, 2>), dim3(grid), dim3(64 * 2), 0, st, e, be, k); break;
+    }
+}
+
+// Size class of an instance: floor(log2(air cells)).
+int size_class_of(uint64_t air) {
+    int c = 0;
+    while (air > 1) {
+        air >>= 1;
+        ++c;
+    }
+    return c;
+}
+
+// Autotuned shapes, shared by every batch and server of the process:
+// z-chunk per (precision, size class) and rows-per-tile variant per precision.
+std::mutex g_tune_mu;
+std::map<std::pair<int, int>, uint32_t> g_tune_zchunk;
+std::map<int, int> g_tune_variant;
+
+uint32_t tuned_zchunk(int prec, int cls) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune_zchunk.find({prec, cls});
+    return it == g_tune_zchunk.end() ? 0u : it->second;
+}
+
+int tuned_variant(int prec) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune_variant.find(prec);
+    return it == g_tune_variant.end() ? kDefaultVariant : it->second;
+}
+
+struct Instance {
+    int32_t id = -1;
+    std::unique_ptr<wv_plan> plan;
+    std::vector<wv_tap> taps;
+    std::vector<double> signal;
+    std::vector<uint64_t> recv;
+    DBuf sig;
+    uint32_t steps = 0;
+    uint32_t n0 = 0;  // host mirror of the device block start
+    bool done = false;
+    bool in_table = false;
+    int size_class = 0;
+    uint64_t footprint = 0;
+};
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+struct wv_batch {
+    int prec = WV_FP32;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    int variant = kDefaultVariant;
+    std::vector<std::unique_ptr<Instance>> inst;
+    DBuf entries, block_entry;
+    std::vector<Instance*> table;
+    uint32_t total_blocks = 0;
+    bool dirty = true;
+    bool graph_valid = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    double last_ms = 0.0;
+    uint64_t last_steps = 0, last_cell_updates = 0;
+    std::atomic<uint64_t> graph_launches{0}, captures{0};
+
+    void drop_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        graph_valid = false;
+    }
+    ~wv_batch() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        drop_graph();
+        inst.clear();
+        entries.release();
+        block_entry.release();
+        for (auto e : {t0, t1})
+            if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+void batch_init(wv_batch* b, int prec) {
+    b->prec = prec;
+    WV_HIP(hipGetDevice(&b->device));
+    WV_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    WV_HIP(hipEventCreate(&b->t0));
+    WV_HIP(hipEventCreate(&b->t1));
+    b->variant = tuned_variant(prec);
+}
+
+// Starts (or restarts from rest) an instance's run with its z-chunk.
+void instance_begin(Instance* in, uint32_t zchunk) {
+    wv_plan* pl = in->plan.get();
+    pl->zchunk = zchunk;
+    run_begin(pl, in->taps.data(), in->taps.size(), in->signal.data(), in->signal.size(), in->recv.data(),
+              in->recv.size());
+    in->n0 = 0;
+    in->done = false;
+}
+
+template <typename S>
+std::unique_ptr<Instance> make_instance(int prec, const wv_grid* g, const wv_tap* taps, size_t n_taps,
+                                        const double* signal, size_t steps, const uint64_t* recv, size_t n_recv) {
+    auto in = std::make_unique<Instance>();
+    in->plan = std::make_unique<wv_plan>();
+    in->plan->prec = prec;
+    plan_setup(in->plan.get(), g);
+    in->taps.assign(taps, taps + n_taps);
+    in->signal.assign(signal, signal + steps);
+    in->recv.assign(recv, recv + n_recv);
+    in->steps = (uint32_t)steps;
+    in->size_class = size_class_of(in->plan->air);
+    std::vector<S> sig(std::max<size_t>(steps, 1), S(0));
+    for (size_t n = 0; n < steps; ++n) sig[n] = (S)signal[n];
+    in->sig.alloc(sig.size() * sizeof(S));
+    WV_HIP(hipMemcpy(in->sig.p, sig.data(), sig.size() * sizeof(S), hipMemcpyHostToDevice));
+    instance_begin(in.get(), tuned_zchunk(prec, in->size_class));
+    in->footprint = plan_bytes(in->plan.get()) + in->sig.bytes;
+    return in;
+}
+
+// Rebuilds the launch table from the instances that still have steps to
+// run. The stream must be idle.
+template <typename T, typename S>
+void build_table(wv_batch* b) {
+    const uint32_t ty = kVariantRows[b->variant];
+    std::vector<BatchEntry<T, S>> es;
+    std::vector<uint32_t> be;
+    b->table.clear();
+    for (auto& up : b->inst) {
+        Instance* in = up.get();
+        in->in_table = false;
+        if (in->done || in->n0 >= in->steps) continue;
+        wv_plan* pl = in->plan.get();
+        BatchEntry<T, S> e{};
+        e.a = make_params<T, S>(pl, ty);
+        e.a.p = pl->fa.as<T>();
+        e.a.prev = pl->fb.as<T>();
+        e.a.ir = pl->ir.as<T>();
+        e.sig = in->sig.as<S>();
+        e.steps = in->steps;
+        e.n0 = in->n0;
+        e.block0 = (uint32_t)be.size();
+        const uint32_t blocks = e.a.main_blocks + div_up(pl->n_recv, kTileX * ty);
+        if ((uint64_t)be.size() + blocks >= (1ull << 31)) throw DevErr{"batch launch too large", WV_ERR_LIMIT};
+        be.insert(be.end(), blocks, (uint32_t)es.size());
+        es.push_back(e);
+        b->table.push_back(in);
+        in->in_table = true;
+    }
+    b->total_blocks = (uint32_t)be.size();
+    if (!es.empty()) {
+        b->entries.alloc(es.size() * sizeof(es[0]));
+        b->block_entry.alloc(be.size() * 4);
+        WV_HIP(hipMemcpy(b->entries.p, es.data(), es.size() * sizeof(es[0]), hipMemcpyHostToDevice));
+        WV_HIP(hipMemcpy(b->block_entry.p, be.data(), be.size() * 4, hipMemcpyHostToDevice));
+    }
+    b->drop_graph();
+    b->dirty = false;
+}
+
+// Enqueues one block of kGraphSteps steps of the current table: the captured
+// graph (captured on first use after a table change), or the same launches
+// issued one by one.
+template <typename T, typename S>
+void enqueue_block(wv_batch* b, bool use_graph) {
+    const uint32_t n = (uint32_t)b->table.size();
+    if (n == 0) return;
+    auto* e = b->entries.as<BatchEntry<T, S>>();
+    const uint32_t* be = b->block_entry.as<uint32_t>();
+    auto issue = [&]() {
+        for (uint32_t k = 0; k < kGraphSteps; ++k) launch_batch_step<T, S>(b->variant, b->total_blocks, b->stream, e, be, k);
+        hipLaunchKernelGGL((k_batch_tick<T, S>), dim3(div_up(n, 256)), dim3(256), 0, b->stream, e, n, kGraphSteps);
+    };
+    if (use_graph) {
+        if (!b->graph_valid) {
+            WV_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            issue();
+            hipGraph_t g = nullptr;
+            const hipError_t ce = hipStreamEndCapture(b->stream, &g);
+            WV_HIP(ce);
+            b->graph = g;
+            WV_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+            b->graph_valid = true;
+            ++b->captures;
+        }
+        WV_HIP(hipGraphLaunch(b->exec, b->stream));
+        ++b->graph_launches;
+    } else {
+        issue();
+        WV_HIP(hipGetLastError());
+    }
+    for (Instance* in : b->table) in->n0 += kGraphSteps;
+}
+
+// Marks table instances whose last step has run (after a synchronize).
+// Returns how many finished.
+uint32_t collect_done(wv_batch* b) {
+    uint32_t k = 0;
+    for (Instance* in : b->table)
+        if (!in->done && in->n0 >= in->steps) {
+            in->done = true;
+            in->plan->done = in->steps;
+            in->plan->steps = in->steps;
+            in->plan->active = true;
+            b->dirty = true;
+            ++k;
+        }
+    return k;
+}
+
+template <typename T, typename S>
+wv_status batch_run(wv_batch* b, bool use_graph, const volatile int32_t* cancel, wv_progress_fn progress, void* ctx) {
+    uint64_t total = 0, cell_updates = 0;
+    for (auto& in : b->inst)
+        if (!in->done) {
+            total += in->steps - std::min(in->n0, in->steps);
+            cell_updates += (uint64_t)in->plan->air * (in->steps - std::min(in->n0, in->steps));
+        }
+    uint64_t done_steps = 0;
+    WV_HIP(hipStreamSynchronize(b->stream));
+    WV_HIP(hipEventRecord(b->t0, b->stream));
+    uint64_t max_steps = 0;
+    while (true) {
+        if (cancel && *cancel) return WV_CANCELLED;
+        if (b->dirty) build_table<T, S>(b);
+        if (b->table.empty()) break;
+        // Blocks until the first instance of the table finishes: the table
+        // does not change before then, so they are enqueued back to back.
+        uint32_t blocks = ~0u;
+        for (Instance* in : b->table) blocks = std::min(blocks, div_up(in->steps - in->n0, kGraphSteps));
+        uint32_t issued = 0;
+        while (issued < blocks) {
+            enqueue_block<T, S>(b, use_graph);
+            ++issued;
+            if (cancel && *cancel) break;
+        }
+        max_steps += (uint64_t)issued * kGraphSteps;
+        WV_HIP(hipStreamSynchronize(b->stream));
+        done_steps += (uint64_t)b->table.size() * issued * kGraphSteps;
+        collect_done(b);
+        if (progress && total) progress(std::min(1.0, (double)done_steps / (double)total), ctx);
+    }
+    WV_HIP(hipEventRecord(b->t1, b->stream));
+    WV_HIP(hipEventSynchronize(b->t1));
+    float ms = 0.f;
+    WV_HIP(hipEventElapsedTime(&ms, b->t0, b->t1));
+    b->last_ms = ms;
+    b->last_steps = max_steps;
+    b->last_cell_updates = cell_updates;
+    return WV_OK;
+}
+
+// Device time per step of the whole table, eager launches, after warm-up.
+template <typename T, typename S>
+double time_table(wv_batch* b, uint32_t warmup, uint32_t steps) {
+    build_table<T, S>(b);
+    if (b->table.empty()) return 0.0;
+    auto* e = b->entries.as<BatchEntry<T, S>>();
+    const uint32_t* be = b->block_entry.as<uint32_t>();
+    for (uint32_t k = 0; k < warmup; ++k) launch_batch_step<T, S>(b->variant, b->total_blocks, b->stream, e, be, k);
+    WV_HIP(hipEventRecord(b->t0, b->stream));
+    for (uint32_t k = 0; k < steps; ++k)
+        launch_batch_step<T, S>(b->variant, b->total_blocks, b->stream, e, be, warmup + k);
+    WV_HIP(hipGetLastError());
+    WV_HIP(hipEventRecord(b->t1, b->stream));
+    WV_HIP(hipEventSynchronize(b->t1));
+    float ms = 0.f;
+    WV_HIP(hipEventElapsedTime(&ms, b->t0, b->t1));
+    return ms / std::max(steps, 1u);
+}
+
+// Coordinate search over the batch's tile shape: the rows-per-tile variant
+// (one template for the launch), then each size class's z-chunk, largest
+// class first, each timed on the whole batch. Instances restart from rest.
+template <typename T, typename S>
+void batch_autotune(wv_batch* b, uint32_t trial_steps) {
+    WV_HIP(hipStreamSynchronize(b->stream));
+    for (auto& in : b->inst) instance_begin(in.get(), in->plan->run_zchunk);
+    int best_v = b->variant;
+    double best = 1e300;
+    for (int v : kTuneVariants) {
+        b->variant = v;
+        const double t = time_table<T, S>(b, 2, trial_steps);
+        if (t < best) {
+            best = t;
+            best_v = v;
+        }
+    }
+    b->variant = best_v;
+    std::map<int, uint64_t, std::greater<int>> classes;
+    for (auto& in : b->inst) classes[in->size_class] += in->plan->air;
+    std::map<int, uint32_t> chosen;
+    for (auto& [cls, air] : classes) {
+        (void)air;
+        uint32_t best_z = 0;
+        double best_t = 1e300;
+        for (uint32_t zc : kTuneZChunks) {
+            for (auto& in : b->inst)
+                if (in->size_class == cls) instance_begin(in.get(), zc);
+            const double t = time_table<T, S>(b, 2, trial_steps);
+            if (t < best_t) {
+                best_t = t;
+                best_z = zc;
+            }
+        }
+        chosen[cls] = best_z;
+        for (auto& in : b->inst)
+            if (in->size_class == cls) instance_begin(in.get(), best_z);
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune_variant[b->prec] = best_v;
+        for (auto& [cls, zc] : chosen) g_tune_zchunk[{b->prec, cls}] = zc;
+    }
+    // Every instance restarts from rest with its class's z-chunk.
+    for (auto& in : b->inst) instance_begin(in.get(), chosen[in->size_class]);
+    b->dirty = true;
+}
+
+template <class F>
+wv_status dispatch_prec(int prec, F f) {
+    switch (prec) {
+        case WV_FP32: return f(float(), float());
+        case WV_FP64: return f(double(), double());
+        default: return f(float(), double());
+    }
+}
+
+Instance* find_instance(wv_batch* b, int32_t id) {
+    if (id < 0 || (size_t)id >= b->inst.size()) return nullptr;
+    return b->inst[id].get();
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ batch C ABI
+
+extern "C" wv_status wv_batch_create(int precision, wv_batch** out, char* err, size_t err_len) {
+    if (!out || (precision != WV_FP32 && precision != WV_FP64 && precision != WV_FP32_STATE64)) {
+        set_err(err, err_len, "wv_batch_create: invalid argument");
+        return WV_ERR_INVALID;
+    }
+    return guarded(err, err_len, [&]() -> wv_status {
+        auto b = std::make_unique<wv_batch>();
+        batch_init(b.get(), precision);
+        *out = b.release();
+        return WV_OK;
+    });
+}
+
+extern "C" void wv_batch_free(wv_batch* b) { delete b; }
+
+extern "C" wv_status wv_batch_add(wv_batch* b, const wv_grid* grid, const wv_tap* taps, size_t n_taps,
+                                  const double* signal, size_t steps, const uint64_t* receivers, size_t n_recv,
+                                  int32_t* id, char* err, size_t err_len) {
+    if (!b || !id) {
+        set_err(err, err_len, "wv_batch_add: invalid argument");
+        return WV_ERR_INVALID;
+    }
+    wv_status s = check_grid(grid, err, err_len);
+    if (s != WV_OK) return s;
+    return guarded(err, err_len, [&]() -> wv_status {
+        WV_HIP(hipSetDevice(b->device));
+        WV_HIP(hipStreamSynchronize(b->stream));
+        // Argument checks against the grid's plan layout (tap planes, receivers).
+        wv_plan probe;
+        probe.nx = (uint32_t)grid->n[0];
+        probe.ny = (uint32_t)grid->n[1];
+        probe.nz = (uint32_t)grid->n[2];
+        probe.cells = grid->n[0] * grid->n[1] * grid->n[2];
+        wv_status st = check_run_args(&probe, taps, n_taps, signal, steps, receivers, n_recv, err, err_len);
+        if (st != WV_OK) return st;
+        std::unique_ptr<Instance> in;
+        if (b->prec == WV_FP32) in = make_instance<float>(b->prec, grid, taps, n_taps, signal, steps, receivers, n_recv);
+        else in = make_instance<double>(b->prec, grid, taps, n_taps, signal, steps, receivers, n_recv);
+        in->id = (int32_t)b->inst.size();
+        *id = in->id;
+        b->inst.push_back(std::move(in));
+        b->dirty = true;
+        return WV_OK;
+    });
+}
+
+extern "C" wv_status wv_batch_autotune(wv_batch* b, uint32_t trial_steps, char* err, size_t err_len) {
+    if (!b || trial_steps == 0) {
+        set_err(err, err_len, "wv_batch_autotune: invalid argument");
+        return WV_ERR_INVALID;
+    }
+    return guarded(err, err_len, [&]() -> wv_status {
+        WV_HIP(hipSetDevice(b->device));
+        return dispatch_prec(b->prec, [&](auto t, auto s) -> wv_status {
+            batch_autotune<decltype(t), decltype(s)>(b, trial_steps);
+            return WV_OK;
+        });
+    });
+}
+
+extern "C" wv_status wv_batch_set_tile(wv_batch* b, int32_t variant, uint32_t zchunk, char* err, size_t err_len) {
+    if (!b || variant >= kVariants || zchunk > 32 || (zchunk & (zchunk - 1))) {
+        set_err(err, err_len, "wv_batch_set_tile: invalid argument");
+        return WV_ERR_INVALID;
+    }
+    return guarded(err, err_len, [&]() -> wv_status {
+        WV_HIP(hipSetDevice(b->device));
+        WV_HIP(hipStreamSynchronize(b->stream));
+        b->variant = variant < 0 ? kDefaultVariant : variant;
+        for (auto& in : b->inst) instance_begin(in.get(), zchunk);
+        b->dirty = true;
+        return WV_OK;
+    });
+}
+
+extern "C" wv_status wv_batch_run(wv_batch* b, int use_graphs, const volatile int32_t* cancel,
+                                  wv_progress_fn progress, void* ctx, char* err, size_t err_len) {
+    if (!b) {
+        set_err(err, err_len, "wv_batch_run: invalid argument");
+        return WV_ERR_INVALID;
+    }
+    return guarded(err, err_len, [&]() -> wv_status {
+        WV_HIP(hipSetDevice(b->device));
+        return dispatch_prec(b->prec, [&](auto t, auto s) -> wv_status {
+            return batch_run<decltype(t), decltype(s)>(b, use_graphs != 0, cancel, progress, ctx);
+        });
+    });
+}
+
+extern "C" wv_status wv_batch_result(wv_batch* b, int32_t id, const double* freqs, size_t nf, int mode,
+                                     double* spectra_out, double* ir_out, char* err, size_t err_len) {
+    Instance* in = b ? find_instance(b, id) : nullptr;
+    if (!in || (nf && !freqs)) {
+        set_err(err, err_len, "wv_batch_result: invalid argument");
+        return WV_ERR_INVALID;
+    }
+    if (!in->done) {
+        set_err(err, err_len, "wv_batch_result: instance has not finished");
+        return WV_ERR_INVALID;
+    }
+    return guarded(err, err_len, [&]() -> wv_status {
+        WV_HIP(hipSetDevice(b->device));
+        run_finish(in->plan.get(), freqs, nf, mode, spectra_out, ir_out);
+        return WV_OK;
+    });
+}
+
+extern "C" wv_status wv_batch_info(const wv_batch* b, int32_t id, uint64_t out[6]) {
+    if (!b || !out || id < 0 || (size_t)id >= b->inst.size()) return WV_ERR_INVALID;
+    const Instance* in = b->inst[id].get();
+    out[0] = in->plan->cells;
+    out[1] = in->plan->air;
+    out[2] = in->steps;
+    out[3] = in->plan->run_zchunk;
+    out[4] = (uint64_t)in->size_class;
+    out[5] = in->footprint;
+    return WV_OK;
+}
+
+extern "C" wv_status wv_batch_timing(const wv_batch* b, double* ms, uint64_t out[5]) {
+    if (!b || !ms || !out) return WV_ERR_INVALID;
+    *ms = b->last_ms;
+    out[0] = b->last_steps;
+    out[1] = b->last_cell_updates;
+    out[2] = b->graph_launches;
+    out[3] = b->captures;
+    out[4] = (uint64_t)b->variant;
+    return WV_OK;
+}
+
+// ------------------------------------------------------------------ server
+
+namespace {
+
+struct Request {
+    uint64_t id = 0;
+    const wv_grid* grid = nullptr;  // borrowed until wv_serve_wait returns
+    std::vector<wv_tap> taps;
+    std::vector<double> signal;
+    std::vector<uint64_t> recv;
+    std::vector<double> freqs;
+    int mode = WV_MODE_IR;
+    double t_submit = 0, t_admit = 0, t_done = 0;
+    bool done = false;
+    wv_status status = WV_OK;
+    std::string err;
+    std::vector<double> ir, spectra;
+    Instance* inst = nullptr;
+    uint64_t reserved = 0;
+};
+
+}  // namespace
+
+struct wv_server {
+    int prec = WV_FP32;
+    uint64_t cap = 0;
+    int device = 0;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<std::shared_ptr<Request>> pending;
+    std::unordered_map<uint64_t, std::shared_ptr<Request>> reqs;
+    uint64_t next_id = 1;
+    bool stop = false;
+    std::thread worker;
+    std::unique_ptr<wv_batch> batch;
+    std::vector<std::shared_ptr<Request>> running;
+    uint64_t used = 0, peak_used = 0, cell_updates = 0, completed = 0, failed = 0;
+    uint32_t max_active = 0;
+    double t_first = -1.0, t_last = 0.0;
+    std::vector<double> latencies;
+    std::string fatal;
+
+    ~wv_server() {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            stop = true;
+        }
+        cv_work.notify_all();
+        if (worker.joinable()) worker.join();
+    }
+};
+
+namespace {
+
+void fail_request(wv_server* sv, const std::shared_ptr<Request>& r, wv_status st, const std::string& msg) {
+    r->status = st;
+    r->err = msg;
+    r->done = true;
+    r->t_done = now_s();
+    ++sv->failed;
+}
+
+template <typename T, typename S>
+void serve_loop(wv_server* sv) {
+    wv_batch* b = sv->batch.get();
+    std::unique_lock<std::mutex> lk(sv->mu);
+    while (true) {
+        const bool busy = !sv->running.empty();
+        lk.unlock();
+        // The next block of the running batch goes to the device first, so
+        // admission work below overlaps with it.
+        if (busy) {
+            if (b->dirty) build_table<T, S>(b);
+            enqueue_block<T, S>(b, true);
+        }
+        lk.lock();
+        bool admitted = false;
+        while (!sv->pending.empty()) {
+            auto r = sv->pending.front();
+            const uint64_t est = estimate_bytes(r->grid, sv->prec, r->signal.size(), r->recv.size(), r->taps.size());
+            if (est > sv->cap) {
+                sv->pending.pop_front();
+                fail_request(sv, r, WV_ERR_LIMIT, "request needs more device memory than the admission cap");
+                sv->cv_done.notify_all();
+                continue;
+            }
+            if (sv->used + est > sv->cap) break;  // waits for running requests to free memory
+            sv->pending.pop_front();
+            sv->used += est;
+            r->reserved = est;
+            lk.unlock();
+            std::unique_ptr<Instance> in;
+            std::string msg;
+            wv_status st = WV_OK;
+            try {
+                in = make_instance<S>(sv->prec, r->grid, r->taps.data(), r->taps.size(), r->signal.data(),
+                                      r->signal.size(), r->recv.data(), r->recv.size());
+            } catch (const DevErr& e) {
+                st = e.code;
+                msg = e.msg;
+            } catch (const std::bad_alloc&) {
+                st = WV_ERR_NOMEM;
+                msg = "host allocation failed";
+            }
+            lk.lock();
+            if (st != WV_OK) {
+                sv->used -= r->reserved;
+                fail_request(sv, r, st, msg);
+                sv->cv_done.notify_all();
+                continue;
+            }
+            sv->used += in->footprint;
+            sv->used -= r->reserved;
+            r->reserved = in->footprint;
+            sv->peak_used = std::max(sv->peak_used, sv->used);
+            r->t_admit = now_s();
+            in->id = (int32_t)r->id;
+            r->inst = in.get();
+            b->inst.push_back(std::move(in));
+            b->dirty = true;
+            sv->running.push_back(r);
+            sv->max_active = std::max<uint32_t>(sv->max_active, (uint32_t)sv->running.size());
+            admitted = true;
+        }
+        lk.unlock();
+        if (busy) {
+            WV_HIP(hipStreamSynchronize(b->stream));
+            if (collect_done(b)) {
+                // Finished instances: results to host memory, device memory freed.
+                std::vector<std::shared_ptr<Request>> fin;
+                for (auto& r : sv->running)
+                    if (r->inst && r->inst->done) fin.push_back(r);
+                for (auto& r : fin) {
+                    Instance* in = r->inst;
+                    const size_t R = in->recv.size(), Tn = in->steps, nf = r->freqs.size();
+                    if (r->mode & WV_MODE_IR) r->ir.assign(R * Tn, 0.0);
+                    if ((r->mode & WV_MODE_SPECTRA) && nf) r->spectra.assign(2 * R * nf, 0.0);
+                    run_finish(in->plan.get(), r->freqs.data(), nf, r->mode, r->spectra.empty() ? nullptr : r->spectra.data(),
+                               r->ir.empty() ? nullptr : r->ir.data());
+                    const uint64_t air = in->plan->air;
+                    auto it = std::find_if(b->inst.begin(), b->inst.end(),
+                                           [&](const std::unique_ptr<Instance>& p) { return p.get() == in; });
+                    b->inst.erase(it);  // frees the instance's device memory
+                    lk.lock();
+                    r->inst = nullptr;
+                    r->done = true;
+                    r->t_done = now_s();
+                    sv->t_last = r->t_done;
+                    sv->used -= r->reserved;
+                    sv->cell_updates += air * Tn;
+                    sv->latencies.push_back(r->t_done - r->t_submit);
+                    ++sv->completed;
+                    sv->running.erase(std::find(sv->running.begin(), sv->running.end(), r));
+                    lk.unlock();
+                }
+                b->dirty = true;
+                sv->cv_done.notify_all();
+            }
+        }
+        lk.lock();
+        // Idle: sleep until a request arrives (with nothing running, any
+        // pending request fits under the cap and is admitted above).
+        if (sv->running.empty() && !admitted) {
+            if (sv->stop && sv->pending.empty()) break;
+            if (sv->pending.empty()) sv->cv_work.wait(lk);
+        }
+    }
+}
+
+void serve_main(wv_server* sv) {
+    try {
+        WV_HIP(hipSetDevice(sv->device));
+        auto run = [&](auto t, auto s) -> wv_status {
+            serve_loop<decltype(t), decltype(s)>(sv);
+            return WV_OK;
+        };
+        dispatch_prec(sv->prec, run);
+    } catch (const DevErr& e) {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->fatal = e.msg;
+        for (auto& r : sv->running) fail_request(sv, r, e.code, e.msg);
+        for (auto& r : sv->pending) fail_request(sv, r, e.code, e.msg);
+        sv->running.clear();
+        sv->pending.clear();
+    } catch (const std::bad_alloc&) {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->fatal = "host allocation failed";
+    }
+    sv->cv_done.notify_all();
+}
+
+double percentile(std::vector<double> v, double q) {
+    if (v.empty()) return 0.0;
+    std::sort(v.begin(), v.end());
+    const double pos = q * (double)(v.size() - 1);
+    const size_t i = (size_t)pos;
+    const double f = pos - (double)i;
+    return i + 1 < v.size() ? v[i] * (1.0 - f) + v[i + 1] * f : v[i];
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ server C ABI
+
+extern "C" wv_status wv_serve_create(int precision, uint64_t hbm_cap_bytes, wv_server** out, char* err,
+                                     size_t err_len) {
+    if (!out || hbm_cap_bytes == 0 ||
+        (precision != WV_FP32 && precision != WV_FP64 && precision != WV_FP32_STATE64)) {
+        set_err(err, err_len, "wv_serve_create: invalid argument");
+        return WV_ERR_INVALID;
+    }
+    return guarded(err, err_len, [&]() -> wv_status {
+        auto sv = std::make_unique<wv_server>();
+        sv->prec = precision;
+        sv->cap = hbm_cap_bytes;
+        WV_HIP(hipGetDevice(&sv->device));
+        sv->batch = std::make_unique<wv_batch>();
+        batch_init(sv->batch.get(), precision);
+        wv_server* p = sv.get();
+        sv->worker = std::thread([p]() { serve_main(p); });
+        *out = sv.release();
+        return WV_OK;
+    });
+}
+
+extern "C" void wv_serve_free(wv_server* sv) { delete sv; }
+
+extern "C" wv_status wv_serve_submit(wv_server* sv, const wv_grid* grid, const wv_tap* taps, size_t n_taps,
+                                     const double* signal, size_t steps, const uint64_t* receivers, size_t n_recv,
+                                     const double* freqs, size_t nf, int mode, uint64_t* id, char* err,
+                                     size_t err_len) {
+    if (!sv || !id || (nf && !freqs) || !(mode & (WV_MODE_IR | WV_MODE_SPECTRA))) {
+        set_err(err, err_len, "wv_serve_submit: invalid argument");
+        return WV_ERR_INVALID;
+    }
+    wv_status s = check_grid(grid, err, err_len);
+    if (s != WV_OK) return s;
+    wv_plan probe;
+    probe.nx = (uint32_t)grid->n[0];
+    probe.ny = (uint32_t)grid->n[1];
+    probe.nz = (uint32_t)grid->n[2];
+    probe.cells = grid->n[0] * grid->n[1] * grid->n[2];
+    s = check_run_args(&probe, taps, n_taps, signal, steps, receivers, n_recv, err, err_len);
+    if (s != WV_OK) return s;
+    try {
+        auto r = std::make_shared<Request>();
+        r->grid = grid;
+        r->taps.assign(taps, taps + n_taps);
+        r->signal.assign(signal, signal + steps);
+        r->recv.assign(receivers, receivers + n_recv);
+        if (nf) r->freqs.assign(freqs, freqs + nf);
+        r->mode = mode;
+        r->t_submit = now_s();
+        {
+            std::lock_guard<std::mutex> lk(sv->mu);
+            if (!sv->fatal.empty()) {
+                set_err(err, err_len, "server stopped: " + sv->fatal);
+                return WV_ERR_DEVICE;
+            }
+            r->id = sv->next_id++;
+            if (sv->t_first < 0) sv->t_first = r->t_submit;
+            sv->reqs[r->id] = r;
+            sv->pending.push_back(r);
+            *id = r->id;
+        }
+        sv->cv_work.notify_all();
+        return WV_OK;
+    } catch (const std::bad_alloc&) {
+        set_err(err, err_len, "host allocation failed");
+        return WV_ERR_NOMEM;
+    }
+}
+
+extern "C" wv_status wv_serve_wait(wv_server* sv, uint64_t id, double* spectra_out, double* ir_out,
+                                   double* latency_s, char* err, size_t err_len) {
+    if (!sv) return WV_ERR_INVALID;
+    std::unique_lock<std::mutex> lk(sv->mu);
+    auto it = sv->reqs.find(id);
+    if (it == sv->reqs.end()) {
+        set_err(err, err_len, "wv_serve_wait: unknown request id");
+        return WV_ERR_INVALID;
+    }
+    auto r = it->second;
+    sv->cv_done.wait(lk, [&]() { return r->done; });
+    sv->reqs.erase(id);
+    if (r->status != WV_OK) {
+        set_err(err, err_len, r->err);
+        return r->status;
+    }
+    if (latency_s) *latency_s = r->t_done - r->t_submit;
+    const size_t R = r->recv.size(), Tn = r->signal.size();
+    if (ir_out && !r->ir.empty()) std::memcpy(ir_out, r->ir.data(), R * Tn * sizeof(double));
+    if (spectra_out && !r->spectra.empty()) std::memcpy(spectra_out, r->spectra.data(), r->spectra.size() * 8);
+    return WV_OK;
+}
+
+extern "C" wv_status wv_serve_stats(wv_server* sv, double out[10]) {
+    if (!sv || !out) return WV_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(sv->mu);
+    out[0] = (double)sv->completed;
+    out[1] = percentile(sv->latencies, 0.50);
+    out[2] = percentile(sv->latencies, 0.95);
+    out[3] = percentile(sv->latencies, 0.99);
+    out[4] = sv->t_first >= 0 ? sv->t_last - sv->t_first : 0.0;
+    out[5] = (double)sv->cell_updates;
+    out[6] = (double)sv->max_active;
+    out[7] = (double)sv->peak_used;
+    out[8] = (double)sv->failed;
+    out[9] = (double)(sv->batch ? sv->batch->captures.load() : 0);
+    return WV_OK;
+}

@@ -1,29 +1,21 @@
-// MI355X (gfx950) FDTD wave-solver run: device layout, kernels, run loop.
+// MI355X (gfx950) FDTD wave-solver run: the plan's device layout, its set-up
+// kernels, the plain run loop and its C ABI.
 //
-// The scheme is Wavey's (core/wavey-solver/src/fdtd.rs:1-30): the 3D
-// leapfrog p+ = (2 - K l2) p + l2 sum_air p_nb - p-, with locally reacting
-// frequency-dependent boundaries solved per node in closed form
-// (fdtd.rs:581-598). The layout is this build's own:
+// Layout (the step kernel itself is in step_kernels.h):
 //
 //  * Fields p, prev live in a pitched grid: row pitch = nx rounded up to 64
 //    cells, so a wave's 16-byte-per-lane accesses cover aligned row segments.
-//  * One launch per time step. A work-group owns a tile of the xy plane
-//    (256 FP32 or 128 FP64 cells by TY rows, one 64-lane wave per row, 16
-//    bytes per lane) and marches a z-chunk; the column's p at z-1 and z stay
-//    in registers. Per plane a lane reads p at z+1, prev and the two
-//    y-neighbour rows, and writes prev: 12 bytes per FP32 cell.
-//  * Two bitmaps, packed along z (one 32-bit word per column per 32 planes,
-//    read once per chunk), say which cells are plain interior air (six air
-//    neighbours, no boundary slots, no source tap) and which are "special"
-//    (every other air cell, and taps). Special cells are updated by the same
-//    lane from the neighbour values it already holds: their metadata sit in
-//    a list compacted (wave ballot + prefix sums) in the order the waves
-//    consume them, so a lane finds its cell's entry from the bitmap alone.
+//  * An interior bitmap, packed along z (one 32-bit word per column per 32
+//    planes), marks plain interior air: six air neighbours, no boundary
+//    slots, no source tap.
+//  * Every other air cell, and every tap, is a special cell. Their records
+//    are compacted (wave ballot + prefix sums) in the order the step
+//    kernel's waves consume them: a wave's records are those of its row
+//    segment and z-chunk, sorted by plane, and a record gives the cell's
+//    position in the chunk.
 //  * Boundary slot constants are deduplicated into a small table of slot
 //    sets; slot state is stored slot-major, state[j][rank], so the reads of a
 //    wave's special cells are contiguous.
-//  * Receivers are recorded by extra blocks of the same launch. Within a step
-//    p is only read and each prev cell is written by one thread.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -35,429 +27,11 @@
 #include <unordered_map>
 #include <vector>
 
-#include "wavey_fdtd.h"
-#include "wv_internal.h"
+#include "plan.h"
 
 using namespace wv;
 
 namespace {
-
-struct DevErr {
-    std::string msg;
-    wv_status code;
-};
-
-#define WV_HIP(expr)                                                                              \
-    do {                                                                                          \
-        hipError_t e_ = (expr);                                                                   \
-        if (e_ != hipSuccess)                                                                     \
-            throw DevErr{std::string(#expr) + ": " + hipGetErrorString(e_),                      \
-                         e_ == hipErrorOutOfMemory ? WV_ERR_NOMEM : WV_ERR_DEVICE};               \
-    } while (0)
-
-// Owned device allocation.
-struct DBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    DBuf() = default;
-    DBuf(const DBuf&) = delete;
-    DBuf& operator=(const DBuf&) = delete;
-    ~DBuf() { release(); }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    void alloc(size_t n) {
-        if (n <= bytes && p) return;
-        release();
-        size_t b = std::max<size_t>(n, 256);
-        WV_HIP(hipMalloc(&p, b));
-        bytes = b;
-    }
-    template <class T>
-    T* as() const { return static_cast<T*>(p); }
-};
-
-constexpr uint32_t kBlock = 256;          // setup kernels and compaction blocks
-constexpr uint32_t kTileX = 64;           // one 64-lane wave per tile row
-constexpr uint32_t kStepsPerBlock = 128;  // cancel/progress granularity, as gpu_fdtd.rs:35
-constexpr uint32_t kSubPlanes = 4;        // planes per special-cell pass of a z-chunk
-
-// Special-cell record: {meta, first constant of its slot set}. Meta bits:
-// 0-5 air neighbours (-x, +x, -y, +y, -z, +z), 6 solid, 7 tap, 8-13 slot
-// count, 14-26 position in its wave's chunk (dz * tile width + x in tile).
-constexpr uint32_t kMetaSolid = 1u << 6;  // a tap on a non-air cell: injection only
-constexpr uint32_t kMetaTap = 1u << 7;    // has a tap weight
-constexpr uint32_t kMetaSlotShift = 8;
-constexpr uint32_t kMetaPosShift = 14;
-constexpr uint32_t kMaxSlots = 63;
-
-// ------------------------------------------------------------------ kernels
-
-template <typename T, typename S>
-struct StepParams {
-    const T* __restrict__ p;
-    T* __restrict__ prev;
-    const uint32_t* __restrict__ zbits;  // interior bits [nzb][ny][pitch]
-    uint32_t nx, ny, nz, pitch, sz;
-    uint32_t tiles_x, tiles_y, zchunk, main_blocks;
-    // Special cells, in wave order: wave w owns [seg_base[w * seg_stride],
-    // seg_base[(w + 1) * seg_stride]).
-    const uint32_t* __restrict__ seg_base;
-    uint32_t seg_stride;
-    const uint2* __restrict__ sp_rec;
-    const S* __restrict__ sp_tapw;
-    uint32_t sp_pad;  // row length of the slot-major state arrays
-    const S* __restrict__ set_inv_d;
-    const S* __restrict__ set_cv;
-    const S* __restrict__ set_k;
-    const S* __restrict__ set_w;
-    S* __restrict__ st_u;  // [slot][rank]
-    S* __restrict__ st_g;
-    uint32_t has_k;
-    uint32_t n_recv;
-    const uint32_t* __restrict__ recv;
-    T* __restrict__ ir;  // this step's receiver row
-    T l2, c2;            // interior: p+ = c2 p + l2 sum - p-
-    S l2s, lam, half_dt, s_n;
-};
-
-template <typename T>
-__device__ __forceinline__ T interior_update(T c, T q, T xm, T xp, T ym, T yp, T zm, T zp, T l2, T c2) {
-    if constexpr (sizeof(T) == 8) {
-        // FP64: the reference's form, neighbour order -x, +x, -y, +y, -z, +z
-        // (fdtd.rs:685-706).
-        T s = xm;
-        s += xp;
-        s += ym;
-        s += yp;
-        s += zm;
-        s += zp;
-        return c2 * c + (l2 * s - q);
-    } else {
-        // FP32: the same update as p + (v + l2 * sum (p_nb - p)) with
-        // v = p - p-. Neighbour differences and v are exact for close values
-        // (Sterbenz), so a near-uniform field rounds only in the final add
-        // instead of in l2 * sum ~ 2p: no correlated rounding drives the
-        // room's DC mode.
-        const T d = ((xm - c) + (xp - c)) + ((ym - c) + (yp - c)) + ((zm - c) + (zp - c));
-        return c + ((c - q) + l2 * d);
-    }
-}
-
-// Lane i takes v from lane i-1 (wave_shr:1); lane 0 keeps `fill`.
-__device__ __forceinline__ int dpp_from_lower(int v, int fill) {
-    return __builtin_amdgcn_update_dpp(fill, v, 0x138, 0xF, 0xF, false);
-}
-// Lane i takes v from lane i+1 (wave_shl:1); lane 63 keeps `fill`.
-__device__ __forceinline__ int dpp_from_upper(int v, int fill) {
-    return __builtin_amdgcn_update_dpp(fill, v, 0x130, 0xF, 0xF, false);
-}
-template <typename T>
-__device__ __forceinline__ T from_lower_lane(T v, T fill) {
-    if constexpr (sizeof(T) == 4) {
-        return __int_as_float(dpp_from_lower(__float_as_int(v), __float_as_int(fill)));
-    } else {
-        const long long a = __double_as_longlong(v), f = __double_as_longlong(fill);
-        const int lo = dpp_from_lower((int)a, (int)f), hi = dpp_from_lower((int)(a >> 32), (int)(f >> 32));
-        return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-    }
-}
-template <typename T>
-__device__ __forceinline__ T from_upper_lane(T v, T fill) {
-    if constexpr (sizeof(T) == 4) {
-        return __int_as_float(dpp_from_upper(__float_as_int(v), __float_as_int(fill)));
-    } else {
-        const long long a = __double_as_longlong(v), f = __double_as_longlong(fill);
-        const int lo = dpp_from_upper((int)a, (int)f), hi = dpp_from_upper((int)(a >> 32), (int)(f >> 32));
-        return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-    }
-}
-
-// W consecutive cells of a row per lane, moved as one 16-byte access.
-template <typename T>
-struct Cells {
-    static constexpr int W = 16 / sizeof(T);
-    T v[W];
-    __device__ __forceinline__ void load(const T* __restrict__ p) {
-        if constexpr (W == 4) {
-            const float4 q = *reinterpret_cast<const float4*>(p);
-            v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
-        } else {
-            const double2 q = *reinterpret_cast<const double2*>(p);
-            v[0] = q.x; v[1] = q.y;
-        }
-    }
-    __device__ __forceinline__ void store(T* __restrict__ p) const {
-        if constexpr (W == 4) *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
-        else *reinterpret_cast<double2*>(p) = make_double2(v[0], v[1]);
-    }
-};
-
-template <int W>
-__device__ __forceinline__ void load_words(const uint32_t* __restrict__ p, uint32_t (&w)[W]) {
-    if constexpr (W == 4) {
-        const uint4 q = *reinterpret_cast<const uint4*>(p);
-        w[0] = q.x; w[1] = q.y; w[2] = q.z; w[3] = q.w;
-    } else {
-        const uint2 q = *reinterpret_cast<const uint2*>(p);
-        w[0] = q.x; w[1] = q.y;
-    }
-}
-
-template <int W>
-__device__ __forceinline__ uint32_t plane_bits(const uint32_t (&w)[W], uint32_t sh) {
-    uint32_t b = 0;
-#pragma unroll
-    for (int e = 0; e < W; ++e) b |= ((w[e] >> sh) & 1u) << e;
-    return b;
-}
-
-// Constants and state of slots jb .. jb+G-1 of a node (indices clamped to
-// the last slot; the caller masks the extras).
-template <typename S, uint32_t G>
-struct SlotGroup {
-    S inv_d[G], w[G], cv[G], u[G], kk[G], g[G];
-    template <typename T>
-    __device__ __forceinline__ void load(const StepParams<T, S>& a, uint32_t c0, uint32_t rank, uint32_t jb,
-                                         uint32_t ns) {
-#pragma unroll
-        for (uint32_t k = 0; k < G; ++k) {
-            const uint32_t j = min(jb + k, ns - 1);
-            inv_d[k] = a.set_inv_d[c0 + j];
-            w[k] = a.set_w[c0 + j];
-            cv[k] = a.set_cv[c0 + j];
-            u[k] = a.st_u[j * a.sp_pad + rank];
-            kk[k] = 0;
-            g[k] = 0;
-            if (a.has_k) {
-                kk[k] = a.set_k[c0 + j];
-                g[k] = a.st_g[j * a.sp_pad + rank];
-            }
-        }
-    }
-    // The part of slot k's u+ that does not depend on p+ (fdtd.rs:589-592).
-    __device__ __forceinline__ S rest(uint32_t k, S pc, S pp) const {
-        return (cv[k] * u[k] + S(0.5) * pc + w[k] * pp - kk[k] * g[k]) * inv_d[k];
-    }
-    __device__ __forceinline__ void sums(uint32_t jb, uint32_t ns, S pc, S pp, S& s1, S& s2) const {
-#pragma unroll
-        for (uint32_t k = 0; k < G; ++k)
-            if (jb + k < ns) {
-                s1 += (S(0.5) - w[k]) * inv_d[k];
-                s2 += rest(k, pc, pp) - u[k];
-            }
-    }
-    template <typename T>
-    __device__ __forceinline__ void advance(const StepParams<T, S>& a, uint32_t rank, uint32_t jb, uint32_t ns, S pc,
-                                            S pp, S next) const {
-#pragma unroll
-        for (uint32_t k = 0; k < G; ++k)
-            if (jb + k < ns) {
-                const uint32_t j = jb + k;
-                const S un = (S(0.5) - w[k]) * inv_d[k] * next + rest(k, pc, pp);
-                if (a.has_k) a.st_g[j * a.sp_pad + rank] = g[k] + a.half_dt * (un + u[k]);
-                a.st_u[j * a.sp_pad + rank] = un;
-            }
-    }
-};
-
-// One special cell: the air-neighbour Laplacian, the boundary solve of
-// fdtd.rs:581-598 when it has slots, and its tap (injection after the
-// update, fdtd.rs:718-720). nb = -x, +x, -y, +y, -z, +z; c0 is the first
-// constant of the node's slot set and rank its index in the state arrays.
-template <typename T, typename S>
-__device__ __forceinline__ S special_update(const StepParams<T, S>& a, uint32_t meta, uint32_t c0, uint32_t rank,
-                                            S pc, S pp, const S (&nb)[6]) {
-    S next;
-    if (meta & kMetaSolid) {
-        next = pp;  // a tap on a solid cell: the reference only injects there
-    } else {
-        S lap;
-        if constexpr (sizeof(S) == 8) {
-            // fdtd.rs:685-696: l2 (sum_air - k p), neighbours in -x..+z order.
-            S sum = 0, k = 0;
-#pragma unroll
-            for (int d = 0; d < 6; ++d)
-                if ((meta >> d) & 1u) {
-                    sum += nb[d];
-                    k += 1;
-                }
-            lap = a.l2s * (sum - k * pc);
-        } else {
-            // FP32: sum of neighbour differences (see interior_update).
-            S d = 0;
-#pragma unroll
-            for (int e = 0; e < 6; ++e)
-                if ((meta >> e) & 1u) d += nb[e] - pc;
-            lap = a.l2s * d;
-        }
-        const uint32_t ns = (meta >> kMetaSlotShift) & kMaxSlots;
-        if (ns == 0) {
-            if constexpr (sizeof(S) == 8) next = S(2) * pc - pp + lap;
-            else next = pc + ((pc - pp) + lap);
-        } else {
-            // Each slot's u+ is linear in p+, so p+ is solved first from the
-            // summed face flux, then the states advance. The first G slots
-            // stay in registers (one pass for nodes with up to G slots);
-            // further groups are read again for the second pass. G = 3 in
-            // FP32 (a wall face with a three-branch admittance) keeps the
-            // kernel within 64 VGPRs, 8 waves per SIMD.
-            constexpr uint32_t G = sizeof(S) == 4 ? 3 : 2;
-            SlotGroup<S, G> g0;
-            g0.load(a, c0, rank, 0, ns);
-            S s1 = 0, s2 = 0;
-            g0.sums(0, ns, pc, pp, s1, s2);
-            for (uint32_t jb = G; jb < ns; jb += G) {
-                SlotGroup<S, G> gj;
-                gj.load(a, c0, rank, jb, ns);
-                gj.sums(jb, ns, pc, pp, s1, s2);
-            }
-            if constexpr (sizeof(S) == 8) {
-                next = (S(2) * pc - pp + lap - a.lam * s2) / (S(1) + a.lam * s1);
-            } else {
-                // The same solve written as pc + (small terms).
-                next = pc + ((pc - pp) + lap - a.lam * (s2 + s1 * pc)) / (S(1) + a.lam * s1);
-            }
-            g0.advance(a, rank, 0, ns, pc, pp, next);
-            for (uint32_t jb = G; jb < ns; jb += G) {
-                SlotGroup<S, G> gj;
-                gj.load(a, c0, rank, jb, ns);
-                gj.advance(a, rank, jb, ns, pc, pp, next);
-            }
-        }
-    }
-    if (meta & kMetaTap) next += a.sp_tapw[rank] * a.s_n;
-    return next;
-}
-
-// A tile of 64*W cells (x) by TY rows (y), one 64-lane wave per row, each
-// lane W consecutive cells (16 bytes: 4 FP32 or 2 FP64), marching z over
-// [z0, z1). Per plane a lane loads p at z+1, prev, the two y-neighbour rows
-// and, on lanes 0 and 63, the row-end cells of the neighbour tiles; x
-// neighbours inside a lane are in its registers and move across lanes by
-// wave shifts. All of a plane's loads are issued before any is used and the
-// loop keeps few registers live, so many waves per SIMD hide the memory
-// latency. Interior cells are written here; the wave then updates the
-// special cells of its chunk, 64 at a time, while the rows they read are
-// still in cache.
-template <typename T, typename S, int TY>
-__device__ __forceinline__ void tile_part(const StepParams<T, S>& a, uint32_t t) {
-    constexpr int W = Cells<T>::W;
-    constexpr uint32_t BX = kTileX * W;
-    const uint32_t tx = t % a.tiles_x;
-    const uint32_t r = t / a.tiles_x;
-    const uint32_t ty = r % a.tiles_y;
-    const uint32_t zc = r / a.tiles_y;
-    const uint32_t lane = threadIdx.x & (kTileX - 1);
-    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kTileX);
-    const uint32_t xl = tx * BX + lane * W;  // first cell of this lane
-    const uint32_t y = ty * TY + wave;
-    if (y >= a.ny) return;
-    const uint32_t w_seg = ((zc * a.tiles_x + tx) * a.ny + y) * a.seg_stride;
-    // Lanes past the row pitch (the last tile of a row) re-read the row's
-    // last 16 bytes and update nothing: no traffic beyond the row.
-    const bool live = xl < a.pitch;
-    const uint32_t x = live ? xl : a.pitch - W;
-    const uint32_t z0 = 1 + zc * a.zchunk;
-    const uint32_t z1 = min(z0 + a.zchunk, a.nz - 1);
-    const uint32_t col = y * a.pitch + x;
-    const uint32_t ym_off = y > 0 ? a.pitch : 0u;
-    const uint32_t yp_off = y + 1 < a.ny ? a.pitch : 0u;
-    const uint32_t hl_off = x > 0 ? 1u : 0u;                     // lane 0: x - 1
-    const uint32_t hr_off = x + W < a.pitch ? (uint32_t)W : 0u;  // lane 63: x + W
-    const uint32_t full = (1u << W) - 1u;
-    const uint32_t lane_mask = live ? full : 0u;
-    // The chunk lies inside one 32-plane block of the z-packed bitmap
-    // (z-chunks divide 32 and start at plane 1 + a multiple of the chunk).
-    uint32_t ib[W];
-    load_words<W>(a.zbits + (((z0 - 1) >> 5) * a.ny + y) * a.pitch + x, ib);
-
-    Cells<T> zm, zcur;
-    zm.load(a.p + (z0 - 1) * a.sz + col);
-    zcur.load(a.p + z0 * a.sz + col);
-    // The chunk is marched in sub-chunks of kSubPlanes planes; after each,
-    // the wave updates that sub-chunk's special cells while the rows they
-    // read are still in cache. Records are sorted by plane, so a sub-chunk's
-    // records start at the compaction block of its first plane.
-    const uint32_t row0 = z0 * a.sz + y * a.pitch + tx * BX;
-    uint32_t s_begin = a.seg_base[w_seg];
-    for (uint32_t zs = z0; zs < z1; zs += kSubPlanes) {
-        const uint32_t ze = min(zs + kSubPlanes, z1);
-        const uint32_t s_end =
-            ze == z1 ? a.seg_base[w_seg + a.seg_stride] : a.seg_base[w_seg + (ze - z0) * BX / kBlock];
-        // The first special records are requested now, so they have arrived
-        // when the sub-chunk's z-march ends.
-        uint2 rec = make_uint2(0u, 0u);
-        if (s_begin + lane < s_end) rec = a.sp_rec[s_begin + lane];
-        for (uint32_t z = zs; z < ze; ++z) {
-            const uint32_t i = z * a.sz + col;
-            T hl = 0, hr = 0;
-            if (lane == 0) hl = a.p[i - hl_off];
-            if (lane == kTileX - 1) hr = a.p[i + hr_off];
-            Cells<T> zp, q, ym, yp;
-            zp.load(a.p + i + a.sz);
-            q.load(a.prev + i);
-            ym.load(a.p + i - ym_off);
-            yp.load(a.p + i + yp_off);
-            const T lo = from_lower_lane(zcur.v[W - 1], hl);
-            const T hi = from_upper_lane(zcur.v[0], hr);
-            Cells<T> out;
-#pragma unroll
-            for (int e = 0; e < W; ++e) {
-                const T xm = e == 0 ? lo : zcur.v[e - 1];
-                const T xp = e == W - 1 ? hi : zcur.v[e + 1];
-                out.v[e] =
-                    interior_update(zcur.v[e], q.v[e], xm, xp, ym.v[e], yp.v[e], zm.v[e], zp.v[e], a.l2, a.c2);
-            }
-            const uint32_t bits = plane_bits<W>(ib, (z - 1) & 31u) & lane_mask;
-            if (bits == full) {
-                out.store(a.prev + i);
-            } else if (bits) {
-                // Row ends next to walls: only the interior cells.
-#pragma unroll
-                for (int e = 0; e < W; ++e)
-                    if ((bits >> e) & 1u) a.prev[i + e] = out.v[e];
-            }
-            zm = zcur;
-            zcur = zp;
-        }
-
-        // The sub-chunk's special cells, 64 at a time. A record's position
-        // gives the cell; its neighbour loads are unconditional (a masked-out
-        // neighbour reads the cell itself, and the mask never points outside
-        // the grid).
-        for (uint32_t s = s_begin + lane; s < s_end; s += kTileX) {
-            uint2 next_rec = make_uint2(0u, 0u);
-            if (s + kTileX < s_end) next_rec = a.sp_rec[s + kTileX];
-            const uint32_t meta = rec.x;
-            const uint32_t pos = meta >> kMetaPosShift;
-            const uint32_t i = row0 + (pos / BX) * a.sz + pos % BX;
-            const T c = a.p[i];
-            const T q = a.prev[i];
-            const S nb[6] = {(S)a.p[i - ((meta & 1u) ? 1u : 0u)], (S)a.p[i + ((meta & 2u) ? 1u : 0u)],
-                             (S)a.p[i - ((meta & 4u) ? a.pitch : 0u)], (S)a.p[i + ((meta & 8u) ? a.pitch : 0u)],
-                             (S)a.p[i - ((meta & 16u) ? a.sz : 0u)], (S)a.p[i + ((meta & 32u) ? a.sz : 0u)]};
-            a.prev[i] = (T)special_update(a, meta, rec.y, s, (S)c, (S)q, nb);
-            rec = next_rec;
-        }
-        s_begin = s_end;
-    }
-}
-
-// One time step: tile blocks, then receiver blocks (p^n before its update,
-// fdtd.rs:673-683). Consecutive tiles go to the 8 XCDs round-robin, so
-// neighbouring tiles run at the same time and spread over all HBM stacks.
-template <typename T, typename S, int TY>
-__global__ __launch_bounds__(kTileX* TY) void k_step(StepParams<T, S> a) {
-    if (blockIdx.x < a.main_blocks) {
-        tile_part<T, S, TY>(a, blockIdx.x);
-        return;
-    }
-    const uint32_t r = (blockIdx.x - a.main_blocks) * blockDim.x + threadIdx.x;
-    if (r < a.n_recv) a.ir[r] = a.p[a.recv[r]];
-}
 
 // Marks cells that must not take the interior path (boundary nodes, taps).
 __global__ void k_mark(uint32_t* __restrict__ force, const uint32_t* __restrict__ cells, uint32_t n,
@@ -681,59 +255,7 @@ __global__ void k_fill(T* __restrict__ p, size_t n, T v) {
     if (i < n) p[i] = v;
 }
 
-inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
-
-// Number of tile-kernel variants (rows per tile: see launch_step), the
-// default one, and the default z-chunk.
-constexpr int kVariants = 4;
-constexpr int kDefaultVariant = 0;
-constexpr uint32_t kDefaultZChunk = 4;
-
 }  // namespace
-
-// ------------------------------------------------------------------ plan
-
-struct wv_plan {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    int prec = WV_FP32;
-    uint32_t nx = 0, ny = 0, nz = 0, pitch = 0, sz = 0;
-    uint64_t npitched = 0, cells = 0, air = 0, nb = 0, ns = 0, ns_k = 0;
-    double dx = 0, dt = 0;
-    uint32_t zchunk = 0;      // requested (0 = default)
-    uint32_t run_zchunk = 0;  // the current run's layout
-    uint32_t nzb = 0;         // 32-plane blocks of the z-packed bitmaps
-    int variant = kDefaultVariant;
-    uint32_t max_slots = 0, n_sets = 0;
-
-    DBuf flags, force_base, force, zbits, counts, offs;
-    DBuf b_cell_sorted, b_set_sorted;
-    DBuf set_n, set_off, set_inv_d, set_cv, set_k, set_w, st_u, st_g;
-    DBuf fa, fb;
-    DBuf sp_rec, sp_tapw;
-    DBuf tap_cell, tap_w, recv, ir, freqs, spec;
-    uint32_t n_special = 0, sp_pad = 0, seg_stride = 0;
-
-    // Current run.
-    bool active = false;
-    size_t steps = 0, done = 0;
-    uint32_t n_recv = 0;
-    std::vector<double> signal;
-    hipEvent_t ev[2] = {nullptr, nullptr};
-    hipEvent_t t0 = nullptr, t1 = nullptr;
-    double last_ms = 0.0;
-    uint64_t last_steps = 0, last_launches = 0;
-
-    size_t tsize() const { return prec == WV_FP64 ? 8 : 4; }
-    size_t ssize() const { return prec == WV_FP32 ? 4 : 8; }
-    uint32_t lane_cells() const { return (uint32_t)(16 / tsize()); }
-    ~wv_plan() {
-        if (stream) (void)hipStreamSynchronize(stream);
-        for (auto e : {ev[0], ev[1], t0, t1})
-            if (e) (void)hipEventDestroy(e);
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-};
 
 namespace {
 
@@ -753,7 +275,9 @@ std::vector<S> cast_vec(const std::vector<double>& v) {
 
 Geo geo_of(const wv_plan* pl) { return Geo{pl->npitched, pl->pitch, pl->nx, pl->ny, pl->nz}; }
 
-void plan_setup(wv_plan* pl, const wv_grid* g) {
+}  // namespace
+
+void wv::plan_setup(wv_plan* pl, const wv_grid* g) {
     WV_HIP(hipGetDevice(&pl->device));
     WV_HIP(hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking));
     WV_HIP(hipEventCreate(&pl->ev[0]));
@@ -862,28 +386,13 @@ void plan_setup(wv_plan* pl, const wv_grid* g) {
     WV_HIP(hipStreamSynchronize(pl->stream));
 }
 
+namespace {
+
 template <class T>
 void fill_async(wv_plan* pl, DBuf& d, size_t n) {
     if (!n) return;
     hipLaunchKernelGGL(k_fill<T>, dim3(div_up(n, 256)), dim3(256), 0, pl->stream, d.as<T>(), n, T(0));
     WV_HIP(hipGetLastError());
-}
-
-struct Layout {
-    uint32_t zchunk, bx, tiles_x, nzc, seg;
-    uint64_t nwaves, positions;
-};
-
-Layout layout_of(const wv_plan* pl) {
-    Layout L;
-    L.zchunk = pl->run_zchunk;
-    L.bx = kTileX * pl->lane_cells();
-    L.tiles_x = div_up(pl->nx, L.bx);
-    L.nzc = div_up(pl->nz - 2, L.zchunk);
-    L.seg = (L.zchunk * L.bx + kBlock - 1) / kBlock * kBlock;
-    L.nwaves = (uint64_t)L.nzc * L.tiles_x * pl->ny;
-    L.positions = L.nwaves * L.seg;
-    return L;
 }
 
 template <class S>
@@ -976,67 +485,6 @@ void run_begin_impl(wv_plan* pl, const wv_tap* taps, size_t n_taps, const double
 }
 
 template <class T, class S>
-StepParams<T, S> make_params(wv_plan* pl, uint32_t ty_rows) {
-    StepParams<T, S> a{};
-    const Layout L = layout_of(pl);
-    a.zbits = pl->zbits.as<uint32_t>();
-    a.nx = pl->nx;
-    a.ny = pl->ny;
-    a.nz = pl->nz;
-    a.pitch = pl->pitch;
-    a.sz = pl->sz;
-    a.tiles_x = L.tiles_x;
-    a.tiles_y = div_up(pl->ny, ty_rows);
-    a.zchunk = L.zchunk;
-    a.main_blocks = a.tiles_x * a.tiles_y * L.nzc;
-    a.seg_base = pl->offs.as<uint32_t>();
-    a.seg_stride = pl->seg_stride;
-    a.sp_rec = pl->sp_rec.as<uint2>();
-    a.sp_tapw = pl->sp_tapw.as<S>();
-    a.sp_pad = pl->sp_pad;
-    a.set_inv_d = pl->set_inv_d.as<S>();
-    a.set_cv = pl->set_cv.as<S>();
-    a.set_k = pl->set_k.as<S>();
-    a.set_w = pl->set_w.as<S>();
-    a.st_u = pl->st_u.as<S>();
-    a.st_g = pl->st_g.as<S>();
-    a.has_k = pl->ns_k > 0;
-    a.n_recv = pl->n_recv;
-    a.recv = pl->recv.as<uint32_t>();
-    const double l2 = kCourant * kCourant;
-    if (sizeof(T) == 4) {
-        // FP32 lambda^2: 1/3 rounded toward zero, so 6 l2 <= 2 (the scheme stays
-        // within its stability limit), and c2 = 2 - 6 l2 exactly, so a constant
-        // field is still preserved.
-        float l2f = (float)l2;
-        if ((double)l2f > l2) l2f = std::nextafter(l2f, 0.0f);
-        a.l2 = (T)l2f;
-        a.c2 = (T)(2.0 - 6.0 * (double)l2f);
-        a.l2s = (S)(sizeof(S) == 4 ? (double)l2f : l2);
-    } else {
-        a.l2 = (T)l2;
-        a.c2 = (T)(2.0 - 6.0 * l2);
-        a.l2s = (S)l2;
-    }
-    a.lam = (S)kCourant;
-    a.half_dt = (S)(0.5 * pl->dt);
-    return a;
-}
-
-// Tile-kernel variants: TY rows per tile (block = 64 x TY threads).
-constexpr uint32_t kVariantRows[] = {4, 8, 16, 2};
-
-template <class T, class S>
-void launch_step(int variant, uint32_t grid, hipStream_t st, const StepParams<T, S>& a) {
-    switch (variant) {
-        case 0: hipLaunchKernelGGL((k_step<T, S, 4>), dim3(grid), dim3(64 * 4), 0, st, a); break;
-        case 1: hipLaunchKernelGGL((k_step<T, S, 8>), dim3(grid), dim3(64 * 8), 0, st, a); break;
-        case 2: hipLaunchKernelGGL((k_step<T, S, 16>), dim3(grid), dim3(64 * 16), 0, st, a); break;
-        default: hipLaunchKernelGGL((k_step<T, S, 2>), dim3(grid), dim3(64 * 2), 0, st, a); break;
-    }
-}
-
-template <class T, class S>
 wv_status advance_impl(wv_plan* pl, size_t k, const volatile int32_t* cancel, wv_progress_fn progress, void* ctx) {
     const uint32_t ty_rows = kVariantRows[pl->variant];
     StepParams<T, S> a = make_params<T, S>(pl, ty_rows);
@@ -1108,7 +556,77 @@ void finish_impl(wv_plan* pl, const double* freqs, size_t nf, int mode, double* 
     WV_HIP(hipStreamSynchronize(pl->stream));
 }
 
-wv_status check_grid(const wv_grid* g, char* err, size_t err_len) {
+}  // namespace
+
+void wv::run_begin(wv_plan* pl, const wv_tap* taps, size_t n_taps, const double* signal, size_t steps,
+                   const uint64_t* receivers, size_t n_recv) {
+    if (pl->prec == WV_FP32) run_begin_impl<float>(pl, taps, n_taps, signal, steps, receivers, n_recv);
+    else run_begin_impl<double>(pl, taps, n_taps, signal, steps, receivers, n_recv);
+}
+
+void wv::run_finish(wv_plan* pl, const double* freqs, size_t nf, int mode, double* spectra_out, double* ir_out) {
+    if (pl->prec == WV_FP64) finish_impl<double>(pl, freqs, nf, mode, spectra_out, ir_out);
+    else finish_impl<float>(pl, freqs, nf, mode, spectra_out, ir_out);
+}
+
+wv_status wv::check_run_args(const wv_plan* pl, const wv_tap* taps, size_t n_taps, const double* signal,
+                             size_t steps, const uint64_t* receivers, size_t n_recv, char* err, size_t err_len) {
+    if (!pl || (n_taps && !taps) || (steps && !signal) || (n_recv && !receivers)) {
+        set_err(err, err_len, "invalid run arguments");
+        return WV_ERR_INVALID;
+    }
+    // Taps must lie in the planes the step kernel updates (1 .. nz-2); every
+    // air cell does. The reference injects anywhere (fdtd.rs:718-720), but a
+    // tap on the outer padding planes could only reach a receiver there.
+    const uint64_t plane = (uint64_t)pl->nx * pl->ny;
+    for (size_t t = 0; t < n_taps; ++t)
+        if (taps[t].cell >= pl->cells || taps[t].cell / plane == 0 || taps[t].cell / plane + 1 >= pl->nz) {
+            set_err(err, err_len, "source tap outside the updated planes of the grid");
+            return WV_ERR_INVALID;
+        }
+    for (size_t r = 0; r < n_recv; ++r)
+        if (receivers[r] >= pl->cells) {
+            set_err(err, err_len, "receiver outside the grid");
+            return WV_ERR_INVALID;
+        }
+    if ((uint64_t)steps * n_recv >= (1ull << 34) || steps >= (1ull << 31)) {
+        set_err(err, err_len, "receiver record too large");
+        return WV_ERR_LIMIT;
+    }
+    return WV_OK;
+}
+
+uint64_t wv::plan_bytes(const wv_plan* pl) {
+    uint64_t t = 0;
+    for (const DBuf* d : {&pl->flags, &pl->force_base, &pl->force, &pl->zbits, &pl->counts, &pl->offs,
+                          &pl->b_cell_sorted, &pl->b_set_sorted, &pl->set_n, &pl->set_off, &pl->set_inv_d,
+                          &pl->set_cv, &pl->set_k, &pl->set_w, &pl->st_u, &pl->st_g, &pl->fa, &pl->fb, &pl->sp_rec,
+                          &pl->sp_tapw, &pl->tap_cell, &pl->tap_w, &pl->recv, &pl->ir, &pl->freqs, &pl->spec})
+        t += d->bytes;
+    return t;
+}
+
+uint64_t wv::estimate_bytes(const wv_grid* g, int prec, size_t steps, size_t n_recv, size_t n_taps) {
+    const uint64_t tsz = prec == WV_FP64 ? 8 : 4, ssz = prec == WV_FP32 ? 4 : 8;
+    const uint64_t cells = g->n[0] * g->n[1] * g->n[2];
+    const uint64_t pitch = (g->n[0] + 63) & ~63ull;
+    const uint64_t npitched = pitch * g->n[1] * g->n[2];
+    uint32_t max_slots = 0;
+    for (uint64_t b = 0; b < g->n_boundary; ++b) max_slots = std::max(max_slots, g->b_start[b + 1] - g->b_start[b]);
+    const uint64_t special = g->n_boundary + n_taps;
+    const uint64_t sp_pad = (special + 63) & ~63ull;
+    const uint64_t slots = g->n_boundary ? g->b_start[g->n_boundary] : 0;
+    uint64_t t = cells;                              // flags
+    t += npitched / 4 + npitched / 8;                // boundary marks (two copies), interior bitmap
+    t += 2 * npitched * tsz;                         // fields
+    t += npitched / 16 + 1024;                       // compaction counts and offsets (z-chunk >= 2)
+    t += 8 * g->n_boundary + slots * (4 * ssz + 8);  // sorted boundary, slot-set table (at most one set per node)
+    t += special * (8 + ssz) + 2 * (uint64_t)max_slots * sp_pad * ssz;  // records, tap weights, slot state
+    t += (uint64_t)steps * n_recv * tsz + steps * ssz + 16 * (n_recv + n_taps);
+    return t + t / 20 + (1u << 16);
+}
+
+wv_status wv::check_grid(const wv_grid* g, char* err, size_t err_len) {
     if (!g || !g->flags || (g->n_boundary && (!g->b_cell || !g->b_start || !g->slots)) || !g->b_start) {
         set_err(err, err_len, "invalid grid: null arrays");
         return WV_ERR_INVALID;
@@ -1136,20 +654,6 @@ wv_status check_grid(const wv_grid* g, char* err, size_t err_len) {
     return WV_OK;
 }
 
-template <class F>
-wv_status guarded(char* err, size_t err_len, F f) {
-    try {
-        return f();
-    } catch (const DevErr& e) {
-        set_err(err, err_len, e.msg);
-        return e.code;
-    } catch (const std::bad_alloc&) {
-        set_err(err, err_len, "host allocation failed");
-        return WV_ERR_NOMEM;
-    }
-}
-
-}  // namespace
 
 // ------------------------------------------------------------------ C ABI
 
@@ -1181,32 +685,11 @@ extern "C" wv_status wv_plan_set_tile(wv_plan* plan, uint32_t zchunk, int32_t va
 
 extern "C" wv_status wv_run_begin(wv_plan* pl, const wv_tap* taps, size_t n_taps, const double* signal, size_t steps,
                                   const uint64_t* receivers, size_t n_recv, char* err, size_t err_len) {
-    if (!pl || (n_taps && !taps) || (steps && !signal) || (n_recv && !receivers)) {
-        set_err(err, err_len, "wv_run_begin: invalid argument");
-        return WV_ERR_INVALID;
-    }
-    // Taps must lie in the planes the step kernel updates (1 .. nz-2); every
-    // air cell does. The reference injects anywhere (fdtd.rs:718-720), but a
-    // tap on the outer padding planes could only reach a receiver there.
-    const uint64_t plane = (uint64_t)pl->nx * pl->ny;
-    for (size_t t = 0; t < n_taps; ++t)
-        if (taps[t].cell >= pl->cells || taps[t].cell / plane == 0 || taps[t].cell / plane + 1 >= pl->nz) {
-            set_err(err, err_len, "source tap outside the updated planes of the grid");
-            return WV_ERR_INVALID;
-        }
-    for (size_t r = 0; r < n_recv; ++r)
-        if (receivers[r] >= pl->cells) {
-            set_err(err, err_len, "receiver outside the grid");
-            return WV_ERR_INVALID;
-        }
-    if ((uint64_t)steps * n_recv >= (1ull << 34)) {
-        set_err(err, err_len, "receiver record too large");
-        return WV_ERR_LIMIT;
-    }
+    wv_status st = check_run_args(pl, taps, n_taps, signal, steps, receivers, n_recv, err, err_len);
+    if (st != WV_OK) return st;
     return guarded(err, err_len, [&]() -> wv_status {
         WV_HIP(hipSetDevice(pl->device));
-        if (pl->prec == WV_FP32) run_begin_impl<float>(pl, taps, n_taps, signal, steps, receivers, n_recv);
-        else run_begin_impl<double>(pl, taps, n_taps, signal, steps, receivers, n_recv);
+        run_begin(pl, taps, n_taps, signal, steps, receivers, n_recv);
         return WV_OK;
     });
 }
@@ -1235,8 +718,7 @@ extern "C" wv_status wv_run_finish(wv_plan* pl, const double* freqs, size_t nf, 
     }
     return guarded(err, err_len, [&]() -> wv_status {
         WV_HIP(hipSetDevice(pl->device));
-        if (pl->prec == WV_FP64) finish_impl<double>(pl, freqs, nf, mode, spectra_out, ir_out);
-        else finish_impl<float>(pl, freqs, nf, mode, spectra_out, ir_out);
+        run_finish(pl, freqs, nf, mode, spectra_out, ir_out);
         pl->active = false;
         return WV_OK;
     });
@@ -1308,5 +790,13 @@ extern "C" wv_status wv_plan_stats(const wv_plan* plan, uint64_t out[7]) {
     out[4] = plan->n_special;
     out[5] = plan->npitched;
     out[6] = plan->ns_k;
+    return WV_OK;
+}
+
+extern "C" wv_status wv_footprint_estimate(const wv_grid* grid, int precision, size_t steps, size_t n_recv,
+                                           size_t n_taps, uint64_t* bytes) {
+    char err[8];
+    if (!bytes || check_grid(grid, err, sizeof err) != WV_OK) return WV_ERR_INVALID;
+    *bytes = estimate_bytes(grid, precision, steps, n_recv, n_taps);
     return WV_OK;
 }

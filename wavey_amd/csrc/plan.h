@@ -1,0 +1,200 @@
+// Device plan of one FDTD run (the grid's device layout, a run's state and
+// its stream) and the helpers the plain and batched entry points share.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "step_kernels.h"
+#include "wavey_fdtd.h"
+#include "wv_internal.h"
+
+namespace wv {
+
+struct DevErr {
+    std::string msg;
+    wv_status code;
+};
+
+#define WV_HIP(expr)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            throw DevErr{std::string(#expr) + ": " + hipGetErrorString(e_),                      \
+                         e_ == hipErrorOutOfMemory ? WV_ERR_NOMEM : WV_ERR_DEVICE};               \
+    } while (0)
+
+// Owned device allocation.
+struct DBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void alloc(size_t n) {
+        if (n <= bytes && p) return;
+        release();
+        size_t b = std::max<size_t>(n, 256);
+        WV_HIP(hipMalloc(&p, b));
+        bytes = b;
+    }
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+}  // namespace wv
+
+struct wv_plan {
+    using DBuf = wv::DBuf;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int prec = WV_FP32;
+    uint32_t nx = 0, ny = 0, nz = 0, pitch = 0, sz = 0;
+    uint64_t npitched = 0, cells = 0, air = 0, nb = 0, ns = 0, ns_k = 0;
+    double dx = 0, dt = 0;
+    uint32_t zchunk = 0;      // requested (0 = default)
+    uint32_t run_zchunk = 0;  // the current run's layout
+    uint32_t nzb = 0;         // 32-plane blocks of the z-packed bitmaps
+    int variant = wv::kDefaultVariant;
+    uint32_t max_slots = 0, n_sets = 0;
+
+    DBuf flags, force_base, force, zbits, counts, offs;
+    DBuf b_cell_sorted, b_set_sorted;
+    DBuf set_n, set_off, set_inv_d, set_cv, set_k, set_w, st_u, st_g;
+    DBuf fa, fb;
+    DBuf sp_rec, sp_tapw;
+    DBuf tap_cell, tap_w, recv, ir, freqs, spec;
+    uint32_t n_special = 0, sp_pad = 0, seg_stride = 0;
+
+    // Current run.
+    bool active = false;
+    size_t steps = 0, done = 0;
+    uint32_t n_recv = 0;
+    std::vector<double> signal;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    double last_ms = 0.0;
+    uint64_t last_steps = 0, last_launches = 0;
+
+    size_t tsize() const { return prec == WV_FP64 ? 8 : 4; }
+    size_t ssize() const { return prec == WV_FP32 ? 4 : 8; }
+    uint32_t lane_cells() const { return (uint32_t)(16 / tsize()); }
+    ~wv_plan() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto e : {ev[0], ev[1], t0, t1})
+            if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace wv {
+
+struct Layout {
+    uint32_t zchunk, bx, tiles_x, nzc, seg;
+    uint64_t nwaves, positions;
+};
+
+inline Layout layout_of(const wv_plan* pl) {
+    Layout L;
+    L.zchunk = pl->run_zchunk;
+    L.bx = kTileX * pl->lane_cells();
+    L.tiles_x = div_up(pl->nx, L.bx);
+    L.nzc = div_up(pl->nz - 2, L.zchunk);
+    L.seg = (L.zchunk * L.bx + kBlock - 1) / kBlock * kBlock;
+    L.nwaves = (uint64_t)L.nzc * L.tiles_x * pl->ny;
+    L.positions = L.nwaves * L.seg;
+    return L;
+}
+
+template <class T, class S>
+inline StepParams<T, S> make_params(wv_plan* pl, uint32_t ty_rows) {
+    StepParams<T, S> a{};
+    const Layout L = layout_of(pl);
+    a.zbits = pl->zbits.as<uint32_t>();
+    a.nx = pl->nx;
+    a.ny = pl->ny;
+    a.nz = pl->nz;
+    a.pitch = pl->pitch;
+    a.sz = pl->sz;
+    a.tiles_x = L.tiles_x;
+    a.tiles_y = div_up(pl->ny, ty_rows);
+    a.zchunk = L.zchunk;
+    a.main_blocks = a.tiles_x * a.tiles_y * L.nzc;
+    a.seg_base = pl->offs.as<uint32_t>();
+    a.seg_stride = pl->seg_stride;
+    a.sp_rec = pl->sp_rec.as<uint2>();
+    a.sp_tapw = pl->sp_tapw.as<S>();
+    a.sp_pad = pl->sp_pad;
+    a.set_inv_d = pl->set_inv_d.as<S>();
+    a.set_cv = pl->set_cv.as<S>();
+    a.set_k = pl->set_k.as<S>();
+    a.set_w = pl->set_w.as<S>();
+    a.st_u = pl->st_u.as<S>();
+    a.st_g = pl->st_g.as<S>();
+    a.has_k = pl->ns_k > 0;
+    a.n_recv = pl->n_recv;
+    a.recv = pl->recv.as<uint32_t>();
+    const double l2 = kCourant * kCourant;
+    if (sizeof(T) == 4) {
+        // FP32 lambda^2: 1/3 rounded toward zero, so 6 l2 <= 2 (the scheme stays
+        // within its stability limit), and c2 = 2 - 6 l2 exactly, so a constant
+        // field is still preserved.
+        float l2f = (float)l2;
+        if ((double)l2f > l2) l2f = std::nextafter(l2f, 0.0f);
+        a.l2 = (T)l2f;
+        a.c2 = (T)(2.0 - 6.0 * (double)l2f);
+        a.l2s = (S)(sizeof(S) == 4 ? (double)l2f : l2);
+    } else {
+        a.l2 = (T)l2;
+        a.c2 = (T)(2.0 - 6.0 * l2);
+        a.l2s = (S)l2;
+    }
+    a.lam = (S)kCourant;
+    a.half_dt = (S)(0.5 * pl->dt);
+    return a;
+}
+
+template <class F>
+inline wv_status guarded(char* err, size_t err_len, F f) {
+    try {
+        return f();
+    } catch (const DevErr& e) {
+        set_err(err, err_len, e.msg);
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_err(err, err_len, "host allocation failed");
+        return WV_ERR_NOMEM;
+    }
+}
+
+// Builds the plan's device copy of a grid (fields, flags, boundary marks,
+// slot-set table). Throws DevErr.
+void plan_setup(wv_plan* pl, const wv_grid* g);
+// Starts a run on a plan: taps, signal, receivers; classification, special
+// cell records, zeroed fields and state. Throws DevErr.
+void run_begin(wv_plan* pl, const wv_tap* taps, size_t n_taps, const double* signal, size_t steps,
+               const uint64_t* receivers, size_t n_recv);
+// Spectra (FP64 DFT of the record) and/or the receiver series of the run's
+// completed steps into caller buffers. Throws DevErr.
+void run_finish(wv_plan* pl, const double* freqs, size_t nf, int mode, double* spectra_out, double* ir_out);
+wv_status check_grid(const wv_grid* g, char* err, size_t err_len);
+// Device bytes a plan holds (every buffer), and an estimate, before any
+// allocation, of what a plan of this grid and run will hold (special cells
+// counted as boundary nodes plus taps).
+uint64_t plan_bytes(const wv_plan* pl);
+uint64_t estimate_bytes(const wv_grid* g, int prec, size_t steps, size_t n_recv, size_t n_taps);
+// Checks of wv_run_begin's arguments against a plan (tap planes, receivers,
+// record size); WV_OK or an error with its message.
+wv_status check_run_args(const wv_plan* pl, const wv_tap* taps, size_t n_taps, const double* signal, size_t steps,
+                         const uint64_t* receivers, size_t n_recv, char* err, size_t err_len);
+
+}  // namespace wv

@@ -1,0 +1,418 @@
+// Step kernel of the FDTD scheme on MI355X (gfx950): the per-step parameter
+// block, the cell updates and the tile z-march, shared by the plain run and
+// the batched launch.
+//
+// The scheme is Wavey's (core/wavey-solver/src/fdtd.rs:1-30): the 3D
+// leapfrog p+ = (2 - K l2) p + l2 sum_air p_nb - p-, with locally reacting
+// frequency-dependent boundaries solved per node in closed form
+// (fdtd.rs:581-598).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace wv {
+
+constexpr uint32_t kBlock = 256;          // setup kernels and compaction blocks
+constexpr uint32_t kTileX = 64;           // one 64-lane wave per tile row
+constexpr uint32_t kStepsPerBlock = 128;  // cancel/progress granularity, as gpu_fdtd.rs:35
+constexpr uint32_t kSubPlanes = 4;        // planes per special-cell pass of a z-chunk
+
+// Special-cell record: {meta, first constant of its slot set}. Meta bits:
+// 0-5 air neighbours (-x, +x, -y, +y, -z, +z), 6 solid, 7 tap, 8-13 slot
+// count, 14-26 position in its wave's chunk (dz * tile width + x in tile).
+constexpr uint32_t kMetaSolid = 1u << 6;  // a tap on a non-air cell: injection only
+constexpr uint32_t kMetaTap = 1u << 7;    // has a tap weight
+constexpr uint32_t kMetaSlotShift = 8;
+constexpr uint32_t kMetaPosShift = 14;
+constexpr uint32_t kMaxSlots = 63;
+
+// ------------------------------------------------------------------ kernels
+
+template <typename T, typename S>
+struct StepParams {
+    const T* __restrict__ p;
+    T* __restrict__ prev;
+    const uint32_t* __restrict__ zbits;  // interior bits [nzb][ny][pitch]
+    uint32_t nx, ny, nz, pitch, sz;
+    uint32_t tiles_x, tiles_y, zchunk, main_blocks;
+    // Special cells, in wave order: wave w owns [seg_base[w * seg_stride],
+    // seg_base[(w + 1) * seg_stride]).
+    const uint32_t* __restrict__ seg_base;
+    uint32_t seg_stride;
+    const uint2* __restrict__ sp_rec;
+    const S* __restrict__ sp_tapw;
+    uint32_t sp_pad;  // row length of the slot-major state arrays
+    const S* __restrict__ set_inv_d;
+    const S* __restrict__ set_cv;
+    const S* __restrict__ set_k;
+    const S* __restrict__ set_w;
+    S* __restrict__ st_u;  // [slot][rank]
+    S* __restrict__ st_g;
+    uint32_t has_k;
+    uint32_t n_recv;
+    const uint32_t* __restrict__ recv;
+    T* __restrict__ ir;  // this step's receiver row
+    T l2, c2;            // interior: p+ = c2 p + l2 sum - p-
+    S l2s, lam, half_dt, s_n;
+};
+
+template <typename T>
+__device__ __forceinline__ T interior_update(T c, T q, T xm, T xp, T ym, T yp, T zm, T zp, T l2, T c2) {
+    if constexpr (sizeof(T) == 8) {
+        // FP64: the reference's form, neighbour order -x, +x, -y, +y, -z, +z
+        // (fdtd.rs:685-706).
+        T s = xm;
+        s += xp;
+        s += ym;
+        s += yp;
+        s += zm;
+        s += zp;
+        return c2 * c + (l2 * s - q);
+    } else {
+        // FP32: the same update as p + (v + l2 * sum (p_nb - p)) with
+        // v = p - p-. Neighbour differences and v are exact for close values
+        // (Sterbenz), so a near-uniform field rounds only in the final add
+        // instead of in l2 * sum ~ 2p: no correlated rounding drives the
+        // room's DC mode.
+        const T d = ((xm - c) + (xp - c)) + ((ym - c) + (yp - c)) + ((zm - c) + (zp - c));
+        return c + ((c - q) + l2 * d);
+    }
+}
+
+// Lane i takes v from lane i-1 (wave_shr:1); lane 0 keeps `fill`.
+__device__ __forceinline__ int dpp_from_lower(int v, int fill) {
+    return __builtin_amdgcn_update_dpp(fill, v, 0x138, 0xF, 0xF, false);
+}
+// Lane i takes v from lane i+1 (wave_shl:1); lane 63 keeps `fill`.
+__device__ __forceinline__ int dpp_from_upper(int v, int fill) {
+    return __builtin_amdgcn_update_dpp(fill, v, 0x130, 0xF, 0xF, false);
+}
+template <typename T>
+__device__ __forceinline__ T from_lower_lane(T v, T fill) {
+    if constexpr (sizeof(T) == 4) {
+        return __int_as_float(dpp_from_lower(__float_as_int(v), __float_as_int(fill)));
+    } else {
+        const long long a = __double_as_longlong(v), f = __double_as_longlong(fill);
+        const int lo = dpp_from_lower((int)a, (int)f), hi = dpp_from_lower((int)(a >> 32), (int)(f >> 32));
+        return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+    }
+}
+template <typename T>
+__device__ __forceinline__ T from_upper_lane(T v, T fill) {
+    if constexpr (sizeof(T) == 4) {
+        return __int_as_float(dpp_from_upper(__float_as_int(v), __float_as_int(fill)));
+    } else {
+        const long long a = __double_as_longlong(v), f = __double_as_longlong(fill);
+        const int lo = dpp_from_upper((int)a, (int)f), hi = dpp_from_upper((int)(a >> 32), (int)(f >> 32));
+        return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+    }
+}
+
+// W consecutive cells of a row per lane, moved as one 16-byte access.
+template <typename T>
+struct Cells {
+    static constexpr int W = 16 / sizeof(T);
+    T v[W];
+    __device__ __forceinline__ void load(const T* __restrict__ p) {
+        if constexpr (W == 4) {
+            const float4 q = *reinterpret_cast<const float4*>(p);
+            v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+        } else {
+            const double2 q = *reinterpret_cast<const double2*>(p);
+            v[0] = q.x; v[1] = q.y;
+        }
+    }
+    __device__ __forceinline__ void store(T* __restrict__ p) const {
+        if constexpr (W == 4) *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+        else *reinterpret_cast<double2*>(p) = make_double2(v[0], v[1]);
+    }
+};
+
+template <int W>
+__device__ __forceinline__ void load_words(const uint32_t* __restrict__ p, uint32_t (&w)[W]) {
+    if constexpr (W == 4) {
+        const uint4 q = *reinterpret_cast<const uint4*>(p);
+        w[0] = q.x; w[1] = q.y; w[2] = q.z; w[3] = q.w;
+    } else {
+        const uint2 q = *reinterpret_cast<const uint2*>(p);
+        w[0] = q.x; w[1] = q.y;
+    }
+}
+
+template <int W>
+__device__ __forceinline__ uint32_t plane_bits(const uint32_t (&w)[W], uint32_t sh) {
+    uint32_t b = 0;
+#pragma unroll
+    for (int e = 0; e < W; ++e) b |= ((w[e] >> sh) & 1u) << e;
+    return b;
+}
+
+// Constants and state of slots jb .. jb+G-1 of a node (indices clamped to
+// the last slot; the caller masks the extras).
+template <typename S, uint32_t G>
+struct SlotGroup {
+    S inv_d[G], w[G], cv[G], u[G], kk[G], g[G];
+    template <typename T>
+    __device__ __forceinline__ void load(const StepParams<T, S>& a, uint32_t c0, uint32_t rank, uint32_t jb,
+                                         uint32_t ns) {
+#pragma unroll
+        for (uint32_t k = 0; k < G; ++k) {
+            const uint32_t j = min(jb + k, ns - 1);
+            inv_d[k] = a.set_inv_d[c0 + j];
+            w[k] = a.set_w[c0 + j];
+            cv[k] = a.set_cv[c0 + j];
+            u[k] = a.st_u[j * a.sp_pad + rank];
+            kk[k] = 0;
+            g[k] = 0;
+            if (a.has_k) {
+                kk[k] = a.set_k[c0 + j];
+                g[k] = a.st_g[j * a.sp_pad + rank];
+            }
+        }
+    }
+    // The part of slot k's u+ that does not depend on p+ (fdtd.rs:589-592).
+    __device__ __forceinline__ S rest(uint32_t k, S pc, S pp) const {
+        return (cv[k] * u[k] + S(0.5) * pc + w[k] * pp - kk[k] * g[k]) * inv_d[k];
+    }
+    __device__ __forceinline__ void sums(uint32_t jb, uint32_t ns, S pc, S pp, S& s1, S& s2) const {
+#pragma unroll
+        for (uint32_t k = 0; k < G; ++k)
+            if (jb + k < ns) {
+                s1 += (S(0.5) - w[k]) * inv_d[k];
+                s2 += rest(k, pc, pp) - u[k];
+            }
+    }
+    template <typename T>
+    __device__ __forceinline__ void advance(const StepParams<T, S>& a, uint32_t rank, uint32_t jb, uint32_t ns, S pc,
+                                            S pp, S next) const {
+#pragma unroll
+        for (uint32_t k = 0; k < G; ++k)
+            if (jb + k < ns) {
+                const uint32_t j = jb + k;
+                const S un = (S(0.5) - w[k]) * inv_d[k] * next + rest(k, pc, pp);
+                if (a.has_k) a.st_g[j * a.sp_pad + rank] = g[k] + a.half_dt * (un + u[k]);
+                a.st_u[j * a.sp_pad + rank] = un;
+            }
+    }
+};
+
+// One special cell: the air-neighbour Laplacian, the boundary solve of
+// fdtd.rs:581-598 when it has slots, and its tap (injection after the
+// update, fdtd.rs:718-720). nb = -x, +x, -y, +y, -z, +z; c0 is the first
+// constant of the node's slot set and rank its index in the state arrays.
+template <typename T, typename S>
+__device__ __forceinline__ S special_update(const StepParams<T, S>& a, uint32_t meta, uint32_t c0, uint32_t rank,
+                                            S pc, S pp, const S (&nb)[6]) {
+    S next;
+    if (meta & kMetaSolid) {
+        next = pp;  // a tap on a solid cell: the reference only injects there
+    } else {
+        S lap;
+        if constexpr (sizeof(S) == 8) {
+            // fdtd.rs:685-696: l2 (sum_air - k p), neighbours in -x..+z order.
+            S sum = 0, k = 0;
+#pragma unroll
+            for (int d = 0; d < 6; ++d)
+                if ((meta >> d) & 1u) {
+                    sum += nb[d];
+                    k += 1;
+                }
+            lap = a.l2s * (sum - k * pc);
+        } else {
+            // FP32: sum of neighbour differences (see interior_update).
+            S d = 0;
+#pragma unroll
+            for (int e = 0; e < 6; ++e)
+                if ((meta >> e) & 1u) d += nb[e] - pc;
+            lap = a.l2s * d;
+        }
+        const uint32_t ns = (meta >> kMetaSlotShift) & kMaxSlots;
+        if (ns == 0) {
+            if constexpr (sizeof(S) == 8) next = S(2) * pc - pp + lap;
+            else next = pc + ((pc - pp) + lap);
+        } else {
+            // Each slot's u+ is linear in p+, so p+ is solved first from the
+            // summed face flux, then the states advance. The first G slots
+            // stay in registers (one pass for nodes with up to G slots);
+            // further groups are read again for the second pass. G = 3 in
+            // FP32 (a wall face with a three-branch admittance) keeps the
+            // kernel within 64 VGPRs, 8 waves per SIMD.
+            constexpr uint32_t G = sizeof(S) == 4 ? 3 : 2;
+            SlotGroup<S, G> g0;
+            g0.load(a, c0, rank, 0, ns);
+            S s1 = 0, s2 = 0;
+            g0.sums(0, ns, pc, pp, s1, s2);
+            for (uint32_t jb = G; jb < ns; jb += G) {
+                SlotGroup<S, G> gj;
+                gj.load(a, c0, rank, jb, ns);
+                gj.sums(jb, ns, pc, pp, s1, s2);
+            }
+            if constexpr (sizeof(S) == 8) {
+                next = (S(2) * pc - pp + lap - a.lam * s2) / (S(1) + a.lam * s1);
+            } else {
+                // The same solve written as pc + (small terms).
+                next = pc + ((pc - pp) + lap - a.lam * (s2 + s1 * pc)) / (S(1) + a.lam * s1);
+            }
+            g0.advance(a, rank, 0, ns, pc, pp, next);
+            for (uint32_t jb = G; jb < ns; jb += G) {
+                SlotGroup<S, G> gj;
+                gj.load(a, c0, rank, jb, ns);
+                gj.advance(a, rank, jb, ns, pc, pp, next);
+            }
+        }
+    }
+    if (meta & kMetaTap) next += a.sp_tapw[rank] * a.s_n;
+    return next;
+}
+
+// A tile of 64*W cells (x) by TY rows (y), one 64-lane wave per row, each
+// lane W consecutive cells (16 bytes: 4 FP32 or 2 FP64), marching z over
+// [z0, z1). Per plane a lane loads p at z+1, prev, the two y-neighbour rows
+// and, on lanes 0 and 63, the row-end cells of the neighbour tiles; x
+// neighbours inside a lane are in its registers and move across lanes by
+// wave shifts. All of a plane's loads are issued before any is used and the
+// loop keeps few registers live, so many waves per SIMD hide the memory
+// latency. Interior cells are written here; the wave then updates the
+// special cells of its chunk, 64 at a time, while the rows they read are
+// still in cache.
+template <typename T, typename S, int TY>
+__device__ __forceinline__ void tile_part(const StepParams<T, S>& a, uint32_t t) {
+    constexpr int W = Cells<T>::W;
+    constexpr uint32_t BX = kTileX * W;
+    const uint32_t tx = t % a.tiles_x;
+    const uint32_t r = t / a.tiles_x;
+    const uint32_t ty = r % a.tiles_y;
+    const uint32_t zc = r / a.tiles_y;
+    const uint32_t lane = threadIdx.x & (kTileX - 1);
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kTileX);
+    const uint32_t xl = tx * BX + lane * W;  // first cell of this lane
+    const uint32_t y = ty * TY + wave;
+    if (y >= a.ny) return;
+    const uint32_t w_seg = ((zc * a.tiles_x + tx) * a.ny + y) * a.seg_stride;
+    // Lanes past the row pitch (the last tile of a row) re-read the row's
+    // last 16 bytes and update nothing: no traffic beyond the row.
+    const bool live = xl < a.pitch;
+    const uint32_t x = live ? xl : a.pitch - W;
+    const uint32_t z0 = 1 + zc * a.zchunk;
+    const uint32_t z1 = min(z0 + a.zchunk, a.nz - 1);
+    const uint32_t col = y * a.pitch + x;
+    const uint32_t ym_off = y > 0 ? a.pitch : 0u;
+    const uint32_t yp_off = y + 1 < a.ny ? a.pitch : 0u;
+    const uint32_t hl_off = x > 0 ? 1u : 0u;                     // lane 0: x - 1
+    const uint32_t hr_off = x + W < a.pitch ? (uint32_t)W : 0u;  // lane 63: x + W
+    const uint32_t full = (1u << W) - 1u;
+    const uint32_t lane_mask = live ? full : 0u;
+    // The chunk lies inside one 32-plane block of the z-packed bitmap
+    // (z-chunks divide 32 and start at plane 1 + a multiple of the chunk).
+    uint32_t ib[W];
+    load_words<W>(a.zbits + (((z0 - 1) >> 5) * a.ny + y) * a.pitch + x, ib);
+
+    Cells<T> zm, zcur;
+    zm.load(a.p + (z0 - 1) * a.sz + col);
+    zcur.load(a.p + z0 * a.sz + col);
+    // The chunk is marched in sub-chunks of kSubPlanes planes; after each,
+    // the wave updates that sub-chunk's special cells while the rows they
+    // read are still in cache. Records are sorted by plane, so a sub-chunk's
+    // records start at the compaction block of its first plane.
+    const uint32_t row0 = z0 * a.sz + y * a.pitch + tx * BX;
+    uint32_t s_begin = a.seg_base[w_seg];
+    for (uint32_t zs = z0; zs < z1; zs += kSubPlanes) {
+        const uint32_t ze = min(zs + kSubPlanes, z1);
+        const uint32_t s_end =
+            ze == z1 ? a.seg_base[w_seg + a.seg_stride] : a.seg_base[w_seg + (ze - z0) * BX / kBlock];
+        // The first special records are requested now, so they have arrived
+        // when the sub-chunk's z-march ends.
+        uint2 rec = make_uint2(0u, 0u);
+        if (s_begin + lane < s_end) rec = a.sp_rec[s_begin + lane];
+        for (uint32_t z = zs; z < ze; ++z) {
+            const uint32_t i = z * a.sz + col;
+            T hl = 0, hr = 0;
+            if (lane == 0) hl = a.p[i - hl_off];
+            if (lane == kTileX - 1) hr = a.p[i + hr_off];
+            Cells<T> zp, q, ym, yp;
+            zp.load(a.p + i + a.sz);
+            q.load(a.prev + i);
+            ym.load(a.p + i - ym_off);
+            yp.load(a.p + i + yp_off);
+            const T lo = from_lower_lane(zcur.v[W - 1], hl);
+            const T hi = from_upper_lane(zcur.v[0], hr);
+            Cells<T> out;
+#pragma unroll
+            for (int e = 0; e < W; ++e) {
+                const T xm = e == 0 ? lo : zcur.v[e - 1];
+                const T xp = e == W - 1 ? hi : zcur.v[e + 1];
+                out.v[e] =
+                    interior_update(zcur.v[e], q.v[e], xm, xp, ym.v[e], yp.v[e], zm.v[e], zp.v[e], a.l2, a.c2);
+            }
+            const uint32_t bits = plane_bits<W>(ib, (z - 1) & 31u) & lane_mask;
+            if (bits == full) {
+                out.store(a.prev + i);
+            } else if (bits) {
+                // Row ends next to walls: only the interior cells.
+#pragma unroll
+                for (int e = 0; e < W; ++e)
+                    if ((bits >> e) & 1u) a.prev[i + e] = out.v[e];
+            }
+            zm = zcur;
+            zcur = zp;
+        }
+
+        // The sub-chunk's special cells, 64 at a time. A record's position
+        // gives the cell; its neighbour loads are unconditional (a masked-out
+        // neighbour reads the cell itself, and the mask never points outside
+        // the grid).
+        for (uint32_t s = s_begin + lane; s < s_end; s += kTileX) {
+            uint2 next_rec = make_uint2(0u, 0u);
+            if (s + kTileX < s_end) next_rec = a.sp_rec[s + kTileX];
+            const uint32_t meta = rec.x;
+            const uint32_t pos = meta >> kMetaPosShift;
+            const uint32_t i = row0 + (pos / BX) * a.sz + pos % BX;
+            const T c = a.p[i];
+            const T q = a.prev[i];
+            const S nb[6] = {(S)a.p[i - ((meta & 1u) ? 1u : 0u)], (S)a.p[i + ((meta & 2u) ? 1u : 0u)],
+                             (S)a.p[i - ((meta & 4u) ? a.pitch : 0u)], (S)a.p[i + ((meta & 8u) ? a.pitch : 0u)],
+                             (S)a.p[i - ((meta & 16u) ? a.sz : 0u)], (S)a.p[i + ((meta & 32u) ? a.sz : 0u)]};
+            a.prev[i] = (T)special_update(a, meta, rec.y, s, (S)c, (S)q, nb);
+            rec = next_rec;
+        }
+        s_begin = s_end;
+    }
+}
+
+// One time step: tile blocks, then receiver blocks (p^n before its update,
+// fdtd.rs:673-683). Consecutive tiles go to the 8 XCDs round-robin, so
+// neighbouring tiles run at the same time and spread over all HBM stacks.
+template <typename T, typename S, int TY>
+__global__ __launch_bounds__(kTileX* TY) void k_step(StepParams<T, S> a) {
+    if (blockIdx.x < a.main_blocks) {
+        tile_part<T, S, TY>(a, blockIdx.x);
+        return;
+    }
+    const uint32_t r = (blockIdx.x - a.main_blocks) * blockDim.x + threadIdx.x;
+    if (r < a.n_recv) a.ir[r] = a.p[a.recv[r]];
+}
+
+inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+// Number of tile-kernel variants (rows per tile: see launch_step), the
+// default one, and the default z-chunk.
+constexpr int kVariants = 4;
+constexpr int kDefaultVariant = 0;
+constexpr uint32_t kDefaultZChunk = 4;
+
+
+// Tile-kernel variants: TY rows per tile (block = 64 x TY threads).
+constexpr uint32_t kVariantRows[] = {4, 8, 16, 2};
+
+template <class T, class S>
+void launch_step(int variant, uint32_t grid, hipStream_t st, const StepParams<T, S>& a) {
+    switch (variant) {
+        case 0: hipLaunchKernelGGL((k_step<T, S, 4>), dim3(grid), dim3(64 * 4), 0, st, a); break;
+        case 1: hipLaunchKernelGGL((k_step<T, S, 8>), dim3(grid), dim3(64 * 8), 0, st, a); break;
+        case 2: hipLaunchKernelGGL((k_step<T, S, 16>), dim3(grid), dim3(64 * 16), 0, st, a); break;
+        default: hipLaunchKernelGGL((k_step<T, S, 2>), dim3(grid), dim3(64 * 2), 0, st, a); break;
+    }
+}
+
+}  // namespace wv

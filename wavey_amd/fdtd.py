@@ -80,7 +80,10 @@ EXPORTS = [
     "wv_plan_last_timing", "wv_plan_stats", "wv_plan_set_tile", "wv_plan_copy_probe", "wv_plan_free", "wv_grid_build_box",
     "wv_built_grid_view", "wv_grid_free", "wv_source_signal", "wv_nearest_air", "wv_taps_at",
     "wv_source_components", "wv_component_weights", "wv_to_transfer", "wv_admittance_eval", "wv_paris",
-    "wv_fit_real_impedance", "wv_version",
+    "wv_fit_real_impedance", "wv_version", "wv_footprint_estimate", "wv_batch_create", "wv_batch_add",
+    "wv_batch_autotune", "wv_batch_set_tile", "wv_batch_run", "wv_batch_result", "wv_batch_info",
+    "wv_batch_timing", "wv_batch_free", "wv_serve_create", "wv_serve_submit", "wv_serve_wait", "wv_serve_stats",
+    "wv_serve_free",
 ]
 
 _lib = None
@@ -150,6 +153,38 @@ def lib():
             L.wv_fit_real_impedance.restype = C.c_double
             L.wv_version.argtypes = []
             L.wv_version.restype = C.c_char_p
+            L.wv_footprint_estimate.argtypes = [P(WvGrid), C.c_int, sz, sz, sz, P(C.c_uint64)]
+            L.wv_footprint_estimate.restype = st
+            L.wv_batch_create.argtypes = [C.c_int, P(C.c_void_p)] + errbuf
+            L.wv_batch_create.restype = st
+            L.wv_batch_add.argtypes = [C.c_void_p, P(WvGrid), P(WvTap), sz, dp, sz, P(C.c_uint64), sz,
+                                       P(C.c_int32)] + errbuf
+            L.wv_batch_add.restype = st
+            L.wv_batch_autotune.argtypes = [C.c_void_p, C.c_uint32] + errbuf
+            L.wv_batch_autotune.restype = st
+            L.wv_batch_set_tile.argtypes = [C.c_void_p, C.c_int32, C.c_uint32] + errbuf
+            L.wv_batch_set_tile.restype = st
+            L.wv_batch_run.argtypes = [C.c_void_p, C.c_int, P(C.c_int32), PROGRESS_FN, C.c_void_p] + errbuf
+            L.wv_batch_run.restype = st
+            L.wv_batch_result.argtypes = [C.c_void_p, C.c_int32, dp, sz, C.c_int, dp, dp] + errbuf
+            L.wv_batch_result.restype = st
+            L.wv_batch_info.argtypes = [C.c_void_p, C.c_int32, P(C.c_uint64)]
+            L.wv_batch_info.restype = st
+            L.wv_batch_timing.argtypes = [C.c_void_p, dp, P(C.c_uint64)]
+            L.wv_batch_timing.restype = st
+            L.wv_batch_free.argtypes = [C.c_void_p]
+            L.wv_batch_free.restype = None
+            L.wv_serve_create.argtypes = [C.c_int, C.c_uint64, P(C.c_void_p)] + errbuf
+            L.wv_serve_create.restype = st
+            L.wv_serve_submit.argtypes = [C.c_void_p, P(WvGrid), P(WvTap), sz, dp, sz, P(C.c_uint64), sz, dp, sz,
+                                          C.c_int, P(C.c_uint64)] + errbuf
+            L.wv_serve_submit.restype = st
+            L.wv_serve_wait.argtypes = [C.c_void_p, C.c_uint64, dp, dp, dp] + errbuf
+            L.wv_serve_wait.restype = st
+            L.wv_serve_stats.argtypes = [C.c_void_p, dp]
+            L.wv_serve_stats.restype = st
+            L.wv_serve_free.argtypes = [C.c_void_p]
+            L.wv_serve_free.restype = None
             _lib = L
     return _lib
 
@@ -484,3 +519,169 @@ def fdtd_run(g, taps, signal, receivers, freqs=(), mode=MODE_BOTH, precision="fp
     if st == WV_CANCELLED:
         return None
     return ir, (spec[0::2] + 1j * spec[1::2])[: R * F].reshape(R, F)
+
+
+def footprint_estimate(grid, precision, steps, n_recv, n_taps):
+    """Device bytes a run of this grid will hold (wv_footprint_estimate)."""
+    out = C.c_uint64()
+    prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+    st = lib().wv_footprint_estimate(C.byref(grid.view()), prec, int(steps), int(n_recv), int(n_taps),
+                                     C.byref(out))
+    if st != WV_OK:
+        raise FdtdError(st, "invalid grid")
+    return int(out.value)
+
+
+class Batch:
+    """Many runs of different grids advanced together (wv_batch_*): one launch
+    per step over every instance, 128-step blocks replayed as HIP graphs.
+    Each instance's results equal its plain run's (``Plan.run``)."""
+
+    def __init__(self, precision="fp32"):
+        self.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        self._h = C.c_void_p()
+        err = C.create_string_buffer(1024)
+        _check(lib().wv_batch_create(self.precision, C.byref(self._h), err, 1024), err)
+        self._shape = {}
+
+    def close(self):
+        if self._h:
+            lib().wv_batch_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def add(self, grid: FdtdGrid, taps, signal, receivers):
+        """Uploads the grid and starts its run; returns the instance id."""
+        sig = np.ascontiguousarray(signal, dtype=np.float64)
+        rc = np.ascontiguousarray(receivers, dtype=np.uint64)
+        iid = C.c_int32()
+        err = C.create_string_buffer(1024)
+        _check(lib().wv_batch_add(self._h, C.byref(grid.view()), _taps_array(taps), len(taps), _dp(sig), len(sig),
+                                  rc.ctypes.data_as(C.POINTER(C.c_uint64)), len(rc), C.byref(iid), err, 1024), err)
+        self._shape[iid.value] = (len(rc), len(sig))
+        return iid.value
+
+    def autotune(self, trial_steps=16):
+        err = C.create_string_buffer(1024)
+        _check(lib().wv_batch_autotune(self._h, int(trial_steps), err, 1024), err)
+
+    def set_tile(self, variant=-1, zchunk=0):
+        err = C.create_string_buffer(1024)
+        _check(lib().wv_batch_set_tile(self._h, int(variant), int(zchunk), err, 1024), err)
+
+    def run(self, graphs=True, cancel=None, progress=None):
+        """Runs every instance to its end; WV_OK or WV_CANCELLED."""
+        err = C.create_string_buffer(1024)
+        cb = PROGRESS_FN(lambda f, _ctx: progress(f)) if progress else PROGRESS_FN()
+        st = lib().wv_batch_run(self._h, 1 if graphs else 0,
+                                cancel.ctypes.data_as(C.POINTER(C.c_int32)) if cancel is not None else None, cb, None,
+                                err, 1024)
+        return _check(st, err)
+
+    def result(self, iid, freqs=(), mode=MODE_IR):
+        R, T = self._shape[iid]
+        fr = np.ascontiguousarray(freqs, dtype=np.float64)
+        F = len(fr)
+        ir = np.zeros((R, T)) if mode & MODE_IR else None
+        spec = np.zeros(2 * R * F) if (mode & MODE_SPECTRA and F) else None
+        err = C.create_string_buffer(1024)
+        _check(lib().wv_batch_result(self._h, int(iid), _dp(fr) if F else None, F, mode,
+                                     _dp(spec) if spec is not None else None, _dp(ir) if ir is not None else None,
+                                     err, 1024), err)
+        sp = (spec[0::2] + 1j * spec[1::2]).reshape(R, F) if spec is not None else None
+        return ir, sp
+
+    def info(self, iid):
+        out = (C.c_uint64 * 6)()
+        if lib().wv_batch_info(self._h, int(iid), out) != WV_OK:
+            raise FdtdError(-1, "unknown instance")
+        keys = ["cells", "air_cells", "steps", "zchunk", "size_class", "device_bytes"]
+        return dict(zip(keys, [int(v) for v in out]))
+
+    def timing(self):
+        ms = C.c_double()
+        out = (C.c_uint64 * 5)()
+        lib().wv_batch_timing(self._h, C.byref(ms), out)
+        keys = ["steps", "cell_updates", "graph_launches", "graph_captures", "variant"]
+        d = dict(zip(keys, [int(v) for v in out]))
+        d["device_ms"] = ms.value
+        return d
+
+
+class Server:
+    """The request loop (wv_serve_*): admission by device footprint under a
+    cap, continuous batching at 128-step boundaries, results on host memory.
+    Grids passed to ``submit`` must stay alive until ``wait`` returns; this
+    class keeps a reference."""
+
+    def __init__(self, precision="fp32", hbm_cap_bytes=288 * 10**9):
+        self.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        self._h = C.c_void_p()
+        err = C.create_string_buffer(1024)
+        _check(lib().wv_serve_create(self.precision, int(hbm_cap_bytes), C.byref(self._h), err, 1024), err)
+        self._req = {}
+
+    def close(self):
+        if self._h:
+            lib().wv_serve_free(self._h)
+            self._h = C.c_void_p()
+            self._req.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def submit(self, grid: FdtdGrid, taps, signal, receivers, freqs=(), mode=MODE_IR):
+        sig = np.ascontiguousarray(signal, dtype=np.float64)
+        rc = np.ascontiguousarray(receivers, dtype=np.uint64)
+        fr = np.ascontiguousarray(freqs, dtype=np.float64)
+        rid = C.c_uint64()
+        err = C.create_string_buffer(1024)
+        view = grid.view()
+        _check(lib().wv_serve_submit(self._h, C.byref(view), _taps_array(taps), len(taps), _dp(sig), len(sig),
+                                     rc.ctypes.data_as(C.POINTER(C.c_uint64)), len(rc), _dp(fr) if len(fr) else None,
+                                     len(fr), mode, C.byref(rid), err, 1024), err)
+        # The library borrows the grid view (and the arrays behind it) until wait.
+        self._req[rid.value] = (grid, view, len(rc), len(sig), len(fr), mode)
+        return rid.value
+
+    def wait(self, rid):
+        """(ir [R][T] or None, spectra [R][F] or None, latency s)."""
+        grid, view, R, T, F, mode = self._req[rid]
+        ir = np.zeros((R, T)) if mode & MODE_IR else None
+        spec = np.zeros(2 * R * F) if (mode & MODE_SPECTRA and F) else None
+        lat = C.c_double()
+        err = C.create_string_buffer(1024)
+        try:
+            _check(lib().wv_serve_wait(self._h, int(rid), _dp(spec) if spec is not None else None,
+                                       _dp(ir) if ir is not None else None, C.byref(lat), err, 1024), err)
+        finally:
+            del self._req[rid]
+        sp = (spec[0::2] + 1j * spec[1::2]).reshape(R, F) if spec is not None else None
+        return ir, sp, lat.value
+
+    def stats(self):
+        out = (C.c_double * 10)()
+        lib().wv_serve_stats(self._h, out)
+        keys = ["completed", "p50_s", "p95_s", "p99_s", "span_s", "cell_updates", "max_running",
+                "peak_device_bytes", "failed", "graph_captures"]
+        return dict(zip(keys, [float(v) for v in out]))
