@@ -127,12 +127,221 @@ def measure_traffic(args, copy_bytes):
     return out["FETCH_SIZE"]["bytes"] + out["WRITE_SIZE"]["bytes"], out
 
 
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(np.asarray(b)), 1e-300))
+
+
+def _instances(cases):
+    """One instance per (room, source run): the per-speaker solves."""
+    return [(ci, c, taps) for ci, c in enumerate(cases) for (_, _, taps) in c.runs]
+
+
+def bench_room_mix(args):
+    """The 48-room uneven mix: sequential plain runs (one plan per room, its
+    speakers one after another) against one batch (all instances in one
+    launch per step, autotuned shapes, 128-step blocks as HIP graphs). Every
+    batched IR is compared with its plain IR (bit-identical expected); a
+    sample of instances is also run in FP64 for the FP32 error."""
+    from wavey_amd import configs, fdtd, workload
+
+    rooms = configs.room_mix(n_rooms=args.rooms)
+    t0 = time.perf_counter()
+    cases = [workload.build_case(cfg) for cfg in rooms]
+    t_build = time.perf_counter() - t0
+    inst = _instances(cases)
+    work = sum(c.grid.air_cells() * len(c.signal) for _, c, _ in inst)
+    print(f"bench: room mix {len(cases)} rooms, {len(inst)} instances, {work / 1e12:.2f} T cell-updates, "
+          f"built in {t_build:.1f} s", file=sys.stderr, flush=True)
+
+    # Sequential plain runs.
+    plain = []
+    t0 = time.perf_counter()
+    dev_ms = 0.0
+    for ci, c in enumerate(cases):
+        with fdtd.Plan(c.grid, args.precision) as p:
+            for (_, _, taps) in c.runs:
+                ir, _ = p.run(taps, c.signal, c.receivers, mode=fdtd.MODE_IR)
+                dev_ms += p.last_timing()[0]
+                plain.append(ir)
+    t_plain = time.perf_counter() - t0
+    print(f"bench: sequential plain {t_plain:.2f} s", file=sys.stderr, flush=True)
+
+    # One batch.
+    t0 = time.perf_counter()
+    b = fdtd.Batch(args.precision)
+    ids = [b.add(c.grid, taps, c.signal, c.receivers) for _, c, taps in inst]
+    t_add = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    if not args.no_autotune:
+        b.autotune(trial_steps=16)
+    t_tune = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    b.run(graphs=not args.no_graphs)
+    t_run = time.perf_counter() - t0
+    bt = b.timing()
+    t0 = time.perf_counter()
+    batched = [b.result(i)[0] for i in ids]
+    t_read = time.perf_counter() - t0
+    zch = sorted({b.info(i)["zchunk"] for i in ids})
+    b.close()
+    t_batch = t_add + t_tune + t_run + t_read
+    identical = sum(int(np.array_equal(x, y)) for x, y in zip(batched, plain))
+    max_diff = max(float(np.max(np.abs(x - y))) for x, y in zip(batched, plain))
+
+    # FP32 against an FP64 run of the same scheme, on a sample of instances.
+    errs = []
+    if args.precision != "fp64":
+        for k in range(0, len(inst), max(1, len(inst) // args.fp64_sample)):
+            _, c, taps = inst[k]
+            with fdtd.Plan(c.grid, "fp64") as p:
+                ref, _ = p.run(taps, c.signal, c.receivers, mode=fdtd.MODE_IR)
+            errs.append(rel_l2(batched[k], ref))
+    n = len(inst)
+    line = {
+        "metric": "batched solves/s (uneven room mix)",
+        "value": round(n / t_batch, 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": {"fp32": "f32", "fp64": "f64", "fp32_state64": "f32+f64state"}[args.precision],
+        "data": "synthetic (seeded rooms, materials, speakers, receivers)",
+        "config": {"workload": f"room mix: {len(cases)} rooms, dx 0.05 m, {n} per-speaker instances, "
+                               f"{len(cases[0].signal)} steps (0.5 s), 16 receivers each",
+                   "instances": n, "cell_updates": int(work), "precision": args.precision,
+                   "graphs": not args.no_graphs, "autotune": not args.no_autotune},
+        "sequential_plain": {"solves_per_s": round(n / t_plain, 3), "seconds": round(t_plain, 3),
+                             "device_step_loop_s": round(dev_ms / 1e3, 3)},
+        "batched": {"seconds": round(t_batch, 3), "add_s": round(t_add, 3), "autotune_s": round(t_tune, 3),
+                    "run_s": round(t_run, 3), "readback_s": round(t_read, 3),
+                    "device_run_s": round(bt["device_ms"] / 1e3, 3),
+                    "run_gcell_per_s": round(work / max(t_run, 1e-9) / 1e9, 1),
+                    "graph_launches": bt["graph_launches"], "graph_captures": bt["graph_captures"],
+                    "variant": bt["variant"], "zchunks": zch},
+        "speedup_vs_sequential_plain": round(t_plain / t_batch, 3),
+        "speedup_step_loop_only": round(t_plain / max(t_run, 1e-9), 3),
+        "parity": {"bit_identical_to_plain": f"{identical}/{n}", "max_abs_diff": max_diff,
+                   "fp32_vs_fp64_rel_l2_max": max(errs) if errs else None, "fp64_sample": len(errs)},
+        "setup_s": {"grid_build": round(t_build, 2)},
+    }
+    print(json.dumps(line), flush=True)
+
+
+def bench_serving(args):
+    """Poisson arrivals of room-mix requests with padded cells log-uniform in
+    0.2M-60M and 0.3-1.0 s of simulated time, at the rate that offers
+    args.load of the given batched throughput (cell-updates/s). The server
+    (admission by footprint, continuous batching) against sequential plain
+    runs of the same trace: the plain latencies are those of a FIFO single
+    queue with the measured per-request times and the same arrivals."""
+    import threading
+
+    from wavey_amd import configs, fdtd, workload
+
+    base = configs.serving_trace(1.0, args.requests)
+    t0 = time.perf_counter()
+    cases = [workload.build_case(cfg) for _, cfg in base]
+    t_build = time.perf_counter() - t0
+    works = [c.grid.air_cells() * len(c.signal) for c in cases]
+    rate = args.load * args.throughput * 1e9 / float(np.mean(works))
+    arrivals = [t / rate for t, _ in base]
+    print(f"bench: serving {len(cases)} requests, {sum(works) / 1e12:.2f} T cell-updates, rate {rate:.2f}/s, "
+          f"built in {t_build:.1f} s", file=sys.stderr, flush=True)
+
+    # Sequential plain: each request alone, in arrival order.
+    plain, service = [], []
+    for c in cases:
+        t0 = time.perf_counter()
+        with fdtd.Plan(c.grid, args.precision) as p:
+            ir, _ = p.run(c.runs[0][2], c.signal, c.receivers, mode=fdtd.MODE_IR)
+        service.append(time.perf_counter() - t0)
+        plain.append(ir)
+    fin, lat_plain = 0.0, []
+    for a, sv in zip(arrivals, service):
+        fin = max(fin, a) + sv
+        lat_plain.append(fin - a)
+    span_plain = fin - arrivals[0]
+    print(f"bench: sequential plain {sum(service):.2f} s of service", file=sys.stderr, flush=True)
+
+    # The server, fed by a client thread at the arrival times.
+    server = fdtd.Server(args.precision, hbm_cap_bytes=int(args.cap_gb * 1e9))
+    ids = [None] * len(cases)
+    start = time.perf_counter()
+
+    def client():
+        for k, (a, c) in enumerate(zip(arrivals, cases)):
+            dt = start + a - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            ids[k] = server.submit(c.grid, c.runs[0][2], c.signal, c.receivers)
+
+    th = threading.Thread(target=client)
+    th.start()
+    th.join()
+    served = []
+    lat = []
+    for k in range(len(cases)):
+        ir, _, l = server.wait(ids[k])
+        served.append(ir)
+        lat.append(l)
+    st = server.stats()
+    server.close()
+    span = st["span_s"]
+    identical = sum(int(np.array_equal(x, y)) for x, y in zip(served, plain))
+    errs = []
+    if args.precision != "fp64":
+        for k in range(0, len(cases), max(1, len(cases) // args.fp64_sample)):
+            c = cases[k]
+            with fdtd.Plan(c.grid, "fp64") as p:
+                ref, _ = p.run(c.runs[0][2], c.signal, c.receivers, mode=fdtd.MODE_IR)
+            errs.append(rel_l2(served[k], ref))
+
+    def pct(v):
+        return {"p50": round(float(np.percentile(v, 50)), 4), "p95": round(float(np.percentile(v, 95)), 4),
+                "p99": round(float(np.percentile(v, 99)), 4)}
+
+    n = len(cases)
+    line = {
+        "metric": "served solves/s and latency (Poisson trace)",
+        "value": round(n / span, 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": {"fp32": "f32", "fp64": "f64", "fp32_state64": "f32+f64state"}[args.precision],
+        "data": "synthetic (seeded rooms, sizes, durations, arrivals)",
+        "config": {"workload": f"serving trace: {n} requests, padded cells log-uniform 0.2M-60M, 0.3-1.0 s, "
+                               f"dx 0.05 m, Poisson rate {rate:.3f}/s = {args.load:.0%} of "
+                               f"{args.throughput} Gcell-updates/s",
+                   "requests": n, "cell_updates": int(sum(works)), "cap_bytes": int(args.cap_gb * 1e9)},
+        "latency_s": pct(lat),
+        "sequential_plain": {"solves_per_s": round(n / span_plain, 3), "latency_s": pct(lat_plain),
+                             "service_s_total": round(sum(service), 3)},
+        "speedup_vs_sequential_plain": round(span_plain / span, 3),
+        "server": {"span_s": round(span, 3), "max_running": int(st["max_running"]),
+                   "peak_device_bytes": int(st["peak_device_bytes"]), "graph_captures": int(st["graph_captures"]),
+                   "failed": int(st["failed"])},
+        "parity": {"bit_identical_to_plain": f"{identical}/{n}",
+                   "fp32_vs_fp64_rel_l2_max": max(errs) if errs else None, "fp64_sample": len(errs)},
+        "setup_s": {"grid_build": round(t_build, 2)},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="hall", choices=["hall", "pillared_room", "shoebox"])
+    ap.add_argument("--workload", default="hall",
+                    choices=["hall", "pillared_room", "shoebox", "room_mix", "serving"])
+    ap.add_argument("--rooms", type=int, default=48, help="room_mix: rooms")
+    ap.add_argument("--requests", type=int, default=40, help="serving: requests")
+    ap.add_argument("--load", type=float, default=0.8, help="serving: offered fraction of --throughput")
+    ap.add_argument("--throughput", type=float, default=300.0,
+                    help="serving: batched throughput to load against, Gcell-updates/s")
+    ap.add_argument("--cap-gb", type=float, default=288.0, help="serving: admission cap, GB")
+    ap.add_argument("--fp64-sample", type=int, default=6, help="instances rerun in FP64 for the error")
+    ap.add_argument("--no-graphs", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64", "fp32_state64"])
     ap.add_argument("--zchunk", type=int, default=0)
     ap.add_argument("--variant", type=int, default=-1)
@@ -143,6 +352,11 @@ def main():
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU z-slab decomposition is a design note, not built)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "room_mix":
+        return bench_room_mix(args)
+    if args.workload == "serving":
+        return bench_serving(args)
 
     from wavey_amd import configs, fdtd, workload
 

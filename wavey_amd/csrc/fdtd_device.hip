@@ -95,12 +95,13 @@ __global__ __launch_bounds__(kBlock) void k_zbits(const uint8_t* __restrict__ fl
     zbits[t] = wi;
 }
 
-// The wave order of special cells: wave w = (zc * tiles_x + tx) * ny + y owns
-// positions [w * seg, (w + 1) * seg); position (w, dz, xi) is cell
-// (tx * BX + xi, y, 1 + zc * zchunk + dz). Every wave segment is a whole
-// number of compaction blocks.
+// The wave order of special cells: wave w = zc * n_tiles + tp (tile tp of
+// the plane, z-chunk zc) owns positions [w * seg, (w + 1) * seg); position
+// (w, dz, xi) is the cell at plane offset tp * BX + xi in plane
+// 1 + zc * zchunk + dz. Every wave segment is a whole number of compaction
+// blocks.
 struct WaveOrder {
-    uint32_t bx, zchunk, seg, tiles_x;
+    uint32_t bx, zchunk, seg, n_tiles;
 };
 
 __device__ __forceinline__ bool wave_cell(uint64_t qw, const WaveOrder& o, const Geo& g, uint32_t* x, uint32_t* y,
@@ -109,10 +110,11 @@ __device__ __forceinline__ bool wave_cell(uint64_t qw, const WaveOrder& o, const
     const uint32_t rr = (uint32_t)(qw % o.seg);
     const uint32_t dz = rr / o.bx, xi = rr % o.bx;
     if (dz >= o.zchunk) return false;
-    *y = (uint32_t)(w % g.ny);
-    const uint64_t t2 = w / g.ny;
-    const uint32_t tx = (uint32_t)(t2 % o.tiles_x), zc = (uint32_t)(t2 / o.tiles_x);
-    *x = tx * o.bx + xi;
+    const uint32_t tp = (uint32_t)(w % o.n_tiles), zc = (uint32_t)(w / o.n_tiles);
+    const uint64_t f = (uint64_t)tp * o.bx + xi;
+    if (f >= (uint64_t)g.pitch * g.ny) return false;
+    *x = (uint32_t)(f % g.pitch);
+    *y = (uint32_t)(f / g.pitch);
     *z = 1 + zc * o.zchunk + dz;
     return *x < g.nx && *z + 1 < g.nz;
 }
@@ -427,7 +429,7 @@ void run_begin_impl(wv_plan* pl, const wv_tap* taps, size_t n_taps, const double
     // Special cells in wave order.
     pl->run_zchunk = pl->zchunk ? pl->zchunk : kDefaultZChunk;
     const Layout L = layout_of(pl);
-    const WaveOrder wo{L.bx, L.zchunk, L.seg, L.tiles_x};
+    const WaveOrder wo{L.bx, L.zchunk, L.seg, L.n_tiles};
     const uint64_t nblocks = L.positions / kBlock;
     if (nblocks >= (1ull << 31)) throw DevErr{"grid too large for the special-cell layout", WV_ERR_LIMIT};
     // offs[b] = special cells before compaction block b; offs[nblocks] = total.

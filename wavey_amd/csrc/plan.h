@@ -99,7 +99,7 @@ struct wv_plan {
 namespace wv {
 
 struct Layout {
-    uint32_t zchunk, bx, tiles_x, nzc, seg;
+    uint32_t zchunk, bx, n_tiles, nzc, seg;
     uint64_t nwaves, positions;
 };
 
@@ -107,10 +107,10 @@ inline Layout layout_of(const wv_plan* pl) {
     Layout L;
     L.zchunk = pl->run_zchunk;
     L.bx = kTileX * pl->lane_cells();
-    L.tiles_x = div_up(pl->nx, L.bx);
+    L.n_tiles = div_up(pl->sz, L.bx);
     L.nzc = div_up(pl->nz - 2, L.zchunk);
     L.seg = (L.zchunk * L.bx + kBlock - 1) / kBlock * kBlock;
-    L.nwaves = (uint64_t)L.nzc * L.tiles_x * pl->ny;
+    L.nwaves = (uint64_t)L.nzc * L.n_tiles;
     L.positions = L.nwaves * L.seg;
     return L;
 }
@@ -125,10 +125,10 @@ inline StepParams<T, S> make_params(wv_plan* pl, uint32_t ty_rows) {
     a.nz = pl->nz;
     a.pitch = pl->pitch;
     a.sz = pl->sz;
-    a.tiles_x = L.tiles_x;
-    a.tiles_y = div_up(pl->ny, ty_rows);
+    a.n_tiles = L.n_tiles;
+    a.tile_blocks = div_up(L.n_tiles, ty_rows);
     a.zchunk = L.zchunk;
-    a.main_blocks = a.tiles_x * a.tiles_y * L.nzc;
+    a.main_blocks = a.tile_blocks * L.nzc;
     a.seg_base = pl->offs.as<uint32_t>();
     a.seg_stride = pl->seg_stride;
     a.sp_rec = pl->sp_rec.as<uint2>();
@@ -141,6 +141,7 @@ inline StepParams<T, S> make_params(wv_plan* pl, uint32_t ty_rows) {
     a.st_u = pl->st_u.as<S>();
     a.st_g = pl->st_g.as<S>();
     a.has_k = pl->ns_k > 0;
+    a.max_slots = pl->max_slots;
     a.n_recv = pl->n_recv;
     a.recv = pl->recv.as<uint32_t>();
     const double l2 = kCourant * kCourant;

@@ -35,7 +35,9 @@ struct StepParams {
     T* __restrict__ prev;
     const uint32_t* __restrict__ zbits;  // interior bits [nzb][ny][pitch]
     uint32_t nx, ny, nz, pitch, sz;
-    uint32_t tiles_x, tiles_y, zchunk, main_blocks;
+    // Tiles are spans of 64 W consecutive cells of the pitched plane (pitch *
+    // ny cells); a block holds TY of them and marches one z-chunk.
+    uint32_t n_tiles, tile_blocks, zchunk, main_blocks;
     // Special cells, in wave order: wave w owns [seg_base[w * seg_stride],
     // seg_base[(w + 1) * seg_stride]).
     const uint32_t* __restrict__ seg_base;
@@ -50,6 +52,7 @@ struct StepParams {
     S* __restrict__ st_u;  // [slot][rank]
     S* __restrict__ st_g;
     uint32_t has_k;
+    uint32_t max_slots;  // rows of the state arrays
     uint32_t n_recv;
     const uint32_t* __restrict__ recv;
     T* __restrict__ ir;  // this step's receiver row
@@ -148,21 +151,28 @@ __device__ __forceinline__ uint32_t plane_bits(const uint32_t (&w)[W], uint32_t 
     return b;
 }
 
+// Slots a special cell keeps in registers: the first group of its boundary
+// solve. Three in FP32 (a wall face with a three-branch admittance) keeps the
+// step kernel within 80 VGPRs; two with FP64 state.
+template <typename S>
+constexpr uint32_t kSlotGroup = sizeof(S) == 4 ? 3 : 2;
+
 // Constants and state of slots jb .. jb+G-1 of a node (indices clamped to
-// the last slot; the caller masks the extras).
+// the last slot; the caller masks the extras). `upre`, when given, holds the
+// states of rows jb .. jb+G-1 read earlier.
 template <typename S, uint32_t G>
 struct SlotGroup {
     S inv_d[G], w[G], cv[G], u[G], kk[G], g[G];
     template <typename T>
     __device__ __forceinline__ void load(const StepParams<T, S>& a, uint32_t c0, uint32_t rank, uint32_t jb,
-                                         uint32_t ns) {
+                                         uint32_t ns, const S* upre = nullptr) {
 #pragma unroll
         for (uint32_t k = 0; k < G; ++k) {
             const uint32_t j = min(jb + k, ns - 1);
             inv_d[k] = a.set_inv_d[c0 + j];
             w[k] = a.set_w[c0 + j];
             cv[k] = a.set_cv[c0 + j];
-            u[k] = a.st_u[j * a.sp_pad + rank];
+            u[k] = upre ? upre[k] : a.st_u[j * a.sp_pad + rank];
             kk[k] = 0;
             g[k] = 0;
             if (a.has_k) {
@@ -200,10 +210,11 @@ struct SlotGroup {
 // One special cell: the air-neighbour Laplacian, the boundary solve of
 // fdtd.rs:581-598 when it has slots, and its tap (injection after the
 // update, fdtd.rs:718-720). nb = -x, +x, -y, +y, -z, +z; c0 is the first
-// constant of the node's slot set and rank its index in the state arrays.
+// constant of the node's slot set, rank its index in the state arrays and u0
+// the states of its first kSlotGroup rows, read ahead by the caller.
 template <typename T, typename S>
 __device__ __forceinline__ S special_update(const StepParams<T, S>& a, uint32_t meta, uint32_t c0, uint32_t rank,
-                                            S pc, S pp, const S (&nb)[6]) {
+                                            S pc, S pp, const S (&nb)[6], const S (&u0)[kSlotGroup<S>]) {
     S next;
     if (meta & kMetaSolid) {
         next = pp;  // a tap on a solid cell: the reference only injects there
@@ -235,12 +246,10 @@ __device__ __forceinline__ S special_update(const StepParams<T, S>& a, uint32_t 
             // Each slot's u+ is linear in p+, so p+ is solved first from the
             // summed face flux, then the states advance. The first G slots
             // stay in registers (one pass for nodes with up to G slots);
-            // further groups are read again for the second pass. G = 3 in
-            // FP32 (a wall face with a three-branch admittance) keeps the
-            // kernel within 64 VGPRs, 8 waves per SIMD.
-            constexpr uint32_t G = sizeof(S) == 4 ? 3 : 2;
+            // further groups are read again for the second pass.
+            constexpr uint32_t G = kSlotGroup<S>;
             SlotGroup<S, G> g0;
-            g0.load(a, c0, rank, 0, ns);
+            g0.load(a, c0, rank, 0, ns, u0);
             S s1 = 0, s2 = 0;
             g0.sums(0, ns, pc, pp, s1, s2);
             for (uint32_t jb = G; jb < ns; jb += G) {
@@ -266,47 +275,51 @@ __device__ __forceinline__ S special_update(const StepParams<T, S>& a, uint32_t 
     return next;
 }
 
-// A tile of 64*W cells (x) by TY rows (y), one 64-lane wave per row, each
-// lane W consecutive cells (16 bytes: 4 FP32 or 2 FP64), marching z over
-// [z0, z1). Per plane a lane loads p at z+1, prev, the two y-neighbour rows
-// and, on lanes 0 and 63, the row-end cells of the neighbour tiles; x
-// neighbours inside a lane are in its registers and move across lanes by
-// wave shifts. All of a plane's loads are issued before any is used and the
-// loop keeps few registers live, so many waves per SIMD hide the memory
-// latency. Interior cells are written here; the wave then updates the
-// special cells of its chunk, 64 at a time, while the rows they read are
+// A tile is a span of 64*W consecutive cells of the pitched xy plane, one
+// 64-lane wave, each lane W consecutive cells (16 bytes: 4 FP32 or 2 FP64),
+// marching z over [z0, z1). Tiles run across row ends, so narrow grids keep
+// their lanes busy: a cell next to a row end (x = 0, x >= nx - 1, the pitch
+// padding) is never interior, so the x-neighbour values it would take from
+// the adjacent row are never stored; interior cells (x in 1..nx-2, y in
+// 1..ny-2) have both x-neighbours in their row and both y-neighbours at
+// +-pitch. Per plane a lane loads p at z+1, prev, the two y-neighbour rows
+// and, on lanes 0 and 63, the cells next to the tile; x neighbours inside a
+// lane are in its registers and move across lanes by wave shifts. All of a
+// plane's loads are issued before any is used and the loop keeps few
+// registers live, so many waves per SIMD hide the memory latency. Interior
+// cells are written here; after each sub-chunk the wave updates that
+// sub-chunk's special cells, 64 at a time, while the rows they read are
 // still in cache.
 template <typename T, typename S, int TY>
 __device__ __forceinline__ void tile_part(const StepParams<T, S>& a, uint32_t t) {
     constexpr int W = Cells<T>::W;
     constexpr uint32_t BX = kTileX * W;
-    const uint32_t tx = t % a.tiles_x;
-    const uint32_t r = t / a.tiles_x;
-    const uint32_t ty = r % a.tiles_y;
-    const uint32_t zc = r / a.tiles_y;
+    const uint32_t tb = t % a.tile_blocks;
+    const uint32_t zc = t / a.tile_blocks;
     const uint32_t lane = threadIdx.x & (kTileX - 1);
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kTileX);
-    const uint32_t xl = tx * BX + lane * W;  // first cell of this lane
-    const uint32_t y = ty * TY + wave;
-    if (y >= a.ny) return;
-    const uint32_t w_seg = ((zc * a.tiles_x + tx) * a.ny + y) * a.seg_stride;
-    // Lanes past the row pitch (the last tile of a row) re-read the row's
-    // last 16 bytes and update nothing: no traffic beyond the row.
-    const bool live = xl < a.pitch;
-    const uint32_t x = live ? xl : a.pitch - W;
+    const uint32_t tp = tb * TY + wave;  // tile of the plane
+    if (tp >= a.n_tiles) return;
+    const uint32_t w_seg = (zc * a.n_tiles + tp) * a.seg_stride;
+    // Lanes past the plane (the last tile) re-read its last 16 bytes and
+    // update nothing. Every neighbour offset below stays inside the field:
+    // planes z0 - 1 .. z1 are within 0 .. nz - 1 and a plane has sz >= pitch
+    // cells.
+    const uint32_t fl = tp * BX + lane * W;  // first cell of this lane in the plane
+    const bool live = fl < a.sz;
+    const uint32_t col = live ? fl : a.sz - W;
     const uint32_t z0 = 1 + zc * a.zchunk;
     const uint32_t z1 = min(z0 + a.zchunk, a.nz - 1);
-    const uint32_t col = y * a.pitch + x;
-    const uint32_t ym_off = y > 0 ? a.pitch : 0u;
-    const uint32_t yp_off = y + 1 < a.ny ? a.pitch : 0u;
-    const uint32_t hl_off = x > 0 ? 1u : 0u;                     // lane 0: x - 1
-    const uint32_t hr_off = x + W < a.pitch ? (uint32_t)W : 0u;  // lane 63: x + W
+    const uint32_t ym_off = a.pitch;
+    const uint32_t yp_off = a.pitch;
+    const uint32_t hl_off = 1u;                // lane 0: the cell before the tile
+    const uint32_t hr_off = (uint32_t)W;       // lane 63: the cell after it
     const uint32_t full = (1u << W) - 1u;
     const uint32_t lane_mask = live ? full : 0u;
     // The chunk lies inside one 32-plane block of the z-packed bitmap
     // (z-chunks divide 32 and start at plane 1 + a multiple of the chunk).
     uint32_t ib[W];
-    load_words<W>(a.zbits + (((z0 - 1) >> 5) * a.ny + y) * a.pitch + x, ib);
+    load_words<W>(a.zbits + ((z0 - 1) >> 5) * a.sz + col, ib);
 
     Cells<T> zm, zcur;
     zm.load(a.p + (z0 - 1) * a.sz + col);
@@ -315,16 +328,26 @@ __device__ __forceinline__ void tile_part(const StepParams<T, S>& a, uint32_t t)
     // the wave updates that sub-chunk's special cells while the rows they
     // read are still in cache. Records are sorted by plane, so a sub-chunk's
     // records start at the compaction block of its first plane.
-    const uint32_t row0 = z0 * a.sz + y * a.pitch + tx * BX;
+    const uint32_t row0 = z0 * a.sz + tp * BX;
     uint32_t s_begin = a.seg_base[w_seg];
     for (uint32_t zs = z0; zs < z1; zs += kSubPlanes) {
         const uint32_t ze = min(zs + kSubPlanes, z1);
         const uint32_t s_end =
             ze == z1 ? a.seg_base[w_seg + a.seg_stride] : a.seg_base[w_seg + (ze - z0) * BX / kBlock];
-        // The first special records are requested now, so they have arrived
-        // when the sub-chunk's z-march ends.
+        // The first special records and their first slot states (rows the
+        // state arrays have, whatever the cell's slot count) are requested
+        // now, so they have arrived when the sub-chunk's z-march ends.
+        constexpr uint32_t G = kSlotGroup<S>;
         uint2 rec = make_uint2(0u, 0u);
-        if (s_begin + lane < s_end) rec = a.sp_rec[s_begin + lane];
+        S u0[G];
+#pragma unroll
+        for (uint32_t j = 0; j < G; ++j) u0[j] = S(0);
+        if (s_begin + lane < s_end) {
+            rec = a.sp_rec[s_begin + lane];
+#pragma unroll
+            for (uint32_t j = 0; j < G; ++j)
+                if (j < a.max_slots) u0[j] = a.st_u[j * a.sp_pad + s_begin + lane];
+        }
         for (uint32_t z = zs; z < ze; ++z) {
             const uint32_t i = z * a.sz + col;
             T hl = 0, hr = 0;
@@ -364,7 +387,15 @@ __device__ __forceinline__ void tile_part(const StepParams<T, S>& a, uint32_t t)
         // the grid).
         for (uint32_t s = s_begin + lane; s < s_end; s += kTileX) {
             uint2 next_rec = make_uint2(0u, 0u);
-            if (s + kTileX < s_end) next_rec = a.sp_rec[s + kTileX];
+            S next_u[G];
+#pragma unroll
+            for (uint32_t j = 0; j < G; ++j) next_u[j] = S(0);
+            if (s + kTileX < s_end) {
+                next_rec = a.sp_rec[s + kTileX];
+#pragma unroll
+                for (uint32_t j = 0; j < G; ++j)
+                    if (j < a.max_slots) next_u[j] = a.st_u[j * a.sp_pad + s + kTileX];
+            }
             const uint32_t meta = rec.x;
             const uint32_t pos = meta >> kMetaPosShift;
             const uint32_t i = row0 + (pos / BX) * a.sz + pos % BX;
@@ -373,8 +404,10 @@ __device__ __forceinline__ void tile_part(const StepParams<T, S>& a, uint32_t t)
             const S nb[6] = {(S)a.p[i - ((meta & 1u) ? 1u : 0u)], (S)a.p[i + ((meta & 2u) ? 1u : 0u)],
                              (S)a.p[i - ((meta & 4u) ? a.pitch : 0u)], (S)a.p[i + ((meta & 8u) ? a.pitch : 0u)],
                              (S)a.p[i - ((meta & 16u) ? a.sz : 0u)], (S)a.p[i + ((meta & 32u) ? a.sz : 0u)]};
-            a.prev[i] = (T)special_update(a, meta, rec.y, s, (S)c, (S)q, nb);
+            a.prev[i] = (T)special_update(a, meta, rec.y, s, (S)c, (S)q, nb, u0);
             rec = next_rec;
+#pragma unroll
+            for (uint32_t j = 0; j < G; ++j) u0[j] = next_u[j];
         }
         s_begin = s_end;
     }
@@ -398,7 +431,7 @@ inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) /
 // Number of tile-kernel variants (rows per tile: see launch_step), the
 // default one, and the default z-chunk.
 constexpr int kVariants = 4;
-constexpr int kDefaultVariant = 0;
+constexpr int kDefaultVariant = 1;
 constexpr uint32_t kDefaultZChunk = 4;
 
 
