@@ -169,11 +169,11 @@ def bench_room_mix(args):
     # One batch.
     t0 = time.perf_counter()
     b = fdtd.Batch(args.precision)
-    ids = [b.add(c.grid, taps, c.signal, c.receivers) for _, c, taps in inst]
+    ids = b.add_many([(c.grid, taps, c.signal, c.receivers) for _, c, taps in inst], threads=8)
     t_add = time.perf_counter() - t0
     t0 = time.perf_counter()
     if not args.no_autotune:
-        b.autotune(trial_steps=16)
+        b.autotune(trial_steps=8)
     t_tune = time.perf_counter() - t0
     t0 = time.perf_counter()
     b.run(graphs=not args.no_graphs)

@@ -148,8 +148,11 @@ wv_status wv_footprint_estimate(const wv_grid* grid, int precision, size_t steps
  * run.rs:645-661) of different grid sizes advanced together: one launch per
  * step covers every instance's tiles and 128-step blocks are replayed as HIP
  * graphs. Each instance runs the same arithmetic as wv_plan_run, so its
- * results equal that run's. The reference runs them one after another. A
- * batch is used by one thread at a time; inputs are copied at wv_batch_add.
+ * results equal that run's. The reference runs them one after another.
+ * wv_batch_add may be called from several threads at once (each instance is
+ * set up on its own stream); every other call on a batch is made by one
+ * thread at a time, with no wv_batch_add in progress. Inputs are copied at
+ * wv_batch_add.
  */
 typedef struct wv_batch wv_batch;
 wv_status wv_batch_create(int precision, wv_batch** out, char* err, size_t err_len);

@@ -42,8 +42,8 @@ using namespace wv;
 namespace {
 
 constexpr uint32_t kGraphSteps = kStepsPerBlock;  // steps per captured block
-constexpr uint32_t kTuneZChunks[] = {2, 4, 8};
-constexpr int kTuneVariants[] = {0, 1, 3};
+constexpr uint32_t kTuneZChunks[] = {4, 8};
+constexpr int kTuneVariants[] = {0, 1, 2};
 
 template <typename T, typename S>
 struct BatchEntry {
@@ -59,7 +59,7 @@ struct BatchEntry {
 // b - block0 of instance block_entry[b] (its tiles, then its receivers) at
 // step n0 + k. Instances past their last step return at once.
 template <typename T, typename S, int TY>
-__global__ __launch_bounds__(kTileX* TY) void k_batch_step(const BatchEntry<T, S>* __restrict__ entries,
+__global__ __launch_bounds__(kTileX*(TY + 1)) void k_batch_step(const BatchEntry<T, S>* __restrict__ entries,
                                                            const uint32_t* __restrict__ block_entry, uint32_t k) {
     const BatchEntry<T, S>& be = entries[block_entry[blockIdx.x]];
     const uint32_t n = be.n0 + k;
@@ -74,7 +74,7 @@ __global__ __launch_bounds__(kTileX* TY) void k_batch_step(const BatchEntry<T, S
     a.ir = be.a.ir + (size_t)n * a.n_recv;
     const uint32_t lb = blockIdx.x - be.block0;
     if (lb < a.main_blocks) {
-        tile_part<T, S, TY>(a, lb);
+        block_part<T, S, TY>(a, lb);  // the whole block takes this branch
         return;
     }
     // p^n before its update (fdtd.rs:673-683).
@@ -91,11 +91,12 @@ __global__ void k_batch_tick(BatchEntry<T, S>* __restrict__ entries, uint32_t n,
 template <typename T, typename S>
 void launch_batch_step(int variant, uint32_t grid, hipStream_t st, const BatchEntry<T, S>* e, const uint32_t* be,
                        uint32_t k) {
+    const dim3 blk(variant_threads(variant));
     switch (variant) {
-        case 0: hipLaunchKernelGGL((k_batch_step<T, S, 4>), dim3(grid), dim3(64 * 4), 0, st, e, be, k); break;
-        case 1: hipLaunchKernelGGL((k_batch_step<T, S, 8>), dim3(grid), dim3(64 * 8), 0, st, e, be, k); break;
-        case 2: hipLaunchKernelGGL((k_batch_step<T, S, 16>), dim3(grid), dim3(64 * 16), 0, st, e, be, k); break;
-        default: hipLaunchKernelGGL((k_batch_step<T, S, 2>), dim3(grid), dim3(64 * 2), 0, st, e, be, k); break;
+        case 0: hipLaunchKernelGGL((k_batch_step<T, S, 7>), dim3(grid), blk, 0, st, e, be, k); break;
+        case 1: hipLaunchKernelGGL((k_batch_step<T, S, 3>), dim3(grid), blk, 0, st, e, be, k); break;
+        case 2: hipLaunchKernelGGL((k_batch_step<T, S, 15>), dim3(grid), blk, 0, st, e, be, k); break;
+        default: hipLaunchKernelGGL((k_batch_step<T, S, 1>), dim3(grid), blk, 0, st, e, be, k); break;
     }
 }
 
@@ -165,6 +166,7 @@ struct wv_batch {
     double last_ms = 0.0;
     uint64_t last_steps = 0, last_cell_updates = 0;
     std::atomic<uint64_t> graph_launches{0}, captures{0};
+    std::mutex add_mu;  // wv_batch_add may be called from several threads at once
 
     void drop_graph() {
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -231,7 +233,7 @@ std::unique_ptr<Instance> make_instance(int prec, const wv_grid* g, const wv_tap
 // run. The stream must be idle.
 template <typename T, typename S>
 void build_table(wv_batch* b) {
-    const uint32_t ty = kVariantRows[b->variant];
+    const uint32_t ty = kVariantTiles[b->variant];
     std::vector<BatchEntry<T, S>> es;
     std::vector<uint32_t> be;
     b->table.clear();
@@ -249,7 +251,7 @@ void build_table(wv_batch* b) {
         e.steps = in->steps;
         e.n0 = in->n0;
         e.block0 = (uint32_t)be.size();
-        const uint32_t blocks = e.a.main_blocks + div_up(pl->n_recv, kTileX * ty);
+        const uint32_t blocks = e.a.main_blocks + div_up(pl->n_recv, variant_threads(b->variant));
         if ((uint64_t)be.size() + blocks >= (1ull << 31)) throw DevErr{"batch launch too large", WV_ERR_LIMIT};
         be.insert(be.end(), blocks, (uint32_t)es.size());
         es.push_back(e);
@@ -470,7 +472,6 @@ extern "C" wv_status wv_batch_add(wv_batch* b, const wv_grid* grid, const wv_tap
     if (s != WV_OK) return s;
     return guarded(err, err_len, [&]() -> wv_status {
         WV_HIP(hipSetDevice(b->device));
-        WV_HIP(hipStreamSynchronize(b->stream));
         // Argument checks against the grid's plan layout (tap planes, receivers).
         wv_plan probe;
         probe.nx = (uint32_t)grid->n[0];
@@ -482,6 +483,7 @@ extern "C" wv_status wv_batch_add(wv_batch* b, const wv_grid* grid, const wv_tap
         std::unique_ptr<Instance> in;
         if (b->prec == WV_FP32) in = make_instance<float>(b->prec, grid, taps, n_taps, signal, steps, receivers, n_recv);
         else in = make_instance<double>(b->prec, grid, taps, n_taps, signal, steps, receivers, n_recv);
+        std::lock_guard<std::mutex> lk(b->add_mu);
         in->id = (int32_t)b->inst.size();
         *id = in->id;
         b->inst.push_back(std::move(in));

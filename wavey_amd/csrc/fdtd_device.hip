@@ -488,9 +488,8 @@ void run_begin_impl(wv_plan* pl, const wv_tap* taps, size_t n_taps, const double
 
 template <class T, class S>
 wv_status advance_impl(wv_plan* pl, size_t k, const volatile int32_t* cancel, wv_progress_fn progress, void* ctx) {
-    const uint32_t ty_rows = kVariantRows[pl->variant];
-    StepParams<T, S> a = make_params<T, S>(pl, ty_rows);
-    const uint32_t grid = a.main_blocks + div_up(pl->n_recv, kTileX * ty_rows);
+    StepParams<T, S> a = make_params<T, S>(pl, kVariantTiles[pl->variant]);
+    const uint32_t grid = a.main_blocks + div_up(pl->n_recv, variant_threads(pl->variant));
     const size_t end = pl->done + k;
     T* A = pl->fa.as<T>();
     T* B = pl->fb.as<T>();

@@ -116,7 +116,7 @@ inline Layout layout_of(const wv_plan* pl) {
 }
 
 template <class T, class S>
-inline StepParams<T, S> make_params(wv_plan* pl, uint32_t ty_rows) {
+inline StepParams<T, S> make_params(wv_plan* pl, uint32_t tiles_per_block) {
     StepParams<T, S> a{};
     const Layout L = layout_of(pl);
     a.zbits = pl->zbits.as<uint32_t>();
@@ -126,7 +126,7 @@ inline StepParams<T, S> make_params(wv_plan* pl, uint32_t ty_rows) {
     a.pitch = pl->pitch;
     a.sz = pl->sz;
     a.n_tiles = L.n_tiles;
-    a.tile_blocks = div_up(L.n_tiles, ty_rows);
+    a.tile_blocks = div_up(L.n_tiles, tiles_per_block);
     a.zchunk = L.zchunk;
     a.main_blocks = a.tile_blocks * L.nzc;
     a.seg_base = pl->offs.as<uint32_t>();

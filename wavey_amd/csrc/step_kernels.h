@@ -16,7 +16,6 @@ namespace wv {
 constexpr uint32_t kBlock = 256;          // setup kernels and compaction blocks
 constexpr uint32_t kTileX = 64;           // one 64-lane wave per tile row
 constexpr uint32_t kStepsPerBlock = 128;  // cancel/progress granularity, as gpu_fdtd.rs:35
-constexpr uint32_t kSubPlanes = 4;        // planes per special-cell pass of a z-chunk
 
 // Special-cell record: {meta, first constant of its slot set}. Meta bits:
 // 0-5 air neighbours (-x, +x, -y, +y, -z, +z), 6 solid, 7 tap, 8-13 slot
@@ -286,21 +285,13 @@ __device__ __forceinline__ S special_update(const StepParams<T, S>& a, uint32_t 
 // and, on lanes 0 and 63, the cells next to the tile; x neighbours inside a
 // lane are in its registers and move across lanes by wave shifts. All of a
 // plane's loads are issued before any is used and the loop keeps few
-// registers live, so many waves per SIMD hide the memory latency. Interior
-// cells are written here; after each sub-chunk the wave updates that
-// sub-chunk's special cells, 64 at a time, while the rows they read are
-// still in cache.
-template <typename T, typename S, int TY>
-__device__ __forceinline__ void tile_part(const StepParams<T, S>& a, uint32_t t) {
+// registers live, so many waves per SIMD hide the memory latency. Only
+// interior cells are written here.
+template <typename T, typename S>
+__device__ __forceinline__ void tile_march(const StepParams<T, S>& a, uint32_t tp, uint32_t z0) {
     constexpr int W = Cells<T>::W;
     constexpr uint32_t BX = kTileX * W;
-    const uint32_t tb = t % a.tile_blocks;
-    const uint32_t zc = t / a.tile_blocks;
     const uint32_t lane = threadIdx.x & (kTileX - 1);
-    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kTileX);
-    const uint32_t tp = tb * TY + wave;  // tile of the plane
-    if (tp >= a.n_tiles) return;
-    const uint32_t w_seg = (zc * a.n_tiles + tp) * a.seg_stride;
     // Lanes past the plane (the last tile) re-read its last 16 bytes and
     // update nothing. Every neighbour offset below stays inside the field:
     // planes z0 - 1 .. z1 are within 0 .. nz - 1 and a plane has sz >= pitch
@@ -308,12 +299,7 @@ __device__ __forceinline__ void tile_part(const StepParams<T, S>& a, uint32_t t)
     const uint32_t fl = tp * BX + lane * W;  // first cell of this lane in the plane
     const bool live = fl < a.sz;
     const uint32_t col = live ? fl : a.sz - W;
-    const uint32_t z0 = 1 + zc * a.zchunk;
     const uint32_t z1 = min(z0 + a.zchunk, a.nz - 1);
-    const uint32_t ym_off = a.pitch;
-    const uint32_t yp_off = a.pitch;
-    const uint32_t hl_off = 1u;                // lane 0: the cell before the tile
-    const uint32_t hr_off = (uint32_t)W;       // lane 63: the cell after it
     const uint32_t full = (1u << W) - 1u;
     const uint32_t lane_mask = live ? full : 0u;
     // The chunk lies inside one 32-plane block of the z-packed bitmap
@@ -324,102 +310,116 @@ __device__ __forceinline__ void tile_part(const StepParams<T, S>& a, uint32_t t)
     Cells<T> zm, zcur;
     zm.load(a.p + (z0 - 1) * a.sz + col);
     zcur.load(a.p + z0 * a.sz + col);
-    // The chunk is marched in sub-chunks of kSubPlanes planes; after each,
-    // the wave updates that sub-chunk's special cells while the rows they
-    // read are still in cache. Records are sorted by plane, so a sub-chunk's
-    // records start at the compaction block of its first plane.
-    const uint32_t row0 = z0 * a.sz + tp * BX;
-    uint32_t s_begin = a.seg_base[w_seg];
-    for (uint32_t zs = z0; zs < z1; zs += kSubPlanes) {
-        const uint32_t ze = min(zs + kSubPlanes, z1);
-        const uint32_t s_end =
-            ze == z1 ? a.seg_base[w_seg + a.seg_stride] : a.seg_base[w_seg + (ze - z0) * BX / kBlock];
-        // The first special records and their first slot states (rows the
-        // state arrays have, whatever the cell's slot count) are requested
-        // now, so they have arrived when the sub-chunk's z-march ends.
-        constexpr uint32_t G = kSlotGroup<S>;
-        uint2 rec = make_uint2(0u, 0u);
-        S u0[G];
+    for (uint32_t z = z0; z < z1; ++z) {
+        const uint32_t i = z * a.sz + col;
+        T hl = 0, hr = 0;
+        if (lane == 0) hl = a.p[i - 1];                // the cell before the tile
+        if (lane == kTileX - 1) hr = a.p[i + W];       // the cell after it
+        Cells<T> zp, q, ym, yp;
+        zp.load(a.p + i + a.sz);
+        q.load(a.prev + i);
+        ym.load(a.p + i - a.pitch);
+        yp.load(a.p + i + a.pitch);
+        const T lo = from_lower_lane(zcur.v[W - 1], hl);
+        const T hi = from_upper_lane(zcur.v[0], hr);
+        Cells<T> out;
 #pragma unroll
-        for (uint32_t j = 0; j < G; ++j) u0[j] = S(0);
-        if (s_begin + lane < s_end) {
-            rec = a.sp_rec[s_begin + lane];
-#pragma unroll
-            for (uint32_t j = 0; j < G; ++j)
-                if (j < a.max_slots) u0[j] = a.st_u[j * a.sp_pad + s_begin + lane];
+        for (int e = 0; e < W; ++e) {
+            const T xm = e == 0 ? lo : zcur.v[e - 1];
+            const T xp = e == W - 1 ? hi : zcur.v[e + 1];
+            out.v[e] = interior_update(zcur.v[e], q.v[e], xm, xp, ym.v[e], yp.v[e], zm.v[e], zp.v[e], a.l2, a.c2);
         }
-        for (uint32_t z = zs; z < ze; ++z) {
-            const uint32_t i = z * a.sz + col;
-            T hl = 0, hr = 0;
-            if (lane == 0) hl = a.p[i - hl_off];
-            if (lane == kTileX - 1) hr = a.p[i + hr_off];
-            Cells<T> zp, q, ym, yp;
-            zp.load(a.p + i + a.sz);
-            q.load(a.prev + i);
-            ym.load(a.p + i - ym_off);
-            yp.load(a.p + i + yp_off);
-            const T lo = from_lower_lane(zcur.v[W - 1], hl);
-            const T hi = from_upper_lane(zcur.v[0], hr);
-            Cells<T> out;
+        const uint32_t bits = plane_bits<W>(ib, (z - 1) & 31u) & lane_mask;
+        if (bits == full) {
+            out.store(a.prev + i);
+        } else if (bits) {
+            // Row ends next to walls: only the interior cells.
 #pragma unroll
-            for (int e = 0; e < W; ++e) {
-                const T xm = e == 0 ? lo : zcur.v[e - 1];
-                const T xp = e == W - 1 ? hi : zcur.v[e + 1];
-                out.v[e] =
-                    interior_update(zcur.v[e], q.v[e], xm, xp, ym.v[e], yp.v[e], zm.v[e], zp.v[e], a.l2, a.c2);
-            }
-            const uint32_t bits = plane_bits<W>(ib, (z - 1) & 31u) & lane_mask;
-            if (bits == full) {
-                out.store(a.prev + i);
-            } else if (bits) {
-                // Row ends next to walls: only the interior cells.
-#pragma unroll
-                for (int e = 0; e < W; ++e)
-                    if ((bits >> e) & 1u) a.prev[i + e] = out.v[e];
-            }
-            zm = zcur;
-            zcur = zp;
+            for (int e = 0; e < W; ++e)
+                if ((bits >> e) & 1u) a.prev[i + e] = out.v[e];
         }
-
-        // The sub-chunk's special cells, 64 at a time. A record's position
-        // gives the cell; its neighbour loads are unconditional (a masked-out
-        // neighbour reads the cell itself, and the mask never points outside
-        // the grid).
-        for (uint32_t s = s_begin + lane; s < s_end; s += kTileX) {
-            uint2 next_rec = make_uint2(0u, 0u);
-            S next_u[G];
-#pragma unroll
-            for (uint32_t j = 0; j < G; ++j) next_u[j] = S(0);
-            if (s + kTileX < s_end) {
-                next_rec = a.sp_rec[s + kTileX];
-#pragma unroll
-                for (uint32_t j = 0; j < G; ++j)
-                    if (j < a.max_slots) next_u[j] = a.st_u[j * a.sp_pad + s + kTileX];
-            }
-            const uint32_t meta = rec.x;
-            const uint32_t pos = meta >> kMetaPosShift;
-            const uint32_t i = row0 + (pos / BX) * a.sz + pos % BX;
-            const T c = a.p[i];
-            const T q = a.prev[i];
-            const S nb[6] = {(S)a.p[i - ((meta & 1u) ? 1u : 0u)], (S)a.p[i + ((meta & 2u) ? 1u : 0u)],
-                             (S)a.p[i - ((meta & 4u) ? a.pitch : 0u)], (S)a.p[i + ((meta & 8u) ? a.pitch : 0u)],
-                             (S)a.p[i - ((meta & 16u) ? a.sz : 0u)], (S)a.p[i + ((meta & 32u) ? a.sz : 0u)]};
-            a.prev[i] = (T)special_update(a, meta, rec.y, s, (S)c, (S)q, nb, u0);
-            rec = next_rec;
-#pragma unroll
-            for (uint32_t j = 0; j < G; ++j) u0[j] = next_u[j];
-        }
-        s_begin = s_end;
+        zm = zcur;
+        zcur = zp;
     }
 }
 
+// Special cell s of a block (tiles tp0 .. tp0 + TY - 1, z-chunk from z0):
+// its tile is found from the block's per-tile record boundaries bnd[k] (first
+// record of tile tp0 + k), its cell from the position the record holds. The
+// neighbour loads are unconditional: a masked-out neighbour reads the cell
+// itself, and the mask never points outside the grid.
+template <typename T, typename S, int TY>
+__device__ __forceinline__ void special_one(const StepParams<T, S>& a, const uint32_t (&bnd)[TY + 1], uint32_t tp0,
+                                            uint32_t z0, uint32_t s) {
+    constexpr int W = Cells<T>::W;
+    constexpr uint32_t BX = kTileX * W;
+    constexpr uint32_t G = kSlotGroup<S>;
+    // The record and the states of its first G slot rows (rows the state
+    // arrays have, whatever the cell's slot count) are read together.
+    const uint2 rec = a.sp_rec[s];
+    S u0[G];
+#pragma unroll
+    for (uint32_t j = 0; j < G; ++j) u0[j] = j < a.max_slots ? a.st_u[j * a.sp_pad + s] : S(0);
+    uint32_t k = 0;
+#pragma unroll
+    for (int m = 1; m < TY; ++m) k += s >= bnd[m] ? 1u : 0u;
+    const uint32_t meta = rec.x;
+    const uint32_t pos = meta >> kMetaPosShift;
+    const uint32_t i = z0 * a.sz + (tp0 + k) * BX + (pos / BX) * a.sz + pos % BX;
+    const T c = a.p[i];
+    const T q = a.prev[i];
+    const S nb[6] = {(S)a.p[i - ((meta & 1u) ? 1u : 0u)], (S)a.p[i + ((meta & 2u) ? 1u : 0u)],
+                     (S)a.p[i - ((meta & 4u) ? a.pitch : 0u)], (S)a.p[i + ((meta & 8u) ? a.pitch : 0u)],
+                     (S)a.p[i - ((meta & 16u) ? a.sz : 0u)], (S)a.p[i + ((meta & 32u) ? a.sz : 0u)]};
+    a.prev[i] = (T)special_update(a, meta, rec.y, s, (S)c, (S)q, nb, u0);
+}
+
+// The special cells of a block's tiles and z-chunk: every wave of the block
+// takes batches of 64 records from a counter in LDS until none are left.
+template <typename T, typename S, int TY>
+__device__ __forceinline__ void special_share(const StepParams<T, S>& a, const uint32_t (&bnd)[TY + 1], uint32_t tp0,
+                                              uint32_t z0, uint32_t* handed) {
+    const uint32_t lane = threadIdx.x & (kTileX - 1);
+    const uint32_t n = bnd[TY] - bnd[0];
+    while (true) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(handed, kTileX);
+        base = __builtin_amdgcn_readfirstlane(base);
+        if (base >= n) break;
+        if (base + lane < n) special_one<T, S, TY>(a, bnd, tp0, z0, bnd[0] + base + lane);
+    }
+}
+
+// One block: TY tiles of the plane over one z-chunk, a wave each, and one
+// more wave that starts on the block's special cells at once; the tile
+// waves join it when their z-march ends, so special cells neither wait for
+// the z-march nor hold it up, and blocks with many of them (floors, walls)
+// share them among all their waves.
+template <typename T, typename S, int TY>
+__device__ __forceinline__ void block_part(const StepParams<T, S>& a, uint32_t t) {
+    __shared__ uint32_t handed;  // special records handed out so far
+    const uint32_t tb = t % a.tile_blocks;
+    const uint32_t zc = t / a.tile_blocks;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kTileX);
+    const uint32_t tp0 = tb * TY;
+    const uint32_t wbase = zc * a.n_tiles;
+    uint32_t bnd[TY + 1];
+#pragma unroll
+    for (int k = 0; k <= TY; ++k) bnd[k] = a.seg_base[(wbase + min(tp0 + k, a.n_tiles)) * a.seg_stride];
+    if (threadIdx.x == 0) handed = 0;
+    __syncthreads();
+    const uint32_t z0 = 1 + zc * a.zchunk;
+    if (wave < TY && tp0 + wave < a.n_tiles) tile_march<T, S>(a, tp0 + wave, z0);
+    special_share<T, S, TY>(a, bnd, tp0, z0, &handed);
+}
+
 // One time step: tile blocks, then receiver blocks (p^n before its update,
-// fdtd.rs:673-683). Consecutive tiles go to the 8 XCDs round-robin, so
+// fdtd.rs:673-683). Consecutive blocks go to the 8 XCDs round-robin, so
 // neighbouring tiles run at the same time and spread over all HBM stacks.
 template <typename T, typename S, int TY>
-__global__ __launch_bounds__(kTileX* TY) void k_step(StepParams<T, S> a) {
+__global__ __launch_bounds__(kTileX*(TY + 1)) void k_step(StepParams<T, S> a) {
     if (blockIdx.x < a.main_blocks) {
-        tile_part<T, S, TY>(a, blockIdx.x);
+        block_part<T, S, TY>(a, blockIdx.x);
         return;
     }
     const uint32_t r = (blockIdx.x - a.main_blocks) * blockDim.x + threadIdx.x;
@@ -428,23 +428,25 @@ __global__ __launch_bounds__(kTileX* TY) void k_step(StepParams<T, S> a) {
 
 inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
-// Number of tile-kernel variants (rows per tile: see launch_step), the
-// default one, and the default z-chunk.
+// Step-kernel variants: TY tiles per block plus the special-cell wave, so
+// blocks of 8, 4, 16 and 2 waves, which divide the 32 wave slots of a CU at
+// the kernel's register use (at most 64 VGPRs: 8 waves per SIMD). The
+// default variant and z-chunk.
 constexpr int kVariants = 4;
-constexpr int kDefaultVariant = 1;
+constexpr int kDefaultVariant = 0;
 constexpr uint32_t kDefaultZChunk = 4;
+constexpr uint32_t kVariantTiles[] = {7, 3, 15, 1};
 
-
-// Tile-kernel variants: TY rows per tile (block = 64 x TY threads).
-constexpr uint32_t kVariantRows[] = {4, 8, 16, 2};
+inline uint32_t variant_threads(int variant) { return kTileX * (kVariantTiles[variant] + 1); }
 
 template <class T, class S>
 void launch_step(int variant, uint32_t grid, hipStream_t st, const StepParams<T, S>& a) {
+    const dim3 blk(variant_threads(variant));
     switch (variant) {
-        case 0: hipLaunchKernelGGL((k_step<T, S, 4>), dim3(grid), dim3(64 * 4), 0, st, a); break;
-        case 1: hipLaunchKernelGGL((k_step<T, S, 8>), dim3(grid), dim3(64 * 8), 0, st, a); break;
-        case 2: hipLaunchKernelGGL((k_step<T, S, 16>), dim3(grid), dim3(64 * 16), 0, st, a); break;
-        default: hipLaunchKernelGGL((k_step<T, S, 2>), dim3(grid), dim3(64 * 2), 0, st, a); break;
+        case 0: hipLaunchKernelGGL((k_step<T, S, 7>), dim3(grid), blk, 0, st, a); break;
+        case 1: hipLaunchKernelGGL((k_step<T, S, 3>), dim3(grid), blk, 0, st, a); break;
+        case 2: hipLaunchKernelGGL((k_step<T, S, 15>), dim3(grid), blk, 0, st, a); break;
+        default: hipLaunchKernelGGL((k_step<T, S, 1>), dim3(grid), blk, 0, st, a); break;
     }
 }
 

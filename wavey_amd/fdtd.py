@@ -572,6 +572,14 @@ class Batch:
         self._shape[iid.value] = (len(rc), len(sig))
         return iid.value
 
+    def add_many(self, items, threads=8):
+        """add() for [(grid, taps, signal, receivers)], set up on several host
+        threads at once; returns the ids in the order of items."""
+        from concurrent.futures import ThreadPoolExecutor
+
+        with ThreadPoolExecutor(max_workers=max(1, int(threads))) as ex:
+            return list(ex.map(lambda it: self.add(*it), items))
+
     def autotune(self, trial_steps=16):
         err = C.create_string_buffer(1024)
         _check(lib().wv_batch_autotune(self._h, int(trial_steps), err, 1024), err)
