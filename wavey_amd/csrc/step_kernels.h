@@ -343,23 +343,17 @@ __device__ __forceinline__ void tile_march(const StepParams<T, S>& a, uint32_t t
     }
 }
 
-// Special cell s of a block (tiles tp0 .. tp0 + TY - 1, z-chunk from z0):
-// its tile is found from the block's per-tile record boundaries bnd[k] (first
-// record of tile tp0 + k), its cell from the position the record holds. The
-// neighbour loads are unconditional: a masked-out neighbour reads the cell
-// itself, and the mask never points outside the grid.
+// Special cell s of a block (tiles tp0 .. tp0 + TY - 1, z-chunk from z0),
+// given its record and the states of its first G slot rows: its tile is
+// found from the block's per-tile record boundaries bnd[k] (first record of
+// tile tp0 + k), its cell from the position the record holds. The neighbour
+// loads are unconditional: a masked-out neighbour reads the cell itself, and
+// the mask never points outside the grid.
 template <typename T, typename S, int TY>
 __device__ __forceinline__ void special_one(const StepParams<T, S>& a, const uint32_t (&bnd)[TY + 1], uint32_t tp0,
-                                            uint32_t z0, uint32_t s) {
+                                            uint32_t z0, uint32_t s, uint2 rec, const S (&u0)[kSlotGroup<S>]) {
     constexpr int W = Cells<T>::W;
     constexpr uint32_t BX = kTileX * W;
-    constexpr uint32_t G = kSlotGroup<S>;
-    // The record and the states of its first G slot rows (rows the state
-    // arrays have, whatever the cell's slot count) are read together.
-    const uint2 rec = a.sp_rec[s];
-    S u0[G];
-#pragma unroll
-    for (uint32_t j = 0; j < G; ++j) u0[j] = j < a.max_slots ? a.st_u[j * a.sp_pad + s] : S(0);
     uint32_t k = 0;
 #pragma unroll
     for (int m = 1; m < TY; ++m) k += s >= bnd[m] ? 1u : 0u;
@@ -374,19 +368,52 @@ __device__ __forceinline__ void special_one(const StepParams<T, S>& a, const uin
     a.prev[i] = (T)special_update(a, meta, rec.y, s, (S)c, (S)q, nb, u0);
 }
 
+// A batch of 64 special records: the record and the states of its first G
+// slot rows (rows the state arrays have, whatever the cell's slot count).
+template <typename S>
+struct SpecialBatch {
+    uint32_t base;
+    uint2 rec;
+    S u0[kSlotGroup<S>];
+    template <typename T>
+    __device__ __forceinline__ void load(const StepParams<T, S>& a, uint32_t first, uint32_t n, uint32_t lane) {
+        rec = make_uint2(0u, 0u);
+#pragma unroll
+        for (uint32_t j = 0; j < kSlotGroup<S>; ++j) u0[j] = S(0);
+        if (base + lane < n) {
+            const uint32_t s = first + base + lane;
+            rec = a.sp_rec[s];
+#pragma unroll
+            for (uint32_t j = 0; j < kSlotGroup<S>; ++j)
+                if (j < a.max_slots) u0[j] = a.st_u[j * a.sp_pad + s];
+        }
+    }
+};
+
 // The special cells of a block's tiles and z-chunk: every wave of the block
-// takes batches of 64 records from a counter in LDS until none are left.
+// takes batches of 64 records from a counter in LDS until none are left. A
+// wave claims and reads its next batch before it updates the current one, so
+// the record reads overlap the neighbour reads.
 template <typename T, typename S, int TY>
 __device__ __forceinline__ void special_share(const StepParams<T, S>& a, const uint32_t (&bnd)[TY + 1], uint32_t tp0,
                                               uint32_t z0, uint32_t* handed) {
     const uint32_t lane = threadIdx.x & (kTileX - 1);
-    const uint32_t n = bnd[TY] - bnd[0];
-    while (true) {
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(handed, kTileX);
-        base = __builtin_amdgcn_readfirstlane(base);
-        if (base >= n) break;
-        if (base + lane < n) special_one<T, S, TY>(a, bnd, tp0, z0, bnd[0] + base + lane);
+    const uint32_t first = bnd[0], n = bnd[TY] - bnd[0];
+    auto claim = [&]() -> uint32_t {
+        uint32_t b = 0;
+        if (lane == 0) b = atomicAdd(handed, kTileX);
+        return __builtin_amdgcn_readfirstlane(b);
+    };
+    SpecialBatch<S> cur;
+    cur.base = claim();
+    if (cur.base >= n) return;
+    cur.load(a, first, n, lane);
+    while (cur.base < n) {
+        SpecialBatch<S> nxt;
+        nxt.base = claim();
+        nxt.load(a, first, n, lane);
+        if (cur.base + lane < n) special_one<T, S, TY>(a, bnd, tp0, z0, first + cur.base + lane, cur.rec, cur.u0);
+        cur = nxt;
     }
 }
 
