@@ -129,6 +129,13 @@ wv_status wv_plan_stats(const wv_plan* plan, uint64_t out[7]);
  * each row 256 FP32 or 128 FP64 cells). No reference counterpart: the
  * autotuner's knobs. */
 wv_status wv_plan_set_tile(wv_plan* plan, uint32_t zchunk, int32_t variant);
+/* Steps per pass over the fields: 1, or 2 (two steps per z-march: deep
+ * interior cells advance twice in one pass, the cells next to boundaries in
+ * a second, sparse launch; four field buffers); 0 = default. pair_zchunk:
+ * z planes per block of a two-step pass (0 = default, else a power of two up
+ * to 32). Results are bit-identical either way. Takes effect at the next
+ * wv_run_begin. No reference counterpart: a tuning knob. */
+wv_status wv_plan_set_pass(wv_plan* plan, uint32_t steps_per_pass, uint32_t pair_zchunk);
 /* Copies `bytes` (at most one field buffer) between the plan's field
  * buffers `reps` times with a plain streaming kernel and returns the mean
  * time per copy. It overwrites the fields: call it outside a run. No

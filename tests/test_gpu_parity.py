@@ -104,3 +104,33 @@ def test_interior_variants_agree(pillared_mini, variant):
             plan.set_tile(zchunk=16, variant=variant)
             ir, _ = plan.run(taps, case.signal, case.receivers, mode=fdtd.MODE_IR)
         assert rel_l2(ir, refs[1]) < TOL[prec]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "fp32_state64"])
+def test_two_step_passes_bit_identical(pillared_mini, prec):
+    # Two steps per pass (deep cells in one z-march, the rest in a sparse
+    # second launch) compute every cell with the single-step arithmetic:
+    # the receiver series must be identical, also with an odd step count
+    # (the last step runs alone) and for every pair z-chunk.
+    case, refs = pillared_mini
+    _, ci, taps = case.runs[2]
+    sig = case.signal[:1499]
+    print(f"case: pillared room mini, two-step passes, {prec}", flush=True)
+    with fdtd.Plan(case.grid, prec) as plan:
+        plan.set_pass(1)
+        one, _ = plan.run(taps, sig, case.receivers, mode=fdtd.MODE_IR)
+        for zc in (8, 16, 32):
+            plan.set_pass(2, zc)
+            two, _ = plan.run(taps, sig, case.receivers, mode=fdtd.MODE_IR)
+            assert np.array_equal(one, two), f"pair z-chunk {zc}"
+    assert rel_l2(one, refs[2][:, :1499]) < TOL[prec]
+
+
+def test_two_step_spectra_and_shoebox(shoebox):
+    case, ref, spec = shoebox
+    freqs = [31.25, 62.5, 125.0, 250.0]
+    with fdtd.Plan(case.grid, "fp64") as plan:
+        plan.set_pass(2)
+        ir, sp = plan.run(case.runs[0][2], case.signal, case.receivers, freqs, mode=fdtd.MODE_BOTH)
+    assert rel_l2(ir, ref) < TOL["fp64"]
+    assert np.max(np.abs(sp - spec)) / np.max(np.abs(spec)) < 1e-9

@@ -214,6 +214,7 @@ std::unique_ptr<Instance> make_instance(int prec, const wv_grid* g, const wv_tap
     auto in = std::make_unique<Instance>();
     in->plan = std::make_unique<wv_plan>();
     in->plan->prec = prec;
+    in->plan->steps_per_pass = 1;  // the batched launch is single-step
     plan_setup(in->plan.get(), g);
     in->taps.assign(taps, taps + n_taps);
     in->signal.assign(signal, signal + steps);

@@ -77,22 +77,88 @@ __device__ __forceinline__ Cls classify(const uint8_t* __restrict__ flags, const
     return c;
 }
 
-// The z-packed interior bitmap: word (kb, y, x) holds the bits of planes
-// 1 + 32 kb .. 32 + 32 kb of column (x, y), bit k for plane 1 + 32 kb + k.
+__device__ __forceinline__ bool is_interior(const uint8_t* __restrict__ flags, const uint32_t* __restrict__ force,
+                                            uint32_t x, uint32_t y, uint32_t z, const Geo& g) {
+    return classify(flags, force, x, y, z, g).interior;
+}
+
+// A deep cell is interior and so are its six neighbours: its second step of
+// a two-step pass needs no special cell's value.
+__device__ __forceinline__ bool is_deep(const uint8_t* __restrict__ flags, const uint32_t* __restrict__ force,
+                                        uint32_t x, uint32_t y, uint32_t z, const Geo& g) {
+    return is_interior(flags, force, x, y, z, g) && is_interior(flags, force, x - 1, y, z, g) &&
+           is_interior(flags, force, x + 1, y, z, g) && is_interior(flags, force, x, y - 1, z, g) &&
+           is_interior(flags, force, x, y + 1, z, g) && is_interior(flags, force, x, y, z - 1, g) &&
+           is_interior(flags, force, x, y, z + 1, g);
+}
+
+// The z-packed interior and deep bitmaps: word (kb, y, x) holds the bits of
+// planes 1 + 32 kb .. 32 + 32 kb of column (x, y), bit k for plane
+// 1 + 32 kb + k. (Coordinates 0 - 1 wrap to large values, which classify
+// treats as outside the grid.)
 __global__ __launch_bounds__(kBlock) void k_zbits(const uint8_t* __restrict__ flags, const uint32_t* __restrict__ force,
-                                                  uint32_t* __restrict__ zbits, Geo g, uint32_t nzb) {
+                                                  uint32_t* __restrict__ zbits, uint32_t* __restrict__ dbits, Geo g,
+                                                  uint32_t nzb) {
     const uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     if (t >= (uint64_t)nzb * g.ny * g.pitch) return;
     const uint32_t x = (uint32_t)(t % g.pitch);
     const uint64_t r = t / g.pitch;
     const uint32_t y = (uint32_t)(r % g.ny), kb = (uint32_t)(r / g.ny);
-    uint32_t wi = 0;
+    uint32_t wi = 0, wd = 0;
     for (uint32_t k = 0; k < 32; ++k) {
         const uint32_t z = 1 + kb * 32 + k;
         if (z + 1 >= g.nz) break;
-        wi |= (uint32_t)classify(flags, force, x, y, z, g).interior << k;
+        wi |= (uint32_t)is_interior(flags, force, x, y, z, g) << k;
+        wd |= (uint32_t)is_deep(flags, force, x, y, z, g) << k;
     }
     zbits[t] = wi;
+    dbits[t] = wd;
+}
+
+// Ring cells (interior, not deep) in pitched order, for the second half of a
+// two-step pass: pass 1 counts them per 256 cells of planes 1 .. nz-2, pass 2
+// writes their pitched indices.
+__device__ __forceinline__ bool ring_at(const uint8_t* __restrict__ flags, const uint32_t* __restrict__ force,
+                                        uint64_t q, const Geo& g, uint64_t end) {
+    if (q >= end) return false;
+    const uint32_t x = (uint32_t)(q % g.pitch);
+    const uint64_t r = q / g.pitch;
+    const uint32_t y = (uint32_t)(r % g.ny), z = (uint32_t)(r / g.ny);
+    return is_interior(flags, force, x, y, z, g) && !is_deep(flags, force, x, y, z, g);
+}
+
+__global__ __launch_bounds__(kBlock) void k_ring_count(const uint8_t* __restrict__ flags,
+                                                       const uint32_t* __restrict__ force,
+                                                       uint32_t* __restrict__ counts, Geo g) {
+    __shared__ uint32_t wc[kBlock / 64];
+    const uint64_t q = (uint64_t)g.pitch * g.ny + (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    const bool rg = ring_at(flags, force, q, g, (uint64_t)g.pitch * g.ny * (g.nz - 1));
+    const uint64_t b = __ballot(rg);
+    if ((threadIdx.x & 63u) == 0) wc[threadIdx.x >> 6] = (uint32_t)__popcll(b);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t s = 0;
+        for (uint32_t w = 0; w < kBlock / 64; ++w) s += wc[w];
+        counts[blockIdx.x] = s;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_ring_write(const uint8_t* __restrict__ flags,
+                                                       const uint32_t* __restrict__ force,
+                                                       const uint32_t* __restrict__ offs, Geo g,
+                                                       uint32_t* __restrict__ ring_cell) {
+    __shared__ uint32_t wc[kBlock / 64];
+    const uint64_t q = (uint64_t)g.pitch * g.ny + (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    const bool rg = ring_at(flags, force, q, g, (uint64_t)g.pitch * g.ny * (g.nz - 1));
+    const uint64_t b = __ballot(rg);
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    if (lane == 0) wc[wave] = (uint32_t)__popcll(b);
+    __syncthreads();
+    if (!rg) return;
+    uint32_t rank = offs[blockIdx.x];
+    for (uint32_t w = 0; w < wave; ++w) rank += wc[w];
+    rank += (uint32_t)__popcll(b & ((1ull << lane) - 1ull));
+    ring_cell[rank] = (uint32_t)q;
 }
 
 // The wave order of special cells: wave w = zc * n_tiles + tp (tile tp of
@@ -186,7 +252,7 @@ __global__ __launch_bounds__(kBlock) void k_sp_write(const uint8_t* __restrict__
                                                      const uint32_t* __restrict__ set_off,
                                                      const uint32_t* __restrict__ tap_cell, const S* __restrict__ tap_w,
                                                      uint32_t n_tap, uint2* __restrict__ sp_rec,
-                                                     S* __restrict__ sp_tapw) {
+                                                     S* __restrict__ sp_tapw, uint32_t* __restrict__ sp_cell) {
     __shared__ uint32_t wc[kBlock / 64];
     const uint64_t qw = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     uint32_t x = 0, y = 0, z = 0;
@@ -218,6 +284,7 @@ __global__ __launch_bounds__(kBlock) void k_sp_write(const uint8_t* __restrict__
         w = tap_w[t];
     }
     sp_rec[rank] = make_uint2(meta, c0);
+    sp_cell[rank] = (uint32_t)(((uint64_t)z * g.ny + y) * g.pitch + x);
     sp_tapw[rank] = w;
 }
 
@@ -383,6 +450,7 @@ void wv::plan_setup(wv_plan* pl, const wv_grid* g) {
     WV_HIP(hipGetLastError());
     pl->nzb = (pl->nz - 2 + 31) / 32;
     pl->zbits.alloc((size_t)pl->nzb * pl->ny * pl->pitch * 4);
+    pl->dbits.alloc((size_t)pl->nzb * pl->ny * pl->pitch * 4);
     pl->fa.alloc(pl->npitched * pl->tsize());
     pl->fb.alloc(pl->npitched * pl->tsize());
     WV_HIP(hipStreamSynchronize(pl->stream));
@@ -425,7 +493,7 @@ void run_begin_impl(wv_plan* pl, const wv_tap* taps, size_t n_taps, const double
     const Geo geo = geo_of(pl);
     hipLaunchKernelGGL(k_zbits, dim3(div_up((uint64_t)pl->nzb * pl->ny * pl->pitch, kBlock)), dim3(kBlock), 0,
                        pl->stream, pl->flags.as<uint8_t>(), pl->force.as<uint32_t>(), pl->zbits.as<uint32_t>(),
-                       geo, pl->nzb);
+                       pl->dbits.as<uint32_t>(), geo, pl->nzb);
     // Special cells in wave order.
     pl->run_zchunk = pl->zchunk ? pl->zchunk : kDefaultZChunk;
     const Layout L = layout_of(pl);
@@ -448,12 +516,40 @@ void run_begin_impl(wv_plan* pl, const wv_tap* taps, size_t n_taps, const double
     pl->sp_pad = (n_special + 63u) & ~63u;
     pl->sp_rec.alloc((size_t)n_special * 8);
     pl->sp_tapw.alloc((size_t)n_special * sizeof(S));
+    pl->sp_cell.alloc((size_t)n_special * 4);
     hipLaunchKernelGGL(k_sp_write<S>, dim3((uint32_t)nblocks), dim3(kBlock), 0, pl->stream, pl->flags.as<uint8_t>(),
                        pl->force.as<uint32_t>(), pl->offs.as<uint32_t>(), geo, wo, pl->b_cell_sorted.as<uint32_t>(),
                        pl->b_set_sorted.as<uint32_t>(), (uint32_t)pl->nb, pl->set_n.as<uint32_t>(),
                        pl->set_off.as<uint32_t>(), pl->tap_cell.as<uint32_t>(), pl->tap_w.as<S>(), (uint32_t)tc.size(),
-                       pl->sp_rec.as<uint2>(), pl->sp_tapw.as<S>());
+                       pl->sp_rec.as<uint2>(), pl->sp_tapw.as<S>(), pl->sp_cell.as<uint32_t>());
     WV_HIP(hipGetLastError());
+    // Two-step passes: the ring list and the two more field buffers.
+    pl->run_steps_per_pass = pl->steps_per_pass ? pl->steps_per_pass : kDefaultStepsPerPass;
+    pl->n_ring = 0;
+    if (pl->run_steps_per_pass == 2) {
+        const uint64_t span = (uint64_t)pl->sz * (pl->nz - 2);
+        const uint64_t rblocks = (span + kBlock - 1) / kBlock;
+        if (rblocks >= (1ull << 31)) throw DevErr{"grid too large for the ring layout", WV_ERR_LIMIT};
+        pl->ring_counts.alloc(rblocks * 4);
+        pl->ring_offs.alloc((rblocks + 1) * 4);
+        hipLaunchKernelGGL(k_ring_count, dim3((uint32_t)rblocks), dim3(kBlock), 0, pl->stream, pl->flags.as<uint8_t>(),
+                           pl->force.as<uint32_t>(), pl->ring_counts.as<uint32_t>(), geo);
+        hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, pl->stream, pl->ring_counts.as<uint32_t>(),
+                           pl->ring_offs.as<uint32_t>(), (uint32_t)rblocks, pl->ring_offs.as<uint32_t>() + rblocks);
+        WV_HIP(hipGetLastError());
+        uint32_t n_ring = 0;
+        WV_HIP(hipMemcpyAsync(&n_ring, pl->ring_offs.as<uint32_t>() + rblocks, 4, hipMemcpyDeviceToHost, pl->stream));
+        WV_HIP(hipStreamSynchronize(pl->stream));
+        pl->n_ring = n_ring;
+        pl->ring_cell.alloc((size_t)n_ring * 4);
+        hipLaunchKernelGGL(k_ring_write, dim3((uint32_t)rblocks), dim3(kBlock), 0, pl->stream, pl->flags.as<uint8_t>(),
+                           pl->force.as<uint32_t>(), pl->ring_offs.as<uint32_t>(), geo, pl->ring_cell.as<uint32_t>());
+        WV_HIP(hipGetLastError());
+        pl->fc.alloc(pl->npitched * pl->tsize());
+        pl->fd.alloc(pl->npitched * pl->tsize());
+    }
+    pl->buf_p = 0;
+    pl->buf_q = 1;
     // Slot state, slot-major: state[j][rank].
     const size_t nstate = (size_t)pl->max_slots * pl->sp_pad;
     pl->st_u.alloc(nstate * pl->ssize());
@@ -467,13 +563,14 @@ void run_begin_impl(wv_plan* pl, const wv_tap* taps, size_t n_taps, const double
     upload(pl, pl->recv, rc);
     pl->ir.alloc(std::max<size_t>(1, steps * n_recv) * pl->tsize());
     // Zeroed fields, state and records.
+    const bool four = pl->run_steps_per_pass == 2;
     if (pl->prec == WV_FP64) {
-        fill_async<double>(pl, pl->fa, pl->npitched);
-        fill_async<double>(pl, pl->fb, pl->npitched);
+        for (DBuf* f : {&pl->fa, &pl->fb, &pl->fc, &pl->fd})
+            if (f == &pl->fa || f == &pl->fb || four) fill_async<double>(pl, *f, pl->npitched);
         fill_async<double>(pl, pl->ir, steps * n_recv);
     } else {
-        fill_async<float>(pl, pl->fa, pl->npitched);
-        fill_async<float>(pl, pl->fb, pl->npitched);
+        for (DBuf* f : {&pl->fa, &pl->fb, &pl->fc, &pl->fd})
+            if (f == &pl->fa || f == &pl->fb || four) fill_async<float>(pl, *f, pl->npitched);
         fill_async<float>(pl, pl->ir, steps * n_recv);
     }
     fill_async<S>(pl, pl->st_u, nstate);
@@ -486,13 +583,44 @@ void run_begin_impl(wv_plan* pl, const wv_tap* taps, size_t n_taps, const double
     pl->active = true;
 }
 
+// Parameters of a two-step pass: pass 1 (tiles of deep cells, special
+// cells' step n, receivers' p^n) and pass 2 (ring and special cells' step
+// n+1, receivers' p^{n+1}).
+template <class T, class S>
+PairParams<T, S> make_pair_params(wv_plan* pl, const StepParams<T, S>& base) {
+    PairParams<T, S> pp{};
+    pp.a = base;
+    constexpr uint32_t BX = kTileX * kPairLane<T>;
+    const uint32_t zch = pl->pair_zchunk ? pl->pair_zchunk : kDefaultPairZChunk;
+    pp.a.zchunk = zch;
+    pp.dbits = pl->dbits.as<uint32_t>();
+    pp.sp_cell = pl->sp_cell.as<uint32_t>();
+    pp.ring_cell = pl->ring_cell.as<uint32_t>();
+    pp.n_special = pl->n_special;
+    pp.n_ring = pl->n_ring;
+    pp.tiles_x = div_up(pl->pitch, BX);
+    pp.row_blocks = div_up(pl->ny, kPairRows);
+    pp.tile_blocks = pp.tiles_x * pp.row_blocks * div_up(pl->nz - 2, zch);
+    pp.special_blocks = div_up(pl->n_special, kPairThreads);
+    return pp;
+}
+
 template <class T, class S>
 wv_status advance_impl(wv_plan* pl, size_t k, const volatile int32_t* cancel, wv_progress_fn progress, void* ctx) {
     StepParams<T, S> a = make_params<T, S>(pl, kVariantTiles[pl->variant]);
     const uint32_t grid = a.main_blocks + div_up(pl->n_recv, variant_threads(pl->variant));
+    const bool pairs = pl->run_steps_per_pass == 2;
+    PairParams<T, S> pp{};
+    uint32_t grid1 = 0, items2 = 0, grid2 = 0;
+    if (pairs) {
+        pp = make_pair_params<T, S>(pl, a);
+        grid1 = pp.tile_blocks + pp.special_blocks + div_up(pl->n_recv, kPairThreads);
+        items2 = div_up((uint64_t)pl->n_special + pl->n_ring, 256);
+        grid2 = items2 + div_up(pl->n_recv, 256);
+    }
+    T* bufs[4] = {pl->fa.as<T>(), pl->fb.as<T>(), pl->fc.as<T>(), pl->fd.as<T>()};
+    auto sig = [&](size_t n) -> S { return (S)(sizeof(S) == 4 ? (double)(float)pl->signal[n] : pl->signal[n]); };
     const size_t end = pl->done + k;
-    T* A = pl->fa.as<T>();
-    T* B = pl->fb.as<T>();
     T* ir = pl->ir.as<T>();
     uint64_t launches = 0;
     int blk = 0;
@@ -502,14 +630,48 @@ wv_status advance_impl(wv_plan* pl, size_t k, const volatile int32_t* cancel, wv
     while (pl->done < end) {
         if (cancel && *cancel) { st = WV_CANCELLED; break; }
         const size_t stop = std::min(end, pl->done + kStepsPerBlock);
-        for (size_t n = pl->done; n < stop; ++n) {
-            // Even steps: p = A, prev = B; odd steps swap (gpu_fdtd.rs:242-243).
-            a.p = (n & 1) ? B : A;
-            a.prev = (n & 1) ? A : B;
-            a.s_n = (S)(sizeof(S) == 4 ? (double)(float)pl->signal[n] : pl->signal[n]);
-            a.ir = ir + n * pl->n_recv;
-            if (grid) launch_step<T, S>(pl->variant, grid, pl->stream, a);
-            ++launches;
+        size_t n = pl->done;
+        while (n < stop) {
+            T* P = bufs[pl->buf_p];
+            T* Q = bufs[pl->buf_q];
+            if (pairs && n + 2 <= stop) {
+                // Steps n, n+1 in two launches; outputs go to the two free
+                // buffers, which then hold p (p^{n+2}) and prev (p^{n+1}).
+                int o1 = 0;
+                while (o1 == pl->buf_p || o1 == pl->buf_q) ++o1;
+                int o2 = o1 + 1;
+                while (o2 == pl->buf_p || o2 == pl->buf_q) ++o2;
+                pp.a.p = P;
+                pp.a.prev = Q;
+                pp.a.s_n = sig(n);
+                pp.a.ir = ir + n * pl->n_recv;
+                pp.p1 = bufs[o1];
+                pp.p2 = bufs[o2];
+                hipLaunchKernelGGL((k_pair1<T, S>), dim3(grid1), dim3(kPairThreads), 0, pl->stream, pp);
+                StepParams<T, S> a2 = a;
+                a2.p = bufs[o1];
+                a2.prev = P;
+                a2.s_n = sig(n + 1);
+                a2.ir = ir + (n + 1) * pl->n_recv;
+                hipLaunchKernelGGL((k_pair2<T, S>), dim3(grid2), dim3(256), 0, pl->stream, a2, bufs[o2],
+                                   pl->sp_cell.as<uint32_t>(), pl->ring_cell.as<uint32_t>(), pl->n_special,
+                                   pl->n_ring, items2);
+                pl->buf_p = o2;
+                pl->buf_q = o1;
+                launches += 2;
+                n += 2;
+            } else {
+                // One step, updating prev in place (the reference swaps
+                // buffers every step, gpu_fdtd.rs:242-243).
+                a.p = P;
+                a.prev = Q;
+                a.s_n = sig(n);
+                a.ir = ir + n * pl->n_recv;
+                if (grid) launch_step<T, S>(pl->variant, grid, pl->stream, a);
+                std::swap(pl->buf_p, pl->buf_q);
+                ++launches;
+                ++n;
+            }
         }
         WV_HIP(hipGetLastError());
         WV_HIP(hipEventRecord(pl->ev[blk], pl->stream));
@@ -529,7 +691,7 @@ wv_status advance_impl(wv_plan* pl, size_t k, const volatile int32_t* cancel, wv
     float ms = 0.f;
     WV_HIP(hipEventElapsedTime(&ms, pl->t0, pl->t1));
     pl->last_ms = ms;
-    pl->last_steps = launches;
+    pl->last_steps = k;
     pl->last_launches = launches;
     return st;
 }
@@ -617,9 +779,11 @@ uint64_t wv::estimate_bytes(const wv_grid* g, int prec, size_t steps, size_t n_r
     const uint64_t special = g->n_boundary + n_taps;
     const uint64_t sp_pad = (special + 63) & ~63ull;
     const uint64_t slots = g->n_boundary ? g->b_start[g->n_boundary] : 0;
+    const uint64_t fields = kDefaultStepsPerPass == 2 ? 4 : 2;
     uint64_t t = cells;                              // flags
-    t += npitched / 4 + npitched / 8;                // boundary marks (two copies), interior bitmap
-    t += 2 * npitched * tsz;                         // fields
+    t += npitched / 4 + npitched / 4;                // boundary marks (two copies), interior and deep bitmaps
+    t += fields * npitched * tsz;                    // fields
+    t += fields == 4 ? 2 * 4 * special : 0;          // special cells' indices and the ring list (about as long)
     t += npitched / 16 + 1024;                       // compaction counts and offsets (z-chunk >= 2)
     t += 8 * g->n_boundary + slots * (4 * ssz + 8);  // sorted boundary, slot-set table (at most one set per node)
     t += special * (8 + ssz) + 2 * (uint64_t)max_slots * sp_pad * ssz;  // records, tap weights, slot state
@@ -681,6 +845,14 @@ extern "C" wv_status wv_plan_set_tile(wv_plan* plan, uint32_t zchunk, int32_t va
     if (!plan || zchunk > 32 || (zchunk & (zchunk - 1)) || variant >= kVariants) return WV_ERR_INVALID;
     plan->zchunk = zchunk;
     plan->variant = variant < 0 ? kDefaultVariant : variant;
+    return WV_OK;
+}
+
+extern "C" wv_status wv_plan_set_pass(wv_plan* plan, uint32_t steps_per_pass, uint32_t pair_zchunk) {
+    // Pair z-chunks must divide the 32-plane blocks of the bitmaps.
+    if (!plan || steps_per_pass > 2 || pair_zchunk > 32 || (pair_zchunk & (pair_zchunk - 1))) return WV_ERR_INVALID;
+    plan->steps_per_pass = steps_per_pass;
+    plan->pair_zchunk = pair_zchunk;
     return WV_OK;
 }
 

@@ -8,6 +8,7 @@
 #include <string>
 #include <vector>
 
+#include "pair_kernels.h"
 #include "step_kernels.h"
 #include "wavey_fdtd.h"
 #include "wv_internal.h"
@@ -67,13 +68,21 @@ struct wv_plan {
     int variant = wv::kDefaultVariant;
     uint32_t max_slots = 0, n_sets = 0;
 
-    DBuf flags, force_base, force, zbits, counts, offs;
+    DBuf flags, force_base, force, zbits, dbits, counts, offs;
     DBuf b_cell_sorted, b_set_sorted;
     DBuf set_n, set_off, set_inv_d, set_cv, set_k, set_w, st_u, st_g;
-    DBuf fa, fb;
+    DBuf fa, fb, fc, fd;  // fields; fc, fd only for two-step passes
     DBuf sp_rec, sp_tapw;
     DBuf tap_cell, tap_w, recv, ir, freqs, spec;
     uint32_t n_special = 0, sp_pad = 0, seg_stride = 0;
+    // Two-step passes: steps per pass (1 or 2; 0 = default), their z-chunk,
+    // the special cells' pitched indices and the ring (interior, not deep).
+    uint32_t steps_per_pass = 0, run_steps_per_pass = 1;
+    uint32_t pair_zchunk = 0;
+    DBuf sp_cell, ring_cell, ring_counts, ring_offs;
+    uint32_t n_ring = 0;
+    // Which field buffer (0..3: fa, fb, fc, fd) holds p and which prev.
+    int buf_p = 0, buf_q = 1;
 
     // Current run.
     bool active = false;

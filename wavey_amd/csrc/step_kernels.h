@@ -111,23 +111,39 @@ __device__ __forceinline__ T from_upper_lane(T v, T fill) {
     }
 }
 
-// W consecutive cells of a row per lane, moved as one 16-byte access.
-template <typename T>
+// W consecutive cells of a row per lane, moved as one access of W * sizeof(T)
+// bytes (16 by default: 4 FP32 or 2 FP64 cells).
+template <typename T, int Wd = 16 / (int)sizeof(T)>
 struct Cells {
-    static constexpr int W = 16 / sizeof(T);
+    static constexpr int W = Wd;
     T v[W];
     __device__ __forceinline__ void load(const T* __restrict__ p) {
-        if constexpr (W == 4) {
-            const float4 q = *reinterpret_cast<const float4*>(p);
-            v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
-        } else {
-            const double2 q = *reinterpret_cast<const double2*>(p);
+        if constexpr (W * sizeof(T) == 16) {
+            if constexpr (sizeof(T) == 4) {
+                const float4 q = *reinterpret_cast<const float4*>(p);
+                v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+            } else {
+                const double2 q = *reinterpret_cast<const double2*>(p);
+                v[0] = q.x; v[1] = q.y;
+            }
+        } else if constexpr (W * sizeof(T) == 8 && sizeof(T) == 4) {
+            const float2 q = *reinterpret_cast<const float2*>(p);
             v[0] = q.x; v[1] = q.y;
+        } else {
+#pragma unroll
+            for (int e = 0; e < W; ++e) v[e] = p[e];
         }
     }
     __device__ __forceinline__ void store(T* __restrict__ p) const {
-        if constexpr (W == 4) *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
-        else *reinterpret_cast<double2*>(p) = make_double2(v[0], v[1]);
+        if constexpr (W * sizeof(T) == 16) {
+            if constexpr (sizeof(T) == 4) *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+            else *reinterpret_cast<double2*>(p) = make_double2(v[0], v[1]);
+        } else if constexpr (W * sizeof(T) == 8 && sizeof(T) == 4) {
+            *reinterpret_cast<float2*>(p) = make_float2(v[0], v[1]);
+        } else {
+#pragma unroll
+            for (int e = 0; e < W; ++e) p[e] = v[e];
+        }
     }
 };
 
@@ -136,9 +152,12 @@ __device__ __forceinline__ void load_words(const uint32_t* __restrict__ p, uint3
     if constexpr (W == 4) {
         const uint4 q = *reinterpret_cast<const uint4*>(p);
         w[0] = q.x; w[1] = q.y; w[2] = q.z; w[3] = q.w;
-    } else {
+    } else if constexpr (W == 2) {
         const uint2 q = *reinterpret_cast<const uint2*>(p);
         w[0] = q.x; w[1] = q.y;
+    } else {
+#pragma unroll
+        for (int e = 0; e < W; ++e) w[e] = p[e];
     }
 }
 
@@ -346,9 +365,23 @@ __device__ __forceinline__ void tile_march(const StepParams<T, S>& a, uint32_t t
 // Special cell s of a block (tiles tp0 .. tp0 + TY - 1, z-chunk from z0),
 // given its record and the states of its first G slot rows: its tile is
 // found from the block's per-tile record boundaries bnd[k] (first record of
-// tile tp0 + k), its cell from the position the record holds. The neighbour
-// loads are unconditional: a masked-out neighbour reads the cell itself, and
-// the mask never points outside the grid.
+// tile tp0 + k), its cell from the position the record holds.
+// Special cell of record s at pitched index i: p from a.p, the cell's old
+// value from a.prev, the result written to out[i]. The neighbour loads are
+// unconditional: a masked-out neighbour reads the cell itself, and the mask
+// never points outside the grid.
+template <typename T, typename S>
+__device__ __forceinline__ void special_at(const StepParams<T, S>& a, uint32_t s, uint32_t i, uint2 rec,
+                                           const S (&u0)[kSlotGroup<S>], T* __restrict__ out) {
+    const uint32_t meta = rec.x;
+    const T c = a.p[i];
+    const T q = a.prev[i];
+    const S nb[6] = {(S)a.p[i - ((meta & 1u) ? 1u : 0u)], (S)a.p[i + ((meta & 2u) ? 1u : 0u)],
+                     (S)a.p[i - ((meta & 4u) ? a.pitch : 0u)], (S)a.p[i + ((meta & 8u) ? a.pitch : 0u)],
+                     (S)a.p[i - ((meta & 16u) ? a.sz : 0u)], (S)a.p[i + ((meta & 32u) ? a.sz : 0u)]};
+    out[i] = (T)special_update(a, meta, rec.y, s, (S)c, (S)q, nb, u0);
+}
+
 template <typename T, typename S, int TY>
 __device__ __forceinline__ void special_one(const StepParams<T, S>& a, const uint32_t (&bnd)[TY + 1], uint32_t tp0,
                                             uint32_t z0, uint32_t s, uint2 rec, const S (&u0)[kSlotGroup<S>]) {
@@ -357,15 +390,9 @@ __device__ __forceinline__ void special_one(const StepParams<T, S>& a, const uin
     uint32_t k = 0;
 #pragma unroll
     for (int m = 1; m < TY; ++m) k += s >= bnd[m] ? 1u : 0u;
-    const uint32_t meta = rec.x;
-    const uint32_t pos = meta >> kMetaPosShift;
+    const uint32_t pos = rec.x >> kMetaPosShift;
     const uint32_t i = z0 * a.sz + (tp0 + k) * BX + (pos / BX) * a.sz + pos % BX;
-    const T c = a.p[i];
-    const T q = a.prev[i];
-    const S nb[6] = {(S)a.p[i - ((meta & 1u) ? 1u : 0u)], (S)a.p[i + ((meta & 2u) ? 1u : 0u)],
-                     (S)a.p[i - ((meta & 4u) ? a.pitch : 0u)], (S)a.p[i + ((meta & 8u) ? a.pitch : 0u)],
-                     (S)a.p[i - ((meta & 16u) ? a.sz : 0u)], (S)a.p[i + ((meta & 32u) ? a.sz : 0u)]};
-    a.prev[i] = (T)special_update(a, meta, rec.y, s, (S)c, (S)q, nb, u0);
+    special_at(a, s, i, rec, u0, a.prev);
 }
 
 // A batch of 64 special records: the record and the states of its first G
@@ -462,6 +489,8 @@ inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) /
 constexpr int kVariants = 4;
 constexpr int kDefaultVariant = 0;
 constexpr uint32_t kDefaultZChunk = 4;
+constexpr uint32_t kDefaultPairZChunk = 16;  // z-chunk of two-step passes
+constexpr uint32_t kDefaultStepsPerPass = 1;
 constexpr uint32_t kVariantTiles[] = {7, 3, 15, 1};
 
 inline uint32_t variant_threads(int variant) { return kTileX * (kVariantTiles[variant] + 1); }

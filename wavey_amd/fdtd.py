@@ -77,7 +77,7 @@ PROGRESS_FN = C.CFUNCTYPE(None, C.c_double, C.c_void_p)
 
 EXPORTS = [
     "wv_fdtd_run", "wv_plan_create", "wv_plan_run", "wv_run_begin", "wv_run_advance", "wv_run_finish",
-    "wv_plan_last_timing", "wv_plan_stats", "wv_plan_set_tile", "wv_plan_copy_probe", "wv_plan_free", "wv_grid_build_box",
+    "wv_plan_last_timing", "wv_plan_stats", "wv_plan_set_tile", "wv_plan_set_pass", "wv_plan_copy_probe", "wv_plan_free", "wv_grid_build_box",
     "wv_built_grid_view", "wv_grid_free", "wv_source_signal", "wv_nearest_air", "wv_taps_at",
     "wv_source_components", "wv_component_weights", "wv_to_transfer", "wv_admittance_eval", "wv_paris",
     "wv_fit_real_impedance", "wv_version", "wv_footprint_estimate", "wv_batch_create", "wv_batch_add",
@@ -121,6 +121,8 @@ def lib():
             L.wv_plan_stats.restype = st
             L.wv_plan_set_tile.argtypes = [C.c_void_p, C.c_uint32, C.c_int32]
             L.wv_plan_set_tile.restype = st
+            L.wv_plan_set_pass.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32]
+            L.wv_plan_set_pass.restype = st
             L.wv_plan_copy_probe.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, dp]
             L.wv_plan_copy_probe.restype = st
             L.wv_plan_free.argtypes = [C.c_void_p]
@@ -419,6 +421,13 @@ class Plan:
         st = lib().wv_plan_set_tile(self._h, int(zchunk), int(variant))
         if st != WV_OK:
             raise FdtdError(st, "invalid tile settings")
+
+    def set_pass(self, steps_per_pass=0, pair_zchunk=0):
+        """Steps per pass (1, or 2 for two-step passes; 0 = default); takes
+        effect at the next begin/run."""
+        st = lib().wv_plan_set_pass(self._h, int(steps_per_pass), int(pair_zchunk))
+        if st != WV_OK:
+            raise FdtdError(st, "invalid pass settings")
 
     def copy_probe(self, nbytes, reps=10):
         """Mean ms of a plain copy of nbytes between the field buffers (clobbers them)."""
