@@ -1,7 +1,10 @@
 """Benchmark of the wave-solver step on one MI355X.
 
-A step is one time step of the FDTD run over the whole grid: one launch of
-the fused step kernel (interior tiles, boundary/special cells, receivers).
+A step is one time step of the FDTD run over the whole grid. Grids of 4M
+cells and more (the hall) take two steps per pass: k_pair1 (deep cells twice
+in one z-march, special cells' first step, receivers) and k_pair2 (special
+and ring cells' second step, receivers), so per-step figures are half a
+pair; smaller grids take one launch of k_step per step.
 Workload (default): the 40 x 25 x 15 m hall at dx = 0.04 m (1002 x 627 x 377
 cells, 232.5M air cells, 8 pillars, 10 three-section admittances on surface
 patches, 256 receivers), FP32 fields, one cardioid source component. Inputs
@@ -94,11 +97,13 @@ def _counter_kib(db_dir, counter, kernel):
     return sum(vals) / len(vals), len(vals)
 
 
-def measure_traffic(args, copy_bytes):
-    """HBM bytes per step-kernel launch from two rocprofv3 counter passes
-    (FETCH_SIZE, then WRITE_SIZE) over a short run of this workload. Each
-    counter is scaled by what it reads for the plain copy kernel of the same
-    run, which moves copy_bytes in each direction."""
+def measure_traffic(args, copy_bytes, kernels, steps_per_launch_set):
+    """HBM bytes per step from two rocprofv3 counter passes (FETCH_SIZE, then
+    WRITE_SIZE) over a short run of this workload: the per-dispatch means of
+    the step kernels (one kernel per step, or the two kernels of a two-step
+    pass), summed and divided by the steps they advance. Each counter is
+    scaled by what it reads for the plain copy kernel of the same run, which
+    moves copy_bytes in each direction."""
     exe = shutil.which("rocprofv3")
     if exe is None:
         return None, "rocprofv3 not found"
@@ -115,14 +120,19 @@ def measure_traffic(args, copy_bytes):
                                timeout=180)
             if r.returncode != 0:
                 return None, f"rocprofv3 {ctr} pass exited with {r.returncode}"
-            step_kib, n = _counter_kib(d, ctr, "k_step")
+            step_kib, n = 0.0, 0
+            for k in kernels:
+                kib, cnt = _counter_kib(d, ctr, k)
+                step_kib += kib
+                n += cnt
+            step_kib /= steps_per_launch_set
             copy_kib, _ = _counter_kib(d, ctr, "k_copy")
         except (subprocess.TimeoutExpired, RuntimeError, sqlite3.Error) as e:
             return None, f"{ctr} pass: {e}"
         finally:
             shutil.rmtree(d, ignore_errors=True)
         scale = copy_bytes / (copy_kib * 1024.0)
-        out[ctr] = {"kib_per_dispatch": round(step_kib, 1), "dispatches": n, "scale": round(scale, 4),
+        out[ctr] = {"kib_per_step": round(step_kib, 1), "dispatches": n, "scale": round(scale, 4),
                     "bytes": int(step_kib * 1024.0 * scale)}
     return out["FETCH_SIZE"]["bytes"] + out["WRITE_SIZE"]["bytes"], out
 
@@ -384,12 +394,14 @@ def main():
     plan.begin(taps, case.signal, case.receivers)
     t_setup = time.perf_counter() - t0
     st = plan.stats()
+    pinfo = plan.pass_info()
+    pairs = pinfo["steps_per_pass"] == 2
     if args.warmup:
         plan.advance(args.warmup)
     t0 = time.perf_counter()
     plan.advance(args.steps)
     wall = time.perf_counter() - t0
-    dev_ms, launches, _ = plan.last_timing()
+    dev_ms, steps_timed, _ = plan.last_timing()
     ir, _ = plan.finish((), fdtd.MODE_IR)
     if not np.all(np.isfinite(ir)):
         raise RuntimeError("non-finite receiver samples")
@@ -397,9 +409,9 @@ def main():
 
     air = st["air_cells"]
     value = air * args.steps / wall / 1e9
-    launch_s = dev_ms * 1e-3 / max(launches, 1)
+    step_s = dev_ms * 1e-3 / max(steps_timed, 1)
     nbytes = algorithmic_bytes(st, len(case.receivers), len(taps), args.precision == "fp64")
-    achieved = nbytes / launch_s / 1e9
+    achieved = nbytes / step_s / 1e9
     line = {
         "metric": "FDTD Gcell-updates/s",
         "value": round(value, 3),
@@ -420,13 +432,16 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_step", "bytes_per_launch": int(nbytes),
-                     "launch_us": round(launch_s * 1e6, 2),
+                     "kernel": ("k_pair1 + k_pair2 (two steps per pass, z-chunk %d)" % pinfo["pair_zchunk"])
+                     if pairs else "k_step",
+                     "per": "step", "bytes_per_step": int(nbytes),
+                     "step_us": round(step_s * 1e6, 2),
                      "copy_GBs": round(2 * copy_bytes / (copy_ms * 1e-3) / 1e9, 1), "copy_bytes": 2 * copy_bytes},
         "setup_s": {"grid_build": round(t_build, 2), "upload_and_layout": round(t_setup, 2)},
     }
     if not args.no_traffic:
-        traffic, detail = measure_traffic(args, copy_bytes)
+        traffic, detail = measure_traffic(args, copy_bytes, ["k_pair1", "k_pair2"] if pairs else ["k_step"],
+                                          2 if pairs else 1)
         line["roofline"]["traffic"] = traffic
         line["roofline"]["traffic_detail"] = detail
     if not args.no_cpu:

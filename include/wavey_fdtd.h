@@ -136,6 +136,11 @@ wv_status wv_plan_set_tile(wv_plan* plan, uint32_t zchunk, int32_t variant);
  * to 32). Results are bit-identical either way. Takes effect at the next
  * wv_run_begin. No reference counterpart: a tuning knob. */
 wv_status wv_plan_set_pass(wv_plan* plan, uint32_t steps_per_pass, uint32_t pair_zchunk);
+/* The pass mode in use (the current run's, else the next run's): out =
+ * [steps per pass, pair z-chunk, ring cells of the current run]. By default
+ * grids of 4M cells and more take two steps per pass (z-chunk 32 from 64M
+ * cells, else 16), smaller ones one. No reference counterpart. */
+wv_status wv_plan_pass_info(const wv_plan* plan, uint32_t out[3]);
 /* Copies `bytes` (at most one field buffer) between the plan's field
  * buffers `reps` times with a plain streaming kernel and returns the mean
  * time per copy. It overwrites the fields: call it outside a run. No

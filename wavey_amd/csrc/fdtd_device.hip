@@ -524,7 +524,7 @@ void run_begin_impl(wv_plan* pl, const wv_tap* taps, size_t n_taps, const double
                        pl->sp_rec.as<uint2>(), pl->sp_tapw.as<S>(), pl->sp_cell.as<uint32_t>());
     WV_HIP(hipGetLastError());
     // Two-step passes: the ring list and the two more field buffers.
-    pl->run_steps_per_pass = pl->steps_per_pass ? pl->steps_per_pass : kDefaultStepsPerPass;
+    pl->run_steps_per_pass = pl->steps_per_pass ? pl->steps_per_pass : default_steps_per_pass(pl->cells);
     pl->n_ring = 0;
     if (pl->run_steps_per_pass == 2) {
         const uint64_t span = (uint64_t)pl->sz * (pl->nz - 2);
@@ -591,7 +591,7 @@ PairParams<T, S> make_pair_params(wv_plan* pl, const StepParams<T, S>& base) {
     PairParams<T, S> pp{};
     pp.a = base;
     constexpr uint32_t BX = kTileX * kPairLane<T>;
-    const uint32_t zch = pl->pair_zchunk ? pl->pair_zchunk : kDefaultPairZChunk;
+    const uint32_t zch = pl->pair_zchunk ? pl->pair_zchunk : default_pair_zchunk(pl->cells);
     pp.a.zchunk = zch;
     pp.dbits = pl->dbits.as<uint32_t>();
     pp.sp_cell = pl->sp_cell.as<uint32_t>();
@@ -779,7 +779,7 @@ uint64_t wv::estimate_bytes(const wv_grid* g, int prec, size_t steps, size_t n_r
     const uint64_t special = g->n_boundary + n_taps;
     const uint64_t sp_pad = (special + 63) & ~63ull;
     const uint64_t slots = g->n_boundary ? g->b_start[g->n_boundary] : 0;
-    const uint64_t fields = kDefaultStepsPerPass == 2 ? 4 : 2;
+    const uint64_t fields = default_steps_per_pass(cells) == 2 ? 4 : 2;
     uint64_t t = cells;                              // flags
     t += npitched / 4 + npitched / 4;                // boundary marks (two copies), interior and deep bitmaps
     t += fields * npitched * tsz;                    // fields
@@ -963,6 +963,15 @@ extern "C" wv_status wv_plan_stats(const wv_plan* plan, uint64_t out[7]) {
     out[4] = plan->n_special;
     out[5] = plan->npitched;
     out[6] = plan->ns_k;
+    return WV_OK;
+}
+
+extern "C" wv_status wv_plan_pass_info(const wv_plan* plan, uint32_t out[3]) {
+    if (!plan || !out) return WV_ERR_INVALID;
+    const uint32_t spp = plan->steps_per_pass ? plan->steps_per_pass : default_steps_per_pass(plan->cells);
+    out[0] = plan->active ? plan->run_steps_per_pass : spp;
+    out[1] = plan->pair_zchunk ? plan->pair_zchunk : default_pair_zchunk(plan->cells);
+    out[2] = plan->n_ring;
     return WV_OK;
 }
 

@@ -489,8 +489,14 @@ inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) /
 constexpr int kVariants = 4;
 constexpr int kDefaultVariant = 0;
 constexpr uint32_t kDefaultZChunk = 4;
-constexpr uint32_t kDefaultPairZChunk = 16;  // z-chunk of two-step passes
-constexpr uint32_t kDefaultStepsPerPass = 1;
+// Default steps per pass and pair z-chunk by grid size (measured on the
+// hall, a 7M-cell room and smaller ones): small grids are launch- and
+// latency-bound, where one step per launch wins; from a few million cells two
+// steps per pass win, with longer z-chunks on the largest grids.
+constexpr uint64_t kPairMinCells = 4ull << 20;
+constexpr uint64_t kPairLongChunkCells = 64ull << 20;
+inline uint32_t default_steps_per_pass(uint64_t cells) { return cells >= kPairMinCells ? 2u : 1u; }
+inline uint32_t default_pair_zchunk(uint64_t cells) { return cells >= kPairLongChunkCells ? 32u : 16u; }
 constexpr uint32_t kVariantTiles[] = {7, 3, 15, 1};
 
 inline uint32_t variant_threads(int variant) { return kTileX * (kVariantTiles[variant] + 1); }

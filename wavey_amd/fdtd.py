@@ -77,7 +77,7 @@ PROGRESS_FN = C.CFUNCTYPE(None, C.c_double, C.c_void_p)
 
 EXPORTS = [
     "wv_fdtd_run", "wv_plan_create", "wv_plan_run", "wv_run_begin", "wv_run_advance", "wv_run_finish",
-    "wv_plan_last_timing", "wv_plan_stats", "wv_plan_set_tile", "wv_plan_set_pass", "wv_plan_copy_probe", "wv_plan_free", "wv_grid_build_box",
+    "wv_plan_last_timing", "wv_plan_stats", "wv_plan_set_tile", "wv_plan_set_pass", "wv_plan_pass_info", "wv_plan_copy_probe", "wv_plan_free", "wv_grid_build_box",
     "wv_built_grid_view", "wv_grid_free", "wv_source_signal", "wv_nearest_air", "wv_taps_at",
     "wv_source_components", "wv_component_weights", "wv_to_transfer", "wv_admittance_eval", "wv_paris",
     "wv_fit_real_impedance", "wv_version", "wv_footprint_estimate", "wv_batch_create", "wv_batch_add",
@@ -123,6 +123,8 @@ def lib():
             L.wv_plan_set_tile.restype = st
             L.wv_plan_set_pass.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32]
             L.wv_plan_set_pass.restype = st
+            L.wv_plan_pass_info.argtypes = [C.c_void_p, P(C.c_uint32)]
+            L.wv_plan_pass_info.restype = st
             L.wv_plan_copy_probe.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, dp]
             L.wv_plan_copy_probe.restype = st
             L.wv_plan_free.argtypes = [C.c_void_p]
@@ -428,6 +430,11 @@ class Plan:
         st = lib().wv_plan_set_pass(self._h, int(steps_per_pass), int(pair_zchunk))
         if st != WV_OK:
             raise FdtdError(st, "invalid pass settings")
+
+    def pass_info(self):
+        out = (C.c_uint32 * 3)()
+        lib().wv_plan_pass_info(self._h, out)
+        return {"steps_per_pass": int(out[0]), "pair_zchunk": int(out[1]), "ring_cells": int(out[2])}
 
     def copy_probe(self, nbytes, reps=10):
         """Mean ms of a plain copy of nbytes between the field buffers (clobbers them)."""
