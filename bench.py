@@ -112,7 +112,8 @@ def measure_traffic(args, copy_bytes, kernels, steps_per_launch_set):
         d = tempfile.mkdtemp(prefix=f"wv_{ctr.lower()}_", dir="/tmp")
         cmd = [exe, "--pmc", ctr, "-d", d, "-o", "run", "--", sys.executable, os.path.abspath(__file__),
                "--no-cpu", "--no-traffic", "--steps", "10", "--warmup", "2", "--workload", args.workload,
-               "--precision", args.precision, "--zchunk", str(args.zchunk), "--variant", str(args.variant)]
+               "--precision", args.precision, "--zchunk", str(args.zchunk), "--variant", str(args.variant),
+               "--steps-per-pass", str(args.steps_per_pass), "--pair-zchunk", str(args.pair_zchunk)]
         print(f"bench: counter pass {ctr}", file=sys.stderr, flush=True)
         env = dict(os.environ, TMPDIR="/tmp")
         try:
@@ -356,6 +357,8 @@ def main():
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64", "fp32_state64"])
     ap.add_argument("--zchunk", type=int, default=0)
     ap.add_argument("--variant", type=int, default=-1)
+    ap.add_argument("--steps-per-pass", type=int, default=0, help="1 or 2; 0 = the plan's default for the grid")
+    ap.add_argument("--pair-zchunk", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-traffic", action="store_true", help="skip the HBM counter passes")
     args = ap.parse_args()
@@ -386,6 +389,8 @@ def main():
     plan = fdtd.Plan(case.grid, args.precision)
     if args.zchunk or args.variant >= 0:
         plan.set_tile(args.zchunk, args.variant)
+    if args.steps_per_pass or args.pair_zchunk:
+        plan.set_pass(args.steps_per_pass, args.pair_zchunk)
     taps = case.runs[0][2]
     # A plain copy of one field buffer: the attainable-bandwidth reference and
     # the calibration dispatch for HBM traffic counters.
