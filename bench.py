@@ -337,6 +337,35 @@ def bench_serving(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_error(args, cfg):
+    """FP32 (and FP32 with FP64 slot state) against an FP64 run of the same
+    scheme and inputs, on the device, over args.error_steps steps of the
+    workload's first source component: relative L2 of the receiver series,
+    per receiver (max) and over all."""
+    from wavey_amd import fdtd, workload
+
+    case = workload.build_case(cfg, steps=args.error_steps)
+    taps = case.runs[0][2]
+    out = {}
+    irs = {}
+    for prec in ("fp64", "fp32", "fp32_state64"):
+        t0 = time.perf_counter()
+        with fdtd.Plan(case.grid, prec) as plan:
+            irs[prec], _ = plan.run(taps, case.signal, case.receivers, mode=fdtd.MODE_IR)
+        out[prec + "_s"] = round(time.perf_counter() - t0, 2)
+        print(f"bench: {prec} run {out[prec + '_s']} s", file=sys.stderr, flush=True)
+    ref = irs["fp64"]
+    for prec in ("fp32", "fp32_state64"):
+        per = [rel_l2(irs[prec][r], ref[r]) for r in range(len(ref)) if np.linalg.norm(ref[r]) > 0]
+        out[prec] = {"rel_l2_all": rel_l2(irs[prec], ref), "rel_l2_max_receiver": max(per),
+                     "rel_l2_median_receiver": float(np.median(per))}
+    line = {"metric": "FP32 receiver error against FP64 (same scheme, same inputs)",
+            "config": {"workload": f"{cfg['name']} {'x'.join(str(v) for v in case.grid.n)} cells, "
+                                   f"{args.error_steps} steps, {len(case.receivers)} receivers, one source component"},
+            "tolerance": 1e-4, **out}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -353,6 +382,8 @@ def main():
     ap.add_argument("--cap-gb", type=float, default=288.0, help="serving: admission cap, GB")
     ap.add_argument("--fp64-sample", type=int, default=6, help="instances rerun in FP64 for the error")
     ap.add_argument("--no-graphs", action="store_true")
+    ap.add_argument("--error-steps", type=int, default=0,
+                    help="instead of timing: FP32 against FP64 over this many steps of the workload")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64", "fp32_state64"])
     ap.add_argument("--zchunk", type=int, default=0)
@@ -380,6 +411,8 @@ def main():
         cfg = configs.pillared_room()
     else:
         cfg = configs.shoebox(fdtd.fit_real_impedance)
+    if args.error_steps:
+        return bench_error(args, cfg)
     total = args.warmup + args.steps
     t0 = time.perf_counter()
     case = workload.build_case(cfg, steps=max(total, 1))
