@@ -1,0 +1,115 @@
+"""GPU: edge cases of the run against the FP64 oracle and between pass modes.
+
+* no receivers; runs of 1, 2 and 3 steps in both pass modes;
+* cancellation between step blocks and the progress callback
+  (gpu_fdtd.rs:202-205, 259);
+* a tap on a solid cell and the same cell given twice (the reference injects
+  into any cell and adds duplicate taps, fdtd.rs:718-720, gpu_fdtd.rs:67-75),
+  with a receiver on the solid cell;
+* the hall at its full size: two steps per pass bit-identical to single
+  steps over 40 steps, every sample finite.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from wavey_amd import configs, fdtd, workload
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+@pytest.fixture(scope="module")
+def shoebox():
+    cfg = configs.shoebox(fdtd.fit_real_impedance)
+    case = workload.build_case(cfg)
+    og = oracle.build_box(cfg["room"], cfg["c"], cfg["dx"])
+    return case, og
+
+
+def test_no_receivers(shoebox):
+    case, _ = shoebox
+    print("case: shoebox, no receivers", flush=True)
+    with fdtd.Plan(case.grid, "fp32") as plan:
+        ir, sp = plan.run(case.runs[0][2], case.signal[:300], [], [62.5], mode=fdtd.MODE_BOTH)
+    assert ir.shape == (0, 300) and sp.shape == (0, 1)
+
+
+@pytest.mark.parametrize("steps", [1, 2, 3, 130])
+def test_short_runs_both_pass_modes(shoebox, steps):
+    case, og = shoebox
+    print(f"case: shoebox, {steps} steps, fp64, one and two steps per pass", flush=True)
+    sig = case.signal[:steps]
+    ref = oracle.run(og, case.runs[0][2], sig, case.receivers, threads=4)[0]
+    with fdtd.Plan(case.grid, "fp64") as plan:
+        plan.set_pass(1)
+        one, _ = plan.run(case.runs[0][2], sig, case.receivers, mode=fdtd.MODE_IR)
+        plan.set_pass(2, 8)
+        two, _ = plan.run(case.runs[0][2], sig, case.receivers, mode=fdtd.MODE_IR)
+    assert np.array_equal(one, two)
+    if np.linalg.norm(ref) > 0:
+        assert rel_l2(one, ref) < 1e-9
+    else:
+        assert np.all(one == 0)
+
+
+def test_cancel_from_progress(shoebox):
+    case, _ = shoebox
+    print("case: shoebox, cancel from the progress callback", flush=True)
+    flag = np.zeros(1, dtype=np.int32)
+    seen = []
+
+    def progress(f):
+        seen.append(f)
+        flag[0] = 1
+
+    with fdtd.Plan(case.grid, "fp32") as plan:
+        out = plan.run(case.runs[0][2], case.signal, case.receivers, cancel=flag, progress=progress)
+    assert out is None
+    assert seen and seen[0] < 1.0
+
+
+def test_progress_reaches_one(shoebox):
+    case, _ = shoebox
+    seen = []
+    with fdtd.Plan(case.grid, "fp32") as plan:
+        out = plan.run(case.runs[0][2], case.signal, case.receivers, progress=seen.append)
+    assert out is not None
+    assert seen == sorted(seen) and seen[-1] == 1.0
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-9), ("fp32", 1e-4)])
+@pytest.mark.parametrize("spp", [1, 2])
+def test_solid_and_duplicate_taps(shoebox, prec, tol, spp):
+    case, og = shoebox
+    g = case.grid
+    nx, ny, nz = g.n
+    solid = g.index(0, ny // 2, nz // 2)  # the padding layer at x = 0
+    assert not (g.flags[solid] & fdtd.AIR)
+    air = case.runs[0][2][0][0]
+    taps = [(solid, 0.5), (air, 0.7), (air, 0.3)]
+    rx = list(case.receivers) + [solid]
+    print(f"case: shoebox, tap on a solid cell and a duplicate tap, {prec}, steps/pass {spp}", flush=True)
+    ref = oracle.run(og, taps, case.signal[:800], rx, threads=4)[0]
+    with fdtd.Plan(g, prec) as plan:
+        plan.set_pass(spp, 8)
+        ir, _ = plan.run(taps, case.signal[:800], rx, mode=fdtd.MODE_IR)
+    assert rel_l2(ir, ref) < tol
+    assert rel_l2(ir[-1], ref[-1]) < tol  # the solid cell: its injections accumulate
+
+
+def test_hall_full_size_pass_modes_agree():
+    cfg = configs.hall()
+    print("case: hall 1002x627x377, 40 steps, fp32, one and two steps per pass", flush=True)
+    case = workload.build_case(cfg, steps=40)
+    taps = case.runs[0][2]
+    with fdtd.Plan(case.grid, "fp32") as plan:
+        assert plan.pass_info()["steps_per_pass"] == 2
+        two, _ = plan.run(taps, case.signal, case.receivers, mode=fdtd.MODE_IR)
+        plan.set_pass(1)
+        one, _ = plan.run(taps, case.signal, case.receivers, mode=fdtd.MODE_IR)
+    assert np.all(np.isfinite(one))
+    assert np.array_equal(one, two)
