@@ -138,7 +138,6 @@ struct Instance {
     uint32_t steps = 0;
     uint32_t n0 = 0;  // host mirror of the device block start
     bool done = false;
-    bool in_table = false;
     int size_class = 0;
     uint64_t footprint = 0;
 };
@@ -240,7 +239,6 @@ void build_table(wv_batch* b) {
     b->table.clear();
     for (auto& up : b->inst) {
         Instance* in = up.get();
-        in->in_table = false;
         if (in->done || in->n0 >= in->steps) continue;
         wv_plan* pl = in->plan.get();
         BatchEntry<T, S> e{};
@@ -257,7 +255,6 @@ void build_table(wv_batch* b) {
         be.insert(be.end(), blocks, (uint32_t)es.size());
         es.push_back(e);
         b->table.push_back(in);
-        in->in_table = true;
     }
     b->total_blocks = (uint32_t)be.size();
     if (!es.empty()) {
