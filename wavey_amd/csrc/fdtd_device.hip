@@ -422,21 +422,28 @@ void wv::plan_setup(wv_plan* pl, const wv_grid* g) {
     upload(pl, pl->b_set_sorted, bset);
     upload(pl, pl->set_n, set_n);
     upload(pl, pl->set_off, set_off);
-    std::vector<float> f_inv, f_cv, f_k, f_w;
+    // Slot constants as one record per slot, {inv_d, cv, k, w}: one 16-byte
+    // (FP32) or two (FP64 state) loads per slot.
+    std::vector<float> f_c;
+    std::vector<double> d_c;
     if (pl->prec == WV_FP32) {
-        f_inv = cast_vec<float>(inv_d);
-        f_cv = cast_vec<float>(cv);
-        f_k = cast_vec<float>(kk);
-        f_w = cast_vec<float>(w);
-        upload(pl, pl->set_inv_d, f_inv);
-        upload(pl, pl->set_cv, f_cv);
-        upload(pl, pl->set_k, f_k);
-        upload(pl, pl->set_w, f_w);
+        f_c.resize(4 * inv_d.size());
+        for (size_t j = 0; j < inv_d.size(); ++j) {
+            f_c[4 * j] = (float)inv_d[j];
+            f_c[4 * j + 1] = (float)cv[j];
+            f_c[4 * j + 2] = (float)kk[j];
+            f_c[4 * j + 3] = (float)w[j];
+        }
+        upload(pl, pl->set_c, f_c);
     } else {
-        upload(pl, pl->set_inv_d, inv_d);
-        upload(pl, pl->set_cv, cv);
-        upload(pl, pl->set_k, kk);
-        upload(pl, pl->set_w, w);
+        d_c.resize(4 * inv_d.size());
+        for (size_t j = 0; j < inv_d.size(); ++j) {
+            d_c[4 * j] = inv_d[j];
+            d_c[4 * j + 1] = cv[j];
+            d_c[4 * j + 2] = kk[j];
+            d_c[4 * j + 3] = w[j];
+        }
+        upload(pl, pl->set_c, d_c);
     }
 
     // Boundary marks, reused by every run.
@@ -762,8 +769,8 @@ wv_status wv::check_run_args(const wv_plan* pl, const wv_tap* taps, size_t n_tap
 uint64_t wv::plan_bytes(const wv_plan* pl) {
     uint64_t t = 0;
     for (const DBuf* d : {&pl->flags, &pl->force_base, &pl->force, &pl->zbits, &pl->counts, &pl->offs,
-                          &pl->b_cell_sorted, &pl->b_set_sorted, &pl->set_n, &pl->set_off, &pl->set_inv_d,
-                          &pl->set_cv, &pl->set_k, &pl->set_w, &pl->st_u, &pl->st_g, &pl->fa, &pl->fb, &pl->sp_rec,
+                          &pl->b_cell_sorted, &pl->b_set_sorted, &pl->set_n, &pl->set_off, &pl->set_c,
+                          &pl->st_u, &pl->st_g, &pl->fa, &pl->fb, &pl->sp_rec,
                           &pl->sp_tapw, &pl->tap_cell, &pl->tap_w, &pl->recv, &pl->ir, &pl->freqs, &pl->spec})
         t += d->bytes;
     return t;

@@ -70,7 +70,7 @@ struct wv_plan {
 
     DBuf flags, force_base, force, zbits, dbits, counts, offs;
     DBuf b_cell_sorted, b_set_sorted;
-    DBuf set_n, set_off, set_inv_d, set_cv, set_k, set_w, st_u, st_g;
+    DBuf set_n, set_off, set_c, st_u, st_g;  // set_c: {inv_d, cv, k, w} per slot
     DBuf fa, fb, fc, fd;  // fields; fc, fd only for two-step passes
     DBuf sp_rec, sp_tapw;
     DBuf tap_cell, tap_w, recv, ir, freqs, spec;
@@ -143,10 +143,7 @@ inline StepParams<T, S> make_params(wv_plan* pl, uint32_t tiles_per_block) {
     a.sp_rec = pl->sp_rec.as<uint2>();
     a.sp_tapw = pl->sp_tapw.as<S>();
     a.sp_pad = pl->sp_pad;
-    a.set_inv_d = pl->set_inv_d.as<S>();
-    a.set_cv = pl->set_cv.as<S>();
-    a.set_k = pl->set_k.as<S>();
-    a.set_w = pl->set_w.as<S>();
+    a.set_c = pl->set_c.as<S>();
     a.st_u = pl->st_u.as<S>();
     a.st_g = pl->st_g.as<S>();
     a.has_k = pl->ns_k > 0;

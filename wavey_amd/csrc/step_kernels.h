@@ -44,10 +44,7 @@ struct StepParams {
     const uint2* __restrict__ sp_rec;
     const S* __restrict__ sp_tapw;
     uint32_t sp_pad;  // row length of the slot-major state arrays
-    const S* __restrict__ set_inv_d;
-    const S* __restrict__ set_cv;
-    const S* __restrict__ set_k;
-    const S* __restrict__ set_w;
+    const S* __restrict__ set_c;  // per slot {inv_d, cv, k, w}
     S* __restrict__ st_u;  // [slot][rank]
     S* __restrict__ st_g;
     uint32_t has_k;
@@ -187,14 +184,24 @@ struct SlotGroup {
 #pragma unroll
         for (uint32_t k = 0; k < G; ++k) {
             const uint32_t j = min(jb + k, ns - 1);
-            inv_d[k] = a.set_inv_d[c0 + j];
-            w[k] = a.set_w[c0 + j];
-            cv[k] = a.set_cv[c0 + j];
+            const S* c = a.set_c + 4 * (c0 + j);
+            if constexpr (sizeof(S) == 4) {
+                const float4 q = *reinterpret_cast<const float4*>(c);
+                inv_d[k] = q.x;
+                cv[k] = q.y;
+                w[k] = q.w;
+            } else {
+                const double2 q0 = *reinterpret_cast<const double2*>(c);
+                const double2 q1 = *reinterpret_cast<const double2*>(c + 2);
+                inv_d[k] = q0.x;
+                cv[k] = q0.y;
+                w[k] = q1.y;
+            }
             u[k] = upre ? upre[k] : a.st_u[j * a.sp_pad + rank];
             kk[k] = 0;
             g[k] = 0;
             if (a.has_k) {
-                kk[k] = a.set_k[c0 + j];
+                kk[k] = c[2];
                 g[k] = a.st_g[j * a.sp_pad + rank];
             }
         }
