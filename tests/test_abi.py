@@ -112,3 +112,41 @@ def test_footprint_estimate_covers_fields():
     e64 = fdtd.footprint_estimate(grid, "fp64", 2000, 4, 8)
     assert e32 > 2 * 4 * pitched + 2000 * 4 * 4
     assert e64 > 2 * 8 * pitched and e64 > e32
+
+
+C_PROGRAM = r"""
+#include <stdio.h>
+#include <string.h>
+#include "wavey_fdtd.h"
+
+int main(void) {
+    double sig[16];
+    wv_source_signal(250.0, 1.6832e-4, 16, sig);
+    /* An invalid grid must come back as an error with a message, not a crash. */
+    wv_grid g;
+    memset(&g, 0, sizeof g);
+    wv_plan* plan = NULL;
+    char err[256] = {0};
+    wv_status st = wv_plan_create(&g, WV_FP32, &plan, err, sizeof err);
+    printf("%.17g %d %s | %s\n", sig[5], (int)st, err, wv_version());
+    return st == WV_ERR_INVALID && plan == NULL && err[0] ? 0 : 1;
+}
+"""
+
+
+def test_header_builds_and_links_as_c(tmp_path):
+    """A C99 program includes the header, links the library and calls it
+    (no GPU needed for these calls)."""
+    import subprocess
+
+    src = tmp_path / "abi.c"
+    src.write_text(C_PROGRAM)
+    exe = tmp_path / "abi"
+    libdir = os.path.dirname(fdtd.LIB_PATH)
+    inc = os.path.join(ROOT, "include")
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", inc, str(src), "-o", str(exe), "-L", libdir,
+                    "-lwavey_fdtd", f"-Wl,-rpath,{libdir}"], check=True, capture_output=True, text=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    ref = fdtd.source_signal(250.0, 1.6832e-4, 16)
+    assert float(r.stdout.split()[0]) == ref[5]
