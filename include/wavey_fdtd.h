@@ -123,10 +123,10 @@ wv_status wv_plan_last_timing(const wv_plan* plan, double* step_loop_ms, uint64_
  * counterpart. out: [cells, air_cells, boundary_nodes, slots, special_cells,
  * pitched_cells, slots_with_k]. */
 wv_status wv_plan_stats(const wv_plan* plan, uint64_t out[7]);
-/* Step-kernel shape: z planes per work-group column (0 = default, else a
- * power of two up to 32) and the
- * interior-kernel variant (-1 = default; 0..3 = tiles of 4, 8, 16, 2 rows,
- * each row 256 FP32 or 128 FP64 cells). No reference counterpart: the
+/* Single-step kernel shape: z planes per block (0 = default, else a power of
+ * two up to 32) and the block variant (-1 = default; 0..3 = blocks of 7, 3,
+ * 15 and 1 plane tiles of 256 FP32 or 128 FP64 cells, plus one special-cell
+ * wave). Takes effect at the next wv_run_begin. No reference counterpart: the
  * autotuner's knobs. */
 wv_status wv_plan_set_tile(wv_plan* plan, uint32_t zchunk, int32_t variant);
 /* Steps per pass over the fields: 1, or 2 (two steps per z-march: deep
@@ -172,7 +172,7 @@ wv_status wv_batch_create(int precision, wv_batch** out, char* err, size_t err_l
 wv_status wv_batch_add(wv_batch* batch, const wv_grid* grid, const wv_tap* taps, size_t n_taps,
                        const double* signal, size_t steps, const uint64_t* receivers, size_t n_recv,
                        int32_t* id, char* err, size_t err_len);
-/* Times the whole batch over trial_steps steps per candidate: rows per tile
+/* Times the whole batch over trial_steps steps per candidate: the block variant
  * for the launch, then the z-chunk of each size class (floor(log2(air
  * cells)), largest first). The choices are kept for later batches and servers
  * of the process; every instance restarts from rest. */
@@ -192,7 +192,7 @@ wv_status wv_batch_result(wv_batch* batch, int32_t id, const double* freqs, size
 wv_status wv_batch_info(const wv_batch* batch, int32_t id, uint64_t out[6]);
 /* Last wv_batch_run: device ms (HIP events on the batch stream); out: [step
  * launches per instance block sequence, cell updates, graph launches, graph
- * captures, rows-per-tile variant]. */
+ * captures, block variant]. */
 wv_status wv_batch_timing(const wv_batch* batch, double* ms, uint64_t out[5]);
 void wv_batch_free(wv_batch* batch);
 

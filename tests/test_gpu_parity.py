@@ -94,8 +94,9 @@ def test_pillared_room_components_match_oracle(pillared_mini, prec):
 
 @pytest.mark.parametrize("variant", range(4))
 def test_interior_variants_agree(pillared_mini, variant):
-    # Every interior-kernel variant (tile rows 4, 8, 16, 2) computes the
-    # same update: FP64 against the oracle at 1e-9, FP32 at 1e-4.
+    # Every single-step block variant (7, 3, 15, 1 plane tiles and a
+    # special-cell wave) computes the same update: FP64 against the oracle at
+    # 1e-9, FP32 at 1e-4.
     case, refs = pillared_mini
     _, ci, taps = case.runs[1]
     for prec in ("fp64", "fp32"):

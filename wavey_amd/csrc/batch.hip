@@ -10,7 +10,7 @@
 // series is bit-identical alone and in a batch.
 //
 // Tile shapes come from an autotuner that times the whole batch: first the
-// rows per tile (one template for the launch), then the z-chunk of each size
+// block variant (one template for the launch), then the z-chunk of each size
 // class of instances, coordinate by coordinate.
 //
 // The server admits queued requests by HBM footprint into a running batch at
@@ -111,7 +111,7 @@ int size_class_of(uint64_t air) {
 }
 
 // Autotuned shapes, shared by every batch and server of the process:
-// z-chunk per (precision, size class) and rows-per-tile variant per precision.
+// z-chunk per (precision, size class) and block variant per precision.
 std::mutex g_tune_mu;
 std::map<std::pair<int, int>, uint32_t> g_tune_zchunk;
 std::map<int, int> g_tune_variant;
@@ -378,7 +378,7 @@ double time_table(wv_batch* b, uint32_t warmup, uint32_t steps) {
     return ms / std::max(steps, 1u);
 }
 
-// Coordinate search over the batch's tile shape: the rows-per-tile variant
+// Coordinate search over the batch's tile shape: the block variant
 // (one template for the launch), then each size class's z-chunk, largest
 // class first, each timed on the whole batch. Instances restart from rest.
 template <typename T, typename S>
