@@ -225,6 +225,31 @@ wv_status wv_serve_stats(wv_server* server, double out[10]);
 /* Finishes every submitted request, then stops the worker. */
 void wv_serve_free(wv_server* server);
 
+/* -------------------------------------------------------------- wave job
+ *
+ * The wave solver's part of a wave job (run.rs:593-661): for each source, its
+ * runs on one grid (one component for an omni source's taps, four for a
+ * first-order directional source: fdtd.rs:730-743), each run's spectra turned
+ * into a transfer function (to_transfer, fdtd.rs:819-840) and summed with the
+ * source's per-frequency complex component weights (fdtd.rs:812-815).
+ */
+typedef struct {
+    const wv_tap* taps;        /* every component's taps, back to back */
+    const uint32_t* comp_off;  /* [n_comps + 1]: component c = taps[comp_off[c] .. comp_off[c+1]) */
+    uint32_t n_comps;
+    const double* weights;     /* [nf][n_comps] complex (re, im); NULL for a single component (weight 1) */
+    const double* h0;          /* [nf] complex source spectrum; NULL for 1 */
+    double r_ref;              /* reference distance of the source's sensitivity, m */
+} wv_wave_source;
+/* transfer_out: [n_sources][n_recv][nf] complex (re, im). batched = 0 runs
+ * the components one after another on one plan (the reference's order),
+ * 1 hands every run to one batch; the results are the same bit for bit. */
+wv_status wv_wave_job(const wv_grid* grid, const wv_wave_source* sources, size_t n_sources,
+                      const double* signal, size_t steps, const uint64_t* receivers, size_t n_recv,
+                      const double* freqs, size_t nf, double c, int precision, int batched,
+                      const volatile int32_t* cancel, double* transfer_out, char* err,
+                      size_t err_len);
+
 /* ------------------------------------------------------------ grid build */
 
 /* impedance_fit.rs:27-44 (Branch, AdmittanceFit). */

@@ -75,6 +75,11 @@ class WvBoxRoom(C.Structure):
 
 PROGRESS_FN = C.CFUNCTYPE(None, C.c_double, C.c_void_p)
 
+class WvWaveSource(C.Structure):
+    _fields_ = [("taps", C.POINTER(WvTap)), ("comp_off", C.POINTER(C.c_uint32)), ("n_comps", C.c_uint32),
+                ("weights", C.POINTER(C.c_double)), ("h0", C.POINTER(C.c_double)), ("r_ref", C.c_double)]
+
+
 EXPORTS = [
     "wv_fdtd_run", "wv_plan_create", "wv_plan_run", "wv_run_begin", "wv_run_advance", "wv_run_finish",
     "wv_plan_last_timing", "wv_plan_stats", "wv_plan_set_tile", "wv_plan_set_pass", "wv_plan_pass_info", "wv_plan_copy_probe", "wv_plan_free", "wv_grid_build_box",
@@ -83,7 +88,7 @@ EXPORTS = [
     "wv_fit_real_impedance", "wv_version", "wv_footprint_estimate", "wv_batch_create", "wv_batch_add",
     "wv_batch_autotune", "wv_batch_set_tile", "wv_batch_run", "wv_batch_result", "wv_batch_info",
     "wv_batch_timing", "wv_batch_free", "wv_serve_create", "wv_serve_submit", "wv_serve_wait", "wv_serve_stats",
-    "wv_serve_free",
+    "wv_serve_free", "wv_wave_job",
 ]
 
 _lib = None
@@ -189,6 +194,9 @@ def lib():
             L.wv_serve_stats.restype = st
             L.wv_serve_free.argtypes = [C.c_void_p]
             L.wv_serve_free.restype = None
+            L.wv_wave_job.argtypes = [P(WvGrid), P(WvWaveSource), sz, dp, sz, P(C.c_uint64), sz, dp, sz, C.c_double,
+                                      C.c_int, C.c_int, P(C.c_int32), dp] + errbuf
+            L.wv_wave_job.restype = st
             _lib = L
     return _lib
 
@@ -709,3 +717,50 @@ class Server:
         keys = ["completed", "p50_s", "p95_s", "p99_s", "span_s", "cell_updates", "max_running",
                 "peak_device_bytes", "failed", "graph_captures"]
         return dict(zip(keys, [float(v) for v in out]))
+
+
+def wave_job(grid: FdtdGrid, sources, signal, receivers, freqs, c, precision="fp32", batched=True, cancel=None):
+    """The wave solver's part of a wave job (run.rs:593-661, wv_wave_job).
+
+    sources: [{"comps": [taps, ...], "weights": [nf][n_comps] complex or None
+    (one component), "h0": [nf] complex or None, "r_ref": m}]. Returns the
+    transfer functions [n_sources][n_recv][nf] complex, or None when cancelled."""
+    sig = np.ascontiguousarray(signal, dtype=np.float64)
+    rc = np.ascontiguousarray(receivers, dtype=np.uint64)
+    fr = np.ascontiguousarray(freqs, dtype=np.float64)
+    R, F = len(rc), len(fr)
+    keep = []
+    arr = (WvWaveSource * max(1, len(sources)))()
+    for k, src in enumerate(sources):
+        comps = src["comps"]
+        taps = [t for comp in comps for t in comp]
+        off = np.zeros(len(comps) + 1, dtype=np.uint32)
+        off[1:] = np.cumsum([len(comp) for comp in comps])
+        tp = _taps_array(taps)
+        keep += [tp, off]
+        e = arr[k]
+        e.taps = tp
+        e.comp_off = off.ctypes.data_as(C.POINTER(C.c_uint32))
+        e.n_comps = len(comps)
+        if src.get("weights") is not None:
+            w = np.ascontiguousarray(np.asarray(src["weights"], dtype=np.complex128).reshape(F, len(comps)))
+            wv = np.ascontiguousarray(w.view(np.float64))
+            keep.append(wv)
+            e.weights = _dp(wv)
+        if src.get("h0") is not None:
+            h = np.ascontiguousarray(np.asarray(src["h0"], dtype=np.complex128).view(np.float64))
+            keep.append(h)
+            e.h0 = _dp(h)
+        e.r_ref = float(src.get("r_ref", 1.0))
+    out = np.zeros(2 * len(sources) * R * F)
+    err = C.create_string_buffer(1024)
+    prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+    st = lib().wv_wave_job(C.byref(grid.view()), arr, len(sources), _dp(sig), len(sig),
+                           rc.ctypes.data_as(C.POINTER(C.c_uint64)), R, _dp(fr) if F else None, F, float(c), prec,
+                           1 if batched else 0,
+                           cancel.ctypes.data_as(C.POINTER(C.c_int32)) if cancel is not None else None, _dp(out),
+                           err, 1024)
+    _check(st, err)
+    if st == WV_CANCELLED:
+        return None
+    return (out[0::2] + 1j * out[1::2]).reshape(len(sources), R, F)

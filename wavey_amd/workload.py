@@ -49,3 +49,20 @@ def build_case(cfg, threads=0, steps=None):
             for ci, taps in enumerate(comps):
                 runs.append((si, ci, taps))
     return Case(cfg=cfg, grid=g, signal=signal, receivers=receivers, runs=runs)
+
+
+def wave_sources(case, freqs):
+    """The wave job's sources of a case (run.rs:611-643): an omni source is
+    one run of its taps; a cardioid D = 1/2 + 1/2 d.axis is its four
+    component runs with per-frequency weights (fdtd.rs:812-815)."""
+    from . import configs
+
+    out = []
+    for si, s in enumerate(case.cfg["sources"]):
+        comps = [taps for (i, _, taps) in case.runs if i == si]
+        src = {"comps": comps, "weights": None, "h0": None, "r_ref": 1.0}
+        if s["kind"] != "omni" and len(comps) == 4:
+            src["weights"] = configs.cardioid_weights(s["axis"], freqs, case.grid.dx, case.cfg["c"],
+                                                      fdtd.component_weights)
+        out.append(src)
+    return out
