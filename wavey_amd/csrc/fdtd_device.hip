@@ -608,6 +608,7 @@ PairParams<T, S> make_pair_params(wv_plan* pl, const StepParams<T, S>& base) {
     pp.tiles_x = div_up(pl->pitch, BX);
     pp.row_blocks = div_up(pl->ny, kPairRows);
     pp.tile_blocks = pp.tiles_x * pp.row_blocks * div_up(pl->nz - 2, zch);
+    pp.tile_launch = div_up(pp.tile_blocks / pp.tiles_x, 8) * 8 * pp.tiles_x;
     pp.special_blocks = div_up(pl->n_special, kPairThreads);
     return pp;
 }
@@ -621,7 +622,7 @@ wv_status advance_impl(wv_plan* pl, size_t k, const volatile int32_t* cancel, wv
     uint32_t grid1 = 0, items2 = 0, grid2 = 0;
     if (pairs) {
         pp = make_pair_params<T, S>(pl, a);
-        grid1 = pp.tile_blocks + pp.special_blocks + div_up(pl->n_recv, kPairThreads);
+        grid1 = pp.tile_launch + pp.special_blocks + div_up(pl->n_recv, kPairThreads);
         items2 = div_up((uint64_t)pl->n_special + pl->n_ring, 256);
         grid2 = items2 + div_up(pl->n_recv, 256);
     }

@@ -46,6 +46,7 @@ struct PairParams {
     uint32_t n_special, n_ring;
     uint32_t tiles_x, row_blocks;  // pass-1 tiles: 64 W cells by kPairRows rows
     uint32_t tile_blocks;          // tiles_x * row_blocks * z-chunks
+    uint32_t tile_launch;          // launched tile blocks: tile_blocks / tiles_x rounded up to 8, times tiles_x
     uint32_t special_blocks;       // pass-1 blocks of kPairThreads special cells
 };
 
@@ -64,8 +65,14 @@ __device__ __forceinline__ void pair_tile(const PairParams<T, S>& pp, uint32_t t
     // plane later; the third buffer lets a wave write the next plane while
     // others still read the previous one, with one barrier per plane).
     __shared__ __align__(16) T rows[3][kPairWaves][BX];
-    const uint32_t tx = t % pp.tiles_x;
-    const uint32_t r = t / pp.tiles_x;
+    // Blocks b, b+8, b+16, ... run on the same XCD (the hardware's
+    // round-robin): one XCD takes all x-tiles of a (row block, z-chunk) in
+    // turn, so the columns the tiles share at their ends stay in its L2;
+    // neighbouring row blocks run on the other XCDs at the same time.
+    const uint32_t xcd = t & 7u, q = t >> 3;
+    const uint32_t tx = q % pp.tiles_x;
+    const uint32_t r = (q / pp.tiles_x) * 8u + xcd;
+    if (r >= pp.tile_blocks / pp.tiles_x) return;  // padding of the last group (the whole block)
     const uint32_t rb = r % pp.row_blocks;
     const uint32_t zc = r / pp.row_blocks;
     const uint32_t lane = threadIdx.x & (kTileX - 1);
@@ -220,11 +227,11 @@ __device__ __forceinline__ void pair_tile(const PairParams<T, S>& pp, uint32_t t
 template <typename T, typename S>
 __global__ __launch_bounds__(kPairThreads) void k_pair1(PairParams<T, S> pp) {
     const StepParams<T, S>& a = pp.a;
-    if (blockIdx.x < pp.tile_blocks) {
+    if (blockIdx.x < pp.tile_launch) {
         pair_tile<T, S>(pp, blockIdx.x);
         return;
     }
-    const uint32_t b = blockIdx.x - pp.tile_blocks;
+    const uint32_t b = blockIdx.x - pp.tile_launch;
     if (b < pp.special_blocks) {
         const uint32_t s = b * kPairThreads + threadIdx.x;
         if (s >= pp.n_special) return;
