@@ -385,6 +385,8 @@ def main():
     ap.add_argument("--error-steps", type=int, default=0,
                     help="instead of timing: FP32 against FP64 over this many steps of the workload")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--grid-builder", default="box", choices=["box", "mesh", "prism"],
+                    help="build the room with the box builder, or as triangles (a closed mesh, or an extruded plan)")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64", "fp32_state64"])
     ap.add_argument("--zchunk", type=int, default=0)
     ap.add_argument("--variant", type=int, default=-1)
@@ -415,9 +417,10 @@ def main():
         return bench_error(args, cfg)
     total = args.warmup + args.steps
     t0 = time.perf_counter()
-    case = workload.build_case(cfg, steps=max(total, 1))
+    case = workload.build_case(cfg, steps=max(total, 1), builder=args.grid_builder)
     t_build = time.perf_counter() - t0
-    print(f"bench: {cfg['name']} grid {case.grid.n}, built in {t_build:.1f} s", file=sys.stderr, flush=True)
+    print(f"bench: {cfg['name']} grid {case.grid.n}, built ({args.grid_builder}) in {t_build:.1f} s",
+          file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     plan = fdtd.Plan(case.grid, args.precision)
     if args.zchunk or args.variant >= 0:
@@ -475,7 +478,8 @@ def main():
                      "per": "step", "bytes_per_step": int(nbytes),
                      "step_us": round(step_s * 1e6, 2),
                      "copy_GBs": round(2 * copy_bytes / (copy_ms * 1e-3) / 1e9, 1), "copy_bytes": 2 * copy_bytes},
-        "setup_s": {"grid_build": round(t_build, 2), "upload_and_layout": round(t_setup, 2)},
+        "setup_s": {"grid_build": round(t_build, 2), "grid_builder": args.grid_builder,
+                    "upload_and_layout": round(t_setup, 2)},
     }
     if not args.no_traffic:
         traffic, detail = measure_traffic(args, copy_bytes, ["k_pair1", "k_pair2"] if pairs else ["k_step"],

@@ -291,6 +291,61 @@ wv_status wv_grid_build_box(const wv_box_room* room, double c, double dx, int th
 const wv_grid* wv_built_grid_view(const wv_built_grid* g);
 void wv_grid_free(wv_built_grid* g);
 
+/*
+ * A general room: the triangles of Room (core/wavey-geom/src/room.rs:133-148)
+ * with the surface of each triangle and one admittance fit per surface.
+ * prism = 1 is a room extruded from a plan sketch: a cell is inside when
+ * 0 < z < height and its (x, y) lies in one of the plan regions
+ * (Room::contains, room.rs:151-157, with Region::contains and
+ * point_in_polygon, regions.rs:44-47, 237-249); a region is cycle
+ * region_cycles[r] (its outer boundary) followed by cycles up to
+ * region_cycles[r+1] (its holes), cycle k being the points
+ * cycle_xy[2*cycle_off[k] .. 2*cycle_off[k+1]). prism = 0 is a closed mesh:
+ * inside by vertical ray parity (column_fill, fdtd.rs:385-422).
+ */
+typedef struct {
+    int32_t prism;
+    double height;
+    uint32_t n_positions;
+    const double* positions;     /* [n_positions][3] */
+    uint32_t n_triangles;
+    const uint32_t* triangles;   /* [n_triangles][3] vertex indices */
+    const uint32_t* tri_surface; /* [n_triangles] surface of each triangle */
+    uint32_t n_fits;             /* one fit per surface: > every tri_surface entry */
+    const wv_admittance* fits;
+    uint32_t n_regions;          /* prism rooms only */
+    const uint32_t* region_cycles; /* [n_regions + 1] */
+    const uint32_t* cycle_off;     /* [region_cycles[n_regions] + 1] */
+    const double* cycle_xy;
+} wv_room_mesh;
+
+/* A rigid convex obstacle (a speaker cabinet): the points with
+ * n . p <= d for every plane (nx, ny, nz, d) inside [lo, hi]
+ * (Solid, fdtd.rs:424-461). */
+typedef struct {
+    uint32_t n_planes;
+    const double* planes; /* [n_planes][4] */
+    double lo[3], hi[3];
+} wv_solid;
+
+/* Solid::from_mesh (fdtd.rs:433-456): a convex triangle mesh with outward
+ * winding, vertices turned by rot (row-major 3x3, applied as rot * v) and
+ * moved by pos. planes_out holds up to n_tris planes; the distinct planes
+ * are written and counted in *n_planes. WV_ERR_INVALID for no triangles. */
+wv_status wv_solid_from_mesh(const double* verts, uint32_t n_verts, const uint32_t* tris,
+                             uint32_t n_tris, const double rot[9], const double pos[3],
+                             double* planes_out, uint32_t* n_planes, double lo[3], double hi[3]);
+/* FdtdGrid::build_with's spacing (fdtd.rs:282-286): c / (f_max * ppw), made
+ * finer where a solid is less than 8 cells across its smallest side. */
+double wv_grid_spacing(double c, double f_max, double ppw, const wv_solid* solids, size_t n_solids);
+/* FdtdGrid::build_dx (fdtd.rs:289-377) for a general room with cabinets:
+ * same flags, boundary order and slots. The surface behind a missing face
+ * is the nearest triangle hit within 1.5 dx along the face's axis; a miss
+ * or a solid neighbour leaves the face rigid. */
+wv_status wv_grid_build_room(const wv_room_mesh* room, const wv_solid* solids, size_t n_solids,
+                             double c, double dx, int threads, wv_built_grid** out, char* err,
+                             size_t err_len);
+
 /* ---------------------------------------------------- source / receivers */
 
 /* fdtd.rs:601-610 */

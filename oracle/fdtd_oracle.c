@@ -333,6 +333,348 @@ int ob_build_box(const ob_room* room, double c, double dx, ob_grid* g) {
     return 0;
 }
 
+/* ---------------------------------------------------------- general rooms */
+
+/* glam's DVec3 arithmetic: cross (a.y*b.z - b.y*a.z, ...), dot summed left
+ * to right, normalize = v * (1 / sqrt(v.v)). */
+static void v3sub(const double a[3], const double b[3], double o[3]) {
+    o[0] = a[0] - b[0];
+    o[1] = a[1] - b[1];
+    o[2] = a[2] - b[2];
+}
+static void v3cross(const double a[3], const double b[3], double o[3]) {
+    o[0] = a[1] * b[2] - b[1] * a[2];
+    o[1] = a[2] * b[0] - b[2] * a[0];
+    o[2] = a[0] * b[1] - b[0] * a[1];
+}
+static double v3dot(const double a[3], const double b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+/* bvh.rs:209-226 */
+int ob_ray_triangle(const double o[3], const double d[3], const double a[3], const double b[3],
+                    const double c[3], double* t) {
+    double e1[3], e2[3], p[3], s[3], q[3];
+    v3sub(b, a, e1);
+    v3sub(c, a, e2);
+    v3cross(d, e2, p);
+    double det = v3dot(e1, p);
+    if (fabs(det) < 1e-14) return 0;
+    double inv = 1.0 / det;
+    v3sub(o, a, s);
+    double u = v3dot(s, p) * inv;
+    if (!(u >= -1e-10 && u <= 1.0 + 1e-10)) return 0;
+    v3cross(s, e1, q);
+    double v = v3dot(d, q) * inv;
+    if (v < -1e-10 || u + v > 1.0 + 1e-10) return 0;
+    *t = v3dot(e2, q) * inv;
+    return 1;
+}
+
+static const double* tri_vertex(const ob_mesh_room* room, int32_t k, int j) {
+    return room->pos + 3 * (int64_t)room->tri[3 * k + j];
+}
+
+/* Bvh::intersect (bvh.rs:63-73): closest hit with t_min < t < t_max. */
+static int32_t closest_hit(const ob_mesh_room* room, const double o[3], const double d[3], double t_min,
+                           double t_max) {
+    int32_t best = -1;
+    double lim = t_max;
+    for (int32_t k = 0; k < room->n_tri; ++k) {
+        double t;
+        if (ob_ray_triangle(o, d, tri_vertex(room, k, 0), tri_vertex(room, k, 1), tri_vertex(room, k, 2), &t) &&
+            t > t_min && t < lim) {
+            lim = t;
+            best = k;
+        }
+    }
+    return best;
+}
+
+typedef struct { double t; int32_t k; } ob_hit;
+
+static int hit_cmp(const void* a, const void* b) {
+    const ob_hit* x = (const ob_hit*)a;
+    const ob_hit* y = (const ob_hit*)b;
+    if (x->t < y->t) return -1;
+    if (x->t > y->t) return 1;
+    return (x->k > y->k) - (x->k < y->k);
+}
+
+/* Bvh::all_hits (bvh.rs:75-87): every hit with t_min < t < t_max, sorted.
+ * hits must hold n_tri entries. */
+static int32_t all_hits(const ob_mesh_room* room, const double o[3], const double d[3], double t_min,
+                        double t_max, ob_hit* hits) {
+    int32_t n = 0;
+    for (int32_t k = 0; k < room->n_tri; ++k) {
+        double t;
+        if (ob_ray_triangle(o, d, tri_vertex(room, k, 0), tri_vertex(room, k, 1), tri_vertex(room, k, 2), &t) &&
+            t > t_min && t < t_max) {
+            hits[n].t = t;
+            hits[n].k = k;
+            n++;
+        }
+    }
+    qsort(hits, (size_t)n, sizeof(ob_hit), hit_cmp);
+    return n;
+}
+
+/* room.rs:355-367 */
+int ob_parity_inside(const ob_mesh_room* room, const double p[3]) {
+    static const double DIRS[3][3] = {
+        {0.5772156649, 0.6180339887, 0.5338319},
+        {-0.6931471806, 0.4142135624, -0.3010299957},
+        {0.2360679775, -0.7320508076, 0.6457513111},
+    };
+    ob_hit* hits = (ob_hit*)malloc(sizeof(ob_hit) * (size_t)(room->n_tri + 1));
+    int odd = 0;
+    for (int r = 0; r < 3; ++r) {
+        double inv = 1.0 / sqrt(v3dot(DIRS[r], DIRS[r]));
+        double d[3] = {DIRS[r][0] * inv, DIRS[r][1] * inv, DIRS[r][2] * inv};
+        if (all_hits(room, p, d, 1e-9, INFINITY, hits) % 2 == 1) odd++;
+    }
+    free(hits);
+    return odd >= 2;
+}
+
+/* regions.rs:237-249 */
+static int point_in_polygon(double px, double py, const double* xy, int32_t n) {
+    int inside = 0;
+    int32_t j = n - 1;
+    for (int32_t i = 0; i < n; ++i) {
+        double ax = xy[2 * i], ay = xy[2 * i + 1], bx = xy[2 * j], by = xy[2 * j + 1];
+        if ((ay > py) != (by > py) && px < (bx - ax) * (py - ay) / (by - ay) + ax) inside = !inside;
+        j = i;
+    }
+    return inside;
+}
+
+/* Room::contains for an extruded room (room.rs:151-154, regions.rs:44-47). */
+static int prism_contains(const ob_mesh_room* room, const double p[3]) {
+    if (!(p[2] > 0.0 && p[2] < room->height)) return 0;
+    for (int32_t r = 0; r < room->n_regions; ++r) {
+        int32_t c0 = room->region_cycles[r], c1 = room->region_cycles[r + 1];
+        const int32_t* off = room->cycle_off;
+        if (!point_in_polygon(p[0], p[1], room->cycle_xy + 2 * (int64_t)off[c0], off[c0 + 1] - off[c0])) continue;
+        int in_hole = 0;
+        for (int32_t h = c0 + 1; h < c1 && !in_hole; ++h)
+            in_hole = point_in_polygon(p[0], p[1], room->cycle_xy + 2 * (int64_t)off[h], off[h + 1] - off[h]);
+        if (!in_hole) return 1;
+    }
+    return 0;
+}
+
+/* fdtd.rs:433-456 */
+int ob_solid_from_mesh(const double* verts, int32_t n_verts, const int32_t* tris, int32_t n_tris,
+                       const double rot[9], const double pos[3], double* planes_out, double lo[3],
+                       double hi[3]) {
+    if (n_tris <= 0) return -1;
+    double* w = (double*)malloc(sizeof(double) * 3 * (size_t)(n_verts > 0 ? n_verts : 1));
+    for (int32_t i = 0; i < n_verts; ++i) {
+        const double* v = verts + 3 * i;
+        for (int r = 0; r < 3; ++r)
+            w[3 * i + r] = (rot[3 * r] * v[0] + rot[3 * r + 1] * v[1] + rot[3 * r + 2] * v[2]) + pos[r];
+    }
+    int np = 0;
+    for (int32_t k = 0; k < n_tris; ++k) {
+        const double* a = w + 3 * tris[3 * k];
+        const double* b = w + 3 * tris[3 * k + 1];
+        const double* c = w + 3 * tris[3 * k + 2];
+        double e1[3], e2[3], n[3];
+        v3sub(b, a, e1);
+        v3sub(c, a, e2);
+        v3cross(e1, e2, n);
+        if (sqrt(v3dot(n, n)) < 1e-12) continue;
+        double inv = 1.0 / sqrt(v3dot(n, n));
+        n[0] *= inv;
+        n[1] *= inv;
+        n[2] *= inv;
+        double d = v3dot(n, a);
+        int dup = 0;
+        for (int j = 0; j < np && !dup; ++j) {
+            const double* m = planes_out + 4 * j;
+            dup = v3dot(m, n) > 1.0 - 1e-9 && fabs(m[3] - d) < 1e-9;
+        }
+        if (dup) continue;
+        planes_out[4 * np] = n[0];
+        planes_out[4 * np + 1] = n[1];
+        planes_out[4 * np + 2] = n[2];
+        planes_out[4 * np + 3] = d;
+        np++;
+    }
+    for (int r = 0; r < 3; ++r) {
+        lo[r] = INFINITY;
+        hi[r] = -INFINITY;
+    }
+    for (int32_t i = 0; i < n_verts; ++i)
+        for (int r = 0; r < 3; ++r) {
+            lo[r] = fmin(lo[r], w[3 * i + r]);
+            hi[r] = fmax(hi[r], w[3 * i + r]);
+        }
+    free(w);
+    return np;
+}
+
+/* fdtd.rs:458-460 */
+int ob_solid_contains(const ob_solid* s, const double p[3]) {
+    for (int r = 0; r < 3; ++r)
+        if (!(p[r] >= s->lo[r] && p[r] <= s->hi[r])) return 0;
+    for (int32_t j = 0; j < s->n_planes; ++j)
+        if (!(v3dot(s->planes + 4 * j, p) <= s->planes[4 * j + 3])) return 0;
+    return 1;
+}
+
+/* fdtd.rs:385-422: per (x, y) column an upward ray from one cell below the
+ * grid, moved off the cell centres; hits of one orientation less than 1e-9
+ * apart count once; a cell is inside after an odd number of crossings. */
+static void column_fill(const ob_mesh_room* room, const ob_grid* g, uint8_t* inside) {
+    const int64_t nx = g->n[0], ny = g->n[1], nz = g->n[2];
+    const double dx = g->dx;
+    const double off[2] = {0.0001372 * dx, 0.0002918 * dx};
+    const double start = g->origin[2] - dx;
+#pragma omp parallel
+    {
+        ob_hit* hits = (ob_hit*)malloc(sizeof(ob_hit) * (size_t)(room->n_tri + 1));
+        ob_hit* cross = (ob_hit*)malloc(sizeof(ob_hit) * (size_t)(room->n_tri + 1));
+        int* up_of = (int*)malloc(sizeof(int) * (size_t)(room->n_tri + 1));
+#pragma omp for schedule(dynamic, 16)
+        for (int64_t col = 0; col < nx * ny; ++col) {
+            int64_t x = col % nx, y = col / nx;
+            double o[3] = {(g->origin[0] + (double)x * dx) + off[0], (g->origin[1] + (double)y * dx) + off[1],
+                           start};
+            const double up_dir[3] = {0.0, 0.0, 1.0};
+            int32_t nh = all_hits(room, o, up_dir, 0.0, INFINITY, hits);
+            int32_t nc = 0;
+            for (int32_t h = 0; h < nh; ++h) {
+                const double* a = tri_vertex(room, hits[h].k, 0);
+                const double* b = tri_vertex(room, hits[h].k, 1);
+                const double* c = tri_vertex(room, hits[h].k, 2);
+                double e1[3], e2[3], n[3];
+                v3sub(b, a, e1);
+                v3sub(c, a, e2);
+                v3cross(e1, e2, n);
+                int up = n[2] > 0.0;
+                if (nc > 0 && hits[h].t - cross[nc - 1].t < 1e-9 && up_of[nc - 1] == up) continue;
+                cross[nc] = hits[h];
+                up_of[nc] = up;
+                nc++;
+            }
+            int32_t k = 0;
+            for (int64_t z = 0; z < nz; ++z) {
+                double tz = (g->origin[2] + (double)z * dx) - start;
+                while (k < nc && cross[k].t < tz) k++;
+                inside[z * nx * ny + col] = (uint8_t)(k % 2 == 1);
+            }
+        }
+        free(hits);
+        free(cross);
+        free(up_of);
+    }
+}
+
+/* fdtd.rs:289-377 (build_dx) for a general room with rigid solids. */
+int ob_build_room(const ob_mesh_room* room, const ob_solid* solids, int32_t n_solids, double c,
+                  double dx, ob_grid* g) {
+    memset(g, 0, sizeof(*g));
+    if (room->n_pos <= 0) return -4;
+    double dt = COURANT * dx / c;
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int32_t i = 0; i < room->n_pos; ++i)
+        for (int r = 0; r < 3; ++r) {
+            lo[r] = fmin(lo[r], room->pos[3 * i + r]);
+            hi[r] = fmax(hi[r], room->pos[3 * i + r]);
+        }
+    int64_t n[3];
+    for (int k = 0; k < 3; ++k) {
+        g->origin[k] = lo[k] - dx * 0.5;
+        n[k] = (int64_t)ceil((hi[k] - lo[k]) / dx) + 2;
+        g->n[k] = n[k];
+    }
+    int64_t cells = n[0] * n[1] * n[2];
+    if (cells > MAX_CELLS) return -1;
+    g->dx = dx;
+    g->dt = dt;
+    const int64_t nx = n[0], nxy = n[0] * n[1];
+    uint8_t* solid = (uint8_t*)calloc((size_t)cells, 1);
+    uint8_t* inside = (uint8_t*)calloc((size_t)cells, 1);
+    uint8_t* air = (uint8_t*)calloc((size_t)cells, 1);
+    int32_t* faces = (int32_t*)malloc(sizeof(int32_t) * 6 * (size_t)cells);
+    g->flags = (uint8_t*)calloc((size_t)cells, 1);
+    g->bidx = (uint32_t*)malloc((size_t)cells * 4);
+    if (!solid || !inside || !air || !faces || !g->flags || !g->bidx) return -2;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < cells; ++i) {
+        int64_t z = i / nxy, r = i % nxy, y = r / nx, x = r % nx;
+        double p[3] = {g->origin[0] + (double)x * dx, g->origin[1] + (double)y * dx, g->origin[2] + (double)z * dx};
+        for (int32_t s = 0; s < n_solids && !solid[i]; ++s) solid[i] = (uint8_t)ob_solid_contains(&solids[s], p);
+        if (room->prism) inside[i] = (uint8_t)prism_contains(room, p);
+    }
+    if (!room->prism) column_fill(room, g, inside);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < cells; ++i) {
+        int64_t z = i / nxy, r = i % nxy, y = r / nx, x = r % nx;
+        air[i] = !solid[i] && x > 0 && y > 0 && z > 0 && x < n[0] - 1 && y < n[1] - 1 && z < n[2] - 1 && inside[i];
+    }
+    static const int64_t DXI[6] = {-1, 1, 0, 0, 0, 0}, DYI[6] = {0, 0, -1, 1, 0, 0}, DZI[6] = {0, 0, 0, 0, -1, 1};
+    /* Per air node: flags and the surface behind each missing face (-1: none). */
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < cells; ++i) {
+        if (!air[i]) continue;
+        int64_t z = i / nxy, r = i % nxy, y = r / nx, x = r % nx;
+        double p[3] = {g->origin[0] + (double)x * dx, g->origin[1] + (double)y * dx, g->origin[2] + (double)z * dx};
+        uint8_t fl = AIR;
+        for (int k = 0; k < 6; ++k) {
+            int64_t j = (z + DZI[k]) * nxy + (y + DYI[k]) * nx + (x + DXI[k]);
+            faces[6 * i + k] = -1;
+            if (air[j]) {
+                fl |= (uint8_t)(1u << k);
+            } else if (!solid[j]) {
+                double d[3] = {(double)DXI[k], (double)DYI[k], (double)DZI[k]};
+                int32_t h = closest_hit(room, p, d, 0.0, dx * 1.5);
+                if (h >= 0) faces[6 * i + k] = room->tri_surface[h];
+            }
+        }
+        g->flags[i] = fl;
+    }
+    /* Serial, in cell order, as fdtd.rs:368-375 builds the boundary list. */
+    int64_t cap_b = 1024, cap_s = 4096;
+    g->b_cell = (uint32_t*)malloc(cap_b * 4);
+    g->b_start = (uint32_t*)malloc((cap_b + 1) * 4);
+    g->slots = (ob_slot*)malloc(cap_s * sizeof(ob_slot));
+    int rc = 0;
+    for (int64_t i = 0; i < cells && rc == 0; ++i) {
+        g->bidx[i] = NO_BOUNDARY;
+        if (!air[i]) continue;
+        int32_t fs[6];
+        int nf = 0;
+        for (int k = 0; k < 6; ++k)
+            if (faces[6 * i + k] >= 0) fs[nf++] = faces[6 * i + k];
+        ob_slot sl[6 * (OB_MAX_BRANCHES + 1)];
+        int ns = ob_boundary_slots(fs, nf, room->fits, dt, sl, 6 * (OB_MAX_BRANCHES + 1));
+        if (ns < 0) { rc = -3; break; }
+        if (ns == 0) continue;
+        if (g->nb + 1 >= cap_b) {
+            cap_b *= 2;
+            g->b_cell = (uint32_t*)realloc(g->b_cell, cap_b * 4);
+            g->b_start = (uint32_t*)realloc(g->b_start, (cap_b + 1) * 4);
+        }
+        while (g->ns + ns > cap_s) {
+            cap_s *= 2;
+            g->slots = (ob_slot*)realloc(g->slots, cap_s * sizeof(ob_slot));
+        }
+        g->bidx[i] = (uint32_t)g->nb;
+        g->b_cell[g->nb] = (uint32_t)i;
+        g->b_start[g->nb] = (uint32_t)g->ns;
+        memcpy(g->slots + g->ns, sl, ns * sizeof(ob_slot));
+        g->ns += ns;
+        g->nb++;
+    }
+    g->b_start[g->nb] = (uint32_t)g->ns;
+    free(solid);
+    free(inside);
+    free(air);
+    free(faces);
+    return rc;
+}
+
 /* ---------------------------------------------------------------- sources */
 
 /* fdtd.rs:601-610 */

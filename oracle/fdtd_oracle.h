@@ -124,6 +124,59 @@ int ob_run_fields(const ob_grid_view* g, const int64_t* tap_cells, const double*
                   int64_t n_taps, const double* signal, int64_t steps, int threads, double* p_out,
                   double* prev_out, double* state_out);
 
+/*
+ * General rooms (fdtd.rs:289-461, core/wavey-geom/src/room.rs:133-157,
+ * core/wavey-geom/src/bvh.rs:63-226). Every ray cast here tests every
+ * triangle (no BVH): the closest hit, and for all-hits queries the hits
+ * sorted by t, equal t in triangle order. The reference's BVH visits
+ * triangles in tree order, so the two differ only where two triangles are
+ * hit at exactly the same t.
+ *
+ * prism = 1: inside when 0 < z < height and (x, y) is in a plan region
+ * (region r = cycle region_cycles[r] as its outer boundary, the cycles up
+ * to region_cycles[r+1] as holes; cycle k = points cycle_off[k] ..
+ * cycle_off[k+1] of cycle_xy). prism = 0: column ray parity.
+ */
+typedef struct {
+    int32_t prism;
+    double height;
+    int32_t n_pos;
+    const double* pos;        /* [n_pos][3] */
+    int32_t n_tri;
+    const int32_t* tri;       /* [n_tri][3] */
+    const int32_t* tri_surface;
+    int32_t n_fits;           /* one per surface */
+    const ob_fit* fits;
+    int32_t n_regions;
+    const int32_t* region_cycles;
+    const int32_t* cycle_off;
+    const double* cycle_xy;
+} ob_mesh_room;
+
+/* fdtd.rs:424-461 (Solid): n . p <= d for every plane, inside [lo, hi]. */
+typedef struct {
+    int32_t n_planes;
+    const double* planes; /* [n_planes][4] = nx, ny, nz, d */
+    double lo[3], hi[3];
+} ob_solid;
+
+/* bvh.rs:209-226: Moller-Trumbore, two-sided; returns 1 and *t on a hit. */
+int ob_ray_triangle(const double o[3], const double d[3], const double a[3], const double b[3],
+                    const double c[3], double* t);
+/* room.rs:355-367: inside when at least two of three fixed rays cross an odd
+ * number of triangles. */
+int ob_parity_inside(const ob_mesh_room* room, const double p[3]);
+/* fdtd.rs:433-456; rot row-major, applied as rot * v. Returns the number of
+ * distinct planes written to planes_out ([n_tris][4] room), -1 for no
+ * triangles. */
+int ob_solid_from_mesh(const double* verts, int32_t n_verts, const int32_t* tris, int32_t n_tris,
+                       const double rot[9], const double pos[3], double* planes_out, double lo[3],
+                       double hi[3]);
+int ob_solid_contains(const ob_solid* s, const double p[3]);
+/* fdtd.rs:289-377 (build_dx) with column_fill (:385-422). */
+int ob_build_room(const ob_mesh_room* room, const ob_solid* solids, int32_t n_solids, double c,
+                  double dx, ob_grid* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -50,6 +50,19 @@ class ObGrid(C.Structure):
                 ("slots", C.POINTER(ObSlot))]
 
 
+class ObMeshRoom(C.Structure):
+    _fields_ = [("prism", C.c_int32), ("height", C.c_double), ("n_pos", C.c_int32), ("pos", C.POINTER(C.c_double)),
+                ("n_tri", C.c_int32), ("tri", C.POINTER(C.c_int32)), ("tri_surface", C.POINTER(C.c_int32)),
+                ("n_fits", C.c_int32), ("fits", C.POINTER(ObFit)), ("n_regions", C.c_int32),
+                ("region_cycles", C.POINTER(C.c_int32)), ("cycle_off", C.POINTER(C.c_int32)),
+                ("cycle_xy", C.POINTER(C.c_double))]
+
+
+class ObSolid(C.Structure):
+    _fields_ = [("n_planes", C.c_int32), ("planes", C.POINTER(C.c_double)), ("lo", C.c_double * 3),
+                ("hi", C.c_double * 3)]
+
+
 class ObGridView(C.Structure):
     _fields_ = [("n", C.c_int64 * 3), ("dx", C.c_double), ("dt", C.c_double), ("flags", C.POINTER(C.c_uint8)),
                 ("nb", C.c_int64), ("b_cell", C.POINTER(C.c_uint32)), ("b_start", C.POINTER(C.c_uint32)),
@@ -98,6 +111,17 @@ def lib():
         L.ob_run_fields.argtypes = [P(ObGridView), P(C.c_int64), P(C.c_double), C.c_int64, P(C.c_double), C.c_int64,
                                     C.c_int, P(C.c_double), P(C.c_double), P(C.c_double)]
         L.ob_run_fields.restype = C.c_int
+        L.ob_ray_triangle.argtypes = [P(C.c_double)] * 5 + [P(C.c_double)]
+        L.ob_ray_triangle.restype = C.c_int
+        L.ob_parity_inside.argtypes = [P(ObMeshRoom), P(C.c_double)]
+        L.ob_parity_inside.restype = C.c_int
+        L.ob_solid_from_mesh.argtypes = [P(C.c_double), C.c_int32, P(C.c_int32), C.c_int32, P(C.c_double),
+                                         P(C.c_double), P(C.c_double), P(C.c_double), P(C.c_double)]
+        L.ob_solid_from_mesh.restype = C.c_int
+        L.ob_solid_contains.argtypes = [P(ObSolid), P(C.c_double)]
+        L.ob_solid_contains.restype = C.c_int
+        L.ob_build_room.argtypes = [P(ObMeshRoom), P(ObSolid), C.c_int32, C.c_double, C.c_double, P(ObGrid)]
+        L.ob_build_room.restype = C.c_int
         _lib = L
     return _lib
 
@@ -191,6 +215,24 @@ def _room_struct(room):
     return r, keep
 
 
+def _grid_from(g):
+    n = tuple(int(g.n[k]) for k in range(3))
+    N = n[0] * n[1] * n[2]
+    flags = np.ctypeslib.as_array(g.flags, shape=(N,)).copy()
+    nb = int(g.nb)
+    b_cell = np.ctypeslib.as_array(g.b_cell, shape=(nb,)).copy() if nb else np.zeros(0, np.uint32)
+    b_start = np.ctypeslib.as_array(g.b_start, shape=(nb + 1,)).copy()
+    ns = int(g.ns)
+    if ns:
+        raw = np.ctypeslib.as_array(C.cast(g.slots, C.POINTER(C.c_double)), shape=(ns * 4,)).copy()
+        slots = raw.reshape(ns, 4)
+    else:
+        slots = np.zeros((0, 4))
+    return Grid(n=n, origin=tuple(g.origin[k] for k in range(3)), dx=g.dx, dt=g.dt, flags=flags,
+                b_cell=b_cell.astype(np.uint32), b_start=b_start.astype(np.uint32),
+                slots=np.ascontiguousarray(slots))
+
+
 def build_box(room, c, dx):
     """fdtd.rs:289-377 (build_dx) for a box room with pillars."""
     L = lib()
@@ -200,24 +242,116 @@ def build_box(room, c, dx):
     if st != 0:
         raise RuntimeError(f"ob_build_box failed ({st})")
     try:
-        n = tuple(int(g.n[k]) for k in range(3))
-        N = n[0] * n[1] * n[2]
-        flags = np.ctypeslib.as_array(g.flags, shape=(N,)).copy()
-        nb = int(g.nb)
-        b_cell = np.ctypeslib.as_array(g.b_cell, shape=(nb,)).copy() if nb else np.zeros(0, np.uint32)
-        b_start = np.ctypeslib.as_array(g.b_start, shape=(nb + 1,)).copy()
-        ns = int(g.ns)
-        if ns:
-            raw = np.ctypeslib.as_array(C.cast(g.slots, C.POINTER(C.c_double)), shape=(ns * 4,)).copy()
-            slots = raw.reshape(ns, 4)
-        else:
-            slots = np.zeros((0, 4))
-        grid = Grid(n=n, origin=tuple(g.origin[k] for k in range(3)), dx=g.dx, dt=g.dt, flags=flags,
-                    b_cell=b_cell.astype(np.uint32), b_start=b_start.astype(np.uint32),
-                    slots=np.ascontiguousarray(slots))
+        return _grid_from(g)
     finally:
         L.ob_grid_free(C.byref(g))
-    return grid
+
+
+# ------------------------------------------------------------ general rooms
+#
+# A mesh room is a dict
+#     {"positions": (V, 3), "triangles": (T, 3), "tri_surface": (T,),
+#      "fits": [(beta0, [(r, m, k), ...]), ...],       # one per surface
+#      "prism": bool, "height": h,                     # extruded rooms
+#      "regions": [[outer (n, 2), hole (m, 2), ...], ...]}
+# and a solid {"planes": (P, 4), "lo": (3,), "hi": (3,)}.
+
+def _mesh_struct(mesh):
+    r = ObMeshRoom()
+    pos = np.ascontiguousarray(np.asarray(mesh["positions"], dtype=np.float64).reshape(-1, 3))
+    tri = np.ascontiguousarray(np.asarray(mesh["triangles"], dtype=np.int32).reshape(-1, 3))
+    ts = np.ascontiguousarray(np.asarray(mesh["tri_surface"], dtype=np.int32))
+    fits = (ObFit * max(1, len(mesh["fits"])))(*[make_fit(b0, br) for b0, br in mesh["fits"]])
+    regions = mesh.get("regions", []) if mesh.get("prism") else []
+    rc, off, xy = [0], [0], []
+    for reg in regions:
+        for cyc in reg:
+            cyc = np.asarray(cyc, dtype=np.float64).reshape(-1, 2)
+            xy.append(cyc)
+            off.append(off[-1] + len(cyc))
+        rc.append(len(off) - 1)
+    rc = np.array(rc, dtype=np.int32)
+    off = np.array(off, dtype=np.int32)
+    xy = np.ascontiguousarray(np.concatenate(xy) if xy else np.zeros((1, 2)))
+    r.prism = int(bool(mesh.get("prism")))
+    r.height = float(mesh.get("height", 0.0))
+    r.n_pos, r.pos = len(pos), _ptr(pos, C.c_double)
+    r.n_tri, r.tri, r.tri_surface = len(tri), _ptr(tri, C.c_int32), _ptr(ts, C.c_int32)
+    r.n_fits, r.fits = len(mesh["fits"]), fits
+    r.n_regions = len(regions)
+    r.region_cycles, r.cycle_off, r.cycle_xy = _ptr(rc, C.c_int32), _ptr(off, C.c_int32), _ptr(xy, C.c_double)
+    return r, [pos, tri, ts, fits, rc, off, xy]
+
+
+def _solids_struct(solids):
+    arr = (ObSolid * max(1, len(solids)))()
+    keep = []
+    for i, s in enumerate(solids):
+        pl = np.ascontiguousarray(np.asarray(s["planes"], dtype=np.float64).reshape(-1, 4))
+        keep.append(pl)
+        arr[i].n_planes = len(pl)
+        arr[i].planes = _ptr(pl, C.c_double)
+        for k in range(3):
+            arr[i].lo[k], arr[i].hi[k] = float(s["lo"][k]), float(s["hi"][k])
+    return arr, keep
+
+
+def build_room(mesh, c, dx, solids=()):
+    """fdtd.rs:289-377 (build_dx) for a general room, column_fill (:385-422)
+    for meshes, rigid convex solids (:424-461)."""
+    L = lib()
+    r, keep = _mesh_struct(mesh)
+    sa, keep2 = _solids_struct(solids)
+    g = ObGrid()
+    st = L.ob_build_room(C.byref(r), sa, len(solids), c, dx, C.byref(g))
+    if st != 0:
+        L.ob_grid_free(C.byref(g))
+        raise RuntimeError(f"ob_build_room failed ({st})")
+    try:
+        return _grid_from(g)
+    finally:
+        L.ob_grid_free(C.byref(g))
+
+
+def parity_inside(mesh, points):
+    """room.rs:355-367 (Room::contains for meshes) at each of ``points`` (N, 3)."""
+    r, keep = _mesh_struct(mesh)
+    pts = np.ascontiguousarray(np.asarray(points, dtype=np.float64).reshape(-1, 3))
+    L = lib()
+    base = pts.ctypes.data
+    return np.array([bool(L.ob_parity_inside(C.byref(r), C.cast(base + 24 * i, C.POINTER(C.c_double))))
+                     for i in range(len(pts))], dtype=bool)
+
+
+def ray_triangle(o, d, tri):
+    """bvh.rs:209-226; the hit's t or None."""
+    a = [np.ascontiguousarray(np.asarray(v, dtype=np.float64)) for v in (o, d, tri[0], tri[1], tri[2])]
+    t = np.zeros(1)
+    ok = lib().ob_ray_triangle(*[_ptr(v, C.c_double) for v in a], _ptr(t, C.c_double))
+    return float(t[0]) if ok else None
+
+
+def solid_from_mesh(verts, tris, rot=None, pos=(0.0, 0.0, 0.0)):
+    """fdtd.rs:433-456 (Solid::from_mesh); rot row-major, applied as rot @ v."""
+    v = np.ascontiguousarray(np.asarray(verts, dtype=np.float64).reshape(-1, 3))
+    t = np.ascontiguousarray(np.asarray(tris, dtype=np.int32).reshape(-1, 3))
+    R = np.ascontiguousarray(np.eye(3) if rot is None else np.asarray(rot, dtype=np.float64).reshape(3, 3))
+    P = np.ascontiguousarray(np.asarray(pos, dtype=np.float64))
+    planes = np.zeros((max(1, len(t)), 4))
+    lo, hi = np.zeros(3), np.zeros(3)
+    n = lib().ob_solid_from_mesh(_ptr(v, C.c_double), len(v), _ptr(t, C.c_int32), len(t), _ptr(R, C.c_double),
+                                 _ptr(P, C.c_double), _ptr(planes, C.c_double), _ptr(lo, C.c_double),
+                                 _ptr(hi, C.c_double))
+    if n < 0:
+        return None
+    return {"planes": planes[:n].copy(), "lo": lo, "hi": hi}
+
+
+def solid_contains(solid, p):
+    """fdtd.rs:458-460"""
+    sa, keep = _solids_struct([solid])
+    q = np.ascontiguousarray(np.asarray(p, dtype=np.float64))
+    return bool(lib().ob_solid_contains(sa, _ptr(q, C.c_double)))
 
 
 def source_signal(f_max, dt, steps):

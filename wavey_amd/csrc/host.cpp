@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "wavey_fdtd.h"
+#include "grid_internal.h"
 #include "wv_internal.h"
 
 namespace wv {
@@ -41,21 +42,8 @@ int pick_threads(int threads) {
     return h ? (int)std::min(h, 64u) : 4;
 }
 
-// Runs f(lo, hi) over [0, n) split into contiguous chunks, one per thread.
-template <class F>
-static void parallel_chunks(int64_t n, int threads, F f) {
-    int t = std::max(1, std::min<int>(threads, (int)std::max<int64_t>(1, n)));
-    if (t == 1) { f(0, n, 0); return; }
-    std::vector<std::thread> pool;
-    for (int k = 0; k < t; ++k) {
-        int64_t lo = n * k / t, hi = n * (k + 1) / t;
-        pool.emplace_back([=] { f(lo, hi, k); });
-    }
-    for (auto& th : pool) th.join();
-}
-
 // fdtd.rs:539-571 one branch -> slot constants; count faces share the fit.
-static wv_slot slot_of(double r, double m, double k, double count, double dt) {
+wv_slot slot_of(double r, double m, double k, double count, double dt) {
     r = r / count;
     m = m / count;
     k = k / count;
@@ -75,16 +63,56 @@ static wv_slot slot_of(double r, double m, double k, double count, double dt) {
     return s;
 }
 
+bool fits_equal(const wv_admittance& a, const wv_admittance& b) {
+    if (a.beta0 != b.beta0 || a.n_branches != b.n_branches) return false;
+    for (uint32_t i = 0; i < a.n_branches; ++i)
+        if (a.branches[i].r != b.branches[i].r || a.branches[i].m != b.branches[i].m ||
+            a.branches[i].k != b.branches[i].k)
+            return false;
+    return true;
+}
+
+// fdtd.rs:539-571: faces grouped by fit in order of first appearance; each
+// group gives its conductance slot (beta0 > 0) and one slot per branch, with
+// the impedances divided by the group's face count.
+void node_slots(const int* canon_fits, int n_faces, const wv_admittance* fits, double dt,
+                std::vector<wv_slot>& v) {
+    int grp[6], cnt[6], ng = 0;
+    for (int f = 0; f < n_faces; ++f) {
+        int fit = canon_fits[f];
+        int j = 0;
+        while (j < ng && grp[j] != fit) ++j;
+        if (j < ng) cnt[j]++;
+        else { grp[ng] = fit; cnt[ng] = 1; ++ng; }
+    }
+    v.clear();
+    for (int j = 0; j < ng; ++j) {
+        const wv_admittance& a = fits[grp[j]];
+        double n = (double)cnt[j];
+        if (a.beta0 > 0.0) v.push_back(slot_of(1.0 / a.beta0, 0.0, 0.0, n, dt));
+        for (uint32_t b = 0; b < a.n_branches; ++b)
+            v.push_back(slot_of(a.branches[b].r, a.branches[b].m, a.branches[b].k, n, dt));
+    }
+}
+
+void finish_view(wv_built_grid& g, const int64_t n[3], const double origin[3], double dx, double dt) {
+    wv_grid& v = g.view;
+    for (int k = 0; k < 3; ++k) {
+        v.n[k] = (uint64_t)n[k];
+        v.origin[k] = origin[k];
+    }
+    v.dx = dx;
+    v.dt = dt;
+    v.flags = g.flags.data();
+    v.n_boundary = g.b_cell.size();
+    v.b_cell = g.b_cell.data();
+    v.b_start = g.b_start.data();
+    v.slots = g.slots.data();
+}
+
 }  // namespace wv
 
 using namespace wv;
-
-struct wv_built_grid {
-    wv_grid view{};
-    std::vector<uint8_t> flags;
-    std::vector<uint32_t> b_cell, b_start;
-    std::vector<wv_slot> slots;
-};
 
 namespace {
 
@@ -136,15 +164,6 @@ struct BoxGrid {
         }
     }
 };
-
-bool fits_equal(const wv_admittance& a, const wv_admittance& b) {
-    if (a.beta0 != b.beta0 || a.n_branches != b.n_branches) return false;
-    for (uint32_t i = 0; i < a.n_branches; ++i)
-        if (a.branches[i].r != b.branches[i].r || a.branches[i].m != b.branches[i].m ||
-            a.branches[i].k != b.branches[i].k)
-            return false;
-    return true;
-}
 
 // A node's missing-face fits in direction order, 8 bits per face (fit + 1),
 // zero-terminated: the key of its slot list.
@@ -239,23 +258,13 @@ extern "C" wv_status wv_grid_build_box(const wv_box_room* room, double c, double
     auto slots_for = [&](uint64_t key) -> const std::vector<wv_slot>& {
         auto it = cache.find(key);
         if (it != cache.end()) return it->second;
-        int grp[6], cnt[6], ng = 0;
-        for (int f = 0; f < 6; ++f) {
-            int fit = (int)((key >> (8 * f)) & 0xff) - 1;
-            if (fit < 0) break;
-            int j = 0;
-            while (j < ng && grp[j] != fit) ++j;
-            if (j < ng) cnt[j]++;
-            else { grp[ng] = fit; cnt[ng] = 1; ++ng; }
+        int faces[6], nf = 0;
+        while (nf < 6 && ((key >> (8 * nf)) & 0xff)) {
+            faces[nf] = (int)((key >> (8 * nf)) & 0xff) - 1;
+            ++nf;
         }
         std::vector<wv_slot> v;
-        for (int j = 0; j < ng; ++j) {
-            const wv_admittance& a = room->fits[grp[j]];
-            double n = (double)cnt[j];
-            if (a.beta0 > 0.0) v.push_back(slot_of(1.0 / a.beta0, 0.0, 0.0, n, dt));
-            for (uint32_t b = 0; b < a.n_branches; ++b)
-                v.push_back(slot_of(a.branches[b].r, a.branches[b].m, a.branches[b].k, n, dt));
-        }
+        node_slots(faces, nf, room->fits, dt, v);
         return cache.emplace(key, std::move(v)).first->second;
     };
     size_t nb_total = 0;
@@ -271,18 +280,8 @@ extern "C" wv_status wv_grid_build_box(const wv_box_room* room, double c, double
             g->slots.insert(g->slots.end(), sl.begin(), sl.end());
         }
     g->b_start.push_back((uint32_t)g->slots.size());
-    wv_grid& v = g->view;
-    for (int k = 0; k < 3; ++k) {
-        v.n[k] = (uint64_t)B.n[k];
-        v.origin[k] = B.origin;
-    }
-    v.dx = dx;
-    v.dt = dt;
-    v.flags = g->flags.data();
-    v.n_boundary = g->b_cell.size();
-    v.b_cell = g->b_cell.data();
-    v.b_start = g->b_start.data();
-    v.slots = g->slots.data();
+    const double org[3] = {B.origin, B.origin, B.origin};
+    finish_view(*g, B.n, org, dx, dt);
     *out = g.release();
     return WV_OK;
 }

@@ -73,6 +73,20 @@ class WvBoxRoom(C.Structure):
                 ("patch_fit", C.POINTER(C.c_uint32))]
 
 
+class WvRoomMesh(C.Structure):
+    _fields_ = [("prism", C.c_int32), ("height", C.c_double), ("n_positions", C.c_uint32),
+                ("positions", C.POINTER(C.c_double)), ("n_triangles", C.c_uint32),
+                ("triangles", C.POINTER(C.c_uint32)), ("tri_surface", C.POINTER(C.c_uint32)),
+                ("n_fits", C.c_uint32), ("fits", C.POINTER(WvAdmittance)), ("n_regions", C.c_uint32),
+                ("region_cycles", C.POINTER(C.c_uint32)), ("cycle_off", C.POINTER(C.c_uint32)),
+                ("cycle_xy", C.POINTER(C.c_double))]
+
+
+class WvSolid(C.Structure):
+    _fields_ = [("n_planes", C.c_uint32), ("planes", C.POINTER(C.c_double)), ("lo", C.c_double * 3),
+                ("hi", C.c_double * 3)]
+
+
 PROGRESS_FN = C.CFUNCTYPE(None, C.c_double, C.c_void_p)
 
 class WvWaveSource(C.Structure):
@@ -88,7 +102,7 @@ EXPORTS = [
     "wv_fit_real_impedance", "wv_version", "wv_footprint_estimate", "wv_batch_create", "wv_batch_add",
     "wv_batch_autotune", "wv_batch_set_tile", "wv_batch_run", "wv_batch_result", "wv_batch_info",
     "wv_batch_timing", "wv_batch_free", "wv_serve_create", "wv_serve_submit", "wv_serve_wait", "wv_serve_stats",
-    "wv_serve_free", "wv_wave_job",
+    "wv_serve_free", "wv_wave_job", "wv_grid_build_room", "wv_solid_from_mesh", "wv_grid_spacing",
 ]
 
 _lib = None
@@ -136,6 +150,14 @@ def lib():
             L.wv_plan_free.restype = None
             L.wv_grid_build_box.argtypes = [P(WvBoxRoom), C.c_double, C.c_double, C.c_int, P(C.c_void_p)] + errbuf
             L.wv_grid_build_box.restype = st
+            L.wv_grid_build_room.argtypes = [P(WvRoomMesh), P(WvSolid), sz, C.c_double, C.c_double, C.c_int,
+                                             P(C.c_void_p)] + errbuf
+            L.wv_grid_build_room.restype = st
+            L.wv_solid_from_mesh.argtypes = [dp, C.c_uint32, P(C.c_uint32), C.c_uint32, dp, dp, dp, P(C.c_uint32),
+                                             dp, dp]
+            L.wv_solid_from_mesh.restype = st
+            L.wv_grid_spacing.argtypes = [C.c_double, C.c_double, C.c_double, P(WvSolid), sz]
+            L.wv_grid_spacing.restype = C.c_double
             L.wv_built_grid_view.argtypes = [C.c_void_p]
             L.wv_built_grid_view.restype = P(WvGrid)
             L.wv_grid_free.argtypes = [C.c_void_p]
@@ -227,6 +249,71 @@ def _admittances(fits):
     return arr, keep
 
 
+def _room_mesh(mesh):
+    u32 = lambda a: a.ctypes.data_as(C.POINTER(C.c_uint32))  # noqa: E731
+    r = WvRoomMesh()
+    pos = np.ascontiguousarray(np.asarray(mesh["positions"], dtype=np.float64).reshape(-1, 3))
+    tri = np.ascontiguousarray(np.asarray(mesh["triangles"], dtype=np.uint32).reshape(-1, 3))
+    ts = np.ascontiguousarray(np.asarray(mesh["tri_surface"], dtype=np.uint32).reshape(-1))
+    fits, keep = _admittances(mesh["fits"])
+    prism = bool(mesh.get("prism", False))
+    regions = mesh.get("regions", []) if prism else []
+    rc, off, xy = [0], [0], []
+    for reg in regions:
+        for cyc in reg:
+            cyc = np.asarray(cyc, dtype=np.float64).reshape(-1, 2)
+            xy.append(cyc)
+            off.append(off[-1] + len(cyc))
+        rc.append(len(off) - 1)
+    rc = np.array(rc, dtype=np.uint32)
+    off = np.array(off, dtype=np.uint32)
+    xy = np.ascontiguousarray(np.concatenate(xy) if xy else np.zeros((1, 2)))
+    r.prism = int(prism)
+    r.height = float(mesh.get("height", 0.0))
+    r.n_positions, r.positions = len(pos), _dp(pos)
+    r.n_triangles, r.triangles, r.tri_surface = len(tri), u32(tri), u32(ts)
+    r.n_fits, r.fits = len(mesh["fits"]), C.cast(fits, C.POINTER(WvAdmittance))
+    r.n_regions = len(regions)
+    r.region_cycles, r.cycle_off, r.cycle_xy = u32(rc), u32(off), _dp(xy)
+    return r, keep + [pos, tri, ts, fits, rc, off, xy]
+
+
+def _solids(solids):
+    arr = (WvSolid * max(1, len(solids)))()
+    keep = []
+    for i, s in enumerate(solids):
+        pl = np.ascontiguousarray(np.asarray(s["planes"], dtype=np.float64).reshape(-1, 4))
+        keep.append(pl)
+        arr[i].n_planes = len(pl)
+        arr[i].planes = _dp(pl)
+        for k in range(3):
+            arr[i].lo[k], arr[i].hi[k] = float(s["lo"][k]), float(s["hi"][k])
+    return arr, keep
+
+
+def solid_from_mesh(verts, tris, rot=None, pos=(0.0, 0.0, 0.0)):
+    """Solid::from_mesh (fdtd.rs:433-456): a convex mesh (outward winding)
+    turned by ``rot`` (3x3, applied as rot @ v) and moved by ``pos``."""
+    v = np.ascontiguousarray(np.asarray(verts, dtype=np.float64).reshape(-1, 3))
+    t = np.ascontiguousarray(np.asarray(tris, dtype=np.uint32).reshape(-1, 3))
+    R = np.ascontiguousarray(np.eye(3) if rot is None else np.asarray(rot, dtype=np.float64).reshape(3, 3))
+    P = np.ascontiguousarray(np.asarray(pos, dtype=np.float64).reshape(3))
+    planes = np.zeros((max(1, len(t)), 4))
+    lo, hi = np.zeros(3), np.zeros(3)
+    n = C.c_uint32()
+    st = lib().wv_solid_from_mesh(_dp(v), len(v), t.ctypes.data_as(C.POINTER(C.c_uint32)), len(t), _dp(R), _dp(P),
+                                  _dp(planes), C.byref(n), _dp(lo), _dp(hi))
+    if st != 0:
+        return None
+    return {"planes": planes[:n.value].copy(), "lo": lo, "hi": hi}
+
+
+def grid_spacing(c, f_max, ppw, solids=()):
+    """FdtdGrid::build_with's spacing (fdtd.rs:282-286)."""
+    sa, keep = _solids(solids)
+    return float(lib().wv_grid_spacing(c, f_max, ppw, sa, len(solids)))
+
+
 @dataclass
 class FdtdGrid:
     """Voxelized room (fdtd.rs:196-209). Arrays are numpy; ``view()`` is the C struct."""
@@ -266,6 +353,29 @@ class FdtdGrid:
         h = C.c_void_p()
         err = C.create_string_buffer(512)
         _check(L.wv_grid_build_box(C.byref(r), c, dx, threads, C.byref(h), err, 512), err)
+        return cls._from_built(h)
+
+    @classmethod
+    def build_room(cls, mesh, c, dx, solids=(), threads=0):
+        """General room with rigid convex solids (``wv_room_mesh``,
+        ``wv_solid``): fdtd.rs:289-461.
+
+        ``mesh`` = {"positions": (V, 3), "triangles": (T, 3), "tri_surface":
+        (T,), "fits": [(beta0, [(r, m, k), ...]), ...] one per surface,
+        "prism": bool, "height": h, "regions": [[outer (n, 2), hole, ...],
+        ...]}; ``solids`` = [{"planes": (P, 4), "lo": (3,), "hi": (3,)}].
+        """
+        L = lib()
+        r, keep = _room_mesh(mesh)
+        sa, keep2 = _solids(solids)
+        h = C.c_void_p()
+        err = C.create_string_buffer(512)
+        _check(L.wv_grid_build_room(C.byref(r), sa, len(solids), c, dx, threads, C.byref(h), err, 512), err)
+        return cls._from_built(h)
+
+    @classmethod
+    def _from_built(cls, h):
+        L = lib()
         try:
             v = L.wv_built_grid_view(h).contents
             n = tuple(int(v.n[k]) for k in range(3))

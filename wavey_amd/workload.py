@@ -24,8 +24,17 @@ class Case:
     runs: list      # [(source index, component index, taps)]
 
 
-def build_case(cfg, threads=0, steps=None):
-    g = fdtd.FdtdGrid.build_box(cfg["room"], cfg["c"], cfg["dx"], threads)
+def build_case(cfg, threads=0, steps=None, builder="box"):
+    """Grid, source signal, receivers and source runs of a room config.
+    ``builder``: "box" (the box-room builder), "mesh" or "prism" (the same
+    room as triangles through the general-room builder: same grid)."""
+    if builder == "box":
+        g = fdtd.FdtdGrid.build_box(cfg["room"], cfg["c"], cfg["dx"], threads)
+    else:
+        from . import meshes
+
+        mesh = meshes.box_room(cfg["room"], prism=(builder == "prism"))
+        g = fdtd.FdtdGrid.build_room(mesh, cfg["c"], cfg["dx"], threads=threads)
     T = cfg["steps"] if steps is None else steps
     signal = fdtd.source_signal(cfg["f_src"], g.dt, T)
     receivers = []
