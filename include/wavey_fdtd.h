@@ -250,6 +250,25 @@ wv_status wv_wave_job(const wv_grid* grid, const wv_wave_source* sources, size_t
                       const volatile int32_t* cancel, double* transfer_out, char* err,
                       size_t err_len);
 
+/*
+ * A cabinet's free-field calibration (CabinetCalibration, calibrate_cabinet:
+ * fdtd.rs:873-955): verts/tris relative to the enclosure centre (outward
+ * winding), acoustic centre ac in that frame, +x on axis. Builds a cubic box
+ * with matched walls around it, runs the wave solver once with the cabinet
+ * as a rigid solid and once without, and writes on_axis_out [nf] (gain of
+ * cabinet + driver over the bare driver, complex (re, im)), angles_out [12]
+ * (0, 30, ... 330 degrees, horizontal, +90 = left) and directivity_out
+ * [12][nf] (complex, relative to on axis), faded toward 1 below the
+ * frequency where the cabinet is half a radian across. batched = 1 runs the
+ * two grids as one batch, 0 one after another; the results are the same.
+ * WV_ERR_INVALID where the reference returns None.
+ */
+wv_status wv_calibrate_cabinet(const double* verts, uint32_t n_verts, const uint32_t* tris, uint32_t n_tris,
+                               const double ac[3], double c, double f_max, double ppw, double cells_across,
+                               const double* freqs, size_t nf, int precision, int batched,
+                               const volatile int32_t* cancel, double* on_axis_out, double* angles_out,
+                               double* directivity_out, char* err, size_t err_len);
+
 /* ------------------------------------------------------------ grid build */
 
 /* impedance_fit.rs:27-44 (Branch, AdmittanceFit). */
@@ -352,6 +371,9 @@ wv_status wv_grid_build_room(const wv_room_mesh* room, const wv_solid* solids, s
 void wv_source_signal(double f_max, double dt, size_t steps, double* out);
 /* fdtd.rs:245-272; returns 0 and *cell, or WV_ERR_INVALID when none (None). */
 wv_status wv_nearest_air(const wv_grid* g, const double p[3], uint64_t* cell);
+/* fdtd.rs:497-512: where a driver on a cabinet's baffle meets the air, from
+ * p (inside the cabinet) along axis; p itself when no air is met within 2 m. */
+void wv_baffle_point(const wv_grid* g, const double p[3], const double axis[3], double out[3]);
 /* fdtd.rs:467-492; *n_taps <= 8; WV_ERR_INVALID when None. */
 wv_status wv_taps_at(const wv_grid* g, const double p[3], wv_tap taps[8], size_t* n_taps);
 /* fdtd.rs:730-743: comps[4][2], comp_len[4], *n_comps = 1 or 4. None -> WV_ERR_INVALID. */
@@ -363,6 +385,9 @@ void wv_component_weights(const double a0[2], const double a[6], double k, doubl
 void wv_to_transfer(const double* spectra, size_t n_spec, const double* signal, size_t steps,
                     double dt, double dx, const double* freqs, size_t nf, double c, double r_ref,
                     const double* h0, double* out);
+/* fdtd.rs:847-862: power-complementary crossfade of the image-source and
+ * wave fields with wave weight w; complex values as (re, im). */
+void wv_blend(const double ism[2], const double wave[2], double w, double out[2]);
 
 /* ------------------------------------------------------------- materials */
 

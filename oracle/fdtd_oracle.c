@@ -722,6 +722,50 @@ int64_t ob_nearest_air(const ob_grid* g, const double p[3]) {
     return -1;
 }
 
+/* fdtd.rs:497-512: from p along axis in steps of dx/20 (up to 2 m) to the
+ * first point whose nearest node is air; the face halfway back from there. */
+void ob_baffle_point(const ob_grid* g, const double p[3], const double axis[3], double out[3]) {
+    const double step = g->dx * 0.05;
+    const int64_t kmax = (int64_t)(2.0 / step);
+    for (int64_t k = 0; k < kmax; ++k) {
+        double q[3], r[3];
+        int neg = 0;
+        for (int i = 0; i < 3; ++i) {
+            q[i] = p[i] + axis[i] * ((double)k * step);
+            r[i] = round((q[i] - g->origin[i]) / g->dx);
+            neg |= r[i] < 0.0;
+        }
+        if (neg) continue;
+        int64_t x = (int64_t)r[0], y = (int64_t)r[1], z = (int64_t)r[2];
+        if (x < g->n[0] && y < g->n[1] && z < g->n[2] && (g->flags[gidx(g, x, y, z)] & AIR)) {
+            double m = fmin(g->dx * 0.5, (double)k * step);
+            for (int i = 0; i < 3; ++i) out[i] = q[i] - axis[i] * m;
+            return;
+        }
+    }
+    for (int i = 0; i < 3; ++i) out[i] = p[i];
+}
+
+/* fdtd.rs:847-862: power-complementary crossfade of two complex fields. */
+void ob_blend(const double ism[2], const double wave[2], double w, double out[2]) {
+    if (w <= 0.0) { out[0] = ism[0]; out[1] = ism[1]; return; }
+    if (w >= 1.0) { out[0] = wave[0]; out[1] = wave[1]; return; }
+    double ni = ism[0] * ism[0] + ism[1] * ism[1], nw = wave[0] * wave[0] + wave[1] * wave[1];
+    double target = (1.0 - w) * ni + w * nw;
+    double pr = ism[0] * (1.0 - w) + wave[0] * w, pi = ism[1] * (1.0 - w) + wave[1] * w;
+    double m2 = pr * pr + pi * pi;
+    if (m2 < 1e-6 * target) {
+        const double* d = nw * w >= ni * (1.0 - w) ? wave : ism;
+        double n = fmax(hypot(d[0], d[1]), 1e-300), st = sqrt(target);
+        out[0] = d[0] / n * st;
+        out[1] = d[1] / n * st;
+        return;
+    }
+    double s = sqrt(target / m2);
+    out[0] = pr * s;
+    out[1] = pi * s;
+}
+
 /* fdtd.rs:467-492 */
 int ob_taps_at(const ob_grid* g, const double p[3], int64_t* cells, double* weights) {
     double q[3], f[3], b[3];
@@ -931,4 +975,135 @@ int ob_run_fields(const ob_grid_view* g, const int64_t* tap_cells, const double*
                   double* prev_out, double* state_out) {
     return run_loop(g, tap_cells, tap_w, n_taps, signal, steps, NULL, 0, NULL, threads, NULL, p_out,
                     prev_out, state_out);
+}
+
+/* --------------------------------------------------- cabinet calibration */
+
+static ob_grid_view view_of(const ob_grid* g) {
+    ob_grid_view v;
+    for (int k = 0; k < 3; ++k) v.n[k] = g->n[k];
+    v.dx = g->dx;
+    v.dt = g->dt;
+    v.flags = g->flags;
+    v.nb = g->nb;
+    v.b_cell = g->b_cell;
+    v.b_start = g->b_start;
+    v.slots = g->slots;
+    return v;
+}
+
+/* fdtd.rs:890-955 (calibrate_cabinet), with run_cpu_with as the runner.
+ * on_axis_out [nf] and dir_out [12][nf] are complex (re, im); angles_out
+ * [12]. Returns 0, -1 when a step of the reference returns None, -2 on
+ * allocation failure. */
+int ob_calibrate_cabinet(const double* verts, int32_t n_verts, const int32_t* tris, int32_t n_tris,
+                         const double ac[3], double c, double f_max, double ppw, double cells_across,
+                         const double* freqs, int32_t nf, int threads, double* on_axis_out,
+                         double* angles_out, double* dir_out) {
+    double reach = 0.1;
+    for (int32_t i = 0; i < n_verts; ++i) {
+        double d[3];
+        v3sub(verts + 3 * i, ac, d);
+        reach = fmax(reach, sqrt(v3dot(d, d)));
+    }
+    const double f_cab = 0.5 * c / (2.0 * PI * reach);
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int32_t i = 0; i < n_verts; ++i)
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = fmin(lo[k], verts[3 * i + k]);
+            hi[k] = fmax(hi[k], verts[3 * i + k]);
+        }
+    const double ext = fmin(fmin(hi[0] - lo[0], hi[1] - lo[1]), hi[2] - lo[2]);
+    const double dx = fmin(c / (f_max * ppw), ext / cells_across);
+    const double max_half = 0.5 * dx * 160.0;
+    const double r_m = fmax(reach + 1.0, 3.0);
+    const double half = fmax(fmin(0.5 * c / f_cab, max_half), r_m + 1.5);
+    const double side = 2.0 * half;
+    /* The absorbing box: a square plan extruded to `side`, matched walls. */
+    double pos[8][3];
+    for (int i = 0; i < 8; ++i) {
+        pos[i][0] = (i & 1) ? side : 0.0;
+        pos[i][1] = (i & 2) ? side : 0.0;
+        pos[i][2] = (i & 4) ? side : 0.0;
+    }
+    static const int32_t box_tris[12][3] = {{0, 2, 3}, {0, 3, 1}, {4, 5, 7}, {4, 7, 6}, {0, 1, 5}, {0, 5, 4},
+                                            {2, 6, 7}, {2, 7, 3}, {0, 4, 6}, {0, 6, 2}, {1, 3, 7}, {1, 7, 5}};
+    int32_t tri_surf[12];
+    for (int i = 0; i < 12; ++i) tri_surf[i] = i / 2;
+    ob_fit fits[6];
+    memset(fits, 0, sizeof(fits));
+    for (int i = 0; i < 6; ++i) fits[i].beta0 = 1.0;
+    const double plan[8] = {0.0, 0.0, side, 0.0, side, side, 0.0, side};
+    const int32_t region_cycles[2] = {0, 1}, cycle_off[2] = {0, 4};
+    ob_mesh_room room = {1, side, 8, &pos[0][0], 12, &box_tris[0][0], tri_surf, 6, fits, 1, region_cycles, cycle_off, plan};
+    const double p0[3] = {half, half, half};
+    const double rot[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    double shift[3];
+    v3sub(p0, ac, shift);
+    double* planes = (double*)malloc(sizeof(double) * 4 * (size_t)(n_tris > 0 ? n_tris : 1));
+    ob_solid cab;
+    cab.n_planes = ob_solid_from_mesh(verts, n_verts, tris, n_tris, rot, shift, planes, cab.lo, cab.hi);
+    cab.planes = planes;
+    if (cab.n_planes < 0) { free(planes); return -1; }
+    ob_grid with, bare;
+    int rc = ob_build_room(&room, &cab, 1, c, dx, &with);
+    if (rc == 0) rc = ob_build_room(&room, NULL, 0, c, dx, &bare);
+    free(planes);
+    if (rc != 0) return rc == -2 ? -2 : -1;
+    double src[3];
+    const double axis[3] = {1.0, 0.0, 0.0};
+    ob_baffle_point(&with, p0, axis, src);
+    int64_t tw_c[8], tb_c[8], rx_w[12], rx_b[12];
+    double tw_w[8], tb_w[8];
+    int ntw = ob_taps_at(&with, src, tw_c, tw_w), ntb = ob_taps_at(&bare, src, tb_c, tb_w);
+    int ok = ntw > 0 && ntb > 0;
+    for (int a = 0; a < 12 && ok; ++a) {
+        angles_out[a] = (double)a * 30.0;
+        const double r = angles_out[a] * (PI / 180.0);
+        const double d[3] = {cos(r) * r_m, sin(r) * r_m, 0.0 * r_m};
+        const double pt[3] = {p0[0] + d[0], p0[1] + d[1], p0[2] + d[2]};
+        rx_w[a] = ob_nearest_air(&with, pt);
+        rx_b[a] = ob_nearest_air(&bare, pt);
+        ok = rx_w[a] >= 0 && rx_b[a] >= 0;
+    }
+    if (!ok) {
+        ob_grid_free(&with);
+        ob_grid_free(&bare);
+        return -1;
+    }
+    const double duration = side * sqrt(3.0) * 2.0 / c + 12.0 * 0.342 / f_max;
+    const int64_t steps = (int64_t)ceil(duration / with.dt);
+    double* signal = (double*)malloc(sizeof(double) * (size_t)steps);
+    double* sw = (double*)malloc(sizeof(double) * 2 * 12 * (size_t)nf);
+    double* sb = (double*)malloc(sizeof(double) * 2 * 12 * (size_t)nf);
+    ob_source_signal(f_max, with.dt, steps, signal);
+    ob_grid_view vw = view_of(&with), vb = view_of(&bare);
+    ob_run(&vw, tw_c, tw_w, ntw, signal, steps, rx_w, 12, freqs, nf, NULL, sw, threads, NULL);
+    ob_run(&vb, tb_c, tb_w, ntb, signal, steps, rx_b, 12, freqs, nf, NULL, sb, threads, NULL);
+    /* Below f_cab the measured ratio fades toward 1 as (f / f_cab)^2. */
+    for (int32_t f = 0; f < nf; ++f) {
+        const double x = freqs[f] / f_cab;
+        const double w = fmin(x * x, 1.0);
+        for (int r = 0; r < 12; ++r) {
+            cplx m = cdiv(cmk(sw[2 * (r * nf + f)], sw[2 * (r * nf + f) + 1]),
+                          cmk(sb[2 * (r * nf + f)], sb[2 * (r * nf + f) + 1]));
+            cplx q = from_polar(pow(hypot(m.re, m.im), w), atan2(m.im, m.re) * w);
+            dir_out[2 * (r * nf + f)] = q.re;
+            dir_out[2 * (r * nf + f) + 1] = q.im;
+        }
+        on_axis_out[2 * f] = dir_out[2 * f];
+        on_axis_out[2 * f + 1] = dir_out[2 * f + 1];
+        const cplx on = cmk(on_axis_out[2 * f], on_axis_out[2 * f + 1]);
+        for (int r = 0; r < 12; ++r) {
+            cplx q = cdiv(cmk(dir_out[2 * (r * nf + f)], dir_out[2 * (r * nf + f) + 1]), on);
+            dir_out[2 * (r * nf + f)] = q.re;
+            dir_out[2 * (r * nf + f) + 1] = q.im;
+        }
+    }
+    free(signal);
+    free(sw);
+    free(sb);
+    ob_grid_free(&with);
+    ob_grid_free(&bare);
+    return 0;
 }

@@ -7,10 +7,9 @@
  * never as the thing measured or shipped. The product (wavey_amd/) never
  * links or calls it.
  *
- * Each function cites the reference file:line it restates. Where the
- * reference reaches a function through geometry this restatement does not
- * have (BVH ray casts, Room::contains on sketches), the box-room special
- * case is restated and the semantics are named at the function.
+ * Each function cites the reference file:line it restates. Ray casts test
+ * every triangle instead of walking the reference's BVH (see the general
+ * rooms below).
  */
 #ifndef WAVEY_FDTD_ORACLE_H
 #define WAVEY_FDTD_ORACLE_H
@@ -78,6 +77,10 @@ double ob_boundary_update(const ob_slot* slots, int n, double* states, double p,
 void ob_source_signal(double f_max, double dt, int64_t steps, double* out);
 /* fdtd.rs:245-272; returns -1 when no air node is within 16 cells. */
 int64_t ob_nearest_air(const ob_grid* g, const double p[3]);
+/* fdtd.rs:497-512 (FdtdGrid::baffle_point). */
+void ob_baffle_point(const ob_grid* g, const double p[3], const double axis[3], double out[3]);
+/* fdtd.rs:847-862 (blend); complex values as (re, im). */
+void ob_blend(const double ism[2], const double wave[2], double w, double out[2]);
 /* fdtd.rs:467-492; returns the number of taps (0 = None). */
 int ob_taps_at(const ob_grid* g, const double p[3], int64_t* cells, double* weights);
 /* fdtd.rs:730-743; returns the number of components (0 = None).
@@ -176,6 +179,14 @@ int ob_solid_contains(const ob_solid* s, const double p[3]);
 /* fdtd.rs:289-377 (build_dx) with column_fill (:385-422). */
 int ob_build_room(const ob_mesh_room* room, const ob_solid* solids, int32_t n_solids, double c,
                   double dx, ob_grid* out);
+
+/* fdtd.rs:890-955 (calibrate_cabinet) with run_cpu_with as the runner:
+ * on_axis_out [nf] and dir_out [12][nf] complex (re, im), angles_out [12]
+ * degrees. 0, or -1 where the reference returns None. */
+int ob_calibrate_cabinet(const double* verts, int32_t n_verts, const int32_t* tris, int32_t n_tris,
+                         const double ac[3], double c, double f_max, double ppw, double cells_across,
+                         const double* freqs, int32_t nf, int threads, double* on_axis_out,
+                         double* angles_out, double* dir_out);
 
 #ifdef __cplusplus
 }

@@ -122,6 +122,14 @@ def lib():
         L.ob_solid_contains.restype = C.c_int
         L.ob_build_room.argtypes = [P(ObMeshRoom), P(ObSolid), C.c_int32, C.c_double, C.c_double, P(ObGrid)]
         L.ob_build_room.restype = C.c_int
+        L.ob_baffle_point.argtypes = [P(ObGrid), P(C.c_double), P(C.c_double), P(C.c_double)]
+        L.ob_baffle_point.restype = None
+        L.ob_blend.argtypes = [P(C.c_double), P(C.c_double), C.c_double, P(C.c_double)]
+        L.ob_blend.restype = None
+        L.ob_calibrate_cabinet.argtypes = [P(C.c_double), C.c_int32, P(C.c_int32), C.c_int32, P(C.c_double), C.c_double,
+                                           C.c_double, C.c_double, C.c_double, P(C.c_double), C.c_int32, C.c_int,
+                                           P(C.c_double), P(C.c_double), P(C.c_double)]
+        L.ob_calibrate_cabinet.restype = C.c_int
         _lib = L
     return _lib
 
@@ -485,3 +493,41 @@ def run_fields(grid, taps, signal, threads=0):
     lib().ob_run_fields(C.byref(v), _ptr(tc, C.c_int64), _ptr(tw, C.c_double), len(tc), _ptr(sig, C.c_double),
                         len(sig), threads, _ptr(p, C.c_double), _ptr(prev, C.c_double), _ptr(st, C.c_double))
     return p, prev, st[: 2 * len(grid.slots)].reshape(-1, 2)
+
+
+def baffle_point(grid, p, axis):
+    """fdtd.rs:497-512"""
+    pp = np.ascontiguousarray(p, dtype=np.float64)
+    ax = np.ascontiguousarray(axis, dtype=np.float64)
+    out = np.zeros(3)
+    g = grid._obgrid()
+    lib().ob_baffle_point(C.byref(g), _ptr(pp, C.c_double), _ptr(ax, C.c_double), _ptr(out, C.c_double))
+    return out
+
+
+def blend(ism, wave, w):
+    """fdtd.rs:847-862"""
+    a = np.array([complex(ism).real, complex(ism).imag])
+    b = np.array([complex(wave).real, complex(wave).imag])
+    out = np.zeros(2)
+    lib().ob_blend(_ptr(a, C.c_double), _ptr(b, C.c_double), float(w), _ptr(out, C.c_double))
+    return complex(out[0], out[1])
+
+
+def calibrate_cabinet(verts, tris, ac, c, f_max, ppw, cells_across, freqs, threads=0):
+    """fdtd.rs:890-955 with run_cpu_with as the runner; None where the reference returns None."""
+    v = np.ascontiguousarray(np.asarray(verts, dtype=np.float64).reshape(-1, 3))
+    t = np.ascontiguousarray(np.asarray(tris, dtype=np.int32).reshape(-1, 3))
+    a = np.ascontiguousarray(np.asarray(ac, dtype=np.float64).reshape(3))
+    fr = np.ascontiguousarray(freqs, dtype=np.float64)
+    F = len(fr)
+    on = np.zeros(2 * max(F, 1))
+    ang = np.zeros(12)
+    dr = np.zeros(2 * 12 * max(F, 1))
+    st = lib().ob_calibrate_cabinet(_ptr(v, C.c_double), len(v), _ptr(t, C.c_int32), len(t), _ptr(a, C.c_double),
+                                    float(c), float(f_max), float(ppw), float(cells_across), _ptr(fr, C.c_double), F,
+                                    threads, _ptr(on, C.c_double), _ptr(ang, C.c_double), _ptr(dr, C.c_double))
+    if st != 0:
+        return None
+    return {"on_axis": on[:2 * F].view(np.complex128).copy(), "angles_deg": ang,
+            "directivity": dr[:2 * 12 * F].view(np.complex128).reshape(12, F).copy()}

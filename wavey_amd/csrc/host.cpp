@@ -337,6 +337,54 @@ extern "C" wv_status wv_nearest_air(const wv_grid* g, const double p[3], uint64_
     return WV_ERR_INVALID;
 }
 
+extern "C" void wv_baffle_point(const wv_grid* g, const double p[3], const double axis[3], double out[3]) {
+    // fdtd.rs:497-512: march from p along the axis in steps of dx/20 (for
+    // up to 2 m) to the first point whose nearest node is air; the driver
+    // sits halfway back from there, on the face between that node and the
+    // solid one.
+    const double step = g->dx * 0.05;
+    const int64_t kmax = (int64_t)(2.0 / step);
+    for (int64_t k = 0; k < kmax; ++k) {
+        double q[3], r[3];
+        bool neg = false;
+        for (int i = 0; i < 3; ++i) {
+            q[i] = p[i] + axis[i] * ((double)k * step);
+            r[i] = std::round((q[i] - g->origin[i]) / g->dx);
+            neg = neg || r[i] < 0.0;
+        }
+        if (neg) continue;
+        const int64_t x = (int64_t)r[0], y = (int64_t)r[1], z = (int64_t)r[2];
+        if (in_grid(g, x, y, z) && (g->flags[cell_of(g, x, y, z)] & 64)) {
+            const double m = std::fmin(g->dx * 0.5, (double)k * step);
+            for (int i = 0; i < 3; ++i) out[i] = q[i] - axis[i] * m;
+            return;
+        }
+    }
+    for (int i = 0; i < 3; ++i) out[i] = p[i];
+}
+
+extern "C" void wv_blend(const double ism[2], const double wave[2], double w, double out[2]) {
+    // fdtd.rs:847-862: the phase of the complex crossfade with the energy
+    // (1 - w)|ism|^2 + w|wave|^2; where the two nearly cancel, the larger
+    // weighted field's phase.
+    if (w <= 0.0) { out[0] = ism[0]; out[1] = ism[1]; return; }
+    if (w >= 1.0) { out[0] = wave[0]; out[1] = wave[1]; return; }
+    const double ni = ism[0] * ism[0] + ism[1] * ism[1], nw = wave[0] * wave[0] + wave[1] * wave[1];
+    const double target = (1.0 - w) * ni + w * nw;
+    const double pr = ism[0] * (1.0 - w) + wave[0] * w, pi = ism[1] * (1.0 - w) + wave[1] * w;
+    const double m2 = pr * pr + pi * pi;
+    if (m2 < 1e-6 * target) {
+        const double* d = nw * w >= ni * (1.0 - w) ? wave : ism;
+        const double n = std::fmax(std::hypot(d[0], d[1]), 1e-300), st = std::sqrt(target);
+        out[0] = d[0] / n * st;
+        out[1] = d[1] / n * st;
+        return;
+    }
+    const double sc = std::sqrt(target / m2);
+    out[0] = pr * sc;
+    out[1] = pi * sc;
+}
+
 extern "C" wv_status wv_taps_at(const wv_grid* g, const double p[3], wv_tap taps[8], size_t* n_taps) {
     // fdtd.rs:467-492: trilinear weights over the 8 surrounding nodes, solid
     // ones dropped and the rest renormalised; under 1/4 total weight the

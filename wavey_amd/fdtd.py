@@ -103,6 +103,7 @@ EXPORTS = [
     "wv_batch_autotune", "wv_batch_set_tile", "wv_batch_run", "wv_batch_result", "wv_batch_info",
     "wv_batch_timing", "wv_batch_free", "wv_serve_create", "wv_serve_submit", "wv_serve_wait", "wv_serve_stats",
     "wv_serve_free", "wv_wave_job", "wv_grid_build_room", "wv_solid_from_mesh", "wv_grid_spacing",
+    "wv_baffle_point", "wv_blend", "wv_calibrate_cabinet",
 ]
 
 _lib = None
@@ -219,6 +220,14 @@ def lib():
             L.wv_wave_job.argtypes = [P(WvGrid), P(WvWaveSource), sz, dp, sz, P(C.c_uint64), sz, dp, sz, C.c_double,
                                       C.c_int, C.c_int, P(C.c_int32), dp] + errbuf
             L.wv_wave_job.restype = st
+            L.wv_baffle_point.argtypes = [P(WvGrid), dp, dp, dp]
+            L.wv_baffle_point.restype = None
+            L.wv_blend.argtypes = [dp, dp, C.c_double, dp]
+            L.wv_blend.restype = None
+            L.wv_calibrate_cabinet.argtypes = [dp, C.c_uint32, P(C.c_uint32), C.c_uint32, dp, C.c_double, C.c_double,
+                                               C.c_double, C.c_double, dp, sz, C.c_int, C.c_int, P(C.c_int32), dp,
+                                               dp, dp] + errbuf
+            L.wv_calibrate_cabinet.restype = st
             _lib = L
     return _lib
 
@@ -433,6 +442,14 @@ class FdtdGrid:
         pp = np.ascontiguousarray(p, dtype=np.float64)
         st = lib().wv_nearest_air(C.byref(self.view()), _dp(pp), C.byref(out))
         return None if st != WV_OK else int(out.value)
+
+    def baffle_point(self, p, axis):
+        """fdtd.rs:497-512"""
+        pp = np.ascontiguousarray(p, dtype=np.float64)
+        ax = np.ascontiguousarray(axis, dtype=np.float64)
+        out = np.zeros(3)
+        lib().wv_baffle_point(C.byref(self.view()), _dp(pp), _dp(ax), _dp(out))
+        return out
 
     def taps_at(self, p):
         taps = (WvTap * 8)()
@@ -874,3 +891,39 @@ def wave_job(grid: FdtdGrid, sources, signal, receivers, freqs, c, precision="fp
     if st == WV_CANCELLED:
         return None
     return (out[0::2] + 1j * out[1::2]).reshape(len(sources), R, F)
+
+
+def blend(ism, wave, w):
+    """fdtd.rs:847-862: power-complementary crossfade (complex scalars)."""
+    a = np.array([complex(ism).real, complex(ism).imag])
+    b = np.array([complex(wave).real, complex(wave).imag])
+    out = np.zeros(2)
+    lib().wv_blend(_dp(a), _dp(b), float(w), _dp(out))
+    return complex(out[0], out[1])
+
+
+def calibrate_cabinet(verts, tris, ac, c, f_max, ppw, cells_across, freqs, precision="fp32", batched=True,
+                      cancel=None):
+    """CabinetCalibration of a cabinet mesh (fdtd.rs:873-955, wv_calibrate_cabinet):
+    {"on_axis": [nf] complex, "angles_deg": [12], "directivity": [12][nf] complex},
+    or None when cancelled."""
+    v = np.ascontiguousarray(np.asarray(verts, dtype=np.float64).reshape(-1, 3))
+    t = np.ascontiguousarray(np.asarray(tris, dtype=np.uint32).reshape(-1, 3))
+    a = np.ascontiguousarray(np.asarray(ac, dtype=np.float64).reshape(3))
+    fr = np.ascontiguousarray(freqs, dtype=np.float64)
+    F = len(fr)
+    on = np.zeros(2 * max(F, 1))
+    ang = np.zeros(12)
+    dr = np.zeros(2 * 12 * max(F, 1))
+    err = C.create_string_buffer(1024)
+    prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+    st = lib().wv_calibrate_cabinet(_dp(v), len(v), t.ctypes.data_as(C.POINTER(C.c_uint32)), len(t), _dp(a),
+                                    float(c), float(f_max), float(ppw), float(cells_across), _dp(fr) if F else None,
+                                    F, prec, 1 if batched else 0,
+                                    cancel.ctypes.data_as(C.POINTER(C.c_int32)) if cancel is not None else None,
+                                    _dp(on), _dp(ang), _dp(dr), err, 1024)
+    if st == WV_CANCELLED:
+        return None
+    _check(st, err)
+    return {"on_axis": on[:2 * F].view(np.complex128).copy(), "angles_deg": ang,
+            "directivity": dr[:2 * 12 * F].view(np.complex128).reshape(12, F).copy()}
