@@ -398,6 +398,37 @@ double wv_paris(double beta_re, double beta_im, uint32_t n);
 /* material.rs:96-111 */
 double wv_fit_real_impedance(double alpha);
 
+/* spectrum.rs:6-44 (Spectrum): n = 0 is the single value values[0], else a
+ * table over freqs_hz interpolated linearly in ln f and held outside it. */
+typedef struct {
+    uint32_t n;
+    const double* freqs_hz;
+    const double* values;
+} wv_spectrum;
+/* material.rs:7-31 (Material), the fields the absorption reads: random-
+ * incidence absorption, or a normalized impedance (both parts non-NULL). */
+typedef struct {
+    wv_spectrum absorption;
+    const wv_spectrum* impedance_re;
+    const wv_spectrum* impedance_im;
+} wv_material;
+/* material.rs:72-78 (Material::absorption_at). */
+double wv_material_absorption(const wv_material* m, double f);
+/* impedance_fit.rs:34-44 (AdmittanceFit) with at most two branches. */
+typedef struct {
+    double beta0;
+    uint32_t n_branches;
+    wv_branch branches[2];
+    double band[2];
+    double rms_error, max_error;
+} wv_fit_result;
+/* impedance_fit.rs:158-235 (fit_admittance): the admittance whose Paris
+ * absorption follows the material's over band_lo..band_hi, with up to
+ * max_branches (<= 2) resonant branches; fdtd.rs:213-222 (fit_surfaces)
+ * calls it per distinct material with max_branches = 2. */
+wv_status wv_fit_admittance(const wv_material* material, double band_lo, double band_hi,
+                            uint32_t max_branches, wv_fit_result* out);
+
 /* Library version string. */
 const char* wv_version(void);
 

@@ -105,6 +105,262 @@ double ob_fit_real_impedance(double alpha) {
     return sqrt(lo * hi);
 }
 
+/* spectrum.rs:24-44 (Spectrum::eval): a scalar (n = 0, values[0]) or a table
+ * interpolated linearly over ln f, held constant outside its range. */
+double ob_spectrum_eval(const double* freqs_hz, const double* values, int32_t n, double f) {
+    if (n == 0) return values[0];
+    if (f <= freqs_hz[0]) return values[0];
+    if (f >= freqs_hz[n - 1]) return values[n - 1];
+    int32_t i = 0;
+    while (i + 1 < n && freqs_hz[i + 1] <= f) ++i; /* partition_point(x <= f) - 1 */
+    double f0 = log(freqs_hz[i]), f1 = log(freqs_hz[i + 1]);
+    double t = (log(f) - f0) / (f1 - f0);
+    return values[i] + t * (values[i + 1] - values[i]);
+}
+
+/* Rust's f64::total_cmp. */
+static int total_cmp(double a, double b) {
+    int64_t x, y;
+    memcpy(&x, &a, 8);
+    memcpy(&y, &b, 8);
+    x ^= (int64_t)(((uint64_t)(x >> 63)) >> 1);
+    y ^= (int64_t)(((uint64_t)(y >> 63)) >> 1);
+    return (x > y) - (x < y);
+}
+
+#define OB_NM_MAX 7 /* 1 + 3 * 2 parameters */
+
+typedef double (*ob_cost_fn)(const double* x, void* ctx);
+
+/* impedance_fit.rs:87-129 (nelder_mead). */
+static double nelder_mead(ob_cost_fn f, void* ctx, const double* x0, int n, double step, int iters, double* xbest) {
+    double pts[OB_NM_MAX + 1][OB_NM_MAX], vals[OB_NM_MAX + 1];
+    for (int i = 0; i <= n; ++i) {
+        for (int d = 0; d < n; ++d) pts[i][d] = x0[d];
+        if (i > 0) pts[i][i - 1] += step;
+        vals[i] = f(pts[i], ctx);
+    }
+    for (int it = 0; it < iters; ++it) {
+        /* Stable insertion sort of the vertices by value (sort_by is stable). */
+        for (int i = 1; i <= n; ++i) {
+            double pv = vals[i], pp[OB_NM_MAX];
+            memcpy(pp, pts[i], sizeof(double) * n);
+            int j = i - 1;
+            while (j >= 0 && total_cmp(vals[j], pv) > 0) {
+                vals[j + 1] = vals[j];
+                memcpy(pts[j + 1], pts[j], sizeof(double) * n);
+                --j;
+            }
+            vals[j + 1] = pv;
+            memcpy(pts[j + 1], pp, sizeof(double) * n);
+        }
+        if (fabs(vals[n] - vals[0]) < 1e-14) break;
+        double cen[OB_NM_MAX], xr[OB_NM_MAX], xe[OB_NM_MAX], xc[OB_NM_MAX];
+        for (int d = 0; d < n; ++d) {
+            double sum = -0.0;
+            for (int i = 0; i < n; ++i) sum += pts[i][d];
+            cen[d] = sum / (double)n;
+        }
+        for (int d = 0; d < n; ++d) xr[d] = cen[d] + -1.0 * (pts[n][d] - cen[d]);
+        double fr = f(xr, ctx);
+        if (fr < vals[0]) {
+            for (int d = 0; d < n; ++d) xe[d] = cen[d] + -2.0 * (pts[n][d] - cen[d]);
+            double fe = f(xe, ctx);
+            if (fe < fr) { memcpy(pts[n], xe, sizeof(double) * n); vals[n] = fe; }
+            else { memcpy(pts[n], xr, sizeof(double) * n); vals[n] = fr; }
+        } else if (fr < vals[n - 1]) {
+            memcpy(pts[n], xr, sizeof(double) * n);
+            vals[n] = fr;
+        } else {
+            double t = fr < vals[n] ? -0.5 : 0.5;
+            for (int d = 0; d < n; ++d) xc[d] = cen[d] + t * (pts[n][d] - cen[d]);
+            double fc = f(xc, ctx);
+            if (fc < fmin(vals[n], fr)) {
+                memcpy(pts[n], xc, sizeof(double) * n);
+                vals[n] = fc;
+            } else {
+                for (int i = 1; i <= n; ++i) {
+                    for (int d = 0; d < n; ++d) pts[i][d] = pts[0][d] + 0.5 * (pts[i][d] - pts[0][d]);
+                    vals[i] = f(pts[i], ctx);
+                }
+            }
+        }
+    }
+    int best = 0;
+    for (int i = 1; i <= n; ++i)
+        if (total_cmp(vals[i], vals[best]) < 0) best = i;
+    memcpy(xbest, pts[best], sizeof(double) * n);
+    return vals[best];
+}
+
+#define OB_MAX_Q 2.0 /* impedance_fit.rs:135 */
+
+/* impedance_fit.rs:139-150 (decode): ln beta0, then per branch (ln m, ln k, logit(Q / MAX_Q)). */
+static void fit_decode(const double* x, int nb, ob_fit* out) {
+    memset(out, 0, sizeof(*out));
+    out->beta0 = exp(x[0]);
+    out->nb = nb;
+    for (int b = 0; b < nb; ++b) {
+        double m = exp(x[1 + 3 * b]), k = exp(x[2 + 3 * b]);
+        double q = OB_MAX_Q / (1.0 + exp(-x[3 + 3 * b]));
+        out->br[b].r = sqrt(m * k) / fmax(q, 1e-9);
+        out->br[b].m = m;
+        out->br[b].k = k;
+    }
+}
+
+/* impedance_fit.rs:152-155 (encode_branch), with Rust's clamp. */
+static void fit_encode(double r, double m, double k, double* x) {
+    double q = sqrt(m * k) / r / OB_MAX_Q;
+    if (q < 1e-6) q = 1e-6;
+    if (q > 1.0 - 1e-6) q = 1.0 - 1e-6;
+    x[0] = log(m);
+    x[1] = log(k);
+    x[2] = log(q / (1.0 - q));
+}
+
+typedef struct { int nb; const double* freqs; const double* target; int nf; } ob_cost_ctx;
+
+static double fit_cost(const double* x, void* vctx) {
+    const ob_cost_ctx* c = (const ob_cost_ctx*)vctx;
+    ob_fit fit;
+    fit_decode(x, c->nb, &fit);
+    double s = 0.0;
+    for (int i = 0; i < c->nf; ++i) {
+        double y[2];
+        ob_admittance(&fit, c->freqs[i], y);
+        double e = ob_paris(y[0], y[1], 48) - c->target[i];
+        s += e * e;
+    }
+    return s;
+}
+
+static void fit_errors(const ob_fit* fit, const double* freqs, const double* target, int nf, double* rms,
+                       double* mx) {
+    double ss = -0.0, m = 0.0;
+    for (int i = 0; i < nf; ++i) {
+        double y[2];
+        ob_admittance(fit, freqs[i], y);
+        double e = ob_paris(y[0], y[1], 96) - target[i];
+        ss += e * e;
+        m = fmax(m, fabs(e));
+    }
+    *rms = sqrt(ss / (double)nf);
+    *mx = m;
+}
+
+/* impedance_fit.rs:158-235 (fit_admittance). Writes the fit and returns the
+ * number of branches kept; band_out = (lo, hi), err_out = (rms, max). */
+/* material.rs:72-78 (absorption_at): from the impedance when the material
+ * has one (Paris integral of R = (z c - 1)/(z c + 1), material.rs:83-93),
+ * else the absorption spectrum clamped to 0..1. */
+double ob_material_absorption(const ob_material* m, double f) {
+    if (m->has_impedance) {
+        cplx z = cmk(ob_spectrum_eval(m->imp_re.f, m->imp_re.v, m->imp_re.n, f),
+                     ob_spectrum_eval(m->imp_im.f, m->imp_im.v, m->imp_im.n, f));
+        const int N = 256;
+        double h = (PI / 2.0) / (double)N, s = -0.0;
+        for (int i = 0; i < N; ++i) {
+            double th = ((double)i + 0.5) * h;
+            double c = cos(th);
+            cplx zc = cscale(z, c);
+            cplx r = cdiv(cmk(zc.re - 1.0, zc.im), cmk(zc.re + 1.0, zc.im));
+            s += (1.0 - cnorm2(r)) * sin(2.0 * th) * h;
+        }
+        return s;
+    }
+    double a = ob_spectrum_eval(m->absorption.f, m->absorption.v, m->absorption.n, f);
+    return a < 0.0 ? 0.0 : (a > 1.0 ? 1.0 : a);
+}
+
+int ob_fit_admittance(const ob_material* mat, double band_lo, double band_hi, int max_branches, ob_fit* out,
+                      double band_out[2], double err_out[2]) {
+    const double lo = fmax(band_lo, 1.0), hi = fmax(band_hi, band_lo + 1.0);
+    enum { NF = 32 };
+    double freqs[NF], target[NF];
+    int all_small = 1;
+    for (int i = 0; i < NF; ++i) {
+        freqs[i] = lo * pow(hi / lo, (double)i / (double)(NF - 1));
+        double a = ob_material_absorption(mat, freqs[i]);
+        target[i] = a < 0.0 ? 0.0 : (a > 0.95 ? 0.95 : a);
+        all_small &= target[i] < 1e-6;
+    }
+    band_out[0] = lo;
+    band_out[1] = hi;
+    memset(out, 0, sizeof(*out));
+    if (all_small) {
+        double m = 0.0;
+        for (int i = 0; i < NF; ++i) m = fmax(m, target[i]);
+        err_out[0] = err_out[1] = m;
+        return 0;
+    }
+    double mean = -0.0;
+    for (int i = 0; i < NF; ++i) mean += target[i];
+    mean = mean / (double)NF;
+    const double beta_mean = 1.0 / ob_fit_real_impedance(fmax(mean, 1e-4));
+    int have = 0;
+    double best_rms = 0.0;
+    ob_fit best;
+    memset(&best, 0, sizeof(best));
+    for (int nb = 0; nb <= max_branches && nb <= 2; ++nb) {
+        ob_cost_ctx ctx = {nb, freqs, target, NF};
+        const int np = 1 + 3 * nb;
+        double starts[9][OB_NM_MAX];
+        int ns = 0;
+        if (nb == 0) {
+            starts[0][0] = log(beta_mean);
+            ns = 1;
+        } else {
+            static const double corners[3] = {0.3, 1.0, 3.0}, levels[3] = {0.3, 1.0, 3.0};
+            for (int ci = 0; ci < 3; ++ci)
+                for (int li = 0; li < 3; ++li) {
+                    double* x = starts[ns++];
+                    x[0] = log(fmax(beta_mean * 0.3, 1e-6));
+                    for (int b = 0; b < nb; ++b) {
+                        double p3 = 1.0;
+                        for (int e = 0; e < b; ++e) p3 *= 3.0;
+                        double fc = sqrt(lo * hi) * corners[ci] * p3;
+                        double wc = 2.0 * PI * fc;
+                        double r = 1.0 / fmax(beta_mean * levels[li], 1e-6);
+                        fit_encode(r, r / (wc * 3.0), r * wc, x + 1 + 3 * b);
+                    }
+                }
+        }
+        for (int si = 0; si < ns; ++si) {
+            double x1[OB_NM_MAX], x2[OB_NM_MAX];
+            nelder_mead(fit_cost, &ctx, starts[si], np, 1.0, 1500 * (1 + nb), x1);
+            nelder_mead(fit_cost, &ctx, x1, np, 0.2, 1000 * (1 + nb), x2);
+            ob_fit fit;
+            fit_decode(x2, nb, &fit);
+            double rms, mx;
+            fit_errors(&fit, freqs, target, NF, &rms, &mx);
+            if (!have || rms < best_rms - 1e-6) {
+                have = 1;
+                best_rms = rms;
+                best = fit;
+            }
+        }
+        if (best_rms < 0.005) break;
+    }
+    /* Drop branches with |1 / (r + j w m + k / (j w))| <= 1e-5 at every fitted frequency. */
+    ob_fit kept = best;
+    kept.nb = 0;
+    for (int b = 0; b < best.nb; ++b) {
+        int any = 0;
+        for (int i = 0; i < NF && !any; ++i) {
+            cplx jw = cmk(0.0, 2.0 * PI * freqs[i]);
+            cplx den = cadd(cadd(cmk(best.br[b].r, 0.0), cscale(jw, best.br[b].m)), cdiv(cmk(best.br[b].k, 0.0), jw));
+            cplx y = cdiv(cmk(1.0, 0.0), den);
+            any = hypot(y.re, y.im) > 1e-5;
+        }
+        if (any) kept.br[kept.nb++] = best.br[b];
+    }
+    if (kept.beta0 < 1e-7) kept.beta0 = 0.0;
+    *out = kept;
+    fit_errors(&kept, freqs, target, NF, &err_out[0], &err_out[1]);
+    return kept.nb;
+}
+
 /* fdtd.rs:157-162 */
 double ob_phase_error_deg_per_m(double f, double c, double dx) {
     double dt = COURANT * dx / c;

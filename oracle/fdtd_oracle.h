@@ -95,6 +95,22 @@ void ob_to_transfer(const double* spectra, int64_t n_spec, const double* signal,
                     const double* h0, double* out);
 /* fdtd.rs:157-162 */
 double ob_phase_error_deg_per_m(double f, double c, double dx);
+/* spectrum.rs:24-44 (Spectrum::eval); n = 0 is the scalar values[0]. */
+double ob_spectrum_eval(const double* freqs_hz, const double* values, int32_t n, double f);
+/* material.rs:7-31 (Material), the parts absorption_at reads. */
+typedef struct { int32_t n; const double* f; const double* v; } ob_spectrum;
+typedef struct {
+    ob_spectrum absorption;
+    int32_t has_impedance;
+    ob_spectrum imp_re, imp_im;
+} ob_material;
+/* material.rs:72-78 */
+double ob_material_absorption(const ob_material* m, double f);
+/* impedance_fit.rs:158-235 (fit_admittance, with nelder_mead :87-129, decode
+ * and encode_branch :139-155), max_branches <= 2. Returns the number of
+ * branches kept; band_out = (lo, hi), err_out = (rms, max) over the band. */
+int ob_fit_admittance(const ob_material* mat, double band_lo, double band_hi, int max_branches, ob_fit* out,
+                      double band_out[2], double err_out[2]);
 /* impedance_fit.rs:48-55, :73-84; material.rs:96-111 */
 void ob_admittance(const ob_fit* fit, double f, double out[2]);
 double ob_paris(double beta_re, double beta_im, int n);

@@ -63,6 +63,15 @@ class ObSolid(C.Structure):
                 ("hi", C.c_double * 3)]
 
 
+class ObSpectrum(C.Structure):
+    _fields_ = [("n", C.c_int32), ("f", C.POINTER(C.c_double)), ("v", C.POINTER(C.c_double))]
+
+
+class ObMaterial(C.Structure):
+    _fields_ = [("absorption", ObSpectrum), ("has_impedance", C.c_int32), ("imp_re", ObSpectrum),
+                ("imp_im", ObSpectrum)]
+
+
 class ObGridView(C.Structure):
     _fields_ = [("n", C.c_int64 * 3), ("dx", C.c_double), ("dt", C.c_double), ("flags", C.POINTER(C.c_uint8)),
                 ("nb", C.c_int64), ("b_cell", C.POINTER(C.c_uint32)), ("b_start", C.POINTER(C.c_uint32)),
@@ -130,6 +139,13 @@ def lib():
                                            C.c_double, C.c_double, C.c_double, P(C.c_double), C.c_int32, C.c_int,
                                            P(C.c_double), P(C.c_double), P(C.c_double)]
         L.ob_calibrate_cabinet.restype = C.c_int
+        L.ob_spectrum_eval.argtypes = [P(C.c_double), P(C.c_double), C.c_int32, C.c_double]
+        L.ob_spectrum_eval.restype = C.c_double
+        L.ob_material_absorption.argtypes = [P(ObMaterial), C.c_double]
+        L.ob_material_absorption.restype = C.c_double
+        L.ob_fit_admittance.argtypes = [P(ObMaterial), C.c_double, C.c_double, C.c_int, P(ObFit), P(C.c_double),
+                                        P(C.c_double)]
+        L.ob_fit_admittance.restype = C.c_int
         _lib = L
     return _lib
 
@@ -531,3 +547,47 @@ def calibrate_cabinet(verts, tris, ac, c, f_max, ppw, cells_across, freqs, threa
         return None
     return {"on_axis": on[:2 * F].view(np.complex128).copy(), "angles_deg": ang,
             "directivity": dr[:2 * 12 * F].view(np.complex128).reshape(12, F).copy()}
+
+
+def _spectrum(spec, keep):
+    sp = ObSpectrum()
+    if isinstance(spec, dict):
+        f = np.ascontiguousarray(spec["freqs_hz"], dtype=np.float64)
+        v = np.ascontiguousarray(spec["values"], dtype=np.float64)
+        sp.n = len(f)
+    else:
+        f, v = np.zeros(1), np.array([float(spec)])
+        sp.n = 0
+    keep += [f, v]
+    sp.f, sp.v = _ptr(f, C.c_double), _ptr(v, C.c_double)
+    return sp
+
+
+def _material(mat, keep):
+    m = ObMaterial()
+    m.absorption = _spectrum(mat.get("absorption", 0.0), keep)
+    if mat.get("impedance") is not None:
+        m.has_impedance = 1
+        m.imp_re = _spectrum(mat["impedance"][0], keep)
+        m.imp_im = _spectrum(mat["impedance"][1], keep)
+    return m
+
+
+def material_absorption(mat, f):
+    """material.rs:72-78"""
+    keep = []
+    m = _material(mat, keep)
+    return float(lib().ob_material_absorption(C.byref(m), float(f)))
+
+
+def fit_admittance(mat, band, max_branches=2):
+    """impedance_fit.rs:158-235: ((beta0, [(r, m, k), ...]), {"band", "rms_error", "max_error"})."""
+    keep = []
+    m = _material(mat, keep)
+    out = ObFit()
+    b = np.zeros(2)
+    e = np.zeros(2)
+    nb = lib().ob_fit_admittance(C.byref(m), float(band[0]), float(band[1]), int(max_branches), C.byref(out),
+                                 _ptr(b, C.c_double), _ptr(e, C.c_double))
+    fit = (out.beta0, [(out.br[k].r, out.br[k].m, out.br[k].k) for k in range(nb)])
+    return fit, {"band": (b[0], b[1]), "rms_error": e[0], "max_error": e[1]}

@@ -73,6 +73,20 @@ class WvBoxRoom(C.Structure):
                 ("patch_fit", C.POINTER(C.c_uint32))]
 
 
+class WvSpectrum(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("freqs_hz", C.POINTER(C.c_double)), ("values", C.POINTER(C.c_double))]
+
+
+class WvMaterial(C.Structure):
+    _fields_ = [("absorption", WvSpectrum), ("impedance_re", C.POINTER(WvSpectrum)),
+                ("impedance_im", C.POINTER(WvSpectrum))]
+
+
+class WvFitResult(C.Structure):
+    _fields_ = [("beta0", C.c_double), ("n_branches", C.c_uint32), ("branches", WvBranch * 2),
+                ("band", C.c_double * 2), ("rms_error", C.c_double), ("max_error", C.c_double)]
+
+
 class WvRoomMesh(C.Structure):
     _fields_ = [("prism", C.c_int32), ("height", C.c_double), ("n_positions", C.c_uint32),
                 ("positions", C.POINTER(C.c_double)), ("n_triangles", C.c_uint32),
@@ -103,7 +117,7 @@ EXPORTS = [
     "wv_batch_autotune", "wv_batch_set_tile", "wv_batch_run", "wv_batch_result", "wv_batch_info",
     "wv_batch_timing", "wv_batch_free", "wv_serve_create", "wv_serve_submit", "wv_serve_wait", "wv_serve_stats",
     "wv_serve_free", "wv_wave_job", "wv_grid_build_room", "wv_solid_from_mesh", "wv_grid_spacing",
-    "wv_baffle_point", "wv_blend", "wv_calibrate_cabinet",
+    "wv_baffle_point", "wv_blend", "wv_calibrate_cabinet", "wv_material_absorption", "wv_fit_admittance",
 ]
 
 _lib = None
@@ -228,6 +242,10 @@ def lib():
                                                C.c_double, C.c_double, dp, sz, C.c_int, C.c_int, P(C.c_int32), dp,
                                                dp, dp] + errbuf
             L.wv_calibrate_cabinet.restype = st
+            L.wv_material_absorption.argtypes = [P(WvMaterial), C.c_double]
+            L.wv_material_absorption.restype = C.c_double
+            L.wv_fit_admittance.argtypes = [P(WvMaterial), C.c_double, C.c_double, C.c_uint32, P(WvFitResult)]
+            L.wv_fit_admittance.restype = st
             _lib = L
     return _lib
 
@@ -927,3 +945,63 @@ def calibrate_cabinet(verts, tris, ac, c, f_max, ppw, cells_across, freqs, preci
     _check(st, err)
     return {"on_axis": on[:2 * F].view(np.complex128).copy(), "angles_deg": ang,
             "directivity": dr[:2 * 12 * F].view(np.complex128).reshape(12, F).copy()}
+
+
+def _spectrum(spec, keep):
+    """A Spectrum (spectrum.rs:6-44): a number, or {"freqs_hz": [...], "values": [...]}."""
+    sp = WvSpectrum()
+    if isinstance(spec, dict):
+        f = np.ascontiguousarray(spec["freqs_hz"], dtype=np.float64)
+        v = np.ascontiguousarray(spec["values"], dtype=np.float64)
+        sp.n = len(f)
+    else:
+        f = np.zeros(1)
+        v = np.array([float(spec)])
+        sp.n = 0
+    keep += [f, v]
+    sp.freqs_hz, sp.values = _dp(f), _dp(v)
+    return sp
+
+
+def _material(mat, keep):
+    m = WvMaterial()
+    m.absorption = _spectrum(mat.get("absorption", 0.0), keep)
+    if mat.get("impedance") is not None:
+        re, im = (_spectrum(x, keep) for x in mat["impedance"])
+        keep += [re, im]
+        m.impedance_re, m.impedance_im = C.pointer(re), C.pointer(im)
+    return m
+
+
+def material_absorption(mat, f):
+    """Material::absorption_at (material.rs:72-78) for a material dict with the
+    library's fields ("absorption", optional "impedance": (re, im) spectra)."""
+    keep = []
+    m = _material(mat, keep)
+    return float(lib().wv_material_absorption(C.byref(m), float(f)))
+
+
+def fit_admittance(mat, band, max_branches=2):
+    """fit_admittance (impedance_fit.rs:158-235): ((beta0, [(r, m, k), ...]),
+    {"band": (lo, hi), "rms_error": e, "max_error": e})."""
+    keep = []
+    m = _material(mat, keep)
+    out = WvFitResult()
+    st = lib().wv_fit_admittance(C.byref(m), float(band[0]), float(band[1]), int(max_branches), C.byref(out))
+    if st != WV_OK:
+        raise FdtdError(st, "wv_fit_admittance: invalid argument")
+    fit = (out.beta0, [(out.branches[b].r, out.branches[b].m, out.branches[b].k) for b in range(out.n_branches)])
+    return fit, {"band": (out.band[0], out.band[1]), "rms_error": out.rms_error, "max_error": out.max_error}
+
+
+def fit_surfaces(materials, band, threads=8):
+    """fit_surfaces (fdtd.rs:213-222): one fit per distinct material id, fitted
+    in parallel, returned per surface."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    unique = {}
+    for m in materials:
+        unique.setdefault(m["id"], m)
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        fits = dict(zip(unique, ex.map(lambda m: fit_admittance(m, band, 2)[0], unique.values())))
+    return [fits[m["id"]] for m in materials]
