@@ -623,7 +623,7 @@ wv_status advance_impl(wv_plan* pl, size_t k, const volatile int32_t* cancel, wv
     if (pairs) {
         pp = make_pair_params<T, S>(pl, a);
         grid1 = pp.tile_launch + pp.special_blocks + div_up(pl->n_recv, kPairThreads);
-        items2 = div_up((uint64_t)pl->n_special + pl->n_ring, 256);
+        items2 = div_up(pl->n_special, 256u) + div_up(pl->n_ring, 256u);
         grid2 = items2 + div_up(pl->n_recv, 256);
     }
     T* bufs[4] = {pl->fa.as<T>(), pl->fb.as<T>(), pl->fc.as<T>(), pl->fd.as<T>()};

@@ -20,8 +20,10 @@
 //    tap injection), p^{n+1} written to the p^{n+1} buffer; and the
 //    receivers' p^n.
 //  * Pass 2: step n+1 of the special cells and of the ring (interior cells
-//    that are not deep), from the complete p^{n+1}; and the receivers'
-//    p^{n+1}.
+//    that are not deep), from the complete p^{n+1}, in blocks of one kind
+//    interleaved in proportion to the two counts (both lists run through
+//    the grid in z order, so the blocks in flight share the cache lines
+//    around the same walls); and the receivers' p^{n+1}.
 //
 // Every cell's arithmetic is the single-step kernel's (interior_update and
 // special_update with the same operands), so results are bit-identical to
@@ -247,24 +249,34 @@ __global__ __launch_bounds__(kPairThreads) void k_pair1(PairParams<T, S> pp) {
 }
 
 // Pass 2: step n+1 of the special cells and the ring, then the receivers'
-// p^{n+1}. a: p = p^{n+1}, prev = p^n, s_n = s[n+1], ir = row n+1.
+// p^{n+1}. Special and ring blocks are interleaved in proportion to their
+// counts, so both lists advance through the grid together.
 template <typename T, typename S>
 __global__ __launch_bounds__(256) void k_pair2(StepParams<T, S> a, T* __restrict__ p2,
                                                const uint32_t* __restrict__ sp_cell,
                                                const uint32_t* __restrict__ ring_cell, uint32_t n_special,
                                                uint32_t n_ring, uint32_t item_blocks) {
     if (blockIdx.x < item_blocks) {
-        const uint32_t t = blockIdx.x * 256 + threadIdx.x;
-        if (t < n_special) {
-            const uint2 rec = a.sp_rec[t];
-            S u0[kSlotGroup<S>];
+        const uint32_t bs = (n_special + 255) / 256;  // special blocks
+        const uint32_t b = blockIdx.x;
+        const uint32_t s0 = (uint32_t)(((uint64_t)b * bs) / item_blocks);
+        const uint32_t s1 = (uint32_t)(((uint64_t)(b + 1) * bs) / item_blocks);
+        if (s1 > s0) {
+            const uint32_t t = s0 * 256 + threadIdx.x;
+            if (t < n_special) {
+                const uint2 rec = a.sp_rec[t];
+                S u0[kSlotGroup<S>];
 #pragma unroll
-            for (uint32_t j = 0; j < kSlotGroup<S>; ++j) u0[j] = j < a.max_slots ? a.st_u[j * a.sp_pad + t] : S(0);
-            special_at(a, t, sp_cell[t], rec, u0, p2);
-        } else if (t < n_special + n_ring) {
-            const uint32_t i = ring_cell[t - n_special];
-            p2[i] = interior_update(a.p[i], a.prev[i], a.p[i - 1], a.p[i + 1], a.p[i - a.pitch], a.p[i + a.pitch],
-                                    a.p[i - a.sz], a.p[i + a.sz], a.l2, a.c2);
+                for (uint32_t j = 0; j < kSlotGroup<S>; ++j) u0[j] = j < a.max_slots ? a.st_u[j * a.sp_pad + t] : S(0);
+                special_at(a, t, sp_cell[t], rec, u0, p2);
+            }
+        } else {
+            const uint32_t r = (b - s1) * 256 + threadIdx.x;  // ring blocks before this one: b - s1
+            if (r < n_ring) {
+                const uint32_t i = ring_cell[r];
+                p2[i] = interior_update(a.p[i], a.prev[i], a.p[i - 1], a.p[i + 1], a.p[i - a.pitch], a.p[i + a.pitch],
+                                        a.p[i - a.sz], a.p[i + a.sz], a.l2, a.c2);
+            }
         }
         return;
     }
