@@ -379,6 +379,15 @@ wv_status wv_taps_at(const wv_grid* g, const double p[3], wv_tap taps[8], size_t
 /* fdtd.rs:730-743: comps[4][2], comp_len[4], *n_comps = 1 or 4. None -> WV_ERR_INVALID. */
 wv_status wv_source_components(const wv_grid* g, uint64_t c, uint32_t order, wv_tap comps[8],
                                uint32_t comp_len[4], uint32_t* n_comps);
+/* fdtd.rs:753-760: the 400 directions (unit vectors, the speaker's local
+ * frame) at which fit_first_order samples the directivity: out [400][3]. */
+void wv_first_order_directions(double out[1200]);
+/* fdtd.rs:749-807 (fit_first_order): D(u) ~ a0 + a.u with u in room
+ * coordinates, from the complex directivity d [400] (re, im) at those
+ * directions and d_axis on axis (+x local); to_room row-major, applied as
+ * to_room * v. Writes a0 [2], a [3][2]; returns the relative RMS residual. */
+double wv_fit_first_order(const double* d, const double d_axis[2], const double to_room[9], double a0[2],
+                          double a[6]);
 /* fdtd.rs:812-815; complex values interleaved (re, im): a0[2], a[6], out[8]. */
 void wv_component_weights(const double a0[2], const double a[6], double k, double dx, double out[8]);
 /* fdtd.rs:819-840; spectra/h0/out complex interleaved. */

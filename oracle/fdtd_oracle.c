@@ -1084,6 +1084,103 @@ int ob_source_components(const ob_grid* g, int64_t c, int order, int64_t* cc, do
     return 4;
 }
 
+/* fdtd.rs:753-760: the N = 400 golden-spiral directions (local frame). */
+void ob_first_order_directions(double* out) {
+    const int N = 400;
+    const double golden = PI * (3.0 - sqrt(5.0));
+    for (int i = 0; i < N; ++i) {
+        double z = 1.0 - 2.0 * ((double)i + 0.5) / (double)N;
+        double r = sqrt(1.0 - z * z);
+        out[3 * i] = r * cos(golden * (double)i);
+        out[3 * i + 1] = r * sin(golden * (double)i);
+        out[3 * i + 2] = z;
+    }
+}
+
+/* Rust's f64::total_cmp > 0 for |a| vs |b| as in max_by (the last maximum wins). */
+static void fo_solve(const double m0[4][4], const cplx rhs[4], cplx out[4]) {
+    double m[4][4];
+    cplx x[4];
+    memcpy(m, m0, sizeof(m));
+    for (int i = 0; i < 4; ++i) x[i] = rhs[i];
+    for (int col = 0; col < 4; ++col) {
+        int piv = col;
+        for (int r = col + 1; r < 4; ++r)
+            if (total_cmp(fabs(m[r][col]), fabs(m[piv][col])) >= 0) piv = r;
+        for (int c = 0; c < 4; ++c) {
+            double t = m[col][c];
+            m[col][c] = m[piv][c];
+            m[piv][c] = t;
+        }
+        cplx tx = x[col];
+        x[col] = x[piv];
+        x[piv] = tx;
+        for (int row = 0; row < 4; ++row) {
+            if (row == col) continue;
+            double k = m[row][col] / m[col][col];
+            for (int c = 0; c < 4; ++c) m[row][c] -= k * m[col][c];
+            x[row] = cmk(x[row].re - x[col].re * k, x[row].im - x[col].im * k);
+        }
+    }
+    for (int i = 0; i < 4; ++i) out[i] = cmk(x[i].re / m[i][i], x[i].im / m[i][i]);
+}
+
+/* fdtd.rs:749-807 (fit_first_order): d [400] complex directivity at the
+ * spiral directions, d_axis the on-axis (+x local) value, to_room row-major.
+ * out: a0 (2), a (6), returns the relative RMS residual. */
+double ob_fit_first_order(const double* d, const double d_axis[2], const double to_room[9], double a0_out[2],
+                          double a_out[6]) {
+    const int N = 400;
+    double dirs[400 * 3];
+    ob_first_order_directions(dirs);
+    double m[4][4];
+    memset(m, 0, sizeof(m));
+    cplx rhs[4];
+    double bs[400][4];
+    for (int a = 0; a < 4; ++a) rhs[a] = cmk(0.0, 0.0);
+    for (int i = 0; i < N; ++i) {
+        const double* v = dirs + 3 * i;
+        double room[3];
+        for (int r = 0; r < 3; ++r) room[r] = (to_room[3 * r] * v[0] + to_room[3 * r + 1] * v[1]) + to_room[3 * r + 2] * v[2];
+        double b[4] = {1.0, room[0], room[1], room[2]};
+        for (int a = 0; a < 4; ++a) {
+            for (int c = 0; c < 4; ++c) m[a][c] += b[a] * b[c];
+            rhs[a] = cadd(rhs[a], cscale(cmk(d[2 * i], d[2 * i + 1]), b[a]));
+            bs[i][a] = b[a];
+        }
+    }
+    cplx ls[4], me[4], e4[4];
+    fo_solve(m, rhs, ls);
+    const double x[3] = {1.0, 0.0, 0.0};
+    double axis[3];
+    for (int r = 0; r < 3; ++r) axis[r] = (to_room[3 * r] * x[0] + to_room[3 * r + 1] * x[1]) + to_room[3 * r + 2] * x[2];
+    const double e[4] = {1.0, axis[0], axis[1], axis[2]};
+    cplx got = cmk(0.0, 0.0);
+    for (int i = 0; i < 4; ++i) got = cadd(got, cscale(ls[i], e[i]));
+    for (int i = 0; i < 4; ++i) e4[i] = cmk(e[i], 0.0);
+    fo_solve(m, e4, me);
+    double ete = -0.0;
+    for (int i = 0; i < 4; ++i) ete += me[i].re * e[i];
+    const cplx diff = cmk(d_axis[0] - got.re, d_axis[1] - got.im);
+    const cplx q = cmk(diff.re / ete, diff.im / ete);
+    cplx a[4];
+    for (int i = 0; i < 4; ++i) a[i] = cadd(ls[i], cmul(me[i], q));
+    double err = 0.0, tot = 0.0;
+    for (int i = 0; i < N; ++i) {
+        cplx fit = cadd(cadd(cadd(a[0], cscale(a[1], bs[i][1])), cscale(a[2], bs[i][2])), cscale(a[3], bs[i][3]));
+        cplx dv = cmk(d[2 * i], d[2 * i + 1]);
+        err += cnorm2(cmk(fit.re - dv.re, fit.im - dv.im));
+        tot += cnorm2(dv);
+    }
+    a0_out[0] = a[0].re;
+    a0_out[1] = a[0].im;
+    for (int i = 0; i < 3; ++i) {
+        a_out[2 * i] = a[i + 1].re;
+        a_out[2 * i + 1] = a[i + 1].im;
+    }
+    return sqrt(err / fmax(tot, 1e-30));
+}
+
 /* fdtd.rs:812-815 */
 void ob_component_weights(const double a0[2], const double a[6], double k, double dx, double out[8]) {
     cplx dip = cmk(0.0, 2.0 * sin(k * dx));

@@ -87,6 +87,14 @@ int ob_taps_at(const ob_grid* g, const double p[3], int64_t* cells, double* weig
  * comp_cells/comp_w are [4][2]; comp_len[4]. */
 int ob_source_components(const ob_grid* g, int64_t c, int order, int64_t* comp_cells,
                          double* comp_w, int* comp_len);
+/* fdtd.rs:753-760: the 400 golden-spiral directions [400][3], local frame. */
+void ob_first_order_directions(double* out);
+/* fdtd.rs:749-807 (fit_first_order): d [400] complex directivity at those
+ * directions, d_axis the on-axis value, to_room row-major (applied as
+ * to_room * v). Writes a0 (re, im) and a [3] (re, im); returns the relative
+ * RMS residual. */
+double ob_fit_first_order(const double* d, const double d_axis[2], const double to_room[9], double a0_out[2],
+                          double a_out[6]);
 /* fdtd.rs:812-815; a0, a[3] and out[4] are complex (re, im). */
 void ob_component_weights(const double a0[2], const double a[6], double k, double dx, double out[8]);
 /* fdtd.rs:819-840; spectra/h0/out complex interleaved. */

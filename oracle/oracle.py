@@ -139,6 +139,10 @@ def lib():
                                            C.c_double, C.c_double, C.c_double, P(C.c_double), C.c_int32, C.c_int,
                                            P(C.c_double), P(C.c_double), P(C.c_double)]
         L.ob_calibrate_cabinet.restype = C.c_int
+        L.ob_first_order_directions.argtypes = [P(C.c_double)]
+        L.ob_first_order_directions.restype = None
+        L.ob_fit_first_order.argtypes = [P(C.c_double)] * 5
+        L.ob_fit_first_order.restype = C.c_double
         L.ob_spectrum_eval.argtypes = [P(C.c_double), P(C.c_double), C.c_int32, C.c_double]
         L.ob_spectrum_eval.restype = C.c_double
         L.ob_material_absorption.argtypes = [P(ObMaterial), C.c_double]
@@ -591,3 +595,21 @@ def fit_admittance(mat, band, max_branches=2):
                                  _ptr(b, C.c_double), _ptr(e, C.c_double))
     fit = (out.beta0, [(out.br[k].r, out.br[k].m, out.br[k].k) for k in range(nb)])
     return fit, {"band": (b[0], b[1]), "rms_error": e[0], "max_error": e[1]}
+
+
+def first_order_directions():
+    """fdtd.rs:753-760"""
+    out = np.zeros(1200)
+    lib().ob_first_order_directions(_ptr(out, C.c_double))
+    return out.reshape(400, 3)
+
+
+def fit_first_order(d, d_axis, to_room):
+    """fdtd.rs:749-807: (a0, [a_x, a_y, a_z], relative RMS residual)."""
+    dv = np.ascontiguousarray(np.asarray(d, dtype=np.complex128).reshape(400).view(np.float64))
+    ax = np.array([complex(d_axis).real, complex(d_axis).imag])
+    R = np.ascontiguousarray(np.asarray(to_room, dtype=np.float64).reshape(3, 3))
+    a0, a = np.zeros(2), np.zeros(6)
+    res = lib().ob_fit_first_order(_ptr(dv, C.c_double), _ptr(ax, C.c_double), _ptr(R, C.c_double),
+                                   _ptr(a0, C.c_double), _ptr(a, C.c_double))
+    return complex(a0[0], a0[1]), [complex(a[2 * i], a[2 * i + 1]) for i in range(3)], float(res)

@@ -454,6 +454,112 @@ inline std::complex<double> cmul(std::complex<double> a, std::complex<double> b)
 }
 }  // namespace
 
+extern "C" void wv_first_order_directions(double out[1200]) {
+    // fdtd.rs:753-760: 400 directions on a golden-angle spiral, even in z.
+    const double golden = kPi * (3.0 - std::sqrt(5.0));
+    for (int i = 0; i < 400; ++i) {
+        const double z = 1.0 - 2.0 * ((double)i + 0.5) / 400.0;
+        const double r = std::sqrt(1.0 - z * z);
+        out[3 * i] = r * std::cos(golden * (double)i);
+        out[3 * i + 1] = r * std::sin(golden * (double)i);
+        out[3 * i + 2] = z;
+    }
+}
+
+namespace {
+// f64::total_cmp's order.
+int64_t total_key(double v) {
+    int64_t x;
+    std::memcpy(&x, &v, 8);
+    return x ^ (int64_t)(((uint64_t)(x >> 63)) >> 1);
+}
+
+// fdtd.rs:771-788: Gauss-Jordan on the 4x4 normal matrix with partial
+// pivoting (the last of equal pivots), complex right-hand side.
+std::array<std::complex<double>, 4> solve4(const double (&m0)[4][4], const std::array<std::complex<double>, 4>& rhs) {
+    double m[4][4];
+    std::memcpy(m, m0, sizeof(m));
+    std::array<std::complex<double>, 4> x = rhs;
+    for (int col = 0; col < 4; ++col) {
+        int piv = col;
+        for (int r = col + 1; r < 4; ++r)
+            if (total_key(std::fabs(m[r][col])) >= total_key(std::fabs(m[piv][col]))) piv = r;
+        for (int c = 0; c < 4; ++c) std::swap(m[col][c], m[piv][c]);
+        std::swap(x[col], x[piv]);
+        for (int row = 0; row < 4; ++row) {
+            if (row == col) continue;
+            const double k = m[row][col] / m[col][col];
+            for (int c = 0; c < 4; ++c) m[row][c] -= k * m[col][c];
+            x[row] = {x[row].real() - x[col].real() * k, x[row].imag() - x[col].imag() * k};
+        }
+    }
+    for (int i = 0; i < 4; ++i) x[i] = {x[i].real() / m[i][i], x[i].imag() / m[i][i]};
+    return x;
+}
+}  // namespace
+
+extern "C" double wv_fit_first_order(const double* d, const double d_axis[2], const double to_room[9], double a0[2],
+                                     double a[6]) {
+    // fdtd.rs:749-807: least-squares D(u) ~ a0 + a.u over the spiral
+    // directions (u in room coordinates), then moved along M^-1 e so that it
+    // meets the on-axis value exactly.
+    double dirs[1200];
+    wv_first_order_directions(dirs);
+    auto rot = [&](const double* v, double* o) {
+        for (int r = 0; r < 3; ++r) o[r] = (to_room[3 * r] * v[0] + to_room[3 * r + 1] * v[1]) + to_room[3 * r + 2] * v[2];
+    };
+    double m[4][4] = {};
+    std::array<std::complex<double>, 4> rhs{};
+    std::vector<std::array<double, 4>> bs(400);
+    for (int i = 0; i < 400; ++i) {
+        double room[3];
+        rot(dirs + 3 * i, room);
+        const std::array<double, 4> b{1.0, room[0], room[1], room[2]};
+        const std::complex<double> di(d[2 * i], d[2 * i + 1]);
+        for (int p = 0; p < 4; ++p) {
+            for (int c = 0; c < 4; ++c) m[p][c] += b[p] * b[c];
+            rhs[p] = {rhs[p].real() + di.real() * b[p], rhs[p].imag() + di.imag() * b[p]};
+        }
+        bs[i] = b;
+    }
+    const auto ls = solve4(m, rhs);
+    const double xax[3] = {1.0, 0.0, 0.0};
+    double axis[3];
+    rot(xax, axis);
+    const double e[4] = {1.0, axis[0], axis[1], axis[2]};
+    std::complex<double> got(0.0, 0.0);
+    for (int i = 0; i < 4; ++i) got = {got.real() + ls[i].real() * e[i], got.imag() + ls[i].imag() * e[i]};
+    const auto me = solve4(m, {std::complex<double>(e[0], 0.0), std::complex<double>(e[1], 0.0),
+                               std::complex<double>(e[2], 0.0), std::complex<double>(e[3], 0.0)});
+    double ete = -0.0;
+    for (int i = 0; i < 4; ++i) ete += me[i].real() * e[i];
+    const std::complex<double> q((d_axis[0] - got.real()) / ete, (d_axis[1] - got.imag()) / ete);
+    std::complex<double> c[4];
+    for (int i = 0; i < 4; ++i) {
+        const std::complex<double> t = cmul(me[i], q);
+        c[i] = {ls[i].real() + t.real(), ls[i].imag() + t.imag()};
+    }
+    double err = 0.0, tot = 0.0;
+    for (int i = 0; i < 400; ++i) {
+        const auto& b = bs[i];
+        double fr = c[0].real(), fi = c[0].imag();
+        for (int p = 1; p < 4; ++p) {
+            fr = fr + c[p].real() * b[p];
+            fi = fi + c[p].imag() * b[p];
+        }
+        const double er = fr - d[2 * i], ei = fi - d[2 * i + 1];
+        err += er * er + ei * ei;
+        tot += d[2 * i] * d[2 * i] + d[2 * i + 1] * d[2 * i + 1];
+    }
+    a0[0] = c[0].real();
+    a0[1] = c[0].imag();
+    for (int i = 0; i < 3; ++i) {
+        a[2 * i] = c[i + 1].real();
+        a[2 * i + 1] = c[i + 1].imag();
+    }
+    return std::sqrt(err / std::fmax(tot, 1e-30));
+}
+
 extern "C" void wv_component_weights(const double a0[2], const double a[6], double k, double dx, double out[8]) {
     // fdtd.rs:812-815: the +-dx dipole radiates 2j sin(k dx) cos(theta) times the monopole.
     const std::complex<double> dip(0.0, 2.0 * std::sin(k * dx));

@@ -118,6 +118,7 @@ EXPORTS = [
     "wv_batch_timing", "wv_batch_free", "wv_serve_create", "wv_serve_submit", "wv_serve_wait", "wv_serve_stats",
     "wv_serve_free", "wv_wave_job", "wv_grid_build_room", "wv_solid_from_mesh", "wv_grid_spacing",
     "wv_baffle_point", "wv_blend", "wv_calibrate_cabinet", "wv_material_absorption", "wv_fit_admittance",
+    "wv_first_order_directions", "wv_fit_first_order",
 ]
 
 _lib = None
@@ -246,6 +247,10 @@ def lib():
             L.wv_material_absorption.restype = C.c_double
             L.wv_fit_admittance.argtypes = [P(WvMaterial), C.c_double, C.c_double, C.c_uint32, P(WvFitResult)]
             L.wv_fit_admittance.restype = st
+            L.wv_first_order_directions.argtypes = [dp]
+            L.wv_first_order_directions.restype = None
+            L.wv_fit_first_order.argtypes = [dp, dp, dp, dp, dp]
+            L.wv_fit_first_order.restype = C.c_double
             _lib = L
     return _lib
 
@@ -1005,3 +1010,22 @@ def fit_surfaces(materials, band, threads=8):
     with ThreadPoolExecutor(max_workers=threads) as ex:
         fits = dict(zip(unique, ex.map(lambda m: fit_admittance(m, band, 2)[0], unique.values())))
     return [fits[m["id"]] for m in materials]
+
+
+def first_order_directions():
+    """fdtd.rs:753-760: the 400 local directions fit_first_order samples."""
+    out = np.zeros(1200)
+    lib().wv_first_order_directions(_dp(out))
+    return out.reshape(400, 3)
+
+
+def fit_first_order(d, d_axis, to_room):
+    """fit_first_order (fdtd.rs:749-807): d = the directivity at
+    first_order_directions() (complex, [400]), d_axis on axis; returns
+    (a0, [a_x, a_y, a_z], relative RMS residual)."""
+    dv = np.ascontiguousarray(np.asarray(d, dtype=np.complex128).reshape(400).view(np.float64))
+    ax = np.array([complex(d_axis).real, complex(d_axis).imag])
+    R = np.ascontiguousarray(np.asarray(to_room, dtype=np.float64).reshape(3, 3))
+    a0, a = np.zeros(2), np.zeros(6)
+    res = lib().wv_fit_first_order(_dp(dv), _dp(ax), _dp(R), _dp(a0), _dp(a))
+    return complex(a0[0], a0[1]), [complex(a[2 * i], a[2 * i + 1]) for i in range(3)], float(res)
