@@ -73,6 +73,23 @@ typedef struct {
 
 typedef void (*wv_progress_fn)(double fraction, void* ctx);
 
+/* --------------------------------------------------------------- settings */
+
+/* fdtd.rs:44-82 (WaveAccuracy) and fdtd.rs:141-176 (FdtdSettings). */
+enum { WV_ACCURACY_DRAFT = 0, WV_ACCURACY_STANDARD = 1, WV_ACCURACY_HIGH = 2 };
+/* fdtd.rs:61-67: grid headroom above the highest frequency needed. */
+double wv_accuracy_headroom(int accuracy);
+/* fdtd.rs:72-78: cells across a cabinet's smallest side. */
+double wv_accuracy_cabinet_cells(int accuracy);
+/* fdtd.rs:144-146: the top of the crossover band, crossover * sqrt(2). */
+double wv_settings_f_max(double crossover_hz);
+/* fdtd.rs:148-154 (grid_fmax). */
+double wv_grid_fmax(double crossover_hz, int accuracy, const double* freqs, size_t nf);
+/* fdtd.rs:166-175 (weight): the wave result's weight at f. */
+double wv_crossover_weight(double crossover_hz, double f);
+/* fdtd.rs:157-162 */
+double wv_phase_error_deg_per_m(double f, double c, double dx);
+
 /* ------------------------------------------------------------------ run */
 
 /*

@@ -147,3 +147,32 @@ def test_first_order_fit_meets_target():
         for r, m, k in fit[1]:
             assert r > 0 and m > 0 and k == 0.0
             assert 2 * math.pi * 20 * 0.999 < r / m < 2 * math.pi * 500 * 1.001
+
+
+def test_settings_functions():
+    """fdtd.rs:44-82, 141-176: crossover weight, f_max, grid_fmax, headroom,
+    phase error against their closed forms and the oracle."""
+    import math
+    L = fdtd.lib()
+    assert fdtd.crossover_weight(250.0, 100.0) == 1.0 and fdtd.crossover_weight(250.0, 176.0) == 1.0
+    assert fdtd.crossover_weight(250.0, 354.0) == 0.0 and fdtd.crossover_weight(250.0, 1000.0) == 0.0
+    assert abs(fdtd.crossover_weight(250.0, 250.0) - 0.5) < 1e-15
+    for f in [190.0, 220.0, 280.0, 330.0]:
+        x = math.log2(f / 250.0)
+        assert abs(fdtd.crossover_weight(250.0, f) - 0.5 * (1 + math.cos(math.pi * (x + 0.5)))) < 1e-15
+        # Power complementary with the geometric side mirrored about the crossover.
+        assert abs(fdtd.crossover_weight(250.0, f) + fdtd.crossover_weight(250.0, 250.0 ** 2 / f) - 1.0) < 1e-12
+    assert L.wv_settings_f_max(250.0) == 250.0 * math.sqrt(2.0)
+    assert [L.wv_accuracy_headroom(a) for a in range(3)] == [1.0, 1.5, 2.0]
+    assert [L.wv_accuracy_cabinet_cells(a) for a in range(3)] == [4.0, 6.0, 8.0]
+    freqs = [50.0, 100.0, 150.0, 300.0, 400.0]
+    # Highest frequency with weight > 0 is 300 Hz; times 1.5, capped at 250 sqrt 2.
+    assert fdtd.grid_fmax(250.0, freqs, "standard") == min(300.0 * 1.5, 250.0 * math.sqrt(2.0))
+    assert fdtd.grid_fmax(250.0, [50.0, 100.0], "draft") == 100.0
+    assert fdtd.grid_fmax(250.0, [500.0], "high") == 250.0 * math.sqrt(2.0)
+    for f, dx in [(63.0, 0.04), (250.0, 0.04), (125.0, 0.1)]:
+        e = fdtd.phase_error_deg_per_m(f, 343.0, dx)
+        assert e == oracle.phase_error_deg_per_m(f, 343.0, dx)
+    # The scheme's axial dispersion is second order in dx.
+    e1, e2 = fdtd.phase_error_deg_per_m(100.0, 343.0, 0.04), fdtd.phase_error_deg_per_m(100.0, 343.0, 0.02)
+    assert 3.9 < e1 / e2 < 4.1

@@ -454,6 +454,40 @@ inline std::complex<double> cmul(std::complex<double> a, std::complex<double> b)
 }
 }  // namespace
 
+// fdtd.rs:61-78 (WaveAccuracy::headroom, cabinet_cells): Draft, Standard, High.
+extern "C" double wv_accuracy_headroom(int accuracy) { return accuracy <= 0 ? 1.0 : accuracy == 1 ? 1.5 : 2.0; }
+extern "C" double wv_accuracy_cabinet_cells(int accuracy) { return accuracy <= 0 ? 4.0 : accuracy == 1 ? 6.0 : 8.0; }
+
+// fdtd.rs:144-146: the top of the one-octave crossover band.
+extern "C" double wv_settings_f_max(double crossover_hz) { return crossover_hz * std::sqrt(2.0); }
+
+// fdtd.rs:166-175: 1 below the crossover band, 0 above, a raised cosine over
+// the octave centred on the crossover.
+extern "C" double wv_crossover_weight(double crossover_hz, double f) {
+    const double x = std::log2(f / crossover_hz);
+    if (x <= -0.5) return 1.0;
+    if (x >= 0.5) return 0.0;
+    return 0.5 * (1.0 + std::cos(kPi * (x + 0.5)));
+}
+
+// fdtd.rs:148-154: the highest frequency the wave solver contributes to,
+// times the accuracy's headroom, up to the top of the crossover band.
+extern "C" double wv_grid_fmax(double crossover_hz, int accuracy, const double* freqs, size_t nf) {
+    double top = 0.0;
+    for (size_t i = 0; i < nf; ++i)
+        if (wv_crossover_weight(crossover_hz, freqs[i]) > 0.0) top = std::fmax(top, freqs[i]);
+    const double fm = wv_settings_f_max(crossover_hz);
+    return top > 0.0 ? std::fmin(top * wv_accuracy_headroom(accuracy), fm) : fm;
+}
+
+// fdtd.rs:157-162: the grid-axis phase error of the scheme, degrees per metre.
+extern "C" double wv_phase_error_deg_per_m(double f, double c, double dx) {
+    const double dt = kCourant * dx / c;
+    const double w = 2.0 * kPi * f;
+    const double k_num = 2.0 / dx * std::asin(std::sin(w * dt / 2.0) / kCourant);
+    return std::fabs(k_num - w / c) * (180.0 / kPi);
+}
+
 extern "C" void wv_first_order_directions(double out[1200]) {
     // fdtd.rs:753-760: 400 directions on a golden-angle spiral, even in z.
     const double golden = kPi * (3.0 - std::sqrt(5.0));

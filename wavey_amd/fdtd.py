@@ -118,7 +118,8 @@ EXPORTS = [
     "wv_batch_timing", "wv_batch_free", "wv_serve_create", "wv_serve_submit", "wv_serve_wait", "wv_serve_stats",
     "wv_serve_free", "wv_wave_job", "wv_grid_build_room", "wv_solid_from_mesh", "wv_grid_spacing",
     "wv_baffle_point", "wv_blend", "wv_calibrate_cabinet", "wv_material_absorption", "wv_fit_admittance",
-    "wv_first_order_directions", "wv_fit_first_order",
+    "wv_first_order_directions", "wv_fit_first_order", "wv_accuracy_headroom", "wv_accuracy_cabinet_cells",
+    "wv_settings_f_max", "wv_grid_fmax", "wv_crossover_weight", "wv_phase_error_deg_per_m",
 ]
 
 _lib = None
@@ -251,6 +252,17 @@ def lib():
             L.wv_first_order_directions.restype = None
             L.wv_fit_first_order.argtypes = [dp, dp, dp, dp, dp]
             L.wv_fit_first_order.restype = C.c_double
+            for name in ("wv_accuracy_headroom", "wv_accuracy_cabinet_cells"):
+                getattr(L, name).argtypes = [C.c_int]
+                getattr(L, name).restype = C.c_double
+            L.wv_settings_f_max.argtypes = [C.c_double]
+            L.wv_settings_f_max.restype = C.c_double
+            L.wv_grid_fmax.argtypes = [C.c_double, C.c_int, dp, sz]
+            L.wv_grid_fmax.restype = C.c_double
+            L.wv_crossover_weight.argtypes = [C.c_double, C.c_double]
+            L.wv_crossover_weight.restype = C.c_double
+            L.wv_phase_error_deg_per_m.argtypes = [C.c_double, C.c_double, C.c_double]
+            L.wv_phase_error_deg_per_m.restype = C.c_double
             _lib = L
     return _lib
 
@@ -1029,3 +1041,22 @@ def fit_first_order(d, d_axis, to_room):
     a0, a = np.zeros(2), np.zeros(6)
     res = lib().wv_fit_first_order(_dp(dv), _dp(ax), _dp(R), _dp(a0), _dp(a))
     return complex(a0[0], a0[1]), [complex(a[2 * i], a[2 * i + 1]) for i in range(3)], float(res)
+
+
+ACCURACY = {"draft": 0, "standard": 1, "high": 2}
+
+
+def crossover_weight(crossover_hz, f):
+    """FdtdSettings::weight (fdtd.rs:166-175)."""
+    return float(lib().wv_crossover_weight(float(crossover_hz), float(f)))
+
+
+def grid_fmax(crossover_hz, freqs, accuracy="high"):
+    """FdtdSettings::grid_fmax (fdtd.rs:148-154)."""
+    fr = np.ascontiguousarray(freqs, dtype=np.float64)
+    return float(lib().wv_grid_fmax(float(crossover_hz), ACCURACY[accuracy], _dp(fr), len(fr)))
+
+
+def phase_error_deg_per_m(f, c, dx):
+    """FdtdSettings::phase_error_deg_per_m (fdtd.rs:157-162)."""
+    return float(lib().wv_phase_error_deg_per_m(float(f), float(c), float(dx)))
