@@ -152,7 +152,6 @@ def test_first_order_fit_meets_target():
 def test_settings_functions():
     """fdtd.rs:44-82, 141-176: crossover weight, f_max, grid_fmax, headroom,
     phase error against their closed forms and the oracle."""
-    import math
     L = fdtd.lib()
     assert fdtd.crossover_weight(250.0, 100.0) == 1.0 and fdtd.crossover_weight(250.0, 176.0) == 1.0
     assert fdtd.crossover_weight(250.0, 354.0) == 0.0 and fdtd.crossover_weight(250.0, 1000.0) == 0.0
@@ -172,7 +171,7 @@ def test_settings_functions():
     assert fdtd.grid_fmax(250.0, [500.0], "high") == 250.0 * math.sqrt(2.0)
     for f, dx in [(63.0, 0.04), (250.0, 0.04), (125.0, 0.1)]:
         e = fdtd.phase_error_deg_per_m(f, 343.0, dx)
-        assert e == oracle.phase_error_deg_per_m(f, 343.0, dx)
+        assert e == O.phase_error_deg_per_m(f, 343.0, dx)
     # The scheme's axial dispersion is second order in dx.
     e1, e2 = fdtd.phase_error_deg_per_m(100.0, 343.0, 0.04), fdtd.phase_error_deg_per_m(100.0, 343.0, 0.02)
     assert 3.9 < e1 / e2 < 4.1
