@@ -299,3 +299,35 @@ def test_build_room_errors():
     with pytest.raises(fdtd.FdtdError, match="M cells"):
         fdtd.FdtdGrid.build_room(mesh, C, 0.001)
     assert fdtd.solid_from_mesh(np.zeros((3, 3)), np.zeros((0, 3))) is None
+
+
+def tessellated_box(L, n, R, fit):
+    """A box room turned by R, every face cut into n x n squares (2 n^2 triangles)."""
+    M = meshes.MeshBuilder()
+    s = M.surface(fit)
+    c = np.array(L) / 2
+    for ax in range(3):
+        u, v = [a for a in range(3) if a != ax]
+        for side in (0, 1):
+            for i in range(n):
+                for j in range(n):
+                    q = []
+                    for di, dj in ((0, 0), (1, 0), (1, 1), (0, 1)):
+                        p = np.zeros(3)
+                        p[ax] = side * L[ax]
+                        p[u], p[v] = (i + di) / n * L[u], (j + dj) / n * L[v]
+                        q.append(tuple(R @ (p - c) + c + 1.0))
+                    M.polygon(s, q if side else q[::-1])
+    return M.room()
+
+
+def test_tessellation_does_not_change_the_grid():
+    """A turned box (every wall sloped) cut into 4 x 4 or 13 x 13 squares per
+    face: the same room, so the same grid, and its air volume is the box's."""
+    R = meshes.rotation((0.3, 0.5, 0.8), 0.6)
+    fit0 = fit(0)
+    grids = [fdtd.FdtdGrid.build_room(tessellated_box((3.1, 2.3, 1.9), n, R, fit0), C, 0.05) for n in (4, 13)]
+    assert grids_equal(grids[0], grids[1])
+    assert abs(grids[0].air_cells() * 0.05 ** 3 / (3.1 * 2.3 * 1.9) - 1.0) < 0.01
+    ref = oracle.build_room(tessellated_box((3.1, 2.3, 1.9), 4, R, fit0), C, 0.05)
+    assert grids_equal(grids[0], ref)
