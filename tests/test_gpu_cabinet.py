@@ -6,6 +6,8 @@ FP64 runs: within 1e-9 relative L2 over every on-axis and directivity value.
 FP32 runs: within 1e-4. The batched and the one-after-another forms give the
 same values bit for bit.
 """
+import time
+
 import numpy as np
 import pytest
 
@@ -34,7 +36,9 @@ def reference():
 def test_calibration_matches_oracle(reference, prec, tol):
     v, t, ac, ref = reference
     print(f"case: cabinet 0.4x0.3x0.5 m, dx 0.1 m, {prec}, batched and plan", flush=True)
+    t0 = time.perf_counter()
     got = fdtd.calibrate_cabinet(v, t, ac, C, 100.0, 10.0, 3.0, FREQS, precision=prec, batched=True)
+    print(f"calibration (two runs, one batch): {time.perf_counter() - t0:.3f} s")
     e_on = rel_l2(got["on_axis"], ref["on_axis"])
     e_dir = rel_l2(got["directivity"], ref["directivity"])
     print(f"relative L2: on axis {e_on:.3e}, directivity {e_dir:.3e}")
