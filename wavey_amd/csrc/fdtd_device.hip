@@ -623,8 +623,8 @@ wv_status advance_impl(wv_plan* pl, size_t k, const volatile int32_t* cancel, wv
     if (pairs) {
         pp = make_pair_params<T, S>(pl, a);
         grid1 = pp.tile_launch + pp.special_blocks + div_up(pl->n_recv, kPairThreads);
-        items2 = div_up(pl->n_special, 256u) + div_up(pl->n_ring, 256u);
-        grid2 = items2 + div_up(pl->n_recv, 256);
+        items2 = div_up(pl->n_special, kPair2Block) + div_up(pl->n_ring, kPair2Block);
+        grid2 = items2 + div_up(pl->n_recv, kPair2Block);
     }
     T* bufs[4] = {pl->fa.as<T>(), pl->fb.as<T>(), pl->fc.as<T>(), pl->fd.as<T>()};
     auto sig = [&](size_t n) -> S { return (S)(sizeof(S) == 4 ? (double)(float)pl->signal[n] : pl->signal[n]); };
@@ -661,7 +661,7 @@ wv_status advance_impl(wv_plan* pl, size_t k, const volatile int32_t* cancel, wv
                 a2.prev = P;
                 a2.s_n = sig(n + 1);
                 a2.ir = ir + (n + 1) * pl->n_recv;
-                hipLaunchKernelGGL((k_pair2<T, S>), dim3(grid2), dim3(256), 0, pl->stream, a2, bufs[o2],
+                hipLaunchKernelGGL((k_pair2<T, S>), dim3(grid2), dim3(kPair2Block), 0, pl->stream, a2, bufs[o2],
                                    pl->sp_cell.as<uint32_t>(), pl->ring_cell.as<uint32_t>(), pl->n_special,
                                    pl->n_ring, items2);
                 pl->buf_p = o2;

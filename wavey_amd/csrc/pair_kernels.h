@@ -36,6 +36,7 @@ namespace wv {
 constexpr int kPairRows = 14;                           // output rows of a pass-1 tile block
 constexpr int kPairWaves = kPairRows + 2;               // and a halo row above and below
 constexpr uint32_t kPairThreads = kTileX * kPairWaves;  // 1024
+constexpr uint32_t kPair2Block = 128;  // pass-2 items per block (specials or ring cells)
 
 template <typename T, typename S>
 struct PairParams {
@@ -252,17 +253,17 @@ __global__ __launch_bounds__(kPairThreads) void k_pair1(PairParams<T, S> pp) {
 // p^{n+1}. Special and ring blocks are interleaved in proportion to their
 // counts, so both lists advance through the grid together.
 template <typename T, typename S>
-__global__ __launch_bounds__(256) void k_pair2(StepParams<T, S> a, T* __restrict__ p2,
+__global__ __launch_bounds__(kPair2Block) void k_pair2(StepParams<T, S> a, T* __restrict__ p2,
                                                const uint32_t* __restrict__ sp_cell,
                                                const uint32_t* __restrict__ ring_cell, uint32_t n_special,
                                                uint32_t n_ring, uint32_t item_blocks) {
     if (blockIdx.x < item_blocks) {
-        const uint32_t bs = (n_special + 255) / 256;  // special blocks
+        const uint32_t bs = (n_special + kPair2Block - 1) / kPair2Block;  // special blocks
         const uint32_t b = blockIdx.x;
         const uint32_t s0 = (uint32_t)(((uint64_t)b * bs) / item_blocks);
         const uint32_t s1 = (uint32_t)(((uint64_t)(b + 1) * bs) / item_blocks);
         if (s1 > s0) {
-            const uint32_t t = s0 * 256 + threadIdx.x;
+            const uint32_t t = s0 * kPair2Block + threadIdx.x;
             if (t < n_special) {
                 const uint2 rec = a.sp_rec[t];
                 S u0[kSlotGroup<S>];
@@ -271,7 +272,7 @@ __global__ __launch_bounds__(256) void k_pair2(StepParams<T, S> a, T* __restrict
                 special_at(a, t, sp_cell[t], rec, u0, p2);
             }
         } else {
-            const uint32_t r = (b - s1) * 256 + threadIdx.x;  // ring blocks before this one: b - s1
+            const uint32_t r = (b - s1) * kPair2Block + threadIdx.x;  // ring blocks before this one: b - s1
             if (r < n_ring) {
                 const uint32_t i = ring_cell[r];
                 p2[i] = interior_update(a.p[i], a.prev[i], a.p[i - 1], a.p[i + 1], a.p[i - a.pitch], a.p[i + a.pitch],
@@ -280,7 +281,7 @@ __global__ __launch_bounds__(256) void k_pair2(StepParams<T, S> a, T* __restrict
         }
         return;
     }
-    const uint32_t rr = (blockIdx.x - item_blocks) * 256 + threadIdx.x;
+    const uint32_t rr = (blockIdx.x - item_blocks) * kPair2Block + threadIdx.x;
     if (rr < a.n_recv) a.ir[rr] = a.p[a.recv[rr]];
 }
 
