@@ -274,7 +274,8 @@ def bench_serving(args):
     print(f"bench: sequential plain {sum(service):.2f} s of service", file=sys.stderr, flush=True)
 
     # The server, fed by a client thread at the arrival times.
-    server = fdtd.Server(args.precision, hbm_cap_bytes=int(args.cap_gb * 1e9))
+    server = fdtd.Server(args.precision, hbm_cap_bytes=int(args.cap_gb * 1e9), policy=args.policy,
+                         fill_cells=args.fill, lane_cells=args.lane_cells)
     ids = [None] * len(cases)
     start = time.perf_counter()
 
@@ -297,6 +298,10 @@ def bench_serving(args):
     st = server.stats()
     server.close()
     span = st["span_s"]
+    for k, c in enumerate(cases):
+        print(f"bench: request {k}: {c.grid.air_cells() / 1e6:.2f}M air cells, {len(c.signal)} steps, "
+              f"arrival {arrivals[k]:.3f} s, service alone {service[k]:.3f} s, latency {lat[k]:.3f} s "
+              f"(sequential plain {lat_plain[k]:.3f} s)", file=sys.stderr, flush=True)
     identical = sum(int(np.array_equal(x, y)) for x, y in zip(served, plain))
     errs = []
     if args.precision != "fp64":
@@ -322,6 +327,8 @@ def bench_serving(args):
         "config": {"workload": f"serving trace: {n} requests, padded cells log-uniform 0.2M-60M, 0.3-1.0 s, "
                                f"dx 0.05 m, Poisson rate {rate:.3f}/s = {args.load:.0%} of "
                                f"{args.throughput} Gcell-updates/s",
+                   "policy": args.policy, "fill_cells": args.fill or 16 << 20,
+                   "lane_cells": args.lane_cells or 8 << 20,
                    "requests": n, "cell_updates": int(sum(works)), "cap_bytes": int(args.cap_gb * 1e9)},
         "latency_s": pct(lat),
         "sequential_plain": {"solves_per_s": round(n / span_plain, 3), "latency_s": pct(lat_plain),
@@ -329,6 +336,8 @@ def bench_serving(args):
         "speedup_vs_sequential_plain": round(span_plain / span, 3),
         "server": {"span_s": round(span, 3), "max_running": int(st["max_running"]),
                    "peak_device_bytes": int(st["peak_device_bytes"]), "graph_captures": int(st["graph_captures"]),
+                   "graph_launches": int(st["graph_launches"]), "blocks_alone": int(st["solo_blocks"]),
+                   "large_lane_blocks": int(st["lane_blocks"]),
                    "failed": int(st["failed"])},
         "parity": {"bit_identical_to_plain": f"{identical}/{n}",
                    "fp32_vs_fp64_rel_l2_max": max(errs) if errs else None, "fp64_sample": len(errs)},
@@ -380,6 +389,13 @@ def main():
                     help="serving: batched room-mix throughput to load against, Gcell-updates/s "
                          "(end to end, as the room_mix mode measures it)")
     ap.add_argument("--cap-gb", type=float, default=288.0, help="serving: admission cap, GB")
+    ap.add_argument("--policy", choices=["lanes", "shortest", "all"], default="lanes",
+                    help="serving: requests each block advances (least work first up to --fill, large "
+                         "requests on their own stream; least work first only; or all)")
+    ap.add_argument("--lane-cells", type=int, default=0, help="serving: air cells from which a request takes "
+                                                              "the large lane (0 = the library's 8 Mi)")
+    ap.add_argument("--fill", type=int, default=0, help="serving: air cells per block for --policy shortest "
+                                                        "(0 = the library's 16 Mi)")
     ap.add_argument("--fp64-sample", type=int, default=6, help="instances rerun in FP64 for the error")
     ap.add_argument("--no-graphs", action="store_true")
     ap.add_argument("--error-steps", type=int, default=0,

@@ -199,7 +199,9 @@ wv_status wv_batch_autotune(wv_batch* batch, uint32_t trial_steps, char* err, si
 wv_status wv_batch_set_tile(wv_batch* batch, int32_t variant, uint32_t zchunk, char* err,
                             size_t err_len);
 /* Runs every instance to its last step. use_graphs = 0 issues the same
- * launches one by one. Cancel is polled between step blocks. */
+ * launches one by one. Cancel is polled between step blocks. An instance
+ * left alone on a grid of at least 4 Mi cells runs its remaining steps with
+ * the plain run's two-step kernels (same results). */
 wv_status wv_batch_run(wv_batch* batch, int use_graphs, const volatile int32_t* cancel,
                        wv_progress_fn progress, void* ctx, char* err, size_t err_len);
 /* An instance's results, as wv_run_finish (ir_out [R][T], spectra [R][F][2]). */
@@ -209,8 +211,8 @@ wv_status wv_batch_result(wv_batch* batch, int32_t id, const double* freqs, size
 wv_status wv_batch_info(const wv_batch* batch, int32_t id, uint64_t out[6]);
 /* Last wv_batch_run: device ms (HIP events on the batch stream); out: [step
  * launches per instance block sequence, cell updates, graph launches, graph
- * captures, block variant]. */
-wv_status wv_batch_timing(const wv_batch* batch, double* ms, uint64_t out[5]);
+ * captures, block variant, blocks run alone]. */
+wv_status wv_batch_timing(const wv_batch* batch, double* ms, uint64_t out[6]);
 void wv_batch_free(wv_batch* batch);
 
 /* ----------------------------------------------------------------- serving
@@ -235,10 +237,24 @@ wv_status wv_serve_submit(wv_server* server, const wv_grid* grid, const wv_tap* 
  * completion - submission. Each id can be waited for once. */
 wv_status wv_serve_wait(wv_server* server, uint64_t id, double* spectra_out, double* ir_out,
                         double* latency_s, char* err, size_t err_len);
+/* Which admitted requests each 128-step block advances.
+ * WV_SERVE_ALL: every one, in one batched launch per step.
+ * WV_SERVE_SHORTEST: least remaining work (air cells x steps left) first,
+ * while the chosen air cells stay within fill_cells (0 = 16 Mi); the first
+ * always runs, the others wait on the device.
+ * WV_SERVE_LANES (the default): as WV_SERVE_SHORTEST for requests below
+ * lane_cells air cells (0 = 8 Mi); the larger ones run one at a time, in
+ * admission order and each to its end, on a second stream beside the batch,
+ * which the device runs at the same time.
+ * A request that runs alone on a grid of at least 4 Mi cells runs the plain
+ * run's two-step kernels. Results do not depend on the policy. */
+enum { WV_SERVE_ALL = 0, WV_SERVE_SHORTEST = 1, WV_SERVE_LANES = 2 };
+wv_status wv_serve_set_policy(wv_server* server, int policy, uint64_t fill_cells, uint64_t lane_cells);
 /* out: [completed, p50, p95, p99 latency s, first submission to last
  * completion s, cell updates, most requests running at once, peak device
- * bytes, failed, graph captures]. */
-wv_status wv_serve_stats(wv_server* server, double out[10]);
+ * bytes, failed, graph captures, blocks run alone on the batch stream, graph
+ * launches, blocks on the large-request lane]. */
+wv_status wv_serve_stats(wv_server* server, double out[13]);
 /* Finishes every submitted request, then stops the worker. */
 void wv_serve_free(wv_server* server);
 

@@ -147,3 +147,96 @@ def test_server_rejects_oversized_request(rooms):
         rid = sv.submit(c.grid, c.runs[0][2], c.signal, c.receivers)
         with pytest.raises(fdtd.FdtdError):
             sv.wait(rid)
+
+
+def large_room(steps=601, seed=11):
+    """A mix-recipe room on a grid of more than 4 Mi cells, so its plain run
+    takes two steps per pass; an odd step count that is not a multiple of 128."""
+    rng = np.random.default_rng(seed)
+    lib = configs.material_library(seed)
+    cfg = configs.mix_room(rng, lib, dx=0.05, dims=(14.0, 10.0, 4.0), n_rx=6)
+    cfg["steps"] = steps
+    return workload.build_case(cfg)
+
+
+@pytest.fixture(scope="module")
+def big():
+    c = large_room()
+    assert c.grid.cells >= 4 << 20
+    with fdtd.Plan(c.grid, "fp32") as p:
+        ir = p.run(c.runs[0][2], c.signal, c.receivers, mode=fdtd.MODE_IR)[0]
+        assert p.pass_info()["steps_per_pass"] == 2
+    return c, ir
+
+
+@pytest.mark.parametrize("policy,fill", [("all", 0), ("shortest", 0), ("shortest", 1), ("lanes", 0), ("lanes", 1)])
+def test_server_policies_keep_results(rooms, policy, fill):
+    """Whichever requests each block advances (all, or least work first up
+    to a fill; fill 1 runs one request per block), every result equals its
+    plain run."""
+    cases, plain = rooms
+    print(f"case: server policy {policy}, fill {fill}, 5 requests, fp32", flush=True)
+    with fdtd.Server("fp32", policy=policy, fill_cells=fill) as sv:
+        ids = [sv.submit(c.grid, c.runs[0][2], c.signal, c.receivers) for c in cases]
+        out = [sv.wait(r) for r in ids]
+        st = sv.stats()
+    for i, (ir, _, _) in enumerate(out):
+        assert np.array_equal(ir, plain[("fp32", i)]), f"request {i}"
+    assert st["completed"] == 5 and st["failed"] == 0
+    assert st["solo_blocks"] == 0  # no grid here has two-step passes
+
+
+@pytest.mark.parametrize("policy", ["all", "shortest", "lanes"])
+def test_server_large_request_alone_runs_pairs(rooms, big, policy):
+    """A large request (601 steps) shares blocks with a small one (274
+    steps), then runs alone on the plain run's two-step kernels; both results
+    equal their plain runs."""
+    cases, plain = rooms
+    c, ref = big
+    print(f"case: server policy {policy}, a {c.grid.cells}-cell request and a small one, fp32", flush=True)
+    with fdtd.Server("fp32", policy=policy) as sv:
+        ids = [sv.submit(c.grid, c.runs[0][2], c.signal, c.receivers),
+               sv.submit(cases[3].grid, cases[3].runs[0][2], cases[3].signal, cases[3].receivers)]
+        (ir_big, _, _), (ir_small, _, _) = [sv.wait(r) for r in ids]
+        st = sv.stats()
+    assert np.array_equal(ir_big, ref)
+    assert np.array_equal(ir_small, plain[("fp32", 3)])
+    assert st["solo_blocks"] > 0
+
+
+def test_batch_last_large_instance_runs_pairs(rooms, big):
+    """In a batch, the large instance outlives the small ones and finishes
+    alone on the two-step kernels, bit-identical to its plain run."""
+    cases, plain = rooms
+    c, ref = big
+    print(f"case: batch of a {c.grid.cells}-cell instance and two small ones, fp32", flush=True)
+    with fdtd.Batch("fp32") as b:
+        ids = [b.add(c.grid, c.runs[0][2], c.signal, c.receivers)]
+        ids += [b.add(x.grid, x.runs[0][2], x.signal[:150], x.receivers) for x in cases[:2]]
+        b.run()
+        t = b.timing()
+        assert np.array_equal(b.result(ids[0])[0], ref)
+        for i in range(2):
+            assert np.array_equal(b.result(ids[1 + i])[0], plain[("fp32", i)][:, :150])
+    assert t["solo_blocks"] > 0
+
+
+def test_server_large_lane_beside_the_batch(rooms, big):
+    """Policy "lanes" with a 1 Mi-cell lane threshold: two large requests take
+    the second stream one after the other while the small ones run as a batch
+    beside them; every result equals its plain run."""
+    cases, plain = rooms
+    c, ref = big
+    print("case: server policy lanes, lane_cells 1Mi, two large and three small requests, fp32", flush=True)
+    with fdtd.Server("fp32", policy="lanes", lane_cells=1 << 20) as sv:
+        ids = [sv.submit(c.grid, c.runs[0][2], c.signal, c.receivers),
+               sv.submit(c.grid, c.runs[0][2], c.signal[:400], c.receivers)]
+        ids += [sv.submit(x.grid, x.runs[0][2], x.signal, x.receivers) for x in cases[:3]]
+        out = [sv.wait(r)[0] for r in ids]
+        st = sv.stats()
+    assert np.array_equal(out[0], ref)
+    assert np.array_equal(out[1], ref[:, :400])
+    for i in range(3):
+        assert np.array_equal(out[2 + i], plain[("fp32", i)]), f"small request {i}"
+    assert st["completed"] == 5 and st["failed"] == 0
+    assert st["lane_blocks"] >= 8  # 601 and 400 steps in 128-step blocks

@@ -13,10 +13,19 @@
 // block variant (one template for the launch), then the z-chunk of each size
 // class of instances, coordinate by coordinate.
 //
+// An instance that advances alone (the last of a batch, or a large request
+// the server runs by itself) runs the plain run's kernels, two steps per pass
+// where the plain run has them; the results are the same bit for bit.
+//
 // The server admits queued requests by HBM footprint into a running batch at
 // the next 128-step boundary, so requests that arrive while others run share
 // the device (continuous batching); finished instances leave at the same
-// boundaries and their memory is freed.
+// boundaries and their memory is freed. Each block advances either every
+// admitted instance or those with the least work left, up to a fill of air
+// cells, so small requests do not wait in step with large ones. By default
+// large requests run on a second stream beside the batch (their own lane, two
+// steps per pass), one at a time in admission order; the device runs both
+// streams' blocks at once.
 //
 // The reference runs a wave job's sources and components one after another
 // (core/wavey-solver/src/run.rs:645-661); each instance here is one such run.
@@ -42,6 +51,8 @@ using namespace wv;
 namespace {
 
 constexpr uint32_t kGraphSteps = kStepsPerBlock;  // steps per captured block
+constexpr uint64_t kDefaultFillCells = 16ull << 20;  // the server's fill: air cells per block
+constexpr uint64_t kDefaultLaneCells = 8ull << 20;   // air cells from which a request takes the large lane
 constexpr uint32_t kTuneZChunks[] = {4, 8};
 constexpr int kTuneVariants[] = {0, 1, 2};
 
@@ -138,6 +149,8 @@ struct Instance {
     uint32_t steps = 0;
     uint32_t n0 = 0;  // host mirror of the device block start
     bool done = false;
+    bool paused = false;  // left out of the launch table by the server's selection
+    int lane = -1;        // the server lane with a block of this instance in flight, or -1
     int size_class = 0;
     uint64_t footprint = 0;
 };
@@ -164,7 +177,7 @@ struct wv_batch {
     hipGraphExec_t exec = nullptr;
     double last_ms = 0.0;
     uint64_t last_steps = 0, last_cell_updates = 0;
-    std::atomic<uint64_t> graph_launches{0}, captures{0};
+    std::atomic<uint64_t> graph_launches{0}, captures{0}, solo_blocks{0};
     std::mutex add_mu;  // wv_batch_add may be called from several threads at once
 
     void drop_graph() {
@@ -213,7 +226,10 @@ std::unique_ptr<Instance> make_instance(int prec, const wv_grid* g, const wv_tap
     auto in = std::make_unique<Instance>();
     in->plan = std::make_unique<wv_plan>();
     in->plan->prec = prec;
-    in->plan->steps_per_pass = 1;  // the batched launch is single-step
+    // The batched launch is single-step; the plan keeps the plain run's
+    // two-step passes (ring list, two more field buffers) for blocks in which
+    // the instance advances alone.
+    in->plan->steps_per_pass = 0;
     plan_setup(in->plan.get(), g);
     in->taps.assign(taps, taps + n_taps);
     in->signal.assign(signal, signal + steps);
@@ -230,7 +246,10 @@ std::unique_ptr<Instance> make_instance(int prec, const wv_grid* g, const wv_tap
 }
 
 // Rebuilds the launch table from the instances that still have steps to
-// run. The stream must be idle.
+// run and are not paused. The stream must be idle. Every block starts at an
+// even step (a multiple of 128), where p is in the plan's buffer buf_p and
+// prev in buf_q: the batched step updates prev in place, so after a block p
+// is back in buf_p.
 template <typename T, typename S>
 void build_table(wv_batch* b) {
     const uint32_t ty = kVariantTiles[b->variant];
@@ -239,12 +258,13 @@ void build_table(wv_batch* b) {
     b->table.clear();
     for (auto& up : b->inst) {
         Instance* in = up.get();
-        if (in->done || in->n0 >= in->steps) continue;
+        if (in->done || in->paused || in->n0 >= in->steps) continue;
         wv_plan* pl = in->plan.get();
+        const DBuf* fields[4] = {&pl->fa, &pl->fb, &pl->fc, &pl->fd};
         BatchEntry<T, S> e{};
         e.a = make_params<T, S>(pl, ty);
-        e.a.p = pl->fa.as<T>();
-        e.a.prev = pl->fb.as<T>();
+        e.a.p = fields[pl->buf_p]->as<T>();
+        e.a.prev = fields[pl->buf_q]->as<T>();
         e.a.ir = pl->ir.as<T>();
         e.sig = in->sig.as<S>();
         e.steps = in->steps;
@@ -301,11 +321,22 @@ void enqueue_block(wv_batch* b, bool use_graph) {
     for (Instance* in : b->table) in->n0 += kGraphSteps;
 }
 
-// Marks table instances whose last step has run (after a synchronize).
-// Returns how many finished.
+// Enqueues the next block (up to kGraphSteps steps) of one instance alone,
+// with the plain run's kernels.
+void enqueue_solo(wv_batch* b, Instance* in) {
+    const size_t stop = std::min<size_t>((size_t)in->n0 + kGraphSteps, in->steps);
+    enqueue_steps(in->plan.get(), in->n0, stop, b->stream);
+    in->n0 = (uint32_t)stop;
+    b->dirty = true;  // its p and prev may have moved to other buffers
+    ++b->solo_blocks;
+}
+
+// Marks instances whose last step has run (after a synchronize). Returns how
+// many finished.
 uint32_t collect_done(wv_batch* b) {
     uint32_t k = 0;
-    for (Instance* in : b->table)
+    for (auto& up : b->inst) {
+        Instance* in = up.get();
         if (!in->done && in->n0 >= in->steps) {
             in->done = true;
             in->plan->done = in->steps;
@@ -314,6 +345,7 @@ uint32_t collect_done(wv_batch* b) {
             b->dirty = true;
             ++k;
         }
+    }
     return k;
 }
 
@@ -337,9 +369,11 @@ wv_status batch_run(wv_batch* b, bool use_graph, const volatile int32_t* cancel,
         // does not change before then, so they are enqueued back to back.
         uint32_t blocks = ~0u;
         for (Instance* in : b->table) blocks = std::min(blocks, div_up(in->steps - in->n0, kGraphSteps));
+        Instance* solo = b->table.size() == 1 && b->table[0]->plan->run_steps_per_pass == 2 ? b->table[0] : nullptr;
         uint32_t issued = 0;
         while (issued < blocks) {
-            enqueue_block<T, S>(b, use_graph);
+            if (solo) enqueue_solo(b, solo);
+            else enqueue_block<T, S>(b, use_graph);
             ++issued;
             if (cancel && *cancel) break;
         }
@@ -563,7 +597,7 @@ extern "C" wv_status wv_batch_info(const wv_batch* b, int32_t id, uint64_t out[6
     return WV_OK;
 }
 
-extern "C" wv_status wv_batch_timing(const wv_batch* b, double* ms, uint64_t out[5]) {
+extern "C" wv_status wv_batch_timing(const wv_batch* b, double* ms, uint64_t out[6]) {
     if (!b || !ms || !out) return WV_ERR_INVALID;
     *ms = b->last_ms;
     out[0] = b->last_steps;
@@ -571,6 +605,7 @@ extern "C" wv_status wv_batch_timing(const wv_batch* b, double* ms, uint64_t out
     out[2] = b->graph_launches;
     out[3] = b->captures;
     out[4] = (uint64_t)b->variant;
+    out[5] = b->solo_blocks;
     return WV_OK;
 }
 
@@ -600,6 +635,10 @@ struct Request {
 struct wv_server {
     int prec = WV_FP32;
     uint64_t cap = 0;
+    std::atomic<int> policy{WV_SERVE_LANES};
+    std::atomic<uint64_t> fill{kDefaultFillCells};
+    std::atomic<uint64_t> lane_cells{kDefaultLaneCells};
+    std::atomic<uint64_t> submitted{0};  // submissions so far (wakes the worker's wait)
     int device = 0;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
@@ -609,8 +648,10 @@ struct wv_server {
     bool stop = false;
     std::thread worker;
     std::unique_ptr<wv_batch> batch;
+    hipStream_t lane_stream = nullptr;  // the large-request lane
+    hipEvent_t lane_ev[2] = {nullptr, nullptr};
     std::vector<std::shared_ptr<Request>> running;
-    uint64_t used = 0, peak_used = 0, cell_updates = 0, completed = 0, failed = 0;
+    uint64_t used = 0, peak_used = 0, cell_updates = 0, completed = 0, failed = 0, lane_blocks = 0;
     uint32_t max_active = 0;
     double t_first = -1.0, t_last = 0.0;
     std::vector<double> latencies;
@@ -623,10 +664,100 @@ struct wv_server {
         }
         cv_work.notify_all();
         if (worker.joinable()) worker.join();
+        if (lane_stream) (void)hipStreamSynchronize(lane_stream);
+        for (auto e : lane_ev)
+            if (e) (void)hipEventDestroy(e);
+        if (lane_stream) (void)hipStreamDestroy(lane_stream);
     }
 };
 
 namespace {
+
+// A stream with the block in flight on it: lane 0 is the batch's stream
+// (the batched launch, or one instance alone), lane 1 the large-request lane.
+struct Lane {
+    hipStream_t stream = nullptr;
+    hipEvent_t ev = nullptr;
+    bool busy = false;
+    std::vector<Instance*> members;  // instances the block in flight advances
+};
+
+bool lane_finished(const Lane& l) {
+    const hipError_t e = hipEventQuery(l.ev);
+    if (e == hipErrorNotReady) return false;
+    WV_HIP(e);
+    return true;
+}
+
+uint64_t work_left(const Instance* in) { return in->plan->air * (uint64_t)(in->steps - in->n0); }
+
+// The instance for the large-request lane's next block (policy
+// WV_SERVE_LANES): of the instances with two-step passes and at least
+// lane_cells air cells, the one the lane has started, else the earliest
+// admitted (so the lane serves large requests first come, first served, as
+// a queue of plain runs would), or null.
+Instance* pick_lane_instance(wv_server* sv) {
+    Instance* first = nullptr;
+    for (auto& up : sv->batch->inst) {
+        Instance* in = up.get();
+        if (in->done || in->n0 >= in->steps || in->lane >= 0) continue;
+        if (in->plan->run_steps_per_pass != 2 || in->plan->air < sv->lane_cells.load()) continue;
+        if (in->n0 > 0) return in;
+        if (!first) first = in;
+    }
+    return first;
+}
+
+// Chooses the instances the batch stream's next block advances and pauses
+// the others. WV_SERVE_ALL: every admitted instance. WV_SERVE_SHORTEST and
+// WV_SERVE_LANES: least remaining work (air cells x steps left) first,
+// earliest admitted among equals; an instance joins while the chosen air
+// cells stay within the fill (the first always runs). WV_SERVE_LANES leaves
+// out the instances that belong to the large-request lane. Instances in
+// flight on the other lane are never chosen. Returns the chosen instances.
+std::vector<Instance*> select_batch(wv_server* sv) {
+    wv_batch* b = sv->batch.get();
+    const int policy = sv->policy.load();
+    std::vector<Instance*> live;
+    for (auto& up : b->inst) {
+        Instance* in = up.get();
+        if (in->done || in->n0 >= in->steps) continue;
+        if (in->lane >= 0) {
+            if (!in->paused) b->dirty = true;
+            in->paused = true;
+            continue;
+        }
+        live.push_back(in);
+    }
+    std::vector<char> run(live.size(), 1);
+    if (policy != WV_SERVE_ALL) {
+        std::vector<size_t> order;
+        for (size_t k = 0; k < live.size(); ++k) {
+            const wv_plan* pl = live[k]->plan.get();
+            if (policy == WV_SERVE_LANES && pl->run_steps_per_pass == 2 && pl->air >= sv->lane_cells.load()) run[k] = 0;
+            else order.push_back(k);
+        }
+        std::stable_sort(order.begin(), order.end(),
+                         [&](size_t x, size_t y) { return work_left(live[x]) < work_left(live[y]); });
+        const uint64_t fill = sv->fill.load();
+        uint64_t cells = 0;
+        for (size_t j = 0; j < order.size(); ++j) {
+            const uint64_t air = live[order[j]]->plan->air;
+            run[order[j]] = j == 0 || cells + air <= fill;
+            if (run[order[j]]) cells += air;
+        }
+    }
+    std::vector<Instance*> chosen;
+    for (size_t k = 0; k < live.size(); ++k) {
+        const bool paused = !run[k];
+        if (live[k]->paused != paused) {
+            live[k]->paused = paused;
+            b->dirty = true;
+        }
+        if (!paused) chosen.push_back(live[k]);
+    }
+    return chosen;
+}
 
 void fail_request(wv_server* sv, const std::shared_ptr<Request>& r, wv_status st, const std::string& msg) {
     r->status = st;
@@ -636,108 +767,169 @@ void fail_request(wv_server* sv, const std::shared_ptr<Request>& r, wv_status st
     ++sv->failed;
 }
 
+// Admits queued requests whose footprint fits under the cap (the lock is
+// held on entry and exit; released while an instance is set up).
+template <typename S>
+void admit(wv_server* sv, std::unique_lock<std::mutex>& lk) {
+    wv_batch* b = sv->batch.get();
+    while (!sv->pending.empty()) {
+        auto r = sv->pending.front();
+        const uint64_t est = estimate_bytes(r->grid, sv->prec, r->signal.size(), r->recv.size(), r->taps.size());
+        if (est > sv->cap) {
+            sv->pending.pop_front();
+            fail_request(sv, r, WV_ERR_LIMIT, "request needs more device memory than the admission cap");
+            sv->cv_done.notify_all();
+            continue;
+        }
+        if (sv->used + est > sv->cap) break;  // waits for running requests to free memory
+        sv->pending.pop_front();
+        sv->used += est;
+        r->reserved = est;
+        lk.unlock();
+        std::unique_ptr<Instance> in;
+        std::string msg;
+        wv_status st = WV_OK;
+        try {
+            in = make_instance<S>(sv->prec, r->grid, r->taps.data(), r->taps.size(), r->signal.data(),
+                                  r->signal.size(), r->recv.data(), r->recv.size());
+        } catch (const DevErr& e) {
+            st = e.code;
+            msg = e.msg;
+        } catch (const std::bad_alloc&) {
+            st = WV_ERR_NOMEM;
+            msg = "host allocation failed";
+        }
+        lk.lock();
+        if (st != WV_OK) {
+            sv->used -= r->reserved;
+            fail_request(sv, r, st, msg);
+            sv->cv_done.notify_all();
+            continue;
+        }
+        sv->used += in->footprint;
+        sv->used -= r->reserved;
+        r->reserved = in->footprint;
+        sv->peak_used = std::max(sv->peak_used, sv->used);
+        r->t_admit = now_s();
+        in->id = (int32_t)r->id;
+        r->inst = in.get();
+        b->inst.push_back(std::move(in));
+        b->dirty = true;
+        sv->running.push_back(r);
+        sv->max_active = std::max<uint32_t>(sv->max_active, (uint32_t)sv->running.size());
+    }
+}
+
+// Finished instances that no lane has in flight: results to host memory,
+// device memory freed, waiters woken. Called without the lock.
+void finish_done(wv_server* sv) {
+    wv_batch* b = sv->batch.get();
+    std::vector<std::shared_ptr<Request>> fin;
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        for (auto& r : sv->running)
+            if (r->inst && r->inst->lane < 0 && r->inst->n0 >= r->inst->steps) fin.push_back(r);
+    }
+    for (auto& r : fin) {
+        Instance* in = r->inst;
+        in->done = true;
+        in->plan->done = in->steps;
+        in->plan->steps = in->steps;
+        in->plan->active = true;
+        const size_t R = in->recv.size(), Tn = in->steps, nf = r->freqs.size();
+        if (r->mode & WV_MODE_IR) r->ir.assign(R * Tn, 0.0);
+        if ((r->mode & WV_MODE_SPECTRA) && nf) r->spectra.assign(2 * R * nf, 0.0);
+        run_finish(in->plan.get(), r->freqs.data(), nf, r->mode, r->spectra.empty() ? nullptr : r->spectra.data(),
+                   r->ir.empty() ? nullptr : r->ir.data());
+        const uint64_t air = in->plan->air;
+        auto it = std::find_if(b->inst.begin(), b->inst.end(),
+                               [&](const std::unique_ptr<Instance>& p) { return p.get() == in; });
+        b->inst.erase(it);  // frees the instance's device memory
+        b->dirty = true;
+        std::lock_guard<std::mutex> lk(sv->mu);
+        r->inst = nullptr;
+        r->done = true;
+        r->t_done = now_s();
+        sv->t_last = r->t_done;
+        sv->used -= r->reserved;
+        sv->cell_updates += air * Tn;
+        sv->latencies.push_back(r->t_done - r->t_submit);
+        ++sv->completed;
+        sv->running.erase(std::find(sv->running.begin(), sv->running.end(), r));
+    }
+    if (!fin.empty()) sv->cv_done.notify_all();
+}
+
+// The worker: admission, then each idle lane gets its next block (up to 128
+// steps), then a wait until a lane's block ends or a request arrives. Blocks
+// on the two lanes run at the same time on the device.
 template <typename T, typename S>
 void serve_loop(wv_server* sv) {
     wv_batch* b = sv->batch.get();
+    Lane lanes[2];
+    lanes[0].stream = b->stream;
+    lanes[0].ev = sv->lane_ev[0];
+    lanes[1].stream = sv->lane_stream;
+    lanes[1].ev = sv->lane_ev[1];
+    uint64_t seen = 0;
     std::unique_lock<std::mutex> lk(sv->mu);
     while (true) {
-        const bool busy = !sv->running.empty();
+        seen = sv->submitted.load();
+        admit<S>(sv, lk);
         lk.unlock();
-        // The next block of the running batch goes to the device first, so
-        // admission work below overlaps with it.
-        if (busy) {
-            if (b->dirty) build_table<T, S>(b);
-            enqueue_block<T, S>(b, true);
+        for (Lane& l : lanes)
+            if (l.busy && lane_finished(l)) {
+                for (Instance* in : l.members) in->lane = -1;
+                l.members.clear();
+                l.busy = false;
+            }
+        finish_done(sv);
+        // The large-request lane.
+        Lane& big = lanes[1];
+        if (!big.busy && sv->policy.load() == WV_SERVE_LANES) {
+            if (Instance* in = pick_lane_instance(sv)) {
+                const size_t stop = std::min<size_t>((size_t)in->n0 + kGraphSteps, in->steps);
+                enqueue_steps(in->plan.get(), in->n0, stop, big.stream);
+                in->n0 = (uint32_t)stop;
+                in->lane = 1;
+                big.members.push_back(in);
+                WV_HIP(hipEventRecord(big.ev, big.stream));
+                big.busy = true;
+                ++sv->lane_blocks;
+            }
         }
-        lk.lock();
-        bool admitted = false;
-        while (!sv->pending.empty()) {
-            auto r = sv->pending.front();
-            const uint64_t est = estimate_bytes(r->grid, sv->prec, r->signal.size(), r->recv.size(), r->taps.size());
-            if (est > sv->cap) {
-                sv->pending.pop_front();
-                fail_request(sv, r, WV_ERR_LIMIT, "request needs more device memory than the admission cap");
-                sv->cv_done.notify_all();
-                continue;
-            }
-            if (sv->used + est > sv->cap) break;  // waits for running requests to free memory
-            sv->pending.pop_front();
-            sv->used += est;
-            r->reserved = est;
-            lk.unlock();
-            std::unique_ptr<Instance> in;
-            std::string msg;
-            wv_status st = WV_OK;
-            try {
-                in = make_instance<S>(sv->prec, r->grid, r->taps.data(), r->taps.size(), r->signal.data(),
-                                      r->signal.size(), r->recv.data(), r->recv.size());
-            } catch (const DevErr& e) {
-                st = e.code;
-                msg = e.msg;
-            } catch (const std::bad_alloc&) {
-                st = WV_ERR_NOMEM;
-                msg = "host allocation failed";
-            }
-            lk.lock();
-            if (st != WV_OK) {
-                sv->used -= r->reserved;
-                fail_request(sv, r, st, msg);
-                sv->cv_done.notify_all();
-                continue;
-            }
-            sv->used += in->footprint;
-            sv->used -= r->reserved;
-            r->reserved = in->footprint;
-            sv->peak_used = std::max(sv->peak_used, sv->used);
-            r->t_admit = now_s();
-            in->id = (int32_t)r->id;
-            r->inst = in.get();
-            b->inst.push_back(std::move(in));
-            b->dirty = true;
-            sv->running.push_back(r);
-            sv->max_active = std::max<uint32_t>(sv->max_active, (uint32_t)sv->running.size());
-            admitted = true;
-        }
-        lk.unlock();
-        if (busy) {
-            WV_HIP(hipStreamSynchronize(b->stream));
-            if (collect_done(b)) {
-                // Finished instances: results to host memory, device memory freed.
-                std::vector<std::shared_ptr<Request>> fin;
-                for (auto& r : sv->running)
-                    if (r->inst && r->inst->done) fin.push_back(r);
-                for (auto& r : fin) {
-                    Instance* in = r->inst;
-                    const size_t R = in->recv.size(), Tn = in->steps, nf = r->freqs.size();
-                    if (r->mode & WV_MODE_IR) r->ir.assign(R * Tn, 0.0);
-                    if ((r->mode & WV_MODE_SPECTRA) && nf) r->spectra.assign(2 * R * nf, 0.0);
-                    run_finish(in->plan.get(), r->freqs.data(), nf, r->mode, r->spectra.empty() ? nullptr : r->spectra.data(),
-                               r->ir.empty() ? nullptr : r->ir.data());
-                    const uint64_t air = in->plan->air;
-                    auto it = std::find_if(b->inst.begin(), b->inst.end(),
-                                           [&](const std::unique_ptr<Instance>& p) { return p.get() == in; });
-                    b->inst.erase(it);  // frees the instance's device memory
-                    lk.lock();
-                    r->inst = nullptr;
-                    r->done = true;
-                    r->t_done = now_s();
-                    sv->t_last = r->t_done;
-                    sv->used -= r->reserved;
-                    sv->cell_updates += air * Tn;
-                    sv->latencies.push_back(r->t_done - r->t_submit);
-                    ++sv->completed;
-                    sv->running.erase(std::find(sv->running.begin(), sv->running.end(), r));
-                    lk.unlock();
+        // The batch stream: the batched launch, or one instance alone.
+        Lane& bat = lanes[0];
+        if (!bat.busy) {
+            std::vector<Instance*> chosen = select_batch(sv);
+            if (!chosen.empty()) {
+                if (chosen.size() == 1 && chosen[0]->plan->run_steps_per_pass == 2) {
+                    enqueue_solo(b, chosen[0]);
+                } else {
+                    if (b->dirty) build_table<T, S>(b);
+                    enqueue_block<T, S>(b, true);
                 }
-                b->dirty = true;
-                sv->cv_done.notify_all();
+                for (Instance* in : chosen) in->lane = 0;
+                bat.members = std::move(chosen);
+                WV_HIP(hipEventRecord(bat.ev, bat.stream));
+                bat.busy = true;
+            }
+        }
+        const bool busy = lanes[0].busy || lanes[1].busy;
+        if (busy) {
+            // Until a block ends or a request is submitted.
+            for (uint32_t spin = 0;; ++spin) {
+                if ((lanes[0].busy && lane_finished(lanes[0])) || (lanes[1].busy && lane_finished(lanes[1]))) break;
+                if (sv->submitted.load() != seen) break;
+                if (spin < 256) std::this_thread::yield();
+                else std::this_thread::sleep_for(std::chrono::microseconds(20));
             }
         }
         lk.lock();
-        // Idle: sleep until a request arrives (with nothing running, any
-        // pending request fits under the cap and is admitted above).
-        if (sv->running.empty() && !admitted) {
+        if (!busy && sv->running.empty()) {
             if (sv->stop && sv->pending.empty()) break;
+            // Idle: sleep until a request arrives (with nothing running, any
+            // pending request fits under the cap and is admitted above).
             if (sv->pending.empty()) sv->cv_work.wait(lk);
         }
     }
@@ -792,6 +984,8 @@ extern "C" wv_status wv_serve_create(int precision, uint64_t hbm_cap_bytes, wv_s
         WV_HIP(hipGetDevice(&sv->device));
         sv->batch = std::make_unique<wv_batch>();
         batch_init(sv->batch.get(), precision);
+        WV_HIP(hipStreamCreateWithFlags(&sv->lane_stream, hipStreamNonBlocking));
+        for (auto& e : sv->lane_ev) WV_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         wv_server* p = sv.get();
         sv->worker = std::thread([p]() { serve_main(p); });
         *out = sv.release();
@@ -839,6 +1033,7 @@ extern "C" wv_status wv_serve_submit(wv_server* sv, const wv_grid* grid, const w
             sv->pending.push_back(r);
             *id = r->id;
         }
+        sv->submitted.fetch_add(1);
         sv->cv_work.notify_all();
         return WV_OK;
     } catch (const std::bad_alloc&) {
@@ -870,7 +1065,16 @@ extern "C" wv_status wv_serve_wait(wv_server* sv, uint64_t id, double* spectra_o
     return WV_OK;
 }
 
-extern "C" wv_status wv_serve_stats(wv_server* sv, double out[10]) {
+extern "C" wv_status wv_serve_set_policy(wv_server* sv, int policy, uint64_t fill_cells, uint64_t lane_cells) {
+    if (!sv || (policy != WV_SERVE_ALL && policy != WV_SERVE_SHORTEST && policy != WV_SERVE_LANES))
+        return WV_ERR_INVALID;
+    sv->policy.store(policy);
+    sv->fill.store(fill_cells ? fill_cells : kDefaultFillCells);
+    sv->lane_cells.store(lane_cells ? lane_cells : kDefaultLaneCells);
+    return WV_OK;
+}
+
+extern "C" wv_status wv_serve_stats(wv_server* sv, double out[13]) {
     if (!sv || !out) return WV_ERR_INVALID;
     std::lock_guard<std::mutex> lk(sv->mu);
     out[0] = (double)sv->completed;
@@ -883,5 +1087,8 @@ extern "C" wv_status wv_serve_stats(wv_server* sv, double out[10]) {
     out[7] = (double)sv->peak_used;
     out[8] = (double)sv->failed;
     out[9] = (double)(sv->batch ? sv->batch->captures.load() : 0);
+    out[10] = (double)(sv->batch ? sv->batch->solo_blocks.load() : 0);
+    out[11] = (double)(sv->batch ? sv->batch->graph_launches.load() : 0);
+    out[12] = (double)sv->lane_blocks;
     return WV_OK;
 }

@@ -613,8 +613,12 @@ PairParams<T, S> make_pair_params(wv_plan* pl, const StepParams<T, S>& base) {
     return pp;
 }
 
+// Enqueues steps [n, stop) of the plan's run on `stream`: pairs of steps as
+// two-step passes where the run has them, single steps otherwise (and for an
+// odd last step). The plan's buf_p / buf_q follow the field buffers. Returns
+// the number of launches.
 template <class T, class S>
-wv_status advance_impl(wv_plan* pl, size_t k, const volatile int32_t* cancel, wv_progress_fn progress, void* ctx) {
+uint64_t enqueue_range(wv_plan* pl, size_t n, size_t stop, hipStream_t stream) {
     StepParams<T, S> a = make_params<T, S>(pl, kVariantTiles[pl->variant]);
     const uint32_t grid = a.main_blocks + div_up(pl->n_recv, variant_threads(pl->variant));
     const bool pairs = pl->run_steps_per_pass == 2;
@@ -627,9 +631,58 @@ wv_status advance_impl(wv_plan* pl, size_t k, const volatile int32_t* cancel, wv
         grid2 = items2 + div_up(pl->n_recv, kPair2Block);
     }
     T* bufs[4] = {pl->fa.as<T>(), pl->fb.as<T>(), pl->fc.as<T>(), pl->fd.as<T>()};
-    auto sig = [&](size_t n) -> S { return (S)(sizeof(S) == 4 ? (double)(float)pl->signal[n] : pl->signal[n]); };
-    const size_t end = pl->done + k;
+    auto sig = [&](size_t m) -> S { return (S)(sizeof(S) == 4 ? (double)(float)pl->signal[m] : pl->signal[m]); };
     T* ir = pl->ir.as<T>();
+    uint64_t launches = 0;
+    while (n < stop) {
+        T* P = bufs[pl->buf_p];
+        T* Q = bufs[pl->buf_q];
+        if (pairs && n + 2 <= stop) {
+            // Steps n, n+1 in two launches; outputs go to the two free
+            // buffers, which then hold p (p^{n+2}) and prev (p^{n+1}).
+            int o1 = 0;
+            while (o1 == pl->buf_p || o1 == pl->buf_q) ++o1;
+            int o2 = o1 + 1;
+            while (o2 == pl->buf_p || o2 == pl->buf_q) ++o2;
+            pp.a.p = P;
+            pp.a.prev = Q;
+            pp.a.s_n = sig(n);
+            pp.a.ir = ir + n * pl->n_recv;
+            pp.p1 = bufs[o1];
+            pp.p2 = bufs[o2];
+            hipLaunchKernelGGL((k_pair1<T, S>), dim3(grid1), dim3(kPairThreads), 0, stream, pp);
+            StepParams<T, S> a2 = a;
+            a2.p = bufs[o1];
+            a2.prev = P;
+            a2.s_n = sig(n + 1);
+            a2.ir = ir + (n + 1) * pl->n_recv;
+            hipLaunchKernelGGL((k_pair2<T, S>), dim3(grid2), dim3(kPair2Block), 0, stream, a2, bufs[o2],
+                               pl->sp_cell.as<uint32_t>(), pl->ring_cell.as<uint32_t>(), pl->n_special, pl->n_ring,
+                               items2);
+            pl->buf_p = o2;
+            pl->buf_q = o1;
+            launches += 2;
+            n += 2;
+        } else {
+            // One step, updating prev in place (the reference swaps
+            // buffers every step, gpu_fdtd.rs:242-243).
+            a.p = P;
+            a.prev = Q;
+            a.s_n = sig(n);
+            a.ir = ir + n * pl->n_recv;
+            if (grid) launch_step<T, S>(pl->variant, grid, stream, a);
+            std::swap(pl->buf_p, pl->buf_q);
+            ++launches;
+            ++n;
+        }
+    }
+    WV_HIP(hipGetLastError());
+    return launches;
+}
+
+template <class T, class S>
+wv_status advance_impl(wv_plan* pl, size_t k, const volatile int32_t* cancel, wv_progress_fn progress, void* ctx) {
+    const size_t end = pl->done + k;
     uint64_t launches = 0;
     int blk = 0;
     bool pending = false;
@@ -638,50 +691,7 @@ wv_status advance_impl(wv_plan* pl, size_t k, const volatile int32_t* cancel, wv
     while (pl->done < end) {
         if (cancel && *cancel) { st = WV_CANCELLED; break; }
         const size_t stop = std::min(end, pl->done + kStepsPerBlock);
-        size_t n = pl->done;
-        while (n < stop) {
-            T* P = bufs[pl->buf_p];
-            T* Q = bufs[pl->buf_q];
-            if (pairs && n + 2 <= stop) {
-                // Steps n, n+1 in two launches; outputs go to the two free
-                // buffers, which then hold p (p^{n+2}) and prev (p^{n+1}).
-                int o1 = 0;
-                while (o1 == pl->buf_p || o1 == pl->buf_q) ++o1;
-                int o2 = o1 + 1;
-                while (o2 == pl->buf_p || o2 == pl->buf_q) ++o2;
-                pp.a.p = P;
-                pp.a.prev = Q;
-                pp.a.s_n = sig(n);
-                pp.a.ir = ir + n * pl->n_recv;
-                pp.p1 = bufs[o1];
-                pp.p2 = bufs[o2];
-                hipLaunchKernelGGL((k_pair1<T, S>), dim3(grid1), dim3(kPairThreads), 0, pl->stream, pp);
-                StepParams<T, S> a2 = a;
-                a2.p = bufs[o1];
-                a2.prev = P;
-                a2.s_n = sig(n + 1);
-                a2.ir = ir + (n + 1) * pl->n_recv;
-                hipLaunchKernelGGL((k_pair2<T, S>), dim3(grid2), dim3(kPair2Block), 0, pl->stream, a2, bufs[o2],
-                                   pl->sp_cell.as<uint32_t>(), pl->ring_cell.as<uint32_t>(), pl->n_special,
-                                   pl->n_ring, items2);
-                pl->buf_p = o2;
-                pl->buf_q = o1;
-                launches += 2;
-                n += 2;
-            } else {
-                // One step, updating prev in place (the reference swaps
-                // buffers every step, gpu_fdtd.rs:242-243).
-                a.p = P;
-                a.prev = Q;
-                a.s_n = sig(n);
-                a.ir = ir + n * pl->n_recv;
-                if (grid) launch_step<T, S>(pl->variant, grid, pl->stream, a);
-                std::swap(pl->buf_p, pl->buf_q);
-                ++launches;
-                ++n;
-            }
-        }
-        WV_HIP(hipGetLastError());
+        launches += enqueue_range<T, S>(pl, pl->done, stop, pl->stream);
         WV_HIP(hipEventRecord(pl->ev[blk], pl->stream));
         // Wait for the previous block while this one runs, so the device
         // always has a block queued.
@@ -733,6 +743,14 @@ void wv::run_begin(wv_plan* pl, const wv_tap* taps, size_t n_taps, const double*
                    const uint64_t* receivers, size_t n_recv) {
     if (pl->prec == WV_FP32) run_begin_impl<float>(pl, taps, n_taps, signal, steps, receivers, n_recv);
     else run_begin_impl<double>(pl, taps, n_taps, signal, steps, receivers, n_recv);
+}
+
+uint64_t wv::enqueue_steps(wv_plan* pl, size_t n, size_t stop, hipStream_t stream) {
+    switch (pl->prec) {
+        case WV_FP32: return enqueue_range<float, float>(pl, n, stop, stream);
+        case WV_FP64: return enqueue_range<double, double>(pl, n, stop, stream);
+        default: return enqueue_range<float, double>(pl, n, stop, stream);
+    }
 }
 
 void wv::run_finish(wv_plan* pl, const double* freqs, size_t nf, int mode, double* spectra_out, double* ir_out) {

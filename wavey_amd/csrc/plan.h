@@ -190,6 +190,11 @@ void plan_setup(wv_plan* pl, const wv_grid* g);
 // cell records, zeroed fields and state. Throws DevErr.
 void run_begin(wv_plan* pl, const wv_tap* taps, size_t n_taps, const double* signal, size_t steps,
                const uint64_t* receivers, size_t n_recv);
+// Enqueues steps [n, stop) of the plan's current run on `stream` (two-step
+// passes where the run has them) without waiting; the plan's buf_p / buf_q
+// then name the buffers of p^stop and p^(stop-1). Returns the launches.
+// Throws DevErr.
+uint64_t enqueue_steps(wv_plan* pl, size_t n, size_t stop, hipStream_t stream);
 // Spectra (FP64 DFT of the record) and/or the receiver series of the run's
 // completed steps into caller buffers. Throws DevErr.
 void run_finish(wv_plan* pl, const double* freqs, size_t nf, int mode, double* spectra_out, double* ir_out);

@@ -32,6 +32,7 @@ WV_OK, WV_CANCELLED = 0, 1
 MODE_SPECTRA, MODE_IR, MODE_BOTH = 1, 2, 3
 FP32, FP64, FP32_STATE64 = 0, 1, 2
 PRECISIONS = {"fp32": FP32, "fp64": FP64, "fp32_state64": FP32_STATE64}
+SERVE_ALL, SERVE_SHORTEST, SERVE_LANES = 0, 1, 2
 COURANT = 0.5773502691896257
 AIR = 1 << 6
 
@@ -115,7 +116,7 @@ EXPORTS = [
     "wv_source_components", "wv_component_weights", "wv_to_transfer", "wv_admittance_eval", "wv_paris",
     "wv_fit_real_impedance", "wv_version", "wv_footprint_estimate", "wv_batch_create", "wv_batch_add",
     "wv_batch_autotune", "wv_batch_set_tile", "wv_batch_run", "wv_batch_result", "wv_batch_info",
-    "wv_batch_timing", "wv_batch_free", "wv_serve_create", "wv_serve_submit", "wv_serve_wait", "wv_serve_stats",
+    "wv_batch_timing", "wv_batch_free", "wv_serve_create", "wv_serve_submit", "wv_serve_wait", "wv_serve_stats", "wv_serve_set_policy",
     "wv_serve_free", "wv_wave_job", "wv_grid_build_room", "wv_solid_from_mesh", "wv_grid_spacing",
     "wv_baffle_point", "wv_blend", "wv_calibrate_cabinet", "wv_material_absorption", "wv_fit_admittance",
     "wv_first_order_directions", "wv_fit_first_order", "wv_accuracy_headroom", "wv_accuracy_cabinet_cells",
@@ -229,6 +230,8 @@ def lib():
             L.wv_serve_submit.restype = st
             L.wv_serve_wait.argtypes = [C.c_void_p, C.c_uint64, dp, dp, dp] + errbuf
             L.wv_serve_wait.restype = st
+            L.wv_serve_set_policy.argtypes = [C.c_void_p, C.c_int, C.c_uint64, C.c_uint64]
+            L.wv_serve_set_policy.restype = st
             L.wv_serve_stats.argtypes = [C.c_void_p, dp]
             L.wv_serve_stats.restype = st
             L.wv_serve_free.argtypes = [C.c_void_p]
@@ -805,9 +808,9 @@ class Batch:
 
     def timing(self):
         ms = C.c_double()
-        out = (C.c_uint64 * 5)()
+        out = (C.c_uint64 * 6)()
         lib().wv_batch_timing(self._h, C.byref(ms), out)
-        keys = ["steps", "cell_updates", "graph_launches", "graph_captures", "variant"]
+        keys = ["steps", "cell_updates", "graph_launches", "graph_captures", "variant", "solo_blocks"]
         d = dict(zip(keys, [int(v) for v in out]))
         d["device_ms"] = ms.value
         return d
@@ -819,12 +822,23 @@ class Server:
     Grids passed to ``submit`` must stay alive until ``wait`` returns; this
     class keeps a reference."""
 
-    def __init__(self, precision="fp32", hbm_cap_bytes=288 * 10**9):
+    def __init__(self, precision="fp32", hbm_cap_bytes=288 * 10**9, policy="lanes", fill_cells=0, lane_cells=0):
         self.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         self._h = C.c_void_p()
         err = C.create_string_buffer(1024)
         _check(lib().wv_serve_create(self.precision, int(hbm_cap_bytes), C.byref(self._h), err, 1024), err)
         self._req = {}
+        self.set_policy(policy, fill_cells, lane_cells)
+
+    def set_policy(self, policy="lanes", fill_cells=0, lane_cells=0):
+        """Which admitted requests each 128-step block advances: "all";
+        "shortest" remaining work first up to fill_cells air cells; "lanes":
+        as "shortest" below lane_cells air cells, the larger requests one at
+        a time in admission order on a second stream (0 = the library's
+        defaults)."""
+        code = {"all": SERVE_ALL, "shortest": SERVE_SHORTEST, "lanes": SERVE_LANES}[policy]
+        if lib().wv_serve_set_policy(self._h, code, int(fill_cells), int(lane_cells)) != 0:
+            raise FdtdError(-1, "wv_serve_set_policy: invalid argument")
 
     def close(self):
         if self._h:
@@ -874,10 +888,10 @@ class Server:
         return ir, sp, lat.value
 
     def stats(self):
-        out = (C.c_double * 10)()
+        out = (C.c_double * 13)()
         lib().wv_serve_stats(self._h, out)
         keys = ["completed", "p50_s", "p95_s", "p99_s", "span_s", "cell_updates", "max_running",
-                "peak_device_bytes", "failed", "graph_captures"]
+                "peak_device_bytes", "failed", "graph_captures", "solo_blocks", "graph_launches", "lane_blocks"]
         return dict(zip(keys, [float(v) for v in out]))
 
 
