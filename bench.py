@@ -1,10 +1,12 @@
 """Benchmark of the wave-solver step on one MI355X.
 
-A step is one time step of the FDTD run over the whole grid. Grids of 4M
-cells and more (the hall) take two steps per pass: k_pair1 (deep cells twice
-in one z-march, special cells' first step, receivers) and k_pair2 (special
-and ring cells' second step, receivers), so per-step figures are half a
-pair; smaller grids take one launch of k_step per step.
+A step is one time step of the FDTD run over the whole grid. Grids of 64 Mi
+cells and more (the hall) take three steps per pass: k_tri1 (cells three or
+more from a wall three times in one z-march, special cells' first step,
+receivers) and two launches of k_pair2 (special cells and the cells near the
+walls, second and third steps, receivers), so per-step figures are a third
+of a pass; grids of 4 Mi cells and more take two steps per pass (k_pair1 and
+k_pair2), smaller grids one launch of k_step per step.
 Workload (default): the 40 x 25 x 15 m hall at dx = 0.04 m (1002 x 627 x 377
 cells, 232.5M air cells, 8 pillars, 10 three-section admittances on surface
 patches, 256 receivers), FP32 fields, one cardioid source component. Inputs
@@ -97,13 +99,16 @@ def _counter_kib(db_dir, counter, kernel):
     return sum(vals) / len(vals), len(vals)
 
 
-def measure_traffic(args, copy_bytes, kernels, steps_per_launch_set):
+STEP_KERNELS = ["k_tri1", "k_pair1", "k_pair2", "k_step"]
+TRAFFIC_STEPS = 12  # a whole number of passes of one, two or three steps
+
+
+def measure_traffic(args, copy_bytes):
     """HBM bytes per step from two rocprofv3 counter passes (FETCH_SIZE, then
-    WRITE_SIZE) over a short run of this workload: the per-dispatch means of
-    the step kernels (one kernel per step, or the two kernels of a two-step
-    pass), summed and divided by the steps they advance. Each counter is
-    scaled by what it reads for the plain copy kernel of the same run, which
-    moves copy_bytes in each direction."""
+    WRITE_SIZE) over a 12-step run of this workload: the counter summed over
+    every dispatch of the step kernels, divided by the 12 steps. Each counter
+    is scaled by what it reads for the plain copy kernel of the same run,
+    which moves copy_bytes in each direction."""
     exe = shutil.which("rocprofv3")
     if exe is None:
         return None, "rocprofv3 not found"
@@ -111,7 +116,7 @@ def measure_traffic(args, copy_bytes, kernels, steps_per_launch_set):
     for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
         d = tempfile.mkdtemp(prefix=f"wv_{ctr.lower()}_", dir="/tmp")
         cmd = [exe, "--pmc", ctr, "-d", d, "-o", "run", "--", sys.executable, os.path.abspath(__file__),
-               "--no-cpu", "--no-traffic", "--steps", "10", "--warmup", "2", "--workload", args.workload,
+               "--no-cpu", "--no-traffic", "--steps", str(TRAFFIC_STEPS), "--warmup", "0", "--workload", args.workload,
                "--precision", args.precision, "--zchunk", str(args.zchunk), "--variant", str(args.variant),
                "--steps-per-pass", str(args.steps_per_pass), "--pair-zchunk", str(args.pair_zchunk)]
         print(f"bench: counter pass {ctr}", file=sys.stderr, flush=True)
@@ -122,11 +127,16 @@ def measure_traffic(args, copy_bytes, kernels, steps_per_launch_set):
             if r.returncode != 0:
                 return None, f"rocprofv3 {ctr} pass exited with {r.returncode}"
             step_kib, n = 0.0, 0
-            for k in kernels:
-                kib, cnt = _counter_kib(d, ctr, k)
-                step_kib += kib
+            for k in STEP_KERNELS:
+                try:
+                    kib, cnt = _counter_kib(d, ctr, k)
+                except RuntimeError:
+                    continue  # not launched in this mode
+                step_kib += kib * cnt
                 n += cnt
-            step_kib /= steps_per_launch_set
+            if n == 0:
+                raise RuntimeError(f"{ctr} not recorded for any step kernel")
+            step_kib /= TRAFFIC_STEPS
             copy_kib, _ = _counter_kib(d, ctr, "k_copy")
         except (subprocess.TimeoutExpired, RuntimeError, sqlite3.Error) as e:
             return None, f"{ctr} pass: {e}"
@@ -452,7 +462,7 @@ def main():
     t_setup = time.perf_counter() - t0
     st = plan.stats()
     pinfo = plan.pass_info()
-    pairs = pinfo["steps_per_pass"] == 2
+    spp = pinfo["steps_per_pass"]
     if args.warmup:
         plan.advance(args.warmup)
     t0 = time.perf_counter()
@@ -489,8 +499,9 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": ("k_pair1 + k_pair2 (two steps per pass, z-chunk %d)" % pinfo["pair_zchunk"])
-                     if pairs else "k_step",
+                     "kernel": {3: "k_tri1 + 2 x k_pair2 (three steps per pass, z-chunk %d)" % pinfo["pair_zchunk"],
+                                2: "k_pair1 + k_pair2 (two steps per pass, z-chunk %d)" % pinfo["pair_zchunk"],
+                                1: "k_step"}[spp],
                      "per": "step", "bytes_per_step": int(nbytes),
                      "step_us": round(step_s * 1e6, 2),
                      "copy_GBs": round(2 * copy_bytes / (copy_ms * 1e-3) / 1e9, 1), "copy_bytes": 2 * copy_bytes},
@@ -498,8 +509,7 @@ def main():
                     "upload_and_layout": round(t_setup, 2)},
     }
     if not args.no_traffic:
-        traffic, detail = measure_traffic(args, copy_bytes, ["k_pair1", "k_pair2"] if pairs else ["k_step"],
-                                          2 if pairs else 1)
+        traffic, detail = measure_traffic(args, copy_bytes)
         line["roofline"]["traffic"] = traffic
         line["roofline"]["traffic_detail"] = detail
     if not args.no_cpu:

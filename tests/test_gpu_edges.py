@@ -1,13 +1,14 @@
 """GPU: edge cases of the run against the FP64 oracle and between pass modes.
 
-* no receivers; runs of 1, 2 and 3 steps in both pass modes;
+* no receivers; runs of 1, 2, 3 and 130 steps with one, two and three steps
+  per pass;
 * cancellation between step blocks and the progress callback
   (gpu_fdtd.rs:202-205, 259);
 * a tap on a solid cell and the same cell given twice (the reference injects
   into any cell and adds duplicate taps, fdtd.rs:718-720, gpu_fdtd.rs:67-75),
   with a receiver on the solid cell;
-* the hall at its full size: two steps per pass bit-identical to single
-  steps over 40 steps, every sample finite.
+* the hall at its full size: three (its default) and two steps per pass
+  bit-identical to single steps over 40 steps, every sample finite.
 """
 import numpy as np
 import pytest
@@ -41,7 +42,7 @@ def test_no_receivers(shoebox):
 @pytest.mark.parametrize("steps", [1, 2, 3, 130])
 def test_short_runs_both_pass_modes(shoebox, steps):
     case, og = shoebox
-    print(f"case: shoebox, {steps} steps, fp64, one and two steps per pass", flush=True)
+    print(f"case: shoebox, {steps} steps, fp64, one, two and three steps per pass", flush=True)
     sig = case.signal[:steps]
     ref = oracle.run(og, case.runs[0][2], sig, case.receivers, threads=4)[0]
     with fdtd.Plan(case.grid, "fp64") as plan:
@@ -49,7 +50,9 @@ def test_short_runs_both_pass_modes(shoebox, steps):
         one, _ = plan.run(case.runs[0][2], sig, case.receivers, mode=fdtd.MODE_IR)
         plan.set_pass(2, 8)
         two, _ = plan.run(case.runs[0][2], sig, case.receivers, mode=fdtd.MODE_IR)
-    assert np.array_equal(one, two)
+        plan.set_pass(3, 8)
+        three, _ = plan.run(case.runs[0][2], sig, case.receivers, mode=fdtd.MODE_IR)
+    assert np.array_equal(one, two) and np.array_equal(one, three)
     if np.linalg.norm(ref) > 0:
         assert rel_l2(one, ref) < 1e-9
     else:
@@ -82,7 +85,7 @@ def test_progress_reaches_one(shoebox):
 
 
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-9), ("fp32", 1e-4)])
-@pytest.mark.parametrize("spp", [1, 2])
+@pytest.mark.parametrize("spp", [1, 2, 3])
 def test_solid_and_duplicate_taps(shoebox, prec, tol, spp):
     case, og = shoebox
     g = case.grid
@@ -103,13 +106,15 @@ def test_solid_and_duplicate_taps(shoebox, prec, tol, spp):
 
 def test_hall_full_size_pass_modes_agree():
     cfg = configs.hall()
-    print("case: hall 1002x627x377, 40 steps, fp32, one and two steps per pass", flush=True)
+    print("case: hall 1002x627x377, 40 steps, fp32, one, two and three steps per pass", flush=True)
     case = workload.build_case(cfg, steps=40)
     taps = case.runs[0][2]
     with fdtd.Plan(case.grid, "fp32") as plan:
-        assert plan.pass_info()["steps_per_pass"] == 2
+        assert plan.pass_info()["steps_per_pass"] == 3
+        three, _ = plan.run(taps, case.signal, case.receivers, mode=fdtd.MODE_IR)
+        plan.set_pass(2)
         two, _ = plan.run(taps, case.signal, case.receivers, mode=fdtd.MODE_IR)
         plan.set_pass(1)
         one, _ = plan.run(taps, case.signal, case.receivers, mode=fdtd.MODE_IR)
     assert np.all(np.isfinite(one))
-    assert np.array_equal(one, two)
+    assert np.array_equal(one, two) and np.array_equal(one, three)

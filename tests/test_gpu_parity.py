@@ -127,6 +127,57 @@ def test_two_step_passes_bit_identical(pillared_mini, prec):
     assert rel_l2(one, refs[2][:, :1499]) < TOL[prec]
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "fp32_state64"])
+def test_three_step_passes_bit_identical(pillared_mini, prec):
+    # Three steps per pass (cells three or more from a non-interior cell in
+    # one z-march, the rest in two sparse launches) compute every cell with
+    # the single-step arithmetic: identical receiver series for every pass
+    # z-chunk, with 128-step blocks that end in a two-step pass (1499 steps:
+    # 42 triples and a pair per block, a last block of 30 triples and a single
+    # step) and with a last block of one triple (1283 steps).
+    case, refs = pillared_mini
+    _, ci, taps = case.runs[2]
+    print(f"case: pillared room mini, three-step passes, {prec}", flush=True)
+    with fdtd.Plan(case.grid, prec) as plan:
+        plan.set_pass(1)
+        one, _ = plan.run(taps, case.signal[:1499], case.receivers, mode=fdtd.MODE_IR)
+        for zc in (4, 8, 16, 32):
+            plan.set_pass(3, zc)
+            three, _ = plan.run(taps, case.signal[:1499], case.receivers, mode=fdtd.MODE_IR)
+            assert np.array_equal(one, three), f"z-chunk {zc}"
+        info = plan.pass_info()
+        short, _ = plan.run(taps, case.signal[:1283], case.receivers, mode=fdtd.MODE_IR)
+    assert np.array_equal(short, one[:, :1283])
+    assert info["steps_per_pass"] == 3 and info["ring2_cells"] > info["ring_cells"] > 0
+    assert rel_l2(one, refs[2][:, :1499]) < TOL[prec]
+
+
+@pytest.fixture(scope="module")
+def long_room():
+    """A 14.5 m room at dx 0.05 m: more than one pass-1 tile across (two FP32,
+    three FP64), several row blocks and z-chunks, pillars and patches."""
+    rng = np.random.default_rng(21)
+    lib = configs.material_library(21)
+    cfg = configs.mix_room(rng, lib, dx=0.05, dims=(14.5, 3.4, 2.6), n_rx=16)
+    cfg["steps"] = 700
+    return workload.build_case(cfg)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_multi_step_passes_across_tiles(long_room, prec):
+    c = long_room
+    print(f"case: 14.5 m room, grid {c.grid.n}, one, two and three steps per pass, {prec}", flush=True)
+    out = {}
+    with fdtd.Plan(c.grid, prec) as plan:
+        for spp in (1, 2, 3):
+            plan.set_pass(spp, 8)
+            out[spp], _ = plan.run(c.runs[0][2], c.signal, c.receivers, mode=fdtd.MODE_IR)
+        info = plan.pass_info()
+    assert c.grid.n[0] > 256
+    assert np.array_equal(out[1], out[2]) and np.array_equal(out[1], out[3])
+    assert info["ring2_cells"] > info["ring_cells"] > 0
+
+
 def test_two_step_spectra_and_shoebox(shoebox):
     case, ref, spec = shoebox
     freqs = [31.25, 62.5, 125.0, 250.0]

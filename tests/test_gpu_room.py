@@ -52,7 +52,7 @@ def room_run(request):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-@pytest.mark.parametrize("spp", [1, 2])
+@pytest.mark.parametrize("spp", [1, 2, 3])
 def test_room_ir_matches_oracle(room_run, prec, spp):
     name, g, taps, sig, rx, ref = room_run
     print(f"case: {name} {g.n} {len(sig)} steps {prec} steps/pass {spp}", flush=True)

@@ -260,7 +260,7 @@ void build_table(wv_batch* b) {
         Instance* in = up.get();
         if (in->done || in->paused || in->n0 >= in->steps) continue;
         wv_plan* pl = in->plan.get();
-        const DBuf* fields[4] = {&pl->fa, &pl->fb, &pl->fc, &pl->fd};
+        const DBuf* fields[5] = {&pl->fa, &pl->fb, &pl->fc, &pl->fd, &pl->fe};
         BatchEntry<T, S> e{};
         e.a = make_params<T, S>(pl, ty);
         e.a.p = fields[pl->buf_p]->as<T>();

@@ -115,24 +115,79 @@ __global__ __launch_bounds__(kBlock) void k_zbits(const uint8_t* __restrict__ fl
     dbits[t] = wd;
 }
 
+// Deep2 bits: deep cells whose six neighbours are deep (the third step of a
+// three-step pass needs nothing the block does not compute itself), from the
+// deep bitmap; the z neighbours are the word's bits shifted by one, with the
+// end bits from the words of the next 32-plane blocks.
+__global__ __launch_bounds__(kBlock) void k_d2bits(const uint32_t* __restrict__ dbits, uint32_t* __restrict__ d2bits,
+                                                   uint32_t pitch, uint32_t ny, uint32_t nzb) {
+    const uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    const uint64_t plane = (uint64_t)ny * pitch;
+    if (t >= nzb * plane) return;
+    const uint32_t x = (uint32_t)(t % pitch);
+    const uint64_t r = t / pitch;
+    const uint32_t y = (uint32_t)(r % ny), kb = (uint32_t)(r / ny);
+    const uint32_t w = dbits[t];
+    uint32_t m = w;
+    m &= x > 0 ? dbits[t - 1] : 0u;
+    m &= x + 1 < pitch ? dbits[t + 1] : 0u;
+    m &= y > 0 ? dbits[t - pitch] : 0u;
+    m &= y + 1 < ny ? dbits[t + pitch] : 0u;
+    m &= (w << 1) | (kb > 0 ? dbits[t - plane] >> 31 : 0u);        // plane below
+    m &= (w >> 1) | (kb + 1 < nzb ? dbits[t + plane] << 31 : 0u);  // plane above
+    d2bits[t] = m;
+}
+
+// Receiver cells leave the deep2 set (pitched indices), so a three-step pass
+// stores their p^{n+1}.
+__global__ void k_unset_d2(uint32_t* __restrict__ d2bits, const uint32_t* __restrict__ cells, uint32_t n,
+                           uint32_t pitch, uint32_t ny, uint32_t nz) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t plane = (uint64_t)ny * pitch;
+    const uint32_t z = (uint32_t)(cells[i] / plane);
+    if (z < 1 || z + 1 >= nz) return;
+    const uint32_t rem = (uint32_t)(cells[i] % plane);
+    const uint64_t word = ((uint64_t)((z - 1) >> 5) * ny + rem / pitch) * pitch + rem % pitch;
+    atomicAnd(&d2bits[word], ~(1u << ((z - 1) & 31u)));
+}
+
 // Ring cells (interior, not deep) in pitched order, for the second half of a
 // two-step pass: pass 1 counts them per 256 cells of planes 1 .. nz-2, pass 2
 // writes their pitched indices.
-__device__ __forceinline__ bool ring_at(const uint8_t* __restrict__ flags, const uint32_t* __restrict__ force,
-                                        uint64_t q, const Geo& g, uint64_t end) {
-    if (q >= end) return false;
-    const uint32_t x = (uint32_t)(q % g.pitch);
-    const uint64_t r = q / g.pitch;
-    const uint32_t y = (uint32_t)(r % g.ny), z = (uint32_t)(r / g.ny);
-    return is_interior(flags, force, x, y, z, g) && !is_deep(flags, force, x, y, z, g);
-}
+struct RingAt {
+    const uint8_t* __restrict__ flags;
+    const uint32_t* __restrict__ force;
+    Geo g;
+    __device__ __forceinline__ bool operator()(uint64_t q) const {
+        const uint32_t x = (uint32_t)(q % g.pitch);
+        const uint64_t r = q / g.pitch;
+        const uint32_t y = (uint32_t)(r % g.ny), z = (uint32_t)(r / g.ny);
+        return is_interior(flags, force, x, y, z, g) && !is_deep(flags, force, x, y, z, g);
+    }
+};
 
-__global__ __launch_bounds__(kBlock) void k_ring_count(const uint8_t* __restrict__ flags,
-                                                       const uint32_t* __restrict__ force,
-                                                       uint32_t* __restrict__ counts, Geo g) {
+// Interior cells that are not deep2 (the cells of a three-step pass's third
+// launch besides the special ones), from the bitmaps.
+struct Ring2At {
+    const uint32_t* __restrict__ zbits;
+    const uint32_t* __restrict__ d2bits;
+    uint32_t pitch, ny;
+    __device__ __forceinline__ bool operator()(uint64_t q) const {
+        const uint64_t plane = (uint64_t)ny * pitch;
+        const uint32_t z = (uint32_t)(q / plane);  // 1 .. nz-2
+        const uint64_t rem = q % plane;
+        const uint64_t word = (uint64_t)((z - 1) >> 5) * plane + rem;
+        const uint32_t bit = 1u << ((z - 1) & 31u);
+        return (zbits[word] & bit) && !(d2bits[word] & bit);
+    }
+};
+
+template <class At>
+__global__ __launch_bounds__(kBlock) void k_ring_count(At at, uint32_t* __restrict__ counts, Geo g) {
     __shared__ uint32_t wc[kBlock / 64];
     const uint64_t q = (uint64_t)g.pitch * g.ny + (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    const bool rg = ring_at(flags, force, q, g, (uint64_t)g.pitch * g.ny * (g.nz - 1));
+    const bool rg = q < (uint64_t)g.pitch * g.ny * (g.nz - 1) && at(q);
     const uint64_t b = __ballot(rg);
     if ((threadIdx.x & 63u) == 0) wc[threadIdx.x >> 6] = (uint32_t)__popcll(b);
     __syncthreads();
@@ -143,13 +198,12 @@ __global__ __launch_bounds__(kBlock) void k_ring_count(const uint8_t* __restrict
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_ring_write(const uint8_t* __restrict__ flags,
-                                                       const uint32_t* __restrict__ force,
-                                                       const uint32_t* __restrict__ offs, Geo g,
+template <class At>
+__global__ __launch_bounds__(kBlock) void k_ring_write(At at, const uint32_t* __restrict__ offs, Geo g,
                                                        uint32_t* __restrict__ ring_cell) {
     __shared__ uint32_t wc[kBlock / 64];
     const uint64_t q = (uint64_t)g.pitch * g.ny + (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    const bool rg = ring_at(flags, force, q, g, (uint64_t)g.pitch * g.ny * (g.nz - 1));
+    const bool rg = q < (uint64_t)g.pitch * g.ny * (g.nz - 1) && at(q);
     const uint64_t b = __ballot(rg);
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     if (lane == 0) wc[wave] = (uint32_t)__popcll(b);
@@ -472,6 +526,31 @@ void fill_async(wv_plan* pl, DBuf& d, size_t n) {
     WV_HIP(hipGetLastError());
 }
 
+// Compacts the cells of planes 1 .. nz-2 that `at` selects into `cells`, in
+// pitched order (counts and offsets per 256 cells in `counts`, `offs`);
+// returns their number.
+template <class At>
+uint32_t compact_cells(wv_plan* pl, const At& at, const Geo& geo, DBuf& counts, DBuf& offs, DBuf& cells) {
+    const uint64_t span = (uint64_t)pl->sz * (pl->nz - 2);
+    const uint64_t rblocks = (span + kBlock - 1) / kBlock;
+    if (rblocks >= (1ull << 31)) throw DevErr{"grid too large for the ring layout", WV_ERR_LIMIT};
+    counts.alloc(rblocks * 4);
+    offs.alloc((rblocks + 1) * 4);
+    hipLaunchKernelGGL((k_ring_count<At>), dim3((uint32_t)rblocks), dim3(kBlock), 0, pl->stream, at,
+                       counts.as<uint32_t>(), geo);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, pl->stream, counts.as<uint32_t>(), offs.as<uint32_t>(),
+                       (uint32_t)rblocks, offs.as<uint32_t>() + rblocks);
+    WV_HIP(hipGetLastError());
+    uint32_t n = 0;
+    WV_HIP(hipMemcpyAsync(&n, offs.as<uint32_t>() + rblocks, 4, hipMemcpyDeviceToHost, pl->stream));
+    WV_HIP(hipStreamSynchronize(pl->stream));
+    cells.alloc((size_t)n * 4);
+    hipLaunchKernelGGL((k_ring_write<At>), dim3((uint32_t)rblocks), dim3(kBlock), 0, pl->stream, at,
+                       offs.as<uint32_t>(), geo, cells.as<uint32_t>());
+    WV_HIP(hipGetLastError());
+    return n;
+}
+
 template <class S>
 void run_begin_impl(wv_plan* pl, const wv_tap* taps, size_t n_taps, const double* signal, size_t steps,
                     const uint64_t* receivers, size_t n_recv) {
@@ -530,31 +609,17 @@ void run_begin_impl(wv_plan* pl, const wv_tap* taps, size_t n_taps, const double
                        pl->set_off.as<uint32_t>(), pl->tap_cell.as<uint32_t>(), pl->tap_w.as<S>(), (uint32_t)tc.size(),
                        pl->sp_rec.as<uint2>(), pl->sp_tapw.as<S>(), pl->sp_cell.as<uint32_t>());
     WV_HIP(hipGetLastError());
-    // Two-step passes: the ring list and the two more field buffers.
+    // Multi-step passes: the ring list and the two more field buffers (a
+    // third for three steps).
     pl->run_steps_per_pass = pl->steps_per_pass ? pl->steps_per_pass : default_steps_per_pass(pl->cells);
-    pl->n_ring = 0;
-    if (pl->run_steps_per_pass == 2) {
-        const uint64_t span = (uint64_t)pl->sz * (pl->nz - 2);
-        const uint64_t rblocks = (span + kBlock - 1) / kBlock;
-        if (rblocks >= (1ull << 31)) throw DevErr{"grid too large for the ring layout", WV_ERR_LIMIT};
-        pl->ring_counts.alloc(rblocks * 4);
-        pl->ring_offs.alloc((rblocks + 1) * 4);
-        hipLaunchKernelGGL(k_ring_count, dim3((uint32_t)rblocks), dim3(kBlock), 0, pl->stream, pl->flags.as<uint8_t>(),
-                           pl->force.as<uint32_t>(), pl->ring_counts.as<uint32_t>(), geo);
-        hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, pl->stream, pl->ring_counts.as<uint32_t>(),
-                           pl->ring_offs.as<uint32_t>(), (uint32_t)rblocks, pl->ring_offs.as<uint32_t>() + rblocks);
-        WV_HIP(hipGetLastError());
-        uint32_t n_ring = 0;
-        WV_HIP(hipMemcpyAsync(&n_ring, pl->ring_offs.as<uint32_t>() + rblocks, 4, hipMemcpyDeviceToHost, pl->stream));
-        WV_HIP(hipStreamSynchronize(pl->stream));
-        pl->n_ring = n_ring;
-        pl->ring_cell.alloc((size_t)n_ring * 4);
-        hipLaunchKernelGGL(k_ring_write, dim3((uint32_t)rblocks), dim3(kBlock), 0, pl->stream, pl->flags.as<uint8_t>(),
-                           pl->force.as<uint32_t>(), pl->ring_offs.as<uint32_t>(), geo, pl->ring_cell.as<uint32_t>());
-        WV_HIP(hipGetLastError());
+    pl->n_ring = pl->n_ring2 = 0;
+    if (pl->run_steps_per_pass >= 2) {
+        pl->n_ring = compact_cells(pl, RingAt{pl->flags.as<uint8_t>(), pl->force.as<uint32_t>(), geo}, geo,
+                                   pl->ring_counts, pl->ring_offs, pl->ring_cell);
         pl->fc.alloc(pl->npitched * pl->tsize());
         pl->fd.alloc(pl->npitched * pl->tsize());
     }
+    if (pl->run_steps_per_pass == 3) pl->fe.alloc(pl->npitched * pl->tsize());
     pl->buf_p = 0;
     pl->buf_q = 1;
     // Slot state, slot-major: state[j][rank].
@@ -568,18 +633,31 @@ void run_begin_impl(wv_plan* pl, const wv_tap* taps, size_t n_taps, const double
         rc[r] = (uint32_t)((c / pl->nx) * pl->pitch + c % pl->nx);
     }
     upload(pl, pl->recv, rc);
+    // Three-step passes: the deep2 bitmap without the receiver cells, and the
+    // interior cells that are not deep2.
+    if (pl->run_steps_per_pass == 3) {
+        const uint64_t nwords = (uint64_t)pl->nzb * pl->ny * pl->pitch;
+        pl->d2bits.alloc(nwords * 4);
+        hipLaunchKernelGGL(k_d2bits, dim3((uint32_t)div_up(nwords, kBlock)), dim3(kBlock), 0, pl->stream,
+                           pl->dbits.as<uint32_t>(), pl->d2bits.as<uint32_t>(), pl->pitch, pl->ny, pl->nzb);
+        if (n_recv)
+            hipLaunchKernelGGL(k_unset_d2, dim3(div_up(n_recv, 256)), dim3(256), 0, pl->stream,
+                               pl->d2bits.as<uint32_t>(), pl->recv.as<uint32_t>(), (uint32_t)n_recv, pl->pitch,
+                               pl->ny, pl->nz);
+        WV_HIP(hipGetLastError());
+        pl->n_ring2 = compact_cells(pl, Ring2At{pl->zbits.as<uint32_t>(), pl->d2bits.as<uint32_t>(), pl->pitch, pl->ny},
+                                    geo, pl->ring2_counts, pl->ring2_offs, pl->ring2_cell);
+    }
     pl->ir.alloc(std::max<size_t>(1, steps * n_recv) * pl->tsize());
     // Zeroed fields, state and records.
-    const bool four = pl->run_steps_per_pass == 2;
-    if (pl->prec == WV_FP64) {
-        for (DBuf* f : {&pl->fa, &pl->fb, &pl->fc, &pl->fd})
-            if (f == &pl->fa || f == &pl->fb || four) fill_async<double>(pl, *f, pl->npitched);
-        fill_async<double>(pl, pl->ir, steps * n_recv);
-    } else {
-        for (DBuf* f : {&pl->fa, &pl->fb, &pl->fc, &pl->fd})
-            if (f == &pl->fa || f == &pl->fb || four) fill_async<float>(pl, *f, pl->npitched);
-        fill_async<float>(pl, pl->ir, steps * n_recv);
+    const size_t nfields = pl->run_steps_per_pass == 3 ? 5 : pl->run_steps_per_pass == 2 ? 4 : 2;
+    DBuf* fields[5] = {&pl->fa, &pl->fb, &pl->fc, &pl->fd, &pl->fe};
+    for (size_t f = 0; f < nfields; ++f) {
+        if (pl->prec == WV_FP64) fill_async<double>(pl, *fields[f], pl->npitched);
+        else fill_async<float>(pl, *fields[f], pl->npitched);
     }
+    if (pl->prec == WV_FP64) fill_async<double>(pl, pl->ir, steps * n_recv);
+    else fill_async<float>(pl, pl->ir, steps * n_recv);
     fill_async<S>(pl, pl->st_u, nstate);
     fill_async<S>(pl, pl->st_g, nstate);
     WV_HIP(hipStreamSynchronize(pl->stream));
@@ -613,37 +691,102 @@ PairParams<T, S> make_pair_params(wv_plan* pl, const StepParams<T, S>& base) {
     return pp;
 }
 
-// Enqueues steps [n, stop) of the plan's run on `stream`: pairs of steps as
-// two-step passes where the run has them, single steps otherwise (and for an
-// odd last step). The plan's buf_p / buf_q follow the field buffers. Returns
-// the number of launches.
+// Parameters of a three-step pass's first launch (tiles of the bulk, the
+// special cells' step n, the receivers' p^n).
+template <class T, class S>
+TriParams<T, S> make_tri_params(wv_plan* pl, const StepParams<T, S>& base) {
+    TriParams<T, S> tp{};
+    tp.a = base;
+    const uint32_t zch = pl->pair_zchunk ? pl->pair_zchunk : default_pair_zchunk(pl->cells);
+    tp.a.zchunk = zch;
+    tp.dbits = pl->dbits.as<uint32_t>();
+    tp.d2bits = pl->d2bits.as<uint32_t>();
+    tp.sp_cell = pl->sp_cell.as<uint32_t>();
+    tp.n_special = pl->n_special;
+    // Tiles own columns 1 + k * stride .. (k + 1) * stride: every column an
+    // interior cell can be in (1 .. nx-2).
+    tp.tiles_x = div_up(pl->nx - 2, kTriStride<T>);
+    tp.row_blocks = div_up(pl->ny, kTriRows);
+    tp.tile_blocks = tp.tiles_x * tp.row_blocks * div_up(pl->nz - 2, zch);
+    tp.tile_launch = div_up(tp.tile_blocks / tp.tiles_x, 8) * 8 * tp.tiles_x;
+    tp.special_blocks = div_up(pl->n_special, kPairThreads);
+    return tp;
+}
+
+// Enqueues steps [n, stop) of the plan's run on `stream`: three or two steps
+// per pass where the run has them (a remainder of two steps as a two-step
+// pass), single steps otherwise and for a last odd step. The plan's buf_p /
+// buf_q follow the field buffers. Returns the number of launches.
 template <class T, class S>
 uint64_t enqueue_range(wv_plan* pl, size_t n, size_t stop, hipStream_t stream) {
     StepParams<T, S> a = make_params<T, S>(pl, kVariantTiles[pl->variant]);
     const uint32_t grid = a.main_blocks + div_up(pl->n_recv, variant_threads(pl->variant));
-    const bool pairs = pl->run_steps_per_pass == 2;
+    const bool triples = pl->run_steps_per_pass == 3;
+    const bool pairs = pl->run_steps_per_pass >= 2;
     PairParams<T, S> pp{};
-    uint32_t grid1 = 0, items2 = 0, grid2 = 0;
+    TriParams<T, S> tp{};
+    uint32_t grid1 = 0, items2 = 0, grid2 = 0, gridt = 0, items3 = 0, grid3 = 0;
     if (pairs) {
         pp = make_pair_params<T, S>(pl, a);
         grid1 = pp.tile_launch + pp.special_blocks + div_up(pl->n_recv, kPairThreads);
         items2 = div_up(pl->n_special, kPair2Block) + div_up(pl->n_ring, kPair2Block);
         grid2 = items2 + div_up(pl->n_recv, kPair2Block);
     }
-    T* bufs[4] = {pl->fa.as<T>(), pl->fb.as<T>(), pl->fc.as<T>(), pl->fd.as<T>()};
+    if (triples) {
+        tp = make_tri_params<T, S>(pl, a);
+        gridt = tp.tile_launch + tp.special_blocks + div_up(pl->n_recv, kPairThreads);
+        items3 = div_up(pl->n_special, kPair2Block) + div_up(pl->n_ring2, kPair2Block);
+        grid3 = items3 + div_up(pl->n_recv, kPair2Block);
+    }
+    T* bufs[5] = {pl->fa.as<T>(), pl->fb.as<T>(), pl->fc.as<T>(), pl->fd.as<T>(), pl->fe.as<T>()};
     auto sig = [&](size_t m) -> S { return (S)(sizeof(S) == 4 ? (double)(float)pl->signal[m] : pl->signal[m]); };
+    // The k-th field buffer that holds neither p nor prev.
+    auto free_buf = [&](int k) -> int {
+        for (int b = 0;; ++b)
+            if (b != pl->buf_p && b != pl->buf_q && k-- == 0) return b;
+    };
     T* ir = pl->ir.as<T>();
     uint64_t launches = 0;
     while (n < stop) {
         T* P = bufs[pl->buf_p];
         T* Q = bufs[pl->buf_q];
-        if (pairs && n + 2 <= stop) {
-            // Steps n, n+1 in two launches; outputs go to the two free
-            // buffers, which then hold p (p^{n+2}) and prev (p^{n+1}).
-            int o1 = 0;
-            while (o1 == pl->buf_p || o1 == pl->buf_q) ++o1;
-            int o2 = o1 + 1;
-            while (o2 == pl->buf_p || o2 == pl->buf_q) ++o2;
+        if (triples && n + 3 <= stop) {
+            // Steps n, n+1, n+2 in three launches; outputs go to three free
+            // buffers, which then hold p^{n+1} (read by passes 2 and 3 only),
+            // p^{n+2} (prev) and p^{n+3} (p).
+            const int o1 = free_buf(0), o2 = free_buf(1), o3 = free_buf(2);
+            tp.a.p = P;
+            tp.a.prev = Q;
+            tp.a.s_n = sig(n);
+            tp.a.ir = ir + n * pl->n_recv;
+            tp.p1 = bufs[o1];
+            tp.p2 = bufs[o2];
+            tp.p3 = bufs[o3];
+            hipLaunchKernelGGL((k_tri1<T, S>), dim3(gridt), dim3(kPairThreads), 0, stream, tp);
+            StepParams<T, S> a2 = a;
+            a2.p = bufs[o1];
+            a2.prev = P;
+            a2.s_n = sig(n + 1);
+            a2.ir = ir + (n + 1) * pl->n_recv;
+            hipLaunchKernelGGL((k_pair2<T, S>), dim3(grid2), dim3(kPair2Block), 0, stream, a2, bufs[o2],
+                               pl->sp_cell.as<uint32_t>(), pl->ring_cell.as<uint32_t>(), pl->n_special, pl->n_ring,
+                               items2);
+            StepParams<T, S> a3 = a;
+            a3.p = bufs[o2];
+            a3.prev = bufs[o1];
+            a3.s_n = sig(n + 2);
+            a3.ir = ir + (n + 2) * pl->n_recv;
+            hipLaunchKernelGGL((k_pair2<T, S>), dim3(grid3), dim3(kPair2Block), 0, stream, a3, bufs[o3],
+                               pl->sp_cell.as<uint32_t>(), pl->ring2_cell.as<uint32_t>(), pl->n_special,
+                               pl->n_ring2, items3);
+            pl->buf_p = o3;
+            pl->buf_q = o2;
+            launches += 3;
+            n += 3;
+        } else if (pairs && n + 2 <= stop) {
+            // Steps n, n+1 in two launches; outputs go to two free buffers,
+            // which then hold p (p^{n+2}) and prev (p^{n+1}).
+            const int o1 = free_buf(0), o2 = free_buf(1);
             pp.a.p = P;
             pp.a.prev = Q;
             pp.a.s_n = sig(n);
@@ -787,10 +930,12 @@ wv_status wv::check_run_args(const wv_plan* pl, const wv_tap* taps, size_t n_tap
 
 uint64_t wv::plan_bytes(const wv_plan* pl) {
     uint64_t t = 0;
-    for (const DBuf* d : {&pl->flags, &pl->force_base, &pl->force, &pl->zbits, &pl->counts, &pl->offs,
-                          &pl->b_cell_sorted, &pl->b_set_sorted, &pl->set_n, &pl->set_off, &pl->set_c,
-                          &pl->st_u, &pl->st_g, &pl->fa, &pl->fb, &pl->sp_rec,
-                          &pl->sp_tapw, &pl->tap_cell, &pl->tap_w, &pl->recv, &pl->ir, &pl->freqs, &pl->spec})
+    for (const DBuf* d : {&pl->flags, &pl->force_base, &pl->force, &pl->zbits, &pl->dbits, &pl->d2bits, &pl->counts,
+                          &pl->offs, &pl->b_cell_sorted, &pl->b_set_sorted, &pl->set_n, &pl->set_off, &pl->set_c,
+                          &pl->st_u, &pl->st_g, &pl->fa, &pl->fb, &pl->fc, &pl->fd, &pl->fe, &pl->sp_rec,
+                          &pl->sp_tapw, &pl->sp_cell, &pl->ring_cell, &pl->ring_counts, &pl->ring_offs,
+                          &pl->ring2_cell, &pl->ring2_counts, &pl->ring2_offs, &pl->tap_cell, &pl->tap_w, &pl->recv,
+                          &pl->ir, &pl->freqs, &pl->spec})
         t += d->bytes;
     return t;
 }
@@ -805,11 +950,14 @@ uint64_t wv::estimate_bytes(const wv_grid* g, int prec, size_t steps, size_t n_r
     const uint64_t special = g->n_boundary + n_taps;
     const uint64_t sp_pad = (special + 63) & ~63ull;
     const uint64_t slots = g->n_boundary ? g->b_start[g->n_boundary] : 0;
-    const uint64_t fields = default_steps_per_pass(cells) == 2 ? 4 : 2;
+    const uint32_t spp = default_steps_per_pass(cells);
+    const uint64_t fields = spp == 3 ? 5 : spp == 2 ? 4 : 2;
     uint64_t t = cells;                              // flags
     t += npitched / 4 + npitched / 4;                // boundary marks (two copies), interior and deep bitmaps
+    t += spp == 3 ? npitched / 8 : 0;                // deep2 bitmap
     t += fields * npitched * tsz;                    // fields
-    t += fields == 4 ? 2 * 4 * special : 0;          // special cells' indices and the ring list (about as long)
+    t += spp >= 2 ? 2 * 4 * special : 0;             // special cells' indices and the ring list (about as long)
+    t += spp == 3 ? 2 * 4 * special : 0;             // the cells within two of a wall
     t += npitched / 16 + 1024;                       // compaction counts and offsets (z-chunk >= 2)
     t += 8 * g->n_boundary + slots * (4 * ssz + 8);  // sorted boundary, slot-set table (at most one set per node)
     t += special * (8 + ssz) + 2 * (uint64_t)max_slots * sp_pad * ssz;  // records, tap weights, slot state
@@ -875,8 +1023,8 @@ extern "C" wv_status wv_plan_set_tile(wv_plan* plan, uint32_t zchunk, int32_t va
 }
 
 extern "C" wv_status wv_plan_set_pass(wv_plan* plan, uint32_t steps_per_pass, uint32_t pair_zchunk) {
-    // Pair z-chunks must divide the 32-plane blocks of the bitmaps.
-    if (!plan || steps_per_pass > 2 || pair_zchunk > 32 || (pair_zchunk & (pair_zchunk - 1))) return WV_ERR_INVALID;
+    // Pass z-chunks must divide the 32-plane blocks of the bitmaps.
+    if (!plan || steps_per_pass > 3 || pair_zchunk > 32 || (pair_zchunk & (pair_zchunk - 1))) return WV_ERR_INVALID;
     plan->steps_per_pass = steps_per_pass;
     plan->pair_zchunk = pair_zchunk;
     return WV_OK;
@@ -992,12 +1140,13 @@ extern "C" wv_status wv_plan_stats(const wv_plan* plan, uint64_t out[7]) {
     return WV_OK;
 }
 
-extern "C" wv_status wv_plan_pass_info(const wv_plan* plan, uint32_t out[3]) {
+extern "C" wv_status wv_plan_pass_info(const wv_plan* plan, uint32_t out[4]) {
     if (!plan || !out) return WV_ERR_INVALID;
     const uint32_t spp = plan->steps_per_pass ? plan->steps_per_pass : default_steps_per_pass(plan->cells);
     out[0] = plan->active ? plan->run_steps_per_pass : spp;
     out[1] = plan->pair_zchunk ? plan->pair_zchunk : default_pair_zchunk(plan->cells);
     out[2] = plan->n_ring;
+    out[3] = plan->n_ring2;
     return WV_OK;
 }
 

@@ -225,28 +225,36 @@ __device__ __forceinline__ void pair_tile(const PairParams<T, S>& pp, uint32_t t
     }
 }
 
-// Pass 1: tile blocks, then blocks of special cells (step n), then the
+// The blocks of pass 1 after its tile blocks (block b counts from the first
+// of them): step n of the special cells, p^{n+1} into p1; then the
 // receivers' p^n.
 template <typename T, typename S>
-__global__ __launch_bounds__(kPairThreads) void k_pair1(PairParams<T, S> pp) {
-    const StepParams<T, S>& a = pp.a;
-    if (blockIdx.x < pp.tile_launch) {
-        pair_tile<T, S>(pp, blockIdx.x);
-        return;
-    }
-    const uint32_t b = blockIdx.x - pp.tile_launch;
-    if (b < pp.special_blocks) {
+__device__ __forceinline__ void pass1_specials(const StepParams<T, S>& a, const uint32_t* __restrict__ sp_cell,
+                                               uint32_t n_special, uint32_t special_blocks, T* __restrict__ p1,
+                                               uint32_t b) {
+    if (b < special_blocks) {
         const uint32_t s = b * kPairThreads + threadIdx.x;
-        if (s >= pp.n_special) return;
+        if (s >= n_special) return;
         const uint2 rec = a.sp_rec[s];
         S u0[kSlotGroup<S>];
 #pragma unroll
         for (uint32_t j = 0; j < kSlotGroup<S>; ++j) u0[j] = j < a.max_slots ? a.st_u[j * a.sp_pad + s] : S(0);
-        special_at(a, s, pp.sp_cell[s], rec, u0, pp.p1);
+        special_at(a, s, sp_cell[s], rec, u0, p1);
         return;
     }
-    const uint32_t rr = (b - pp.special_blocks) * kPairThreads + threadIdx.x;
+    const uint32_t rr = (b - special_blocks) * kPairThreads + threadIdx.x;
     if (rr < a.n_recv) a.ir[rr] = a.p[a.recv[rr]];
+}
+
+// Pass 1: tile blocks, then blocks of special cells (step n), then the
+// receivers' p^n.
+template <typename T, typename S>
+__global__ __launch_bounds__(kPairThreads) void k_pair1(PairParams<T, S> pp) {
+    if (blockIdx.x < pp.tile_launch) {
+        pair_tile<T, S>(pp, blockIdx.x);
+        return;
+    }
+    pass1_specials<T, S>(pp.a, pp.sp_cell, pp.n_special, pp.special_blocks, pp.p1, blockIdx.x - pp.tile_launch);
 }
 
 // Pass 2: step n+1 of the special cells and the ring, then the receivers'

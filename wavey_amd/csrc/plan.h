@@ -8,7 +8,7 @@
 #include <string>
 #include <vector>
 
-#include "pair_kernels.h"
+#include "tri_kernels.h"
 #include "step_kernels.h"
 #include "wavey_fdtd.h"
 #include "wv_internal.h"
@@ -75,13 +75,17 @@ struct wv_plan {
     DBuf sp_rec, sp_tapw;
     DBuf tap_cell, tap_w, recv, ir, freqs, spec;
     uint32_t n_special = 0, sp_pad = 0, seg_stride = 0;
-    // Two-step passes: steps per pass (1 or 2; 0 = default), their z-chunk,
-    // the special cells' pitched indices and the ring (interior, not deep).
+    // Multi-step passes: steps per pass (1, 2 or 3; 0 = default), their
+    // z-chunk, the special cells' pitched indices, the ring (interior, not
+    // deep) and, for three steps, the deep2 bitmap and the interior cells
+    // that are not deep2.
     uint32_t steps_per_pass = 0, run_steps_per_pass = 1;
     uint32_t pair_zchunk = 0;
     DBuf sp_cell, ring_cell, ring_counts, ring_offs;
     uint32_t n_ring = 0;
-    // Which field buffer (0..3: fa, fb, fc, fd) holds p and which prev.
+    DBuf fe, d2bits, ring2_cell, ring2_counts, ring2_offs;  // fe: the fifth field buffer
+    uint32_t n_ring2 = 0;
+    // Which field buffer (0..4: fa, fb, fc, fd, fe) holds p and which prev.
     int buf_p = 0, buf_q = 1;
 
     // Current run.

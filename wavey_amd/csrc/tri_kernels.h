@@ -1,0 +1,259 @@
+// Three time steps per pass over the fields (temporal blocking in z): the
+// two-step pass of pair_kernels.h taken one step further.
+//
+// Two steps per pass read p^n and p^{n-1} once and write p^{n+1} and p^{n+2}
+// (16 bytes per FP32 cell per two steps). Three steps read the same two
+// fields and write p^{n+2} and p^{n+3} of the bulk; p^{n+1} is stored only
+// where the later passes read it, near the walls (16 bytes per three steps),
+// with five field buffers so no input is overwritten while a neighbouring
+// block still reads it:
+//
+//  * Pass 1, tile blocks: a block covers 256 FP32 (128 FP64) cells of a row
+//    by 16 rows (a wave per row) and marches a z-chunk. Per plane it computes
+//    p^{n+1} of every row (the lanes at the tile ends also the column next to
+//    the tile), one plane behind p^{n+2} of rows 1..14 from the p^{n+1} rows
+//    shared through LDS, and two planes behind p^{n+3} of rows 2..13 from the
+//    p^{n+2} rows shared through LDS. A block owns rows 2..13 and columns
+//    1 .. 252 of its tile (1 .. 126 FP64), where all three levels are right
+//    for the cells that store them; tiles step 252 (126) cells and 12 rows.
+//    It stores p^{n+1} of the interior cells that are not deep2, p^{n+2} of
+//    the deep cells and p^{n+3} of the deep2 cells (deep cells whose six
+//    neighbours are deep: their third step needs only values the block
+//    computes itself).
+//  * Pass 1, special blocks and receivers: as with two steps per pass.
+//  * Pass 2: step n+1 of the special cells and of the ring (interior, not
+//    deep), as k_pair2.
+//  * Pass 3: step n+2 of the special cells and of the interior cells that are
+//    not deep2, as k_pair2 with that list.
+//
+// Receiver cells are kept out of the deep2 set, so their p^{n+1} is stored
+// for pass 2's record. Every cell's arithmetic is the single-step kernel's,
+// so results are bit-identical to single steps.
+#pragma once
+#include "pair_kernels.h"
+
+namespace wv {
+
+constexpr int kTriRows = kPairWaves - 4;  // rows a pass-1 block owns (12)
+
+// Columns a pass-1 tile owns and steps by: the tile less its two end
+// columns, rounded down to whole lanes (252 FP32, 126 FP64).
+template <typename T>
+constexpr uint32_t kTriStride = (kTileX * kPairLane<T> - 2) / kPairLane<T> * kPairLane<T>;
+
+template <typename T, typename S>
+struct TriParams {
+    StepParams<T, S> a;  // step n: p = p^n, prev = p^{n-1}; ir = row n; s_n = s[n]
+    T* __restrict__ p1;  // p^{n+1}
+    T* __restrict__ p2;  // p^{n+2}
+    T* __restrict__ p3;  // p^{n+3}
+    const uint32_t* __restrict__ dbits;   // deep bits, laid out as the interior bits
+    const uint32_t* __restrict__ d2bits;  // deep2 bits (receiver cells cleared)
+    const uint32_t* __restrict__ sp_cell;
+    uint32_t n_special;
+    uint32_t tiles_x, row_blocks;  // pass-1 tiles: kTriStride columns by kTriRows rows
+    uint32_t tile_blocks;          // tiles_x * row_blocks * z-chunks
+    uint32_t tile_launch;          // tile_blocks / tiles_x rounded up to 8, times tiles_x
+    uint32_t special_blocks;       // pass-1 blocks of kPairThreads special cells
+};
+
+// Stores the cells of c whose bit is set (one access when all are).
+template <typename T, int W>
+__device__ __forceinline__ void store_masked(const Cells<T, W>& c, T* __restrict__ p, uint32_t bits) {
+    if (bits == (1u << W) - 1u) {
+        c.store(p);
+    } else if (bits) {
+#pragma unroll
+        for (int e = 0; e < W; ++e)
+            if ((bits >> e) & 1u) p[e] = c.v[e];
+    }
+}
+
+template <typename T, typename S>
+__device__ __forceinline__ void tri_tile(const TriParams<T, S>& tp, uint32_t t) {
+    const StepParams<T, S>& a = tp.a;
+    constexpr int W = kPairLane<T>;
+    constexpr uint32_t BX = kTileX * W;
+    constexpr uint32_t SX = kTriStride<T>;
+    using C = Cells<T, W>;
+    // p^{n+1} and p^{n+2} rows of the last three planes of each (written at
+    // one plane, read one plane later; one barrier per plane).
+    __shared__ __align__(16) T rows1[3][kPairWaves][BX];
+    __shared__ __align__(16) T rows2[3][kPairWaves][BX];
+    // XCD-ordered blocks, as the two-step pass: one XCD takes all x-tiles of
+    // a (row block, z-chunk) in turn.
+    const uint32_t xcd = t & 7u, q = t >> 3;
+    const uint32_t tx = q % tp.tiles_x;
+    const uint32_t r = (q / tp.tiles_x) * 8u + xcd;
+    if (r >= tp.tile_blocks / tp.tiles_x) return;  // padding of the last group (the whole block)
+    const uint32_t rb = r % tp.row_blocks;
+    const uint32_t zc = r / tp.row_blocks;
+    const uint32_t lane = threadIdx.x & (kTileX - 1);
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kTileX);
+    const int ny = (int)a.ny, nz = (int)a.nz;
+    // Row of this wave: two rows of halo above and below the owned ones.
+    const int y = (int)(rb * kTriRows + wave) - 2;
+    const bool own_row = wave >= 2 && wave < 2u + (uint32_t)kTriRows && y < ny;
+    const uint32_t yc = (uint32_t)min(max(y, 0), ny - 1);
+    const uint32_t ymc = (uint32_t)min(max(y - 1, 0), ny - 1);
+    const uint32_t ypc = (uint32_t)min(max(y + 1, 0), ny - 1);
+    const uint32_t x0 = tx * SX;
+    const uint32_t xl = x0 + lane * W;
+    const bool live = xl < a.pitch;
+    const uint32_t x = live ? xl : a.pitch - W;
+    const uint32_t col = yc * a.pitch + x, colm = ymc * a.pitch + x, colp = ypc * a.pitch + x;
+    // Lanes 0 and 63: the column next to the tile (xh) and the one beyond (xh2).
+    const bool edge = lane == 0 || lane == kTileX - 1;
+    const uint32_t xh = lane == 0 ? (x0 > 0 ? x0 - 1 : 0u) : min(x0 + BX, a.pitch - 1);
+    const uint32_t xh2 = lane == 0 ? (x0 > 1 ? x0 - 2 : 0u) : min(x0 + BX + 1, a.pitch - 1);
+    const uint32_t hcol = yc * a.pitch + xh, hcol2 = yc * a.pitch + xh2;
+    const uint32_t hcolm = ymc * a.pitch + xh, hcolp = ypc * a.pitch + xh;
+    const int z0 = 1 + (int)(zc * a.zchunk);
+    const int z1 = min(z0 + (int)a.zchunk, nz - 1);
+    // Owned cells of this lane: columns x0 + 1 .. x0 + SX of an owned row.
+    uint32_t own = 0;
+#pragma unroll
+    for (int e = 0; e < W; ++e) own |= (uint32_t)(xl + e >= x0 + 1 && xl + e <= x0 + SX) << e;
+    const uint32_t lane_mask = (live && own_row) ? own : 0u;
+    auto plane = [&](int z) -> uint32_t { return (uint32_t)min(max(z, 0), nz - 1) * a.sz; };
+    // The chunk lies inside one 32-plane block of the z-packed bitmaps.
+    uint32_t ib[W], db[W], d2b[W];
+    const uint32_t wb = ((z0 - 1) >> 5) * a.sz + col;
+    load_words<W>(a.zbits + wb, ib);
+    load_words<W>(tp.dbits + wb, db);
+    load_words<W>(tp.d2bits + wb, d2b);
+
+    // A plane's loads, as the two-step pass: p^n of the row at zq + 1 and of
+    // the rows above and below at zq, p^{n-1} of the row at zq, and (edge
+    // lanes) the same around the halo column, issued one plane ahead.
+    struct PlaneLoads {
+        C zp, q, ym, yp;
+        T hp, hx2, hym, hyp, hq;
+    };
+    auto issue = [&](int zq, PlaneLoads& L) {
+        const uint32_t pz = plane(zq), pzp = plane(zq + 1);
+        L.zp.load(a.p + pzp + col);
+        L.q.load(a.prev + pz + col);
+        L.ym.load(a.p + pz + colm);
+        L.yp.load(a.p + pz + colp);
+        L.hp = L.hx2 = L.hym = L.hyp = L.hq = T(0);
+        if (edge) {
+            L.hp = a.p[pzp + hcol];
+            L.hx2 = a.p[pz + hcol2];
+            L.hym = a.p[pz + hcolm];
+            L.hyp = a.p[pz + hcolp];
+            L.hq = a.prev[pz + hcol];
+        }
+    };
+
+    // p^n of the row at planes zq - 1 and zq (and of the halo column).
+    C zm, zcur;
+    zm.load(a.p + plane(z0 - 3) + col);
+    zcur.load(a.p + plane(z0 - 2) + col);
+    T hm = 0, hc = 0;
+    if (edge) {
+        hm = a.p[plane(z0 - 3) + hcol];
+        hc = a.p[plane(z0 - 2) + hcol];
+    }
+    // p^{n+1} of the row at planes zq - 2 and zq - 1, of the halo column at
+    // zq - 1; p^{n+2} of the row at planes zq - 3 and zq - 2.
+    C p1m, p1c, p2m, p2c, p2;
+#pragma unroll
+    for (int e = 0; e < W; ++e) p1m.v[e] = p1c.v[e] = p2m.v[e] = p2c.v[e] = p2.v[e] = T(0);
+    T e1c = 0;
+    uint32_t bw = 0;  // LDS buffer of plane zq
+    PlaneLoads nx;
+    issue(z0 - 2, nx);
+    // p^{n+1} at planes z0 - 2 .. z1 + 1, p^{n+2} at z0 - 1 .. z1, p^{n+3} at z0 .. z1 - 1.
+    for (int zq = z0 - 2; zq <= z1 + 1; ++zq) {
+        const PlaneLoads cur = nx;
+        if (zq <= z1) issue(zq + 1, nx);
+        // p^{n+1} at plane zq: the row, and (edge lanes) the halo column.
+        const T lo = from_lower_lane(zcur.v[W - 1], hc);
+        const T hi = from_upper_lane(zcur.v[0], hc);
+        C p1;
+#pragma unroll
+        for (int e = 0; e < W; ++e) {
+            const T xm = e == 0 ? lo : zcur.v[e - 1];
+            const T xp = e == W - 1 ? hi : zcur.v[e + 1];
+            p1.v[e] = interior_update(zcur.v[e], cur.q.v[e], xm, xp, cur.ym.v[e], cur.yp.v[e], zm.v[e], cur.zp.v[e],
+                                      a.l2, a.c2);
+        }
+        T e1 = 0;
+        if (edge) {
+            const T xin = lane == 0 ? zcur.v[0] : zcur.v[W - 1];
+            const T exm = lane == 0 ? cur.hx2 : xin;
+            const T exp = lane == 0 ? xin : cur.hx2;
+            e1 = interior_update(hc, cur.hq, exm, exp, cur.hym, cur.hyp, hm, cur.hp, a.l2, a.c2);
+        }
+        p1.store(&rows1[bw][wave][lane * W]);
+        if (zq >= z0 && zq < z1) {
+            const uint32_t sh = (uint32_t)(zq - 1) & 31u;
+            const uint32_t bits = plane_bits<W>(ib, sh) & ~plane_bits<W>(d2b, sh) & lane_mask;
+            store_masked<T, W>(p1, tp.p1 + plane(zq) + col, bits);
+        }
+        __syncthreads();
+        const uint32_t b1 = bw == 0 ? 2u : bw - 1;  // LDS buffers of plane zq - 1
+        const uint32_t b2 = b1 == 0 ? 2u : b1 - 1;  // and of plane zq - 2
+        // p^{n+2} at plane zq - 1 of rows 1..14.
+        if (zq >= z0 && wave >= 1 && wave <= (uint32_t)kPairWaves - 2) {
+            C ym1, yp1;
+            ym1.load(&rows1[b1][wave - 1][lane * W]);
+            yp1.load(&rows1[b1][wave + 1][lane * W]);
+            const T lo1 = from_lower_lane(p1c.v[W - 1], e1c);
+            const T hi1 = from_upper_lane(p1c.v[0], e1c);
+#pragma unroll
+            for (int e = 0; e < W; ++e) {
+                const T xm = e == 0 ? lo1 : p1c.v[e - 1];
+                const T xp = e == W - 1 ? hi1 : p1c.v[e + 1];
+                p2.v[e] = interior_update(p1c.v[e], zm.v[e], xm, xp, ym1.v[e], yp1.v[e], p1m.v[e], p1.v[e], a.l2, a.c2);
+            }
+            p2.store(&rows2[b1][wave][lane * W]);
+            const int zo = zq - 1;
+            if (zo >= z0 && zo < z1)
+                store_masked<T, W>(p2, tp.p2 + (uint32_t)zo * a.sz + col,
+                                   plane_bits<W>(db, (uint32_t)(zo - 1) & 31u) & lane_mask);
+        }
+        // p^{n+3} at plane zq - 2 of rows 2..13 (the p^{n+2} rows of plane
+        // zq - 2 were written before this plane's barrier).
+        if (zq >= z0 + 2 && own_row) {
+            C ym2, yp2, p3;
+            ym2.load(&rows2[b2][wave - 1][lane * W]);
+            yp2.load(&rows2[b2][wave + 1][lane * W]);
+            const T lo2 = from_lower_lane(p2c.v[W - 1], T(0));
+            const T hi2 = from_upper_lane(p2c.v[0], T(0));
+#pragma unroll
+            for (int e = 0; e < W; ++e) {
+                const T xm = e == 0 ? lo2 : p2c.v[e - 1];
+                const T xp = e == W - 1 ? hi2 : p2c.v[e + 1];
+                p3.v[e] = interior_update(p2c.v[e], p1m.v[e], xm, xp, ym2.v[e], yp2.v[e], p2m.v[e], p2.v[e], a.l2, a.c2);
+            }
+            const int zo3 = zq - 2;
+            store_masked<T, W>(p3, tp.p3 + (uint32_t)zo3 * a.sz + col,
+                               plane_bits<W>(d2b, (uint32_t)(zo3 - 1) & 31u) & lane_mask);
+        }
+        p1m = p1c;
+        p1c = p1;
+        e1c = e1;
+        p2m = p2c;
+        p2c = p2;
+        zm = zcur;
+        zcur = cur.zp;
+        hm = hc;
+        hc = cur.hp;
+        bw = bw == 2 ? 0u : bw + 1;
+    }
+}
+
+// Pass 1 of three steps: tile blocks, then blocks of special cells (step n),
+// then the receivers' p^n.
+template <typename T, typename S>
+__global__ __launch_bounds__(kPairThreads) void k_tri1(TriParams<T, S> tp) {
+    if (blockIdx.x < tp.tile_launch) {
+        tri_tile<T, S>(tp, blockIdx.x);
+        return;
+    }
+    pass1_specials<T, S>(tp.a, tp.sp_cell, tp.n_special, tp.special_blocks, tp.p1, blockIdx.x - tp.tile_launch);
+}
+
+}  // namespace wv

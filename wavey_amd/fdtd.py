@@ -598,16 +598,17 @@ class Plan:
             raise FdtdError(st, "invalid tile settings")
 
     def set_pass(self, steps_per_pass=0, pair_zchunk=0):
-        """Steps per pass (1, or 2 for two-step passes; 0 = default); takes
-        effect at the next begin/run."""
+        """Steps per pass (1, 2 or 3; 0 = default); takes effect at the next
+        begin/run."""
         st = lib().wv_plan_set_pass(self._h, int(steps_per_pass), int(pair_zchunk))
         if st != WV_OK:
             raise FdtdError(st, "invalid pass settings")
 
     def pass_info(self):
-        out = (C.c_uint32 * 3)()
+        out = (C.c_uint32 * 4)()
         lib().wv_plan_pass_info(self._h, out)
-        return {"steps_per_pass": int(out[0]), "pair_zchunk": int(out[1]), "ring_cells": int(out[2])}
+        return {"steps_per_pass": int(out[0]), "pair_zchunk": int(out[1]), "ring_cells": int(out[2]),
+                "ring2_cells": int(out[3])}
 
     def copy_probe(self, nbytes, reps=10):
         """Mean ms of a plain copy of nbytes between the field buffers (clobbers them)."""
