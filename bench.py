@@ -253,7 +253,9 @@ def bench_serving(args):
     args.load of the given batched throughput (cell-updates/s). The server
     (admission by footprint, continuous batching) against sequential plain
     runs of the same trace: the plain latencies are those of a FIFO single
-    queue with the measured per-request times and the same arrivals."""
+    queue with the measured per-request times and the same arrivals. With
+    args.burst every request arrives at once: solves/s is then the capacity
+    of each form (all 40 requests from first submission to last result)."""
     import threading
 
     from wavey_amd import configs, fdtd, workload
@@ -264,7 +266,7 @@ def bench_serving(args):
     t_build = time.perf_counter() - t0
     works = [c.grid.air_cells() * len(c.signal) for c in cases]
     rate = args.load * args.throughput * 1e9 / float(np.mean(works))
-    arrivals = [t / rate for t, _ in base]
+    arrivals = [0.0] * len(base) if args.burst else [t / rate for t, _ in base]
     print(f"bench: serving {len(cases)} requests, {sum(works) / 1e12:.2f} T cell-updates, rate {rate:.2f}/s, "
           f"built in {t_build:.1f} s", file=sys.stderr, flush=True)
 
@@ -327,7 +329,7 @@ def bench_serving(args):
 
     n = len(cases)
     line = {
-        "metric": "served solves/s and latency (Poisson trace)",
+        "metric": "served solves/s and latency (%s)" % ("all requests at once" if args.burst else "Poisson trace"),
         "value": round(n / span, 3),
         "unit": "solves/s",
         "n_gpus": 1,
@@ -335,8 +337,9 @@ def bench_serving(args):
         "dtype": {"fp32": "f32", "fp64": "f64", "fp32_state64": "f32+f64state"}[args.precision],
         "data": "synthetic (seeded rooms, sizes, durations, arrivals)",
         "config": {"workload": f"serving trace: {n} requests, padded cells log-uniform 0.2M-60M, 0.3-1.0 s, "
-                               f"dx 0.05 m, Poisson rate {rate:.3f}/s = {args.load:.0%} of "
-                               f"{args.throughput} Gcell-updates/s",
+                               + ("dx 0.05 m, all submitted at once" if args.burst else
+                                  f"dx 0.05 m, Poisson rate {rate:.3f}/s = {args.load:.0%} of "
+                                  f"{args.throughput} Gcell-updates/s"),
                    "policy": args.policy, "fill_cells": args.fill or 16 << 20,
                    "lane_cells": args.lane_cells or 8 << 20,
                    "requests": n, "cell_updates": int(sum(works)), "cap_bytes": int(args.cap_gb * 1e9)},
@@ -395,6 +398,7 @@ def main():
     ap.add_argument("--rooms", type=int, default=48, help="room_mix: rooms")
     ap.add_argument("--requests", type=int, default=40, help="serving: requests")
     ap.add_argument("--load", type=float, default=0.8, help="serving: offered fraction of --throughput")
+    ap.add_argument("--burst", action="store_true", help="serving: every request arrives at once")
     ap.add_argument("--throughput", type=float, default=200.0,
                     help="serving: batched room-mix throughput to load against, Gcell-updates/s "
                          "(end to end, as the room_mix mode measures it)")

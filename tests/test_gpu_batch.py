@@ -240,3 +240,20 @@ def test_server_large_lane_beside_the_batch(rooms, big):
         assert np.array_equal(out[2 + i], plain[("fp32", i)]), f"small request {i}"
     assert st["completed"] == 5 and st["failed"] == 0
     assert st["lane_blocks"] >= 8  # 601 and 400 steps in 128-step blocks
+
+
+def test_footprint_estimate_bounds_device_bytes(rooms, big):
+    """The admission estimate is at least what an instance holds on the
+    device, in each pass mode: small rooms (one step per pass), a 4.5M-cell
+    room (two) and the hall (three, five field buffers)."""
+    cases, _ = rooms
+    c_big, _ = big
+    hall = workload.build_case(configs.hall(), steps=12)
+    print("case: footprint estimates of small rooms, a 4.5M-cell room and the hall, fp32", flush=True)
+    with fdtd.Batch("fp32") as b:
+        for c in [cases[0], cases[1], c_big, hall]:
+            iid = b.add(c.grid, c.runs[0][2], c.signal, c.receivers)
+            held = b.info(iid)["device_bytes"]
+            est = fdtd.footprint_estimate(c.grid, "fp32", len(c.signal), len(c.receivers), len(c.runs[0][2]))
+            assert held <= est, (c.grid.n, held, est)
+            assert est < 1.5 * held + (64 << 20), (c.grid.n, held, est)
