@@ -205,7 +205,8 @@ wv_status wv_batch_set_tile(wv_batch* batch, int32_t variant, uint32_t zchunk, c
 /* Runs every instance to its last step. use_graphs = 0 issues the same
  * launches one by one. Cancel is polled between step blocks. An instance
  * left alone on a grid of at least 4 Mi cells runs its remaining steps with
- * the plain run's two-step kernels (same results). */
+ * the plain run's multi-step kernels (two or three steps per pass, as its
+ * plain run; same results). */
 wv_status wv_batch_run(wv_batch* batch, int use_graphs, const volatile int32_t* cancel,
                        wv_progress_fn progress, void* ctx, char* err, size_t err_len);
 /* An instance's results, as wv_run_finish (ir_out [R][T], spectra [R][F][2]). */
@@ -251,7 +252,7 @@ wv_status wv_serve_wait(wv_server* server, uint64_t id, double* spectra_out, dou
  * admission order and each to its end, on a second stream beside the batch,
  * which the device runs at the same time.
  * A request that runs alone on a grid of at least 4 Mi cells runs the plain
- * run's two-step kernels. Results do not depend on the policy. */
+ * run's multi-step kernels. Results do not depend on the policy. */
 enum { WV_SERVE_ALL = 0, WV_SERVE_SHORTEST = 1, WV_SERVE_LANES = 2 };
 wv_status wv_serve_set_policy(wv_server* server, int policy, uint64_t fill_cells, uint64_t lane_cells);
 /* out: [completed, p50, p95, p99 latency s, first submission to last

@@ -257,3 +257,31 @@ def test_footprint_estimate_bounds_device_bytes(rooms, big):
             est = fdtd.footprint_estimate(c.grid, "fp32", len(c.signal), len(c.receivers), len(c.runs[0][2]))
             assert held <= est, (c.grid.n, held, est)
             assert est < 1.5 * held + (64 << 20), (c.grid.n, held, est)
+
+
+def test_hall_takes_the_large_lane_and_runs_alone_in_a_batch(rooms):
+    """A request on the full hall grid (three steps per pass in its plain
+    run) takes the server's large lane, and in a batch it finishes alone on
+    the three-step kernels; both give its plain run's series."""
+    cases, plain = rooms
+    hall = workload.build_case(configs.hall(), steps=300)
+    print("case: hall request (300 steps) in the server and in a batch beside a small room, fp32", flush=True)
+    with fdtd.Plan(hall.grid, "fp32") as p:
+        ref = p.run(hall.runs[0][2], hall.signal, hall.receivers, mode=fdtd.MODE_IR)[0]
+        assert p.pass_info()["steps_per_pass"] == 3
+    with fdtd.Server("fp32") as sv:
+        ids = [sv.submit(hall.grid, hall.runs[0][2], hall.signal, hall.receivers),
+               sv.submit(cases[3].grid, cases[3].runs[0][2], cases[3].signal, cases[3].receivers)]
+        out = [sv.wait(r)[0] for r in ids]
+        st = sv.stats()
+    assert np.array_equal(out[0], ref)
+    assert np.array_equal(out[1], plain[("fp32", 3)])
+    assert st["lane_blocks"] >= 3
+    with fdtd.Batch("fp32") as b:
+        ids = [b.add(hall.grid, hall.runs[0][2], hall.signal, hall.receivers),
+               b.add(cases[0].grid, cases[0].runs[0][2], cases[0].signal[:150], cases[0].receivers)]
+        b.run()
+        t = b.timing()
+        assert np.array_equal(b.result(ids[0])[0], ref)
+        assert np.array_equal(b.result(ids[1])[0], plain[("fp32", 0)][:, :150])
+    assert t["solo_blocks"] >= 1

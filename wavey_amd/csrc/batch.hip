@@ -14,8 +14,8 @@
 // class of instances, coordinate by coordinate.
 //
 // An instance that advances alone (the last of a batch, or a large request
-// the server runs by itself) runs the plain run's kernels, two steps per pass
-// where the plain run has them; the results are the same bit for bit.
+// the server runs by itself) runs the plain run's kernels, two or three steps
+// per pass where the plain run has them; the results are the same bit for bit.
 //
 // The server admits queued requests by HBM footprint into a running batch at
 // the next 128-step boundary, so requests that arrive while others run share
@@ -369,7 +369,7 @@ wv_status batch_run(wv_batch* b, bool use_graph, const volatile int32_t* cancel,
         // does not change before then, so they are enqueued back to back.
         uint32_t blocks = ~0u;
         for (Instance* in : b->table) blocks = std::min(blocks, div_up(in->steps - in->n0, kGraphSteps));
-        Instance* solo = b->table.size() == 1 && b->table[0]->plan->run_steps_per_pass == 2 ? b->table[0] : nullptr;
+        Instance* solo = b->table.size() == 1 && b->table[0]->plan->run_steps_per_pass >= 2 ? b->table[0] : nullptr;
         uint32_t issued = 0;
         while (issued < blocks) {
             if (solo) enqueue_solo(b, solo);
@@ -701,7 +701,7 @@ Instance* pick_lane_instance(wv_server* sv) {
     for (auto& up : sv->batch->inst) {
         Instance* in = up.get();
         if (in->done || in->n0 >= in->steps || in->lane >= 0) continue;
-        if (in->plan->run_steps_per_pass != 2 || in->plan->air < sv->lane_cells.load()) continue;
+        if (in->plan->run_steps_per_pass < 2 || in->plan->air < sv->lane_cells.load()) continue;
         if (in->n0 > 0) return in;
         if (!first) first = in;
     }
@@ -734,7 +734,7 @@ std::vector<Instance*> select_batch(wv_server* sv) {
         std::vector<size_t> order;
         for (size_t k = 0; k < live.size(); ++k) {
             const wv_plan* pl = live[k]->plan.get();
-            if (policy == WV_SERVE_LANES && pl->run_steps_per_pass == 2 && pl->air >= sv->lane_cells.load()) run[k] = 0;
+            if (policy == WV_SERVE_LANES && pl->run_steps_per_pass >= 2 && pl->air >= sv->lane_cells.load()) run[k] = 0;
             else order.push_back(k);
         }
         std::stable_sort(order.begin(), order.end(),
@@ -903,7 +903,7 @@ void serve_loop(wv_server* sv) {
         if (!bat.busy) {
             std::vector<Instance*> chosen = select_batch(sv);
             if (!chosen.empty()) {
-                if (chosen.size() == 1 && chosen[0]->plan->run_steps_per_pass == 2) {
+                if (chosen.size() == 1 && chosen[0]->plan->run_steps_per_pass >= 2) {
                     enqueue_solo(b, chosen[0]);
                 } else {
                     if (b->dirty) build_table<T, S>(b);
