@@ -178,6 +178,26 @@ def test_multi_step_passes_across_tiles(long_room, prec):
     assert info["ring2_cells"] > info["ring_cells"] > 0
 
 
+@pytest.mark.parametrize("dims", [(0.45, 6.0, 2.5), (6.0, 0.45, 2.5), (5.0, 4.0, 0.35)])
+def test_multi_step_passes_thin_rooms(dims):
+    # Rooms thinner than a pass-1 block in one direction (a few cells across
+    # x, fewer rows than a block owns in y, fewer planes than a z-chunk):
+    # one, two and three steps per pass agree bit for bit.
+    rng = np.random.default_rng(33)
+    lib = configs.material_library(33)
+    cfg = configs.mix_room(rng, lib, dx=0.05, dims=dims, n_rx=6)
+    cfg["steps"] = 400
+    c = workload.build_case(cfg)
+    print(f"case: thin room {dims} m, grid {c.grid.n}, fp32, one/two/three steps per pass", flush=True)
+    out = {}
+    with fdtd.Plan(c.grid, "fp32") as plan:
+        for spp in (1, 2, 3):
+            plan.set_pass(spp, 8)
+            out[spp], _ = plan.run(c.runs[0][2], c.signal, c.receivers, mode=fdtd.MODE_IR)
+    assert np.all(np.isfinite(out[1]))
+    assert np.array_equal(out[1], out[2]) and np.array_equal(out[1], out[3])
+
+
 def test_two_step_spectra_and_shoebox(shoebox):
     case, ref, spec = shoebox
     freqs = [31.25, 62.5, 125.0, 250.0]
