@@ -162,7 +162,9 @@ def bench_room_mix(args):
     speakers one after another) against one batch (all instances in one
     launch per step, autotuned shapes, 128-step blocks as HIP graphs). Every
     batched IR is compared with its plain IR (bit-identical expected); a
-    sample of instances is also run in FP64 for the FP32 error."""
+    sample of instances is also run in FP64 for the FP32 error. Also timed:
+    one wv_fdtd_run call per solve (the device plan built and freed every
+    call, as the reference calls Gpu::run_fdtd once per component run)."""
     from wavey_amd import configs, fdtd, workload
 
     rooms = configs.room_mix(n_rooms=args.rooms)
@@ -186,6 +188,15 @@ def bench_room_mix(args):
                 plain.append(ir)
     t_plain = time.perf_counter() - t0
     print(f"bench: sequential plain {t_plain:.2f} s", file=sys.stderr, flush=True)
+
+    # One wv_fdtd_run call per solve.
+    t0 = time.perf_counter()
+    oneshot_same = 0
+    for k, (_, c, taps) in enumerate(inst):
+        ir, _ = fdtd.fdtd_run(c.grid, taps, c.signal, c.receivers, mode=fdtd.MODE_IR, precision=args.precision)
+        oneshot_same += int(np.array_equal(ir, plain[k]))
+    t_oneshot = time.perf_counter() - t0
+    print(f"bench: sequential wv_fdtd_run calls {t_oneshot:.2f} s", file=sys.stderr, flush=True)
 
     # One batch.
     t0 = time.perf_counter()
@@ -232,6 +243,9 @@ def bench_room_mix(args):
                    "graphs": not args.no_graphs, "autotune": not args.no_autotune},
         "sequential_plain": {"solves_per_s": round(n / t_plain, 3), "seconds": round(t_plain, 3),
                              "device_step_loop_s": round(dev_ms / 1e3, 3)},
+        "sequential_fdtd_run_calls": {"solves_per_s": round(n / t_oneshot, 3), "seconds": round(t_oneshot, 3),
+                                      "identical_to_plain": f"{oneshot_same}/{n}",
+                                      "speedup_of_batch": round(t_oneshot / t_batch, 3)},
         "batched": {"seconds": round(t_batch, 3), "add_s": round(t_add, 3), "autotune_s": round(t_tune, 3),
                     "run_s": round(t_run, 3), "readback_s": round(t_read, 3),
                     "device_run_s": round(bt["device_ms"] / 1e3, 3),
