@@ -151,16 +151,17 @@ wv_status wv_plan_set_tile(wv_plan* plan, uint32_t zchunk, int32_t variant);
  * a second, sparse launch; four field buffers); or 3 (the cells three or more
  * cells from a boundary advance three times in one pass, the others in two
  * sparse launches; five field buffers); 0 = default. pair_zchunk: z planes
- * per block of a multi-step pass (0 = default, else a power of two up to
- * 32). Results are bit-identical in every mode. Takes effect at the next
+ * per block of a multi-step pass (0 = default, else a power of two up to 64;
+ * two-step and FP64 passes take at most 32). Results are bit-identical in
+ * every mode. Takes effect at the next
  * wv_run_begin. No reference counterpart: a tuning knob. */
 wv_status wv_plan_set_pass(wv_plan* plan, uint32_t steps_per_pass, uint32_t pair_zchunk);
 /* The pass mode in use (the current run's, else the next run's): out =
  * [steps per pass, pass z-chunk, ring cells (interior, next to a
  * non-interior cell) and cells within two of a non-interior cell (three
  * steps per pass only) of the current run]. By default grids of 64 Mi cells
- * and more take three steps per pass (z-chunk 32), from 4 Mi cells two
- * (z-chunk 16), smaller ones one. No reference counterpart. */
+ * and more take three steps per pass (z-chunk 64, FP64 32), from 4 Mi cells
+ * two (z-chunk 16), smaller ones one. No reference counterpart. */
 wv_status wv_plan_pass_info(const wv_plan* plan, uint32_t out[4]);
 /* Copies `bytes` (at most one field buffer) between the plan's field
  * buffers `reps` times with a plain streaming kernel and returns the mean

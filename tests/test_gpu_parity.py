@@ -132,7 +132,8 @@ def test_three_step_passes_bit_identical(pillared_mini, prec):
     # Three steps per pass (cells three or more from a non-interior cell in
     # one z-march, the rest in two sparse launches) compute every cell with
     # the single-step arithmetic: identical receiver series for every pass
-    # z-chunk, with 128-step blocks that end in a two-step pass (1499 steps:
+    # z-chunk (64: two bitmap words per lane in FP32; FP64 takes 32), with
+    # 128-step blocks that end in a two-step pass (1499 steps:
     # 42 triples and a pair per block, a last block of 30 triples and a single
     # step) and with a last block of one triple (1283 steps).
     case, refs = pillared_mini
@@ -141,10 +142,11 @@ def test_three_step_passes_bit_identical(pillared_mini, prec):
     with fdtd.Plan(case.grid, prec) as plan:
         plan.set_pass(1)
         one, _ = plan.run(taps, case.signal[:1499], case.receivers, mode=fdtd.MODE_IR)
-        for zc in (4, 8, 16, 32):
+        for zc in (4, 8, 16, 32, 64):
             plan.set_pass(3, zc)
             three, _ = plan.run(taps, case.signal[:1499], case.receivers, mode=fdtd.MODE_IR)
             assert np.array_equal(one, three), f"z-chunk {zc}"
+            assert plan.pass_info()["pair_zchunk"] == (min(zc, 32) if prec == "fp64" else zc)
         info = plan.pass_info()
         short, _ = plan.run(taps, case.signal[:1283], case.receivers, mode=fdtd.MODE_IR)
     assert np.array_equal(short, one[:, :1283])
@@ -169,12 +171,12 @@ def test_multi_step_passes_across_tiles(long_room, prec):
     print(f"case: 14.5 m room, grid {c.grid.n}, one, two and three steps per pass, {prec}", flush=True)
     out = {}
     with fdtd.Plan(c.grid, prec) as plan:
-        for spp in (1, 2, 3):
-            plan.set_pass(spp, 8)
-            out[spp], _ = plan.run(c.runs[0][2], c.signal, c.receivers, mode=fdtd.MODE_IR)
+        for spp, zc in ((1, 8), (2, 8), (3, 8), (3, 64)):
+            plan.set_pass(spp, zc)
+            out[(spp, zc)], _ = plan.run(c.runs[0][2], c.signal, c.receivers, mode=fdtd.MODE_IR)
         info = plan.pass_info()
     assert c.grid.n[0] > 256
-    assert np.array_equal(out[1], out[2]) and np.array_equal(out[1], out[3])
+    assert all(np.array_equal(out[(1, 8)], v) for v in out.values())
     assert info["ring2_cells"] > info["ring_cells"] > 0
 
 

@@ -676,7 +676,7 @@ PairParams<T, S> make_pair_params(wv_plan* pl, const StepParams<T, S>& base) {
     PairParams<T, S> pp{};
     pp.a = base;
     constexpr uint32_t BX = kTileX * kPairLane<T>;
-    const uint32_t zch = pl->pair_zchunk ? pl->pair_zchunk : default_pair_zchunk(pl->cells);
+    const uint32_t zch = pass_zchunk(pl, 2);
     pp.a.zchunk = zch;
     pp.dbits = pl->dbits.as<uint32_t>();
     pp.sp_cell = pl->sp_cell.as<uint32_t>();
@@ -697,7 +697,7 @@ template <class T, class S>
 TriParams<T, S> make_tri_params(wv_plan* pl, const StepParams<T, S>& base) {
     TriParams<T, S> tp{};
     tp.a = base;
-    const uint32_t zch = pl->pair_zchunk ? pl->pair_zchunk : default_pair_zchunk(pl->cells);
+    const uint32_t zch = pass_zchunk(pl, 3);
     tp.a.zchunk = zch;
     tp.dbits = pl->dbits.as<uint32_t>();
     tp.d2bits = pl->d2bits.as<uint32_t>();
@@ -1023,8 +1023,8 @@ extern "C" wv_status wv_plan_set_tile(wv_plan* plan, uint32_t zchunk, int32_t va
 }
 
 extern "C" wv_status wv_plan_set_pass(wv_plan* plan, uint32_t steps_per_pass, uint32_t pair_zchunk) {
-    // Pass z-chunks must divide the 32-plane blocks of the bitmaps.
-    if (!plan || steps_per_pass > 3 || pair_zchunk > 32 || (pair_zchunk & (pair_zchunk - 1))) return WV_ERR_INVALID;
+    // Pass z-chunks are powers of two up to 64 (pass_zchunk caps them per mode).
+    if (!plan || steps_per_pass > 3 || pair_zchunk > 64 || (pair_zchunk & (pair_zchunk - 1))) return WV_ERR_INVALID;
     plan->steps_per_pass = steps_per_pass;
     plan->pair_zchunk = pair_zchunk;
     return WV_OK;
@@ -1144,7 +1144,7 @@ extern "C" wv_status wv_plan_pass_info(const wv_plan* plan, uint32_t out[4]) {
     if (!plan || !out) return WV_ERR_INVALID;
     const uint32_t spp = plan->steps_per_pass ? plan->steps_per_pass : default_steps_per_pass(plan->cells);
     out[0] = plan->active ? plan->run_steps_per_pass : spp;
-    out[1] = plan->pair_zchunk ? plan->pair_zchunk : default_pair_zchunk(plan->cells);
+    out[1] = pass_zchunk(plan, out[0]);
     out[2] = plan->n_ring;
     out[3] = plan->n_ring2;
     return WV_OK;

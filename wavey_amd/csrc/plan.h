@@ -174,6 +174,16 @@ inline StepParams<T, S> make_params(wv_plan* pl, uint32_t tiles_per_block) {
     return a;
 }
 
+// The z-chunk of a multi-step pass: the requested one, else the default by
+// grid size and mode. Chunks must lie within the 32-plane blocks of the
+// bitmaps, except FP32 three-step passes, which hold two bitmap words per
+// lane and take up to 64 planes (their default).
+inline uint32_t pass_zchunk(const wv_plan* pl, uint32_t steps_per_pass) {
+    const bool wide = steps_per_pass == 3 && pl->prec != WV_FP64;
+    const uint32_t want = pl->pair_zchunk ? pl->pair_zchunk : wide ? 64u : default_pair_zchunk(pl->cells);
+    return std::min(want, wide ? 64u : 32u);
+}
+
 template <class F>
 inline wv_status guarded(char* err, size_t err_len, F f) {
     try {

@@ -116,12 +116,35 @@ __device__ __forceinline__ void tri_tile(const TriParams<T, S>& tp, uint32_t t) 
     for (int e = 0; e < W; ++e) own |= (uint32_t)(xl + e >= x0 + 1 && xl + e <= x0 + SX) << e;
     const uint32_t lane_mask = (live && own_row) ? own : 0u;
     auto plane = [&](int z) -> uint32_t { return (uint32_t)min(max(z, 0), nz - 1) * a.sz; };
-    // The chunk lies inside one 32-plane block of the z-packed bitmaps.
-    uint32_t ib[W], db[W], d2b[W];
-    const uint32_t wb = ((z0 - 1) >> 5) * a.sz + col;
-    load_words<W>(a.zbits + wb, ib);
-    load_words<W>(tp.dbits + wb, db);
-    load_words<W>(tp.d2bits + wb, d2b);
+    // The chunk covers one 32-plane block of the z-packed bitmaps or, for
+    // FP32 chunks of up to 64 planes (which start at a 64-plane boundary),
+    // two: per cell the p^{n+1} store bits (interior, not deep2), the deep
+    // and the deep2 bits of each block.
+    constexpr int NW = sizeof(T) == 4 ? 2 : 1;
+    const uint32_t kb0 = (uint32_t)(z0 - 1) >> 5;
+    const uint32_t nzb = ((uint32_t)nz - 2 + 31) >> 5;
+    uint32_t s1[NW][W], db[NW][W], d2b[NW][W];
+#pragma unroll
+    for (int h = 0; h < NW; ++h) {
+        uint32_t ib[W];
+        if (h == 0 || (a.zchunk > 32 && kb0 + h < nzb)) {
+            const uint32_t wb = (kb0 + h) * a.sz + col;
+            load_words<W>(a.zbits + wb, ib);
+            load_words<W>(tp.dbits + wb, db[h]);
+            load_words<W>(tp.d2bits + wb, d2b[h]);
+        } else {
+#pragma unroll
+            for (int e = 0; e < W; ++e) ib[e] = db[h][e] = d2b[h][e] = 0u;
+        }
+#pragma unroll
+        for (int e = 0; e < W; ++e) s1[h][e] = ib[e] & ~d2b[h][e];
+    }
+    // The W cells' bits of plane z (inside the chunk).
+    auto bits_of = [&](const uint32_t (&w)[NW][W], int z) -> uint32_t {
+        const uint32_t k = (uint32_t)(z - 1) - 32u * kb0;
+        if constexpr (NW == 1) return plane_bits<W>(w[0], k);
+        else return k < 32u ? plane_bits<W>(w[0], k) : plane_bits<W>(w[1], k - 32u);
+    };
 
     // A plane's loads, as the two-step pass: p^n of the row at zq + 1 and of
     // the rows above and below at zq, p^{n-1} of the row at zq, and (edge
@@ -188,9 +211,7 @@ __device__ __forceinline__ void tri_tile(const TriParams<T, S>& tp, uint32_t t) 
         }
         p1.store(&rows1[bw][wave][lane * W]);
         if (zq >= z0 && zq < z1) {
-            const uint32_t sh = (uint32_t)(zq - 1) & 31u;
-            const uint32_t bits = plane_bits<W>(ib, sh) & ~plane_bits<W>(d2b, sh) & lane_mask;
-            store_masked<T, W>(p1, tp.p1 + plane(zq) + col, bits);
+            store_masked<T, W>(p1, tp.p1 + plane(zq) + col, bits_of(s1, zq) & lane_mask);
         }
         __syncthreads();
         const uint32_t b1 = bw == 0 ? 2u : bw - 1;  // LDS buffers of plane zq - 1
@@ -211,8 +232,7 @@ __device__ __forceinline__ void tri_tile(const TriParams<T, S>& tp, uint32_t t) 
             p2.store(&rows2[b1][wave][lane * W]);
             const int zo = zq - 1;
             if (zo >= z0 && zo < z1)
-                store_masked<T, W>(p2, tp.p2 + (uint32_t)zo * a.sz + col,
-                                   plane_bits<W>(db, (uint32_t)(zo - 1) & 31u) & lane_mask);
+                store_masked<T, W>(p2, tp.p2 + (uint32_t)zo * a.sz + col, bits_of(db, zo) & lane_mask);
         }
         // p^{n+3} at plane zq - 2 of rows 2..13 (the p^{n+2} rows of plane
         // zq - 2 were written before this plane's barrier).
@@ -229,8 +249,7 @@ __device__ __forceinline__ void tri_tile(const TriParams<T, S>& tp, uint32_t t) 
                 p3.v[e] = interior_update(p2c.v[e], p1m.v[e], xm, xp, ym2.v[e], yp2.v[e], p2m.v[e], p2.v[e], a.l2, a.c2);
             }
             const int zo3 = zq - 2;
-            store_masked<T, W>(p3, tp.p3 + (uint32_t)zo3 * a.sz + col,
-                               plane_bits<W>(d2b, (uint32_t)(zo3 - 1) & 31u) & lane_mask);
+            store_masked<T, W>(p3, tp.p3 + (uint32_t)zo3 * a.sz + col, bits_of(d2b, zo3) & lane_mask);
         }
         p1m = p1c;
         p1c = p1;
