@@ -434,8 +434,8 @@ def main():
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64", "fp32_state64"])
     ap.add_argument("--zchunk", type=int, default=0)
     ap.add_argument("--variant", type=int, default=-1)
-    ap.add_argument("--steps-per-pass", type=int, default=0, help="1 or 2; 0 = the plan's default for the grid")
-    ap.add_argument("--pair-zchunk", type=int, default=0)
+    ap.add_argument("--steps-per-pass", type=int, default=0, help="1, 2 or 3; 0 = the plan's default for the grid")
+    ap.add_argument("--pair-zchunk", type=int, default=0, help="z-chunk of multi-step passes; 0 = the default")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-traffic", action="store_true", help="skip the HBM counter passes")
     args = ap.parse_args()
