@@ -147,6 +147,14 @@ __device__ __forceinline__ void pair_tile(const PairParams<T, S>& pp, uint32_t t
 #pragma unroll
     for (int e = 0; e < W; ++e) p1m.v[e] = p1c.v[e] = T(0);
     T e1c = 0;
+    // zp - c at the plane below of the p^n and p^{n+1} rows (FP32), carried
+    // from plane to plane.
+    T dz1[W], dz2[W];
+#pragma unroll
+    for (int e = 0; e < W; ++e) {
+        dz1[e] = zcur.v[e] - zm.v[e];
+        dz2[e] = T(0);
+    }
     uint32_t bw = 0;  // LDS buffer of plane zq
     PlaneLoads nx;
     issue(z0 - 1, nx);
@@ -159,13 +167,7 @@ __device__ __forceinline__ void pair_tile(const PairParams<T, S>& pp, uint32_t t
         const T lo = from_lower_lane(zcur.v[W - 1], hc);
         const T hi = from_upper_lane(zcur.v[0], hc);
         C p1;
-#pragma unroll
-        for (int e = 0; e < W; ++e) {
-            const T xm = e == 0 ? lo : zcur.v[e - 1];
-            const T xp = e == W - 1 ? hi : zcur.v[e + 1];
-            p1.v[e] = interior_update(zcur.v[e], cur.q.v[e], xm, xp, cur.ym.v[e], cur.yp.v[e], zm.v[e], cur.zp.v[e],
-                                      a.l2, a.c2);
-        }
+        row_update<T, W>(zcur.v, cur.q.v, lo, hi, cur.ym.v, cur.yp.v, zm.v, cur.zp.v, dz1, dz1, a.l2, a.c2, p1.v);
         T e1 = 0;
         if (edge) {
             const T xin = lane == 0 ? zcur.v[0] : zcur.v[W - 1];
@@ -187,6 +189,7 @@ __device__ __forceinline__ void pair_tile(const PairParams<T, S>& pp, uint32_t t
         __syncthreads();
         // p^{n+2} at plane zo = zq - 1 of the deep cells of the output rows.
         const int zo = zq - 1;
+        bool level2 = false;  // p^{n+2} formed this plane (uniform in the wave)
         if (zo >= z0 && out_row) {
             const uint32_t dmask = plane_bits<W>(db, (uint32_t)(zo - 1) & 31u) & lane_mask;
             if (__ballot(dmask != 0)) {
@@ -197,13 +200,8 @@ __device__ __forceinline__ void pair_tile(const PairParams<T, S>& pp, uint32_t t
                 const T lo1 = from_lower_lane(p1c.v[W - 1], e1c);
                 const T hi1 = from_upper_lane(p1c.v[0], e1c);
                 C p2;
-#pragma unroll
-                for (int e = 0; e < W; ++e) {
-                    const T xm = e == 0 ? lo1 : p1c.v[e - 1];
-                    const T xp = e == W - 1 ? hi1 : p1c.v[e + 1];
-                    p2.v[e] =
-                        interior_update(p1c.v[e], zm.v[e], xm, xp, ym1.v[e], yp1.v[e], p1m.v[e], p1.v[e], a.l2, a.c2);
-                }
+                row_update<T, W>(p1c.v, zm.v, lo1, hi1, ym1.v, yp1.v, p1m.v, p1.v, dz2, dz2, a.l2, a.c2, p2.v);
+                level2 = true;
                 const uint32_t po = (uint32_t)zo * a.sz;
                 if (dmask == full) {
                     p2.store(pp.p2 + po + col);
@@ -213,6 +211,11 @@ __device__ __forceinline__ void pair_tile(const PairParams<T, S>& pp, uint32_t t
                         if ((dmask >> e) & 1u) pp.p2[po + col + e] = p2.v[e];
                 }
             }
+        }
+        if (!level2) {
+            // The next plane's p^{n+2} reads this plane's difference.
+#pragma unroll
+            for (int e = 0; e < W; ++e) dz2[e] = p1.v[e] - p1c.v[e];
         }
         p1m = p1c;
         p1c = p1;

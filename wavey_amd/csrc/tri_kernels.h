@@ -69,39 +69,6 @@ __device__ __forceinline__ void store_masked(const Cells<T, W>& c, T* __restrict
     }
 }
 
-// One row's update of the W cells of a lane: interior_update of each, with
-// the FP32 neighbour differences shared. xm - c of a cell is the negation of
-// xp - c of the cell before it, and zm - c at a plane the negation of zp - c
-// at the plane below (dzm, kept by the caller; this plane's zp - c goes to
-// dzp). Round to nearest is symmetric, so each sum of two differences is the
-// one interior_update forms, bit for bit, with 5 subtractions per lane for x
-// instead of 8 and 4 for z instead of 8. FP64 keeps interior_update's form.
-template <typename T, int W>
-__device__ __forceinline__ void row_update(const T (&c)[W], const T (&q)[W], T lo, T hi, const T (&ym)[W],
-                                           const T (&yp)[W], const T (&zm)[W], const T (&zp)[W],
-                                           const T (&dzm)[W], T (&dzp)[W], T l2, T c2, T (&out)[W]) {
-    if constexpr (sizeof(T) == 8) {
-#pragma unroll
-        for (int e = 0; e < W; ++e)
-            out[e] = interior_update(c[e], q[e], e == 0 ? lo : c[e - 1], e == W - 1 ? hi : c[e + 1], ym[e], yp[e],
-                                     zm[e], zp[e], l2, c2);
-    } else {
-        T dx[W + 1];  // dx[e] = c[e] - (the cell before it)
-        dx[0] = c[0] - lo;
-#pragma unroll
-        for (int e = 1; e < W; ++e) dx[e] = c[e] - c[e - 1];
-        dx[W] = hi - c[W - 1];
-#pragma unroll
-        for (int e = 0; e < W; ++e) {
-            const T zl = dzm[e];  // read before dzp[e] is written: the caller may pass one array as both
-            const T zu = zp[e] - c[e];
-            const T d = (dx[e + 1] - dx[e]) + ((ym[e] - c[e]) + (yp[e] - c[e])) + (zu - zl);
-            out[e] = c[e] + ((c[e] - q[e]) + l2 * d);
-            dzp[e] = zu;
-        }
-    }
-}
-
 template <typename T, typename S>
 __device__ __forceinline__ void tri_tile(const TriParams<T, S>& tp, uint32_t t) {
     const StepParams<T, S>& a = tp.a;
