@@ -178,20 +178,21 @@ __device__ __forceinline__ void tri_tile(const TriParams<T, S>& tp, uint32_t t) 
         hm = a.p[plane(z0 - 3) + hcol];
         hc = a.p[plane(z0 - 2) + hcol];
     }
-    // p^{n+1} of the row at planes zq - 2 and zq - 1, of the halo column at
-    // zq - 1; p^{n+2} of the row at planes zq - 3 and zq - 2.
-    C p1m, p1c, p2m, p2c, p2;
+    // p^{n+2} of the row at the last plane formed (for the next p^{n+3}), and
+    // p^{n+1} of the halo column at zq - 1. The older rows of this wave
+    // (p^{n+1} at planes zq - 1 and zq - 2, p^{n+2} at zq - 2 and zq - 3) are
+    // read back from the LDS rings, which hold every wave's own row.
+    C p2;
 #pragma unroll
-    for (int e = 0; e < W; ++e) p1m.v[e] = p1c.v[e] = p2m.v[e] = p2c.v[e] = p2.v[e] = T(0);
+    for (int e = 0; e < W; ++e) p2.v[e] = T(0);
     T e1c = 0;
-    // zp - c at the plane below of the p^n and p^{n+2} rows (FP32), carried
-    // from plane to plane. The p^{n+1} level forms its own: a third carry
-    // does not fit the 128 registers of four waves per SIMD.
-    T dz1[W], dz3[W];
+    // zp - c at the plane below of the p^n, p^{n+1} and p^{n+2} rows (FP32),
+    // carried from plane to plane.
+    T dz1[W], dz2[W], dz3[W];
 #pragma unroll
     for (int e = 0; e < W; ++e) {
         dz1[e] = zcur.v[e] - zm.v[e];
-        dz3[e] = T(0);
+        dz2[e] = dz3[e] = T(0);
     }
     uint32_t bw = 0;  // LDS buffer of plane zq
     PlaneLoads nx;
@@ -219,6 +220,9 @@ __device__ __forceinline__ void tri_tile(const TriParams<T, S>& tp, uint32_t t) 
         __syncthreads();
         const uint32_t b1 = bw == 0 ? 2u : bw - 1;  // LDS buffers of plane zq - 1
         const uint32_t b2 = b1 == 0 ? 2u : b1 - 1;  // and of plane zq - 2
+        C p1c, p1m;  // this row's p^{n+1} at planes zq - 1 and zq - 2
+        p1c.load(&rows1[b1][wave][lane * W]);
+        p1m.load(&rows1[b2][wave][lane * W]);
         // p^{n+2} at plane zq - 1 of rows 1..14.
         if (zq >= z0 && wave >= 1 && wave <= (uint32_t)kPairWaves - 2) {
             C ym1, yp1;
@@ -226,37 +230,38 @@ __device__ __forceinline__ void tri_tile(const TriParams<T, S>& tp, uint32_t t) 
             yp1.load(&rows1[b1][wave + 1][lane * W]);
             const T lo1 = from_lower_lane(p1c.v[W - 1], e1c);
             const T hi1 = from_upper_lane(p1c.v[0], e1c);
-            T dz2[W];
-#pragma unroll
-            for (int e = 0; e < W; ++e) dz2[e] = p1c.v[e] - p1m.v[e];
             row_update<T, W>(p1c.v, zm.v, lo1, hi1, ym1.v, yp1.v, p1m.v, p1.v, dz2, dz2, a.l2, a.c2, p2.v);
             p2.store(&rows2[b1][wave][lane * W]);
             const int zo = zq - 1;
             if (zo >= z0 && zo < z1)
                 store_masked<T, W>(p2, tp.p2 + (uint32_t)zo * a.sz + col, bits_of(db, zo) & lane_mask);
+        } else {
+            // Planes and rows without p^{n+2} still carry the difference.
+#pragma unroll
+            for (int e = 0; e < W; ++e) dz2[e] = p1.v[e] - p1c.v[e];
         }
         // p^{n+3} at plane zq - 2 of rows 2..13 (the p^{n+2} rows of plane
         // zq - 2 were written before this plane's barrier).
-        if (zq >= z0 + 2 && own_row) {
-            C ym2, yp2, p3;
-            ym2.load(&rows2[b2][wave - 1][lane * W]);
-            yp2.load(&rows2[b2][wave + 1][lane * W]);
-            const T lo2 = from_lower_lane(p2c.v[W - 1], T(0));
-            const T hi2 = from_upper_lane(p2c.v[0], T(0));
-            row_update<T, W>(p2c.v, p1m.v, lo2, hi2, ym2.v, yp2.v, p2m.v, p2.v, dz3, dz3, a.l2, a.c2, p3.v);
-            const int zo3 = zq - 2;
-            store_masked<T, W>(p3, tp.p3 + (uint32_t)zo3 * a.sz + col, bits_of(d2b, zo3) & lane_mask);
-        } else {
-            // Rows and planes without p^{n+3} still carry the difference
-            // (the first p^{n+3} plane reads the one of the plane before).
+        if (own_row) {
+            C p2c;  // this row's p^{n+2} at plane zq - 2
+            p2c.load(&rows2[b2][wave][lane * W]);
+            if (zq >= z0 + 2) {
+                C ym2, yp2, p2m, p3;
+                ym2.load(&rows2[b2][wave - 1][lane * W]);
+                yp2.load(&rows2[b2][wave + 1][lane * W]);
+                p2m.load(&rows2[bw][wave][lane * W]);  // plane zq - 3 (FP64's form only)
+                const T lo2 = from_lower_lane(p2c.v[W - 1], T(0));
+                const T hi2 = from_upper_lane(p2c.v[0], T(0));
+                row_update<T, W>(p2c.v, p1m.v, lo2, hi2, ym2.v, yp2.v, p2m.v, p2.v, dz3, dz3, a.l2, a.c2, p3.v);
+                const int zo3 = zq - 2;
+                store_masked<T, W>(p3, tp.p3 + (uint32_t)zo3 * a.sz + col, bits_of(d2b, zo3) & lane_mask);
+            } else {
+                // The first p^{n+3} plane reads the difference of the plane before.
 #pragma unroll
-            for (int e = 0; e < W; ++e) dz3[e] = p2.v[e] - p2c.v[e];
+                for (int e = 0; e < W; ++e) dz3[e] = p2.v[e] - p2c.v[e];
+            }
         }
-        p1m = p1c;
-        p1c = p1;
         e1c = e1;
-        p2m = p2c;
-        p2c = p2;
         zm = zcur;
         zcur = cur.zp;
         hm = hc;
