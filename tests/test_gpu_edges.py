@@ -8,8 +8,13 @@
   into any cell and adds duplicate taps, fdtd.rs:718-720, gpu_fdtd.rs:67-75),
   with a receiver on the solid cell;
 * the hall at its full size: three (its default) and two steps per pass
-  bit-identical to single steps over 40 steps, every sample finite.
+  bit-identical to single steps over 40 steps, every sample finite;
+* air up to the grid's x ends (a caller's grid without the builder's padding
+  columns), so that cells at x = 1 and x = nx - 2 are interior: every pass
+  mode bit-identical and FP64 against the oracle.
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -118,3 +123,46 @@ def test_hall_full_size_pass_modes_agree():
         one, _ = plan.run(taps, case.signal, case.receivers, mode=fdtd.MODE_IR)
     assert np.all(np.isfinite(one))
     assert np.array_equal(one, two) and np.array_equal(one, three)
+
+
+def without_x_padding(g):
+    """The grid less its columns x = 0 and x = nx - 1: the cells next to the
+    walls become the grid's end columns and the cells beside them interior.
+    Works for the library's and the oracle's grid (same fields)."""
+    nx, ny, nz = g.n
+    keep = np.zeros(nx, bool)
+    keep[1:-1] = True
+    flags = np.ascontiguousarray(g.flags.reshape(nz, ny, nx)[:, :, keep].ravel())
+    z, r = np.divmod(g.b_cell.astype(np.int64), nx * ny)
+    y, x = np.divmod(r, nx)
+    assert np.all((x >= 1) & (x <= nx - 2)), "a boundary cell in a padding column"
+    b_cell = ((z * ny + y) * (nx - 2) + (x - 1)).astype(np.uint32)
+    origin = (g.origin[0] + g.dx, g.origin[1], g.origin[2])
+    return dataclasses.replace(g, n=(nx - 2, ny, nz), origin=origin, flags=flags, b_cell=b_cell)
+
+
+def test_air_at_the_grid_x_ends_all_pass_modes():
+    cfg = dict(configs.pillared_room(), dx=0.08, steps=600)
+    case = workload.build_case(cfg)
+    g = without_x_padding(case.grid)
+    og = without_x_padding(oracle.build_box(cfg["room"], cfg["c"], cfg["dx"]))
+    nx, ny, nz = g.n
+    f = g.flags.reshape(nz, ny, nx)
+    assert np.count_nonzero((f[:, :, 1] & 127) == 127) > 0 and np.count_nonzero((f[:, :, nx - 2] & 127) == 127) > 0
+    src = g.nearest_air(g.center(case.runs[0][2][0][0]) - np.array([g.dx, 0.0, 0.0]))
+    taps = [(src, 1.0)]
+    rx = [g.index(1, y, z) for y, z in ((ny // 2, nz // 2), (ny // 3, nz // 4))] + \
+         [g.index(nx - 2, ny // 2, nz // 3), g.index(2, ny // 4, nz // 2)]
+    sig = case.signal[:600]
+    print(f"case: pillared room without x padding, grid {g.n}, 600 steps, passes of 1, 2, 3", flush=True)
+    ref = oracle.run(og, taps, sig, rx, threads=4)[0]
+    for prec in ("fp64", "fp32"):
+        with fdtd.Plan(g, prec) as plan:
+            outs = []
+            for spp, zc in ((1, 0), (2, 8), (3, 8), (3, 0)):
+                plan.set_pass(spp, zc)
+                outs.append(plan.run(taps, sig, rx, mode=fdtd.MODE_IR)[0])
+        for o in outs[1:]:
+            assert np.array_equal(outs[0], o), prec
+        assert np.all(np.isfinite(outs[0]))
+        assert rel_l2(outs[0], ref) < (1e-9 if prec == "fp64" else 1e-4), prec

@@ -703,9 +703,11 @@ TriParams<T, S> make_tri_params(wv_plan* pl, const StepParams<T, S>& base) {
     tp.d2bits = pl->d2bits.as<uint32_t>();
     tp.sp_cell = pl->sp_cell.as<uint32_t>();
     tp.n_special = pl->n_special;
-    // Tiles own columns 1 + k * stride .. (k + 1) * stride: every column an
-    // interior cell can be in (1 .. nx-2).
-    tp.tiles_x = div_up(pl->nx - 2, kTriStride<T>);
+    // Tile k stores p^{n+2} and p^{n+3} on columns 2 + k * stride ..
+    // (k + 1) * stride + 1, and p^{n+1} one column further on each side: the
+    // tiles cover columns 2 .. nx - 3 (a deep cell is two from the grid's
+    // ends) and 1 .. nx - 2 for p^{n+1} (every interior cell).
+    tp.tiles_x = std::max<uint32_t>(1, div_up(pl->nx > 4 ? pl->nx - 4 : 0, kTriStride<T>));
     tp.row_blocks = div_up(pl->ny, kTriRows);
     tp.tile_blocks = tp.tiles_x * tp.row_blocks * div_up(pl->nz - 2, zch);
     tp.tile_launch = div_up(tp.tile_blocks / tp.tiles_x, 8) * 8 * tp.tiles_x;

@@ -10,14 +10,17 @@
 //
 //  * Pass 1, tile blocks: a block covers 256 FP32 (128 FP64) cells of a row
 //    by 16 rows (a wave per row) and marches a z-chunk. Per plane it computes
-//    p^{n+1} of every row (the lanes at the tile ends also the column next to
-//    the tile), one plane behind p^{n+2} of rows 1..14 from the p^{n+1} rows
+//    p^{n+1} of every row (from p^n of the tile and of the column next to
+//    each end), one plane behind p^{n+2} of rows 1..14 from the p^{n+1} rows
 //    shared through LDS, and two planes behind p^{n+3} of rows 2..13 from the
-//    p^{n+2} rows shared through LDS. A block owns rows 2..13 and columns
-//    1 .. 252 of its tile (1 .. 126 FP64), where all three levels are right
-//    for the cells that store them; tiles step 252 (126) cells and 12 rows.
-//    It stores p^{n+1} of the interior cells that are not deep2, p^{n+2} of
-//    the deep cells and p^{n+3} of the deep2 cells (deep cells whose six
+//    p^{n+2} rows shared through LDS. p^{n+1} is right on every column of the
+//    tile, p^{n+2} on all but the two end columns, p^{n+3} on columns
+//    2 .. 253 (2 .. 125 FP64): a block owns rows 2..13 and those columns, and
+//    tiles step 252 (124) cells and 12 rows, so no tile forms a value beyond
+//    its ends. It stores p^{n+1} of the interior cells that are not deep2
+//    (also on columns 1 and 254 (126), which the tile beside writes with the
+//    same value, so that columns 1 and nx - 2 are covered), p^{n+2} of the
+//    deep cells and p^{n+3} of the deep2 cells (deep cells whose six
 //    neighbours are deep: their third step needs only values the block
 //    computes itself).
 //  * Pass 1, special blocks and receivers: as with two steps per pass.
@@ -36,10 +39,12 @@ namespace wv {
 
 constexpr int kTriRows = kPairWaves - 4;  // rows a pass-1 block owns (12)
 
-// Columns a pass-1 tile owns and steps by: the tile less its two end
-// columns, rounded down to whole lanes (252 FP32, 126 FP64).
+// Columns a pass-1 tile steps by: the tile less two columns at each end
+// (252 FP32, 124 FP64). A tile forms p^{n+1} on all its columns (from p^n of
+// the column beyond each end), p^{n+2} on all but its end columns and
+// p^{n+3} on the stride's columns 2 .. stride + 1 of the tile.
 template <typename T>
-constexpr uint32_t kTriStride = (kTileX * kPairLane<T> - 2) / kPairLane<T> * kPairLane<T>;
+constexpr uint32_t kTriStride = kTileX * kPairLane<T> - 4;
 
 template <typename T, typename S>
 struct TriParams {
@@ -102,19 +107,25 @@ __device__ __forceinline__ void tri_tile(const TriParams<T, S>& tp, uint32_t t) 
     const bool live = xl < a.pitch;
     const uint32_t x = live ? xl : a.pitch - W;
     const uint32_t col = yc * a.pitch + x, colm = ymc * a.pitch + x, colp = ypc * a.pitch + x;
-    // Lanes 0 and 63: the column next to the tile (xh) and the one beyond (xh2).
+    // Lanes 0 and 63: p^n of the column next to the tile (xh).
     const bool edge = lane == 0 || lane == kTileX - 1;
     const uint32_t xh = lane == 0 ? (x0 > 0 ? x0 - 1 : 0u) : min(x0 + BX, a.pitch - 1);
-    const uint32_t xh2 = lane == 0 ? (x0 > 1 ? x0 - 2 : 0u) : min(x0 + BX + 1, a.pitch - 1);
-    const uint32_t hcol = yc * a.pitch + xh, hcol2 = yc * a.pitch + xh2;
-    const uint32_t hcolm = ymc * a.pitch + xh, hcolp = ypc * a.pitch + xh;
+    const uint32_t hcol = yc * a.pitch + xh;
     const int z0 = 1 + (int)(zc * a.zchunk);
     const int z1 = min(z0 + (int)a.zchunk, nz - 1);
-    // Owned cells of this lane: columns x0 + 1 .. x0 + SX of an owned row.
-    uint32_t own = 0;
+    // Stored cells of this lane in an owned row: p^{n+2} and p^{n+3} on
+    // columns x0 + 2 .. x0 + SX + 1 (each column of the grid in one tile), and
+    // p^{n+1}, right on every column of the tile, also on x0 + 1 and
+    // x0 + SX + 2, so that columns 1 and nx - 2 are covered (the tiles beside
+    // write the same values there).
+    uint32_t own = 0, own1 = 0;
 #pragma unroll
-    for (int e = 0; e < W; ++e) own |= (uint32_t)(xl + e >= x0 + 1 && xl + e <= x0 + SX) << e;
+    for (int e = 0; e < W; ++e) {
+        own |= (uint32_t)(xl + e >= x0 + 2 && xl + e <= x0 + SX + 1) << e;
+        own1 |= (uint32_t)(xl + e >= x0 + 1 && xl + e <= x0 + SX + 2) << e;
+    }
     const uint32_t lane_mask = (live && own_row) ? own : 0u;
+    const uint32_t lane_mask1 = (live && own_row) ? own1 : 0u;
     auto plane = [&](int z) -> uint32_t { return (uint32_t)min(max(z, 0), nz - 1) * a.sz; };
     // The chunk covers one 32-plane block of the z-packed bitmaps or, for
     // FP32 chunks of up to 64 planes (which start at a 64-plane boundary),
@@ -146,12 +157,12 @@ __device__ __forceinline__ void tri_tile(const TriParams<T, S>& tp, uint32_t t) 
         else return k < 32u ? plane_bits<W>(w[0], k) : plane_bits<W>(w[1], k - 32u);
     };
 
-    // A plane's loads, as the two-step pass: p^n of the row at zq + 1 and of
-    // the rows above and below at zq, p^{n-1} of the row at zq, and (edge
-    // lanes) the same around the halo column, issued one plane ahead.
+    // A plane's loads, issued one plane ahead: p^n of the row at zq + 1 and
+    // of the rows above and below at zq, p^{n-1} of the row at zq, and (edge
+    // lanes) p^n of the column next to the tile at zq + 1.
     struct PlaneLoads {
         C zp, q, ym, yp;
-        T hp, hx2, hym, hyp, hq;
+        T hp;
     };
     auto issue = [&](int zq, PlaneLoads& L) {
         const uint32_t pz = plane(zq), pzp = plane(zq + 1);
@@ -159,33 +170,24 @@ __device__ __forceinline__ void tri_tile(const TriParams<T, S>& tp, uint32_t t) 
         L.q.load(a.prev + pz + col);
         L.ym.load(a.p + pz + colm);
         L.yp.load(a.p + pz + colp);
-        L.hp = L.hx2 = L.hym = L.hyp = L.hq = T(0);
-        if (edge) {
-            L.hp = a.p[pzp + hcol];
-            L.hx2 = a.p[pz + hcol2];
-            L.hym = a.p[pz + hcolm];
-            L.hyp = a.p[pz + hcolp];
-            L.hq = a.prev[pz + hcol];
-        }
+        L.hp = T(0);
+        if (edge) L.hp = a.p[pzp + hcol];
     };
 
-    // p^n of the row at planes zq - 1 and zq (and of the halo column).
+    // p^n of the row at planes zq - 1 and zq, and of the column next to the
+    // tile at zq.
     C zm, zcur;
     zm.load(a.p + plane(z0 - 3) + col);
     zcur.load(a.p + plane(z0 - 2) + col);
-    T hm = 0, hc = 0;
-    if (edge) {
-        hm = a.p[plane(z0 - 3) + hcol];
-        hc = a.p[plane(z0 - 2) + hcol];
-    }
-    // p^{n+2} of the row at the last plane formed (for the next p^{n+3}), and
-    // p^{n+1} of the halo column at zq - 1. The older rows of this wave
+    T hc = 0;
+    if (edge) hc = a.p[plane(z0 - 2) + hcol];
+    // p^{n+2} of the row at the last plane formed (for the next p^{n+3}).
+    // The older rows of this wave
     // (p^{n+1} at planes zq - 1 and zq - 2, p^{n+2} at zq - 2 and zq - 3) are
     // read back from the LDS rings, which hold every wave's own row.
     C p2;
 #pragma unroll
     for (int e = 0; e < W; ++e) p2.v[e] = T(0);
-    T e1c = 0;
     // zp - c at the plane below of the p^n, p^{n+1} and p^{n+2} rows (FP32),
     // carried from plane to plane.
     T dz1[W], dz2[W], dz3[W];
@@ -201,21 +203,14 @@ __device__ __forceinline__ void tri_tile(const TriParams<T, S>& tp, uint32_t t) 
     for (int zq = z0 - 2; zq <= z1 + 1; ++zq) {
         const PlaneLoads cur = nx;
         if (zq <= z1) issue(zq + 1, nx);
-        // p^{n+1} at plane zq: the row, and (edge lanes) the halo column.
+        // p^{n+1} at plane zq of the row.
         const T lo = from_lower_lane(zcur.v[W - 1], hc);
         const T hi = from_upper_lane(zcur.v[0], hc);
         C p1;
         row_update<T, W>(zcur.v, cur.q.v, lo, hi, cur.ym.v, cur.yp.v, zm.v, cur.zp.v, dz1, dz1, a.l2, a.c2, p1.v);
-        T e1 = 0;
-        if (edge) {
-            const T xin = lane == 0 ? zcur.v[0] : zcur.v[W - 1];
-            const T exm = lane == 0 ? cur.hx2 : xin;
-            const T exp = lane == 0 ? xin : cur.hx2;
-            e1 = interior_update(hc, cur.hq, exm, exp, cur.hym, cur.hyp, hm, cur.hp, a.l2, a.c2);
-        }
         p1.store(&rows1[bw][wave][lane * W]);
         if (zq >= z0 && zq < z1) {
-            store_masked<T, W>(p1, tp.p1 + plane(zq) + col, bits_of(s1, zq) & lane_mask);
+            store_masked<T, W>(p1, tp.p1 + plane(zq) + col, bits_of(s1, zq) & lane_mask1);
         }
         __syncthreads();
         const uint32_t b1 = bw == 0 ? 2u : bw - 1;  // LDS buffers of plane zq - 1
@@ -228,8 +223,8 @@ __device__ __forceinline__ void tri_tile(const TriParams<T, S>& tp, uint32_t t) 
             C ym1, yp1;
             ym1.load(&rows1[b1][wave - 1][lane * W]);
             yp1.load(&rows1[b1][wave + 1][lane * W]);
-            const T lo1 = from_lower_lane(p1c.v[W - 1], e1c);
-            const T hi1 = from_upper_lane(p1c.v[0], e1c);
+            const T lo1 = from_lower_lane(p1c.v[W - 1], T(0));  // the tile's end columns are
+            const T hi1 = from_upper_lane(p1c.v[0], T(0));      // not stored at this level
             row_update<T, W>(p1c.v, zm.v, lo1, hi1, ym1.v, yp1.v, p1m.v, p1.v, dz2, dz2, a.l2, a.c2, p2.v);
             p2.store(&rows2[b1][wave][lane * W]);
             const int zo = zq - 1;
@@ -261,10 +256,8 @@ __device__ __forceinline__ void tri_tile(const TriParams<T, S>& tp, uint32_t t) 
                 for (int e = 0; e < W; ++e) dz3[e] = p2.v[e] - p2c.v[e];
             }
         }
-        e1c = e1;
         zm = zcur;
         zcur = cur.zp;
-        hm = hc;
         hc = cur.hp;
         bw = bw == 2 ? 0u : bw + 1;
     }
