@@ -732,13 +732,13 @@ uint64_t enqueue_range(wv_plan* pl, size_t n, size_t stop, hipStream_t stream) {
         pp = make_pair_params<T, S>(pl, a);
         grid1 = pp.tile_launch + pp.special_blocks + div_up(pl->n_recv, kPairThreads);
         items2 = div_up(pl->n_special, kPair2Block) + div_up(pl->n_ring, kPair2Block);
-        grid2 = items2 + div_up(pl->n_recv, kPair2Block);
+        grid2 = div_up(items2, kPair2Bands) * kPair2Bands + div_up(pl->n_recv, kPair2Block);
     }
     if (triples) {
         tp = make_tri_params<T, S>(pl, a);
         gridt = tp.tile_launch + tp.special_blocks + div_up(pl->n_recv, kPairThreads);
         items3 = div_up(pl->n_special, kPair2Block) + div_up(pl->n_ring2, kPair2Block);
-        grid3 = items3 + div_up(pl->n_recv, kPair2Block);
+        grid3 = div_up(items3, kPair2Bands) * kPair2Bands + div_up(pl->n_recv, kPair2Block);
     }
     T* bufs[5] = {pl->fa.as<T>(), pl->fb.as<T>(), pl->fc.as<T>(), pl->fd.as<T>(), pl->fe.as<T>()};
     auto sig = [&](size_t m) -> S { return (S)(sizeof(S) == 4 ? (double)(float)pl->signal[m] : pl->signal[m]); };

@@ -22,8 +22,8 @@
 //  * Pass 2: step n+1 of the special cells and of the ring (interior cells
 //    that are not deep), from the complete p^{n+1}, in blocks of one kind
 //    interleaved in proportion to the two counts (both lists run through
-//    the grid in z order, so the blocks in flight share the cache lines
-//    around the same walls); and the receivers' p^{n+1}.
+//    the grid in z order, so the blocks of a band, on one XCD, share the
+//    cache lines around the same walls); and the receivers' p^{n+1}.
 //
 // Every cell's arithmetic is the single-step kernel's (interior_update and
 // special_update with the same operands), so results are bit-identical to
@@ -37,6 +37,11 @@ constexpr int kPairRows = 14;                           // output rows of a pass
 constexpr int kPairWaves = kPairRows + 2;               // and a halo row above and below
 constexpr uint32_t kPairThreads = kTileX * kPairWaves;  // 1024
 constexpr uint32_t kPair2Block = 128;  // pass-2 items per block (specials or ring cells)
+// Pass 2's item blocks run in kPair2Bands bands of consecutive blocks (the
+// lists are in cell order), dealt to the eight XCDs in turn: a band's cells
+// share cache lines with their neighbours in one L2, and each XCD's bands
+// lie all along the lists, so dense regions (a floor) spread over the XCDs.
+constexpr uint32_t kPair2Bands = 128;
 
 template <typename T, typename S>
 struct PairParams {
@@ -268,9 +273,14 @@ __global__ __launch_bounds__(kPair2Block) void k_pair2(StepParams<T, S> a, T* __
                                                const uint32_t* __restrict__ sp_cell,
                                                const uint32_t* __restrict__ ring_cell, uint32_t n_special,
                                                uint32_t n_ring, uint32_t item_blocks) {
-    if (blockIdx.x < item_blocks) {
+    // Launch block i runs on XCD i mod 8; its item block is in band
+    // (i mod 8) + 8 k, k = (i / 8) / band.
+    const uint32_t band = (item_blocks + kPair2Bands - 1u) / kPair2Bands;
+    if (blockIdx.x < band * kPair2Bands) {
+        const uint32_t q = blockIdx.x >> 3;
+        const uint32_t b = ((blockIdx.x & 7u) + 8u * (q / band)) * band + q % band;
+        if (b >= item_blocks) return;
         const uint32_t bs = (n_special + kPair2Block - 1) / kPair2Block;  // special blocks
-        const uint32_t b = blockIdx.x;
         const uint32_t s0 = (uint32_t)(((uint64_t)b * bs) / item_blocks);
         const uint32_t s1 = (uint32_t)(((uint64_t)(b + 1) * bs) / item_blocks);
         if (s1 > s0) {
@@ -292,7 +302,7 @@ __global__ __launch_bounds__(kPair2Block) void k_pair2(StepParams<T, S> a, T* __
         }
         return;
     }
-    const uint32_t rr = (blockIdx.x - item_blocks) * kPair2Block + threadIdx.x;
+    const uint32_t rr = (blockIdx.x - band * kPair2Bands) * kPair2Block + threadIdx.x;
     if (rr < a.n_recv) a.ir[rr] = a.p[a.recv[rr]];
 }
 
