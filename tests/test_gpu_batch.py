@@ -165,7 +165,7 @@ def big():
     assert c.grid.cells >= 4 << 20
     with fdtd.Plan(c.grid, "fp32") as p:
         ir = p.run(c.runs[0][2], c.signal, c.receivers, mode=fdtd.MODE_IR)[0]
-        assert p.pass_info()["steps_per_pass"] == 2
+        assert p.pass_info()["steps_per_pass"] >= 2
     return c, ir
 
 

@@ -611,7 +611,7 @@ void run_begin_impl(wv_plan* pl, const wv_tap* taps, size_t n_taps, const double
     WV_HIP(hipGetLastError());
     // Multi-step passes: the ring list and the two more field buffers (a
     // third for three steps).
-    pl->run_steps_per_pass = pl->steps_per_pass ? pl->steps_per_pass : default_steps_per_pass(pl->cells);
+    pl->run_steps_per_pass = pl->steps_per_pass ? pl->steps_per_pass : default_steps_per_pass(pl->cells, pl->prec == WV_FP64);
     pl->n_ring = pl->n_ring2 = 0;
     if (pl->run_steps_per_pass >= 2) {
         pl->n_ring = compact_cells(pl, RingAt{pl->flags.as<uint8_t>(), pl->force.as<uint32_t>(), geo}, geo,
@@ -952,7 +952,7 @@ uint64_t wv::estimate_bytes(const wv_grid* g, int prec, size_t steps, size_t n_r
     const uint64_t special = g->n_boundary + n_taps;
     const uint64_t sp_pad = (special + 63) & ~63ull;
     const uint64_t slots = g->n_boundary ? g->b_start[g->n_boundary] : 0;
-    const uint32_t spp = default_steps_per_pass(cells);
+    const uint32_t spp = default_steps_per_pass(cells, prec == WV_FP64);
     const uint64_t fields = spp == 3 ? 5 : spp == 2 ? 4 : 2;
     uint64_t t = cells;                              // flags
     t += npitched / 4 + npitched / 4;                // boundary marks (two copies), interior and deep bitmaps
@@ -1144,7 +1144,7 @@ extern "C" wv_status wv_plan_stats(const wv_plan* plan, uint64_t out[7]) {
 
 extern "C" wv_status wv_plan_pass_info(const wv_plan* plan, uint32_t out[4]) {
     if (!plan || !out) return WV_ERR_INVALID;
-    const uint32_t spp = plan->steps_per_pass ? plan->steps_per_pass : default_steps_per_pass(plan->cells);
+    const uint32_t spp = plan->steps_per_pass ? plan->steps_per_pass : default_steps_per_pass(plan->cells, plan->prec == WV_FP64);
     out[0] = plan->active ? plan->run_steps_per_pass : spp;
     out[1] = pass_zchunk(plan, out[0]);
     out[2] = plan->n_ring;

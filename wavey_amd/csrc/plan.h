@@ -177,11 +177,19 @@ inline StepParams<T, S> make_params(wv_plan* pl, uint32_t tiles_per_block) {
 // The z-chunk of a multi-step pass: the requested one, else the default by
 // grid size and mode. Chunks must lie within the 32-plane blocks of the
 // bitmaps, except FP32 three-step passes, which hold two bitmap words per
-// lane and take up to 64 planes (their default).
+// lane and take up to 64 planes: by default the longest of 64, 32 and 16
+// that still gives 640 tile blocks (2.5 per CU; the hall takes 64, a 58 Mi
+// cell grid 32, a 29 Mi one 16).
 inline uint32_t pass_zchunk(const wv_plan* pl, uint32_t steps_per_pass) {
     const bool wide = steps_per_pass == 3 && pl->prec != WV_FP64;
-    const uint32_t want = pl->pair_zchunk ? pl->pair_zchunk : wide ? 64u : default_pair_zchunk(pl->cells);
-    return std::min(want, wide ? 64u : 32u);
+    if (pl->pair_zchunk) return std::min(pl->pair_zchunk, wide ? 64u : 32u);
+    if (!wide) return default_pair_zchunk(pl->cells);
+    const uint64_t cols = std::max<uint64_t>(1, pl->nx > 4 ? (pl->nx - 4 + kTriStride<float> - 1) / kTriStride<float> : 1);
+    const uint64_t rows = (pl->ny + kTriRows - 1) / kTriRows;
+    for (uint32_t zc : {64u, 32u}) {
+        if (cols * rows * ((pl->nz - 2 + zc - 1) / zc) >= 640) return zc;
+    }
+    return 16u;
 }
 
 template <class F>

@@ -528,15 +528,17 @@ constexpr int kVariants = 4;
 constexpr int kDefaultVariant = 0;
 constexpr uint32_t kDefaultZChunk = 4;
 // Default steps per pass and pass z-chunk by grid size (measured on the
-// hall, 32M-, 10M- and 7M-cell rooms and smaller ones): small grids are
-// launch- and latency-bound, where one step per launch wins; from a few
-// million cells two steps per pass win, and on the largest grids three, with
-// longer z-chunks.
+// hall and on hall-shaped grids of 58, 29, 15 and 7.6 Mi cells, and smaller
+// rooms): small grids are launch- and latency-bound, where one step per
+// launch wins; from a few million cells two steps per pass win, and three
+// from 8 Mi cells in FP32 (64 Mi in FP64, measured on the hall only), with
+// z-chunks that keep enough tile blocks in flight (pass_zchunk).
 constexpr uint64_t kPairMinCells = 4ull << 20;
 constexpr uint64_t kPairLongChunkCells = 64ull << 20;
-constexpr uint64_t kTriMinCells = 64ull << 20;
-inline uint32_t default_steps_per_pass(uint64_t cells) {
-    return cells >= kTriMinCells ? 3u : cells >= kPairMinCells ? 2u : 1u;
+constexpr uint64_t kTriMinCells = 8ull << 20;
+constexpr uint64_t kTriMinCellsFp64 = 64ull << 20;
+inline uint32_t default_steps_per_pass(uint64_t cells, bool fp64) {
+    return cells >= (fp64 ? kTriMinCellsFp64 : kTriMinCells) ? 3u : cells >= kPairMinCells ? 2u : 1u;
 }
 inline uint32_t default_pair_zchunk(uint64_t cells) { return cells >= kPairLongChunkCells ? 32u : 16u; }
 constexpr uint32_t kVariantTiles[] = {7, 3, 15, 1};
