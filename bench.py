@@ -1,7 +1,7 @@
 """Benchmark of the wave-solver step on one MI355X.
 
 A step is one time step of the FDTD run over the whole grid. FP32 grids of
-8 Mi cells and more (the hall; FP64 from 64 Mi) take three steps per pass:
+8 Mi cells and more (the hall; FP64 from 7 Mi) take three steps per pass:
 k_tri1 (cells three or more from a wall three times in one z-march, special
 cells' first step, receivers) and two launches of k_pair2 (special cells and the cells near the
 walls, second and third steps, receivers), so per-step figures are a third
