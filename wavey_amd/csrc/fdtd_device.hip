@@ -675,7 +675,6 @@ template <class T, class S>
 PairParams<T, S> make_pair_params(wv_plan* pl, const StepParams<T, S>& base) {
     PairParams<T, S> pp{};
     pp.a = base;
-    constexpr uint32_t BX = kTileX * kPairLane<T>;
     const uint32_t zch = pass_zchunk(pl, 2);
     pp.a.zchunk = zch;
     pp.dbits = pl->dbits.as<uint32_t>();
@@ -683,7 +682,10 @@ PairParams<T, S> make_pair_params(wv_plan* pl, const StepParams<T, S>& base) {
     pp.ring_cell = pl->ring_cell.as<uint32_t>();
     pp.n_special = pl->n_special;
     pp.n_ring = pl->n_ring;
-    pp.tiles_x = div_up(pl->pitch, BX);
+    // Tile k stores p^{n+2} on columns 2 + k * step .. (k + 1) * step + 1 and
+    // p^{n+1} one column further on each side: columns 2 .. nx - 3 (a deep
+    // cell is two from the grid's ends) and 1 .. nx - 2 (every interior cell).
+    pp.tiles_x = std::max<uint32_t>(1, div_up(pl->nx > 4 ? pl->nx - 4 : 0, kTileStep<T>));
     pp.row_blocks = div_up(pl->ny, kPairRows);
     pp.tile_blocks = pp.tiles_x * pp.row_blocks * div_up(pl->nz - 2, zch);
     pp.tile_launch = div_up(pp.tile_blocks / pp.tiles_x, 8) * 8 * pp.tiles_x;

@@ -14,8 +14,9 @@
 //    rows (the interior update; only interior cells are stored), shares the
 //    plane's rows through LDS, and one plane behind computes p^{n+2} of its
 //    deep cells: interior cells whose six neighbours are interior, so every
-//    value they need is an interior p^{n+1} the block has itself. The lanes
-//    at the tile ends also compute p^{n+1} of the cell next to the tile.
+//    value they need is an interior p^{n+1} the block has itself. Tiles step
+//    252 (FP64 124) columns and store p^{n+2} on columns 2 .. 253 of the
+//    tile, where it needs no p^{n+1} beyond the tile's ends.
 //  * Pass 1, special blocks: step n of every special cell (boundary solve,
 //    tap injection), p^{n+1} written to the p^{n+1} buffer; and the
 //    receivers' p^n.
@@ -62,6 +63,11 @@ struct PairParams {
 // fit two 16-wave blocks per CU but measured slower.
 template <typename T>
 constexpr int kPairLane = 16 / (int)sizeof(T);
+// Columns a pass-1 tile steps by: the tile less two columns at each end
+// (252 FP32, 124 FP64). A tile forms p^{n+1} on all its columns (from p^n of
+// the column beyond each end) and the later levels on fewer.
+template <typename T>
+constexpr uint32_t kTileStep = kTileX * kPairLane<T> - 4;
 
 template <typename T, typename S>
 __device__ __forceinline__ void pair_tile(const PairParams<T, S>& pp, uint32_t t) {
@@ -93,21 +99,32 @@ __device__ __forceinline__ void pair_tile(const PairParams<T, S>& pp, uint32_t t
     const uint32_t yc = (uint32_t)min(max(y, 0), ny - 1);
     const uint32_t ymc = (uint32_t)min(max(y - 1, 0), ny - 1);
     const uint32_t ypc = (uint32_t)min(max(y + 1, 0), ny - 1);
-    const uint32_t x0 = tx * BX;
+    constexpr uint32_t SX = kTileStep<T>;
+    const uint32_t x0 = tx * SX;
     const uint32_t xl = x0 + lane * W;
     const bool live = xl < a.pitch;
     const uint32_t x = live ? xl : a.pitch - W;
     const uint32_t col = yc * a.pitch + x, colm = ymc * a.pitch + x, colp = ypc * a.pitch + x;
-    // Lanes 0 and 63: the column next to the tile (xh) and the one beyond (xh2).
+    // Lanes 0 and 63: p^n of the column next to the tile (xh).
     const bool edge = lane == 0 || lane == kTileX - 1;
     const uint32_t xh = lane == 0 ? (x0 > 0 ? x0 - 1 : 0u) : min(x0 + BX, a.pitch - 1);
-    const uint32_t xh2 = lane == 0 ? (x0 > 1 ? x0 - 2 : 0u) : min(x0 + BX + 1, a.pitch - 1);
-    const uint32_t hcol = yc * a.pitch + xh, hcol2 = yc * a.pitch + xh2;
-    const uint32_t hcolm = ymc * a.pitch + xh, hcolp = ypc * a.pitch + xh;
+    const uint32_t hcol = yc * a.pitch + xh;
     const int z0 = 1 + (int)(zc * a.zchunk);
     const int z1 = min(z0 + (int)a.zchunk, nz - 1);
     const uint32_t full = (1u << W) - 1u;
-    const uint32_t lane_mask = (live && out_row) ? full : 0u;
+    // Stored cells of this lane in an output row: p^{n+2} on columns x0 + 2
+    // .. x0 + SX + 1 (each column of the grid in one tile), p^{n+1}, right on
+    // every column of the tile, also on x0 + 1 and x0 + SX + 2, so that
+    // columns 1 and nx - 2 are covered (the tiles beside write the same
+    // values there).
+    uint32_t own = 0, own1 = 0;
+#pragma unroll
+    for (int e = 0; e < W; ++e) {
+        own |= (uint32_t)(xl + e >= x0 + 2 && xl + e <= x0 + SX + 1) << e;
+        own1 |= (uint32_t)(xl + e >= x0 + 1 && xl + e <= x0 + SX + 2) << e;
+    }
+    const uint32_t lane_mask = (live && out_row) ? own : 0u;
+    const uint32_t lane_mask1 = (live && out_row) ? own1 : 0u;
     auto plane = [&](int z) -> uint32_t { return (uint32_t)min(max(z, 0), nz - 1) * a.sz; };
     // The chunk lies inside one 32-plane block of the z-packed bitmaps.
     uint32_t ib[W], db[W];
@@ -115,12 +132,12 @@ __device__ __forceinline__ void pair_tile(const PairParams<T, S>& pp, uint32_t t
     load_words<W>(pp.dbits + ((z0 - 1) >> 5) * a.sz + col, db);
 
     // A plane's loads: p^n of the row at zq + 1 and of the rows above and
-    // below at zq, p^{n-1} of the row at zq, and (edge lanes) the same around
-    // the halo column. They are issued one plane ahead, so they are in
-    // flight during the previous plane's compute and barrier.
+    // below at zq, p^{n-1} of the row at zq, and (edge lanes) p^n of the
+    // column next to the tile at zq + 1. They are issued one plane ahead, so
+    // they are in flight during the previous plane's compute and barrier.
     struct PlaneLoads {
         C zp, q, ym, yp;
-        T hp, hx2, hym, hyp, hq;
+        T hp;
     };
     auto issue = [&](int zq, PlaneLoads& L) {
         const uint32_t pz = plane(zq), pzp = plane(zq + 1);
@@ -128,30 +145,21 @@ __device__ __forceinline__ void pair_tile(const PairParams<T, S>& pp, uint32_t t
         L.q.load(a.prev + pz + col);
         L.ym.load(a.p + pz + colm);
         L.yp.load(a.p + pz + colp);
-        L.hp = L.hx2 = L.hym = L.hyp = L.hq = T(0);
-        if (edge) {
-            L.hp = a.p[pzp + hcol];
-            L.hx2 = a.p[pz + hcol2];
-            L.hym = a.p[pz + hcolm];
-            L.hyp = a.p[pz + hcolp];
-            L.hq = a.prev[pz + hcol];
-        }
+        L.hp = T(0);
+        if (edge) L.hp = a.p[pzp + hcol];
     };
 
-    // p^n of the row at planes zq - 1 and zq (and of the halo column).
+    // p^n of the row at planes zq - 1 and zq, and of the column next to the
+    // tile at zq.
     C zm, zcur;
     zm.load(a.p + plane(z0 - 2) + col);
     zcur.load(a.p + plane(z0 - 1) + col);
-    T hm = 0, hc = 0;
-    if (edge) {
-        hm = a.p[plane(z0 - 2) + hcol];
-        hc = a.p[plane(z0 - 1) + hcol];
-    }
-    // p^{n+1} of the row at planes zq - 2 and zq - 1, and of the halo column at zq - 1.
+    T hc = 0;
+    if (edge) hc = a.p[plane(z0 - 1) + hcol];
+    // p^{n+1} of the row at planes zq - 2 and zq - 1.
     C p1m, p1c;
 #pragma unroll
     for (int e = 0; e < W; ++e) p1m.v[e] = p1c.v[e] = T(0);
-    T e1c = 0;
     // zp - c at the plane below of the p^n and p^{n+1} rows (FP32), carried
     // from plane to plane.
     T dz1[W], dz2[W];
@@ -168,21 +176,14 @@ __device__ __forceinline__ void pair_tile(const PairParams<T, S>& pp, uint32_t t
         const PlaneLoads cur = nx;
         if (zq < z1) issue(zq + 1, nx);
         const uint32_t pz = plane(zq);
-        // p^{n+1} at plane zq: the row, and (edge lanes) the halo column.
+        // p^{n+1} at plane zq of the row.
         const T lo = from_lower_lane(zcur.v[W - 1], hc);
         const T hi = from_upper_lane(zcur.v[0], hc);
         C p1;
         row_update<T, W>(zcur.v, cur.q.v, lo, hi, cur.ym.v, cur.yp.v, zm.v, cur.zp.v, dz1, dz1, a.l2, a.c2, p1.v);
-        T e1 = 0;
-        if (edge) {
-            const T xin = lane == 0 ? zcur.v[0] : zcur.v[W - 1];
-            const T exm = lane == 0 ? cur.hx2 : xin;
-            const T exp = lane == 0 ? xin : cur.hx2;
-            e1 = interior_update(hc, cur.hq, exm, exp, cur.hym, cur.hyp, hm, cur.hp, a.l2, a.c2);
-        }
         p1.store(&rows[bw][wave][lane * W]);
         if (zq >= z0 && zq < z1) {
-            const uint32_t bits = plane_bits<W>(ib, (uint32_t)(zq - 1) & 31u) & lane_mask;
+            const uint32_t bits = plane_bits<W>(ib, (uint32_t)(zq - 1) & 31u) & lane_mask1;
             if (bits == full) {
                 p1.store(pp.p1 + pz + col);
             } else if (bits) {
@@ -202,8 +203,8 @@ __device__ __forceinline__ void pair_tile(const PairParams<T, S>& pp, uint32_t t
                 C ym1, yp1;
                 ym1.load(&rows[br][wave - 1][lane * W]);
                 yp1.load(&rows[br][wave + 1][lane * W]);
-                const T lo1 = from_lower_lane(p1c.v[W - 1], e1c);
-                const T hi1 = from_upper_lane(p1c.v[0], e1c);
+                const T lo1 = from_lower_lane(p1c.v[W - 1], T(0));  // the tile's end columns are
+                const T hi1 = from_upper_lane(p1c.v[0], T(0));      // not stored at this level
                 C p2;
                 row_update<T, W>(p1c.v, zm.v, lo1, hi1, ym1.v, yp1.v, p1m.v, p1.v, dz2, dz2, a.l2, a.c2, p2.v);
                 level2 = true;
@@ -224,10 +225,8 @@ __device__ __forceinline__ void pair_tile(const PairParams<T, S>& pp, uint32_t t
         }
         p1m = p1c;
         p1c = p1;
-        e1c = e1;
         zm = zcur;
         zcur = cur.zp;
-        hm = hc;
         hc = cur.hp;
         bw = bw == 2 ? 0u : bw + 1;
     }

@@ -44,7 +44,7 @@ constexpr int kTriRows = kPairWaves - 4;  // rows a pass-1 block owns (12)
 // the column beyond each end), p^{n+2} on all but its end columns and
 // p^{n+3} on the stride's columns 2 .. stride + 1 of the tile.
 template <typename T>
-constexpr uint32_t kTriStride = kTileX * kPairLane<T> - 4;
+constexpr uint32_t kTriStride = kTileStep<T>;
 
 template <typename T, typename S>
 struct TriParams {
