@@ -160,9 +160,10 @@ wv_status wv_plan_set_pass(wv_plan* plan, uint32_t steps_per_pass, uint32_t pair
  * [steps per pass, pass z-chunk, ring cells (interior, next to a
  * non-interior cell) and cells within two of a non-interior cell (three
  * steps per pass only) of the current run]. By default FP32 grids of 8 Mi
- * cells and more (FP64: 7 Mi) take three steps per pass (FP32 z-chunk: the
- * longest of 64, 32, 16 that gives 640 tile blocks; FP64 32), from 4 Mi
- * cells two (z-chunk 16, 32 from 64 Mi), smaller ones one. No reference
+ * cells and more (FP64: 7 Mi) take three steps per pass, from 4 Mi cells
+ * two, smaller ones one. Default z-chunk: FP32 three-step passes the
+ * longest of 64, 32, 16 that gives 640 tile blocks; FP64 three-step and
+ * every two-step pass 16 below 64 Mi cells and 32 from 64 Mi. No reference
  * counterpart. */
 wv_status wv_plan_pass_info(const wv_plan* plan, uint32_t out[4]);
 /* Copies `bytes` (at most one field buffer) between the plan's field

@@ -7,8 +7,6 @@
 * a tap on a solid cell and the same cell given twice (the reference injects
   into any cell and adds duplicate taps, fdtd.rs:718-720, gpu_fdtd.rs:67-75),
   with a receiver on the solid cell;
-* the hall at its full size: three (its default) and two steps per pass
-  bit-identical to single steps over 40 steps, every sample finite;
 * air up to the grid's x ends (a caller's grid without the builder's padding
   columns), so that cells at x = 1 and x = nx - 2 are interior: every pass
   mode bit-identical and FP64 against the oracle.
@@ -107,22 +105,6 @@ def test_solid_and_duplicate_taps(shoebox, prec, tol, spp):
         ir, _ = plan.run(taps, case.signal[:800], rx, mode=fdtd.MODE_IR)
     assert rel_l2(ir, ref) < tol
     assert rel_l2(ir[-1], ref[-1]) < tol  # the solid cell: its injections accumulate
-
-
-def test_hall_full_size_pass_modes_agree():
-    cfg = configs.hall()
-    print("case: hall 1002x627x377, 40 steps, fp32, one, two and three steps per pass", flush=True)
-    case = workload.build_case(cfg, steps=40)
-    taps = case.runs[0][2]
-    with fdtd.Plan(case.grid, "fp32") as plan:
-        assert plan.pass_info()["steps_per_pass"] == 3
-        three, _ = plan.run(taps, case.signal, case.receivers, mode=fdtd.MODE_IR)
-        plan.set_pass(2)
-        two, _ = plan.run(taps, case.signal, case.receivers, mode=fdtd.MODE_IR)
-        plan.set_pass(1)
-        one, _ = plan.run(taps, case.signal, case.receivers, mode=fdtd.MODE_IR)
-    assert np.all(np.isfinite(one))
-    assert np.array_equal(one, two) and np.array_equal(one, three)
 
 
 def without_x_padding(g):
