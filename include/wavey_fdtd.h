@@ -106,6 +106,28 @@ wv_status wv_fdtd_run(const wv_grid* grid, const wv_tap* taps, size_t n_taps, co
                       char* err, size_t err_len);
 
 /*
+ * wv_fdtd_run without any device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * beside HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Every device buffer of the run is carved from one static
+ * device array of 64 MiB per GPU, held from the library's load until the
+ * process ends: FP32 grids up to about 3.5 M cells (FP64 about 1.8 M), less
+ * the receiver record (steps x receivers x 4 or 8 bytes). A run whose
+ * estimated footprint is larger returns WV_ERR_LIMIT with a message.
+ * Arguments, results and status otherwise as wv_fdtd_run, with the same
+ * kernels, so results equal wv_fdtd_run's bit for bit. The first call on a
+ * device looks up the array's device address and creates that device's
+ * stream and events. Calls on one GPU share its array and are run one at a
+ * time (a per-device lock); the array keeps the last call's contents, and
+ * every run sets up all it reads.
+ */
+wv_status wv_fdtd_run_noalloc(const wv_grid* grid, const wv_tap* taps, size_t n_taps, const double* signal,
+                              size_t steps, const uint64_t* receivers, size_t n_recv, const double* freqs,
+                              size_t nf, int mode, int precision, const volatile int32_t* cancel,
+                              wv_progress_fn progress, void* ctx, double* spectra_out, double* ir_out,
+                              char* err, size_t err_len);
+
+/*
  * Plan form of the same run: the grid is uploaded and its device layout built
  * once, then many runs (source components, calibrations: run.rs:645-661,
  * fdtd.rs:941-942) reuse it. A plan is used by one thread at a time.

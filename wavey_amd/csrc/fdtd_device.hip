@@ -22,6 +22,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <unordered_map>
@@ -402,11 +403,13 @@ Geo geo_of(const wv_plan* pl) { return Geo{pl->npitched, pl->pitch, pl->nx, pl->
 
 void wv::plan_setup(wv_plan* pl, const wv_grid* g) {
     WV_HIP(hipGetDevice(&pl->device));
-    WV_HIP(hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking));
-    WV_HIP(hipEventCreate(&pl->ev[0]));
-    WV_HIP(hipEventCreate(&pl->ev[1]));
-    WV_HIP(hipEventCreate(&pl->t0));
-    WV_HIP(hipEventCreate(&pl->t1));
+    if (!pl->borrowed) {
+        WV_HIP(hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking));
+        WV_HIP(hipEventCreate(&pl->ev[0]));
+        WV_HIP(hipEventCreate(&pl->ev[1]));
+        WV_HIP(hipEventCreate(&pl->t0));
+        WV_HIP(hipEventCreate(&pl->t1));
+    }
     pl->nx = (uint32_t)g->n[0];
     pl->ny = (uint32_t)g->n[1];
     pl->nz = (uint32_t)g->n[2];
@@ -1101,6 +1104,119 @@ extern "C" wv_status wv_fdtd_run(const wv_grid* grid, const wv_tap* taps, size_t
                     spectra_out, ir_out, err, err_len);
     wv_plan_free(plan);
     return s;
+}
+
+// ------------------------------------------------ allocation-free entry point
+//
+// wv_fdtd_run_noalloc runs wv_fdtd_run's set-up and kernels with every device
+// buffer of the run (fields, bitmaps, special-cell records, slot state,
+// receiver record, spectra) carved from `scratch`, one static array per GPU,
+// so the call allocates no device memory. The first call on a device looks
+// up that device's address of the array and creates the device's stream and
+// events; they are kept until the process ends. Calls on one device share the
+// array and run one at a time (a per-device lock); every buffer the run reads
+// is written by the run's own set-up (fields, state and record by fill
+// kernels, marks and bitmaps by their kernels), so nothing depends on what
+// the last call left in the array.
+
+static constexpr size_t kNoAllocBytes = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+
+struct NoAllocDevice {
+    char* ptr = nullptr;  // this device's address of `scratch`
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    std::mutex mu;  // one call at a time on this device
+};
+
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+NoAllocDevice* g_noalloc[kMaxDevices] = {};
+
+// The current device's array address, stream and events, set up on the
+// device's first call.
+NoAllocDevice* noalloc_device() {
+    int dev = 0;
+    WV_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) throw DevErr{"device ordinal out of range", WV_ERR_DEVICE};
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc[dev]) {
+        auto d = std::make_unique<NoAllocDevice>();
+        void* ptr = nullptr;
+        WV_HIP(hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) throw DevErr{"no device address for the scratch array", WV_ERR_DEVICE};
+        d->ptr = static_cast<char*>(ptr);
+        WV_HIP(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
+        WV_HIP(hipEventCreate(&d->ev[0]));
+        WV_HIP(hipEventCreate(&d->ev[1]));
+        WV_HIP(hipEventCreate(&d->t0));
+        WV_HIP(hipEventCreate(&d->t1));
+        g_noalloc[dev] = d.release();
+    }
+    return g_noalloc[dev];
+}
+
+// Installs an arena on the calling thread for the lifetime of the object.
+struct ArenaScope {
+    explicit ArenaScope(Arena* a) { t_arena = a; }
+    ~ArenaScope() { t_arena = nullptr; }
+};
+
+}  // namespace
+
+extern "C" wv_status wv_fdtd_run_noalloc(const wv_grid* grid, const wv_tap* taps, size_t n_taps,
+                                         const double* signal, size_t steps, const uint64_t* receivers,
+                                         size_t n_recv, const double* freqs, size_t nf, int mode, int precision,
+                                         const volatile int32_t* cancel, wv_progress_fn progress, void* ctx,
+                                         double* spectra_out, double* ir_out, char* err, size_t err_len) {
+    if (precision != WV_FP32 && precision != WV_FP64 && precision != WV_FP32_STATE64) {
+        set_err(err, err_len, "wv_fdtd_run_noalloc: invalid precision");
+        return WV_ERR_INVALID;
+    }
+    wv_status s = check_grid(grid, err, err_len);
+    if (s != WV_OK) return s;
+    if ((nf && !freqs) || (n_taps && !taps) || (steps && !signal) || (n_recv && !receivers)) {
+        set_err(err, err_len, "wv_fdtd_run_noalloc: invalid argument");
+        return WV_ERR_INVALID;
+    }
+    const uint64_t need = estimate_bytes(grid, precision, steps, n_recv, n_taps);
+    if (need > kNoAllocBytes) {
+        set_err(err, err_len, "wv_fdtd_run_noalloc: the run needs about " + std::to_string((need + (1u << 20) - 1) >> 20) +
+                                  " MiB of device memory; the allocation-free entry point holds " +
+                                  std::to_string(kNoAllocBytes >> 20) + " MiB (use wv_fdtd_run)");
+        return WV_ERR_LIMIT;
+    }
+    return guarded(err, err_len, [&]() -> wv_status {
+        NoAllocDevice* d = noalloc_device();
+        std::lock_guard<std::mutex> lk(d->mu);
+        Arena arena;
+        arena.base = d->ptr;
+        arena.cap = kNoAllocBytes;
+        ArenaScope scope(&arena);
+        wv_plan pl;
+        pl.prec = precision;
+        pl.borrowed = true;
+        pl.stream = d->stream;
+        pl.ev[0] = d->ev[0];
+        pl.ev[1] = d->ev[1];
+        pl.t0 = d->t0;
+        pl.t1 = d->t1;
+        plan_setup(&pl, grid);
+        wv_status st = check_run_args(&pl, taps, n_taps, signal, steps, receivers, n_recv, err, err_len);
+        if (st != WV_OK) return st;
+        run_begin(&pl, taps, n_taps, signal, steps, receivers, n_recv);
+        switch (pl.prec) {
+            case WV_FP32: st = advance_impl<float, float>(&pl, steps, cancel, progress, ctx); break;
+            case WV_FP64: st = advance_impl<double, double>(&pl, steps, cancel, progress, ctx); break;
+            default: st = advance_impl<float, double>(&pl, steps, cancel, progress, ctx); break;
+        }
+        if (st != WV_OK) return st;
+        run_finish(&pl, freqs, nf, mode, spectra_out, ir_out);
+        return WV_OK;
+    });
 }
 
 extern "C" wv_status wv_plan_last_timing(const wv_plan* plan, double* ms, uint64_t* steps, uint64_t* launches) {

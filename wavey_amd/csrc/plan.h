@@ -28,25 +28,50 @@ struct DevErr {
                          e_ == hipErrorOutOfMemory ? WV_ERR_NOMEM : WV_ERR_DEVICE};               \
     } while (0)
 
-// Owned device allocation.
+// A bump arena over device memory the library already holds (the
+// allocation-free entry point's static array). While one is installed on the
+// calling thread, DBuf::alloc carves 256-byte-aligned pieces from it instead
+// of calling hipMalloc, and DBuf::release gives nothing back.
+struct Arena {
+    char* base = nullptr;
+    size_t cap = 0, used = 0;
+};
+inline thread_local Arena* t_arena = nullptr;
+
+// Device allocation: owned (hipMalloc / hipFree), or a piece of the arena
+// installed when it was allocated.
 struct DBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    bool owned = false;
     DBuf() = default;
     DBuf(const DBuf&) = delete;
     DBuf& operator=(const DBuf&) = delete;
     ~DBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p && owned) (void)hipFree(p);
         p = nullptr;
         bytes = 0;
+        owned = false;
     }
     void alloc(size_t n) {
         if (n <= bytes && p) return;
         release();
         size_t b = std::max<size_t>(n, 256);
+        if (Arena* a = t_arena) {
+            const size_t off = (a->used + 255) & ~(size_t)255;
+            if (off > a->cap || b > a->cap - off)
+                throw DevErr{"the run needs more than the " + std::to_string(a->cap >> 20) +
+                                 " MiB of device scratch the allocation-free entry point holds",
+                             WV_ERR_LIMIT};
+            p = a->base + off;
+            a->used = off + b;
+            bytes = b;
+            return;
+        }
         WV_HIP(hipMalloc(&p, b));
         bytes = b;
+        owned = true;
     }
     template <class T>
     T* as() const { return static_cast<T*>(p); }
@@ -98,11 +123,17 @@ struct wv_plan {
     double last_ms = 0.0;
     uint64_t last_steps = 0, last_launches = 0;
 
+    // The stream and events belong to someone else (the allocation-free
+    // entry point's per-device set): plan_setup creates none, and they
+    // outlive the plan.
+    bool borrowed = false;
+
     size_t tsize() const { return prec == WV_FP64 ? 8 : 4; }
     size_t ssize() const { return prec == WV_FP32 ? 4 : 8; }
     uint32_t lane_cells() const { return (uint32_t)(16 / tsize()); }
     ~wv_plan() {
         if (stream) (void)hipStreamSynchronize(stream);
+        if (borrowed) return;
         for (auto e : {ev[0], ev[1], t0, t1})
             if (e) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);

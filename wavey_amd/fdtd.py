@@ -110,7 +110,7 @@ class WvWaveSource(C.Structure):
 
 
 EXPORTS = [
-    "wv_fdtd_run", "wv_plan_create", "wv_plan_run", "wv_run_begin", "wv_run_advance", "wv_run_finish",
+    "wv_fdtd_run", "wv_fdtd_run_noalloc", "wv_plan_create", "wv_plan_run", "wv_run_begin", "wv_run_advance", "wv_run_finish",
     "wv_plan_last_timing", "wv_plan_stats", "wv_plan_set_tile", "wv_plan_set_pass", "wv_plan_pass_info", "wv_plan_copy_probe", "wv_plan_free", "wv_grid_build_box",
     "wv_built_grid_view", "wv_grid_free", "wv_source_signal", "wv_nearest_air", "wv_taps_at",
     "wv_source_components", "wv_component_weights", "wv_to_transfer", "wv_admittance_eval", "wv_paris",
@@ -141,6 +141,8 @@ def lib():
             L.wv_fdtd_run.argtypes = [P(WvGrid), P(WvTap), sz, dp, sz, P(C.c_uint64), sz, dp, sz, C.c_int, C.c_int,
                                       P(C.c_int32), PROGRESS_FN, C.c_void_p, dp, dp] + errbuf
             L.wv_fdtd_run.restype = st
+            L.wv_fdtd_run_noalloc.argtypes = L.wv_fdtd_run.argtypes
+            L.wv_fdtd_run_noalloc.restype = st
             L.wv_plan_create.argtypes = [P(WvGrid), C.c_int, P(C.c_void_p)] + errbuf
             L.wv_plan_create.restype = st
             L.wv_plan_run.argtypes = [C.c_void_p, P(WvTap), sz, dp, sz, P(C.c_uint64), sz, dp, sz, C.c_int,
@@ -690,8 +692,7 @@ class Gpu:
         return None if out is None else out[0]
 
 
-def fdtd_run(g, taps, signal, receivers, freqs=(), mode=MODE_BOTH, precision="fp32", cancel=None, progress=None):
-    """The one-call C entry point wv_fdtd_run."""
+def _one_call(fn, g, taps, signal, receivers, freqs, mode, precision, cancel, progress):
     sig = np.ascontiguousarray(signal, dtype=np.float64)
     rc = np.ascontiguousarray(receivers, dtype=np.uint64)
     fr = np.ascontiguousarray(freqs, dtype=np.float64)
@@ -701,14 +702,28 @@ def fdtd_run(g, taps, signal, receivers, freqs=(), mode=MODE_BOTH, precision="fp
     err = C.create_string_buffer(1024)
     cb = PROGRESS_FN(lambda f, _ctx: progress(f)) if progress else PROGRESS_FN()
     prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
-    st = lib().wv_fdtd_run(C.byref(g.view()), _taps_array(taps), len(taps), _dp(sig), T,
-                           rc.ctypes.data_as(C.POINTER(C.c_uint64)), R, _dp(fr) if F else None, F, mode, prec,
-                           cancel.ctypes.data_as(C.POINTER(C.c_int32)) if cancel is not None else None, cb, None,
-                           _dp(spec), _dp(ir), err, 1024)
+    st = fn(C.byref(g.view()), _taps_array(taps), len(taps), _dp(sig), T,
+            rc.ctypes.data_as(C.POINTER(C.c_uint64)), R, _dp(fr) if F else None, F, mode, prec,
+            cancel.ctypes.data_as(C.POINTER(C.c_int32)) if cancel is not None else None, cb, None,
+            _dp(spec), _dp(ir), err, 1024)
     _check(st, err)
     if st == WV_CANCELLED:
         return None
     return ir, (spec[0::2] + 1j * spec[1::2])[: R * F].reshape(R, F)
+
+
+def fdtd_run(g, taps, signal, receivers, freqs=(), mode=MODE_BOTH, precision="fp32", cancel=None, progress=None):
+    """The one-call C entry point wv_fdtd_run."""
+    return _one_call(lib().wv_fdtd_run, g, taps, signal, receivers, freqs, mode, precision, cancel, progress)
+
+
+def fdtd_run_noalloc(g, taps, signal, receivers, freqs=(), mode=MODE_BOTH, precision="fp32", cancel=None,
+                     progress=None):
+    """wv_fdtd_run_noalloc: the same run with no device allocation (every
+    device buffer in the library's 64 MiB static array; larger runs raise
+    FdtdError with WV_ERR_LIMIT)."""
+    return _one_call(lib().wv_fdtd_run_noalloc, g, taps, signal, receivers, freqs, mode, precision, cancel,
+                     progress)
 
 
 def footprint_estimate(grid, precision, steps, n_recv, n_taps):
