@@ -17,8 +17,13 @@ synchronize on both sides).
 Prints one JSON line with the metric, the roofline of the step kernel
 (algorithmic bytes per launch / measured launch time, HIP events on the
 kernel's stream; HBM traffic per launch from two rocprofv3 counter passes of
-a short run, when rocprofv3 is installed) and a CPU baseline (the FP64
-oracle, threaded, on a bounded sample of the same grid).
+a short run, when rocprofv3 is installed), a CPU baseline (the FP64 oracle,
+threaded, on a bounded sample of the same grid), the FP32 receiver error
+against an FP64 run of the same inputs, and the second half of the metric:
+the batched form on the 48-room uneven mix against sequential plain runs
+(`room_mix`) and the served Poisson trace, loaded at 80 % of the mix's
+measured throughput, with its p50/p95/p99 latency against a sequential
+plain queue (`serving`).
 """
 import argparse
 import glob
@@ -50,35 +55,63 @@ def algorithmic_bytes(st, n_recv, n_taps, fp64=False):
             + 4 * f * n_recv + 8 * n_taps)
 
 
-def cpu_baseline(cfg, case, budget_s=15.0, threads=None):
-    """The FP64 oracle (restatement of fdtd.rs:644-725, z-slab threaded) on the
-    same grid, from rest. Runs of n1 and n2 steps pay the same set-up (field
-    allocation and first touch), so the step rate is the difference quotient;
-    n2 is sized to about budget_s of stepping."""
-    import oracle
+def host_cpu():
+    """CPU model, logical CPUs of the host and of this process's affinity."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count() or 1
+    return model, os.cpu_count() or 1, affinity
 
-    threads = threads or min(16, os.cpu_count() or 1)
+
+def cpu_threads():
+    """Every CPU of this process's affinity, within the host's CPU share for
+    one GPU (OMP_NUM_THREADS where it is set, else 16)."""
+    _, _, affinity = host_cpu()
+    share = int(os.environ.get("OMP_NUM_THREADS", "16") or 16)
+    return max(1, min(affinity, share))
+
+
+def cpu_baseline(cfg, case, n1=20, n2=170, threads=None):
+    """The FP64 oracle (restatement of fdtd.rs:644-725, z-slab threaded) on the
+    same grid, taps and receivers, from rest, for n2 steps and for n1: both
+    pay the same set-up (field allocation and first touch), so the step rate
+    is their difference quotient over n2 - n1 steps. The signal is the
+    workload's source pulse at that length, whatever the GPU run's length."""
+    import oracle
+    from wavey_amd import fdtd
+
+    threads = threads or cpu_threads()
     og = oracle.build_box(cfg["room"], cfg["c"], cfg["dx"])
     taps = case.runs[0][2]
     rx = case.receivers
-    sig = case.signal
+    sig = fdtd.source_signal(cfg["f_src"], og.dt, n2)
 
     def timed(n):
         t0 = time.perf_counter()
         oracle.run(og, taps, sig[:n], rx, threads=threads)
         return time.perf_counter() - t0
 
-    n1 = 2
     t1 = timed(n1)
-    t_probe = max(timed(2 * n1) - t1, 1e-6) / n1
-    n2 = int(min(len(sig), max(2 * n1 + 1, n1 + budget_s / t_probe)))
     t2 = timed(n2)
     dt = max(t2 - t1, 1e-9)
     air = int(np.count_nonzero(og.flags & 64))
+    model, nproc, affinity = host_cpu()
     return {"value": round(air * (n2 - n1) / dt / 1e9, 4), "unit": "Gcell-updates/s", "cores": threads,
             "kind": "port",
-            "sample": f"{cfg['name']} grid, FP64 oracle, {threads} threads: runs of {n2} and {n1} steps from rest, "
-                      f"{n2 - n1} steps in {dt:.1f} s (difference of the two runs)"}
+            "sample": f"{cfg['name']} grid ({air} air cells), FP64 oracle, {threads} threads: runs of {n2} and "
+                      f"{n1} steps from rest, {n2 - n1} steps in {dt:.1f} s (difference of the two runs), "
+                      f"extrapolated per step",
+            "cpu_model": model, "nproc": nproc, "affinity_cpus": affinity}
 
 
 def _counter_kib(db_dir, counter, kernel):
@@ -116,7 +149,7 @@ def measure_traffic(args, copy_bytes):
     for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
         d = tempfile.mkdtemp(prefix=f"wv_{ctr.lower()}_", dir="/tmp")
         cmd = [exe, "--pmc", ctr, "-d", d, "-o", "run", "--", sys.executable, os.path.abspath(__file__),
-               "--no-cpu", "--no-traffic", "--steps", str(TRAFFIC_STEPS), "--warmup", "0", "--workload", args.workload,
+               "--no-cpu", "--no-traffic", "--no-mix", "--precision-steps", "0", "--steps", str(TRAFFIC_STEPS), "--warmup", "0", "--workload", args.workload,
                "--precision", args.precision, "--zchunk", str(args.zchunk), "--variant", str(args.variant),
                "--steps-per-pass", str(args.steps_per_pass), "--pair-zchunk", str(args.pair_zchunk)]
         print(f"bench: counter pass {ctr}", file=sys.stderr, flush=True)
@@ -162,9 +195,10 @@ def bench_room_mix(args):
     speakers one after another) against one batch (all instances in one
     launch per step, autotuned shapes, 128-step blocks as HIP graphs). Every
     batched IR is compared with its plain IR (bit-identical expected); a
-    sample of instances is also run in FP64 for the FP32 error. Also timed:
-    one wv_fdtd_run call per solve (the device plan built and freed every
-    call, as the reference calls Gpu::run_fdtd once per component run)."""
+    sample of instances is also run in FP64 for the FP32 error. With
+    args.oneshot also timed: one wv_fdtd_run call per solve (the device plan
+    built and freed every call, as the reference calls Gpu::run_fdtd once
+    per component run). Returns the result line."""
     from wavey_amd import configs, fdtd, workload
 
     rooms = configs.room_mix(n_rooms=args.rooms)
@@ -190,13 +224,14 @@ def bench_room_mix(args):
     print(f"bench: sequential plain {t_plain:.2f} s", file=sys.stderr, flush=True)
 
     # One wv_fdtd_run call per solve.
-    t0 = time.perf_counter()
-    oneshot_same = 0
-    for k, (_, c, taps) in enumerate(inst):
-        ir, _ = fdtd.fdtd_run(c.grid, taps, c.signal, c.receivers, mode=fdtd.MODE_IR, precision=args.precision)
-        oneshot_same += int(np.array_equal(ir, plain[k]))
-    t_oneshot = time.perf_counter() - t0
-    print(f"bench: sequential wv_fdtd_run calls {t_oneshot:.2f} s", file=sys.stderr, flush=True)
+    t_oneshot, oneshot_same = None, 0
+    if args.oneshot:
+        t0 = time.perf_counter()
+        for k, (_, c, taps) in enumerate(inst):
+            ir, _ = fdtd.fdtd_run(c.grid, taps, c.signal, c.receivers, mode=fdtd.MODE_IR, precision=args.precision)
+            oneshot_same += int(np.array_equal(ir, plain[k]))
+        t_oneshot = time.perf_counter() - t0
+        print(f"bench: sequential wv_fdtd_run calls {t_oneshot:.2f} s", file=sys.stderr, flush=True)
 
     # One batch.
     t0 = time.perf_counter()
@@ -243,9 +278,9 @@ def bench_room_mix(args):
                    "graphs": not args.no_graphs, "autotune": not args.no_autotune},
         "sequential_plain": {"solves_per_s": round(n / t_plain, 3), "seconds": round(t_plain, 3),
                              "device_step_loop_s": round(dev_ms / 1e3, 3)},
-        "sequential_fdtd_run_calls": {"solves_per_s": round(n / t_oneshot, 3), "seconds": round(t_oneshot, 3),
-                                      "identical_to_plain": f"{oneshot_same}/{n}",
-                                      "speedup_of_batch": round(t_oneshot / t_batch, 3)},
+        "sequential_fdtd_run_calls": None if t_oneshot is None else {
+            "solves_per_s": round(n / t_oneshot, 3), "seconds": round(t_oneshot, 3),
+            "identical_to_plain": f"{oneshot_same}/{n}", "speedup_of_batch": round(t_oneshot / t_batch, 3)},
         "batched": {"seconds": round(t_batch, 3), "add_s": round(t_add, 3), "autotune_s": round(t_tune, 3),
                     "run_s": round(t_run, 3), "readback_s": round(t_read, 3),
                     "device_run_s": round(bt["device_ms"] / 1e3, 3),
@@ -256,12 +291,13 @@ def bench_room_mix(args):
         "speedup_step_loop_only": round(t_plain / max(t_run, 1e-9), 3),
         "parity": {"bit_identical_to_plain": f"{identical}/{n}", "max_abs_diff": max_diff,
                    "fp32_vs_fp64_rel_l2_max": max(errs) if errs else None, "fp64_sample": len(errs)},
+        "end_to_end_gcell_per_s": round(work / t_batch / 1e9, 2),
         "setup_s": {"grid_build": round(t_build, 2)},
     }
-    print(json.dumps(line), flush=True)
+    return line
 
 
-def bench_serving(args):
+def bench_serving(args, throughput=None):
     """Poisson arrivals of room-mix requests with padded cells log-uniform in
     0.2M-60M and 0.3-1.0 s of simulated time, at the rate that offers
     args.load of the given batched throughput (cell-updates/s). The server
@@ -269,7 +305,9 @@ def bench_serving(args):
     runs of the same trace: the plain latencies are those of a FIFO single
     queue with the measured per-request times and the same arrivals. With
     args.burst every request arrives at once: solves/s is then the capacity
-    of each form (all 40 requests from first submission to last result)."""
+    of each form (all 40 requests from first submission to last result).
+    `throughput` (Gcell-updates/s) overrides args.throughput. Returns the
+    result line."""
     import threading
 
     from wavey_amd import configs, fdtd, workload
@@ -279,7 +317,8 @@ def bench_serving(args):
     cases = [workload.build_case(cfg) for _, cfg in base]
     t_build = time.perf_counter() - t0
     works = [c.grid.air_cells() * len(c.signal) for c in cases]
-    rate = args.load * args.throughput * 1e9 / float(np.mean(works))
+    throughput = throughput or args.throughput
+    rate = args.load * throughput * 1e9 / float(np.mean(works))
     arrivals = [0.0] * len(base) if args.burst else [t / rate for t, _ in base]
     print(f"bench: serving {len(cases)} requests, {sum(works) / 1e12:.2f} T cell-updates, rate {rate:.2f}/s, "
           f"built in {t_build:.1f} s", file=sys.stderr, flush=True)
@@ -353,7 +392,7 @@ def bench_serving(args):
         "config": {"workload": f"serving trace: {n} requests, padded cells log-uniform 0.2M-60M, 0.3-1.0 s, "
                                + ("dx 0.05 m, all submitted at once" if args.burst else
                                   f"dx 0.05 m, Poisson rate {rate:.3f}/s = {args.load:.0%} of "
-                                  f"{args.throughput} Gcell-updates/s"),
+                                  f"{throughput:.1f} Gcell-updates/s"),
                    "policy": args.policy, "fill_cells": args.fill or 16 << 20,
                    "lane_cells": args.lane_cells or 8 << 20,
                    "requests": n, "cell_updates": int(sum(works)), "cap_bytes": int(args.cap_gb * 1e9)},
@@ -370,7 +409,7 @@ def bench_serving(args):
                    "fp32_vs_fp64_rel_l2_max": max(errs) if errs else None, "fp64_sample": len(errs)},
         "setup_s": {"grid_build": round(t_build, 2)},
     }
-    print(json.dumps(line), flush=True)
+    return line
 
 
 def bench_error(args, cfg):
@@ -399,7 +438,33 @@ def bench_error(args, cfg):
             "config": {"workload": f"{cfg['name']} {'x'.join(str(v) for v in case.grid.n)} cells, "
                                    f"{args.error_steps} steps, {len(case.receivers)} receivers, one source component"},
             "tolerance": 1e-4, **out}
-    print(json.dumps(line), flush=True)
+    return line
+
+
+def precision_check(case, taps, ir32, steps, precision):
+    """The timed run's receiver series against an FP64 device run of the same
+    steps, and a longer FP32/FP64 pair over `steps` steps of the same grid,
+    taps and receivers (the timed run is too short for the pulse to reach
+    most receivers). Relative L2 over all receivers and the worst receiver."""
+    from wavey_amd import fdtd
+
+    def errs(a, ref):
+        per = [rel_l2(a[r], ref[r]) for r in range(len(ref)) if np.linalg.norm(ref[r]) > 0]
+        return {"rel_l2_all": rel_l2(a, ref), "rel_l2_max_receiver": max(per) if per else None,
+                "receivers_reached": len(per)}
+
+    out = {"tolerance": 1e-4, "reference": "FP64 device run of the same scheme and inputs (itself within "
+                                           "1e-9 of the FP64 CPU oracle in the parity tests)"}
+    sig_long = fdtd.source_signal(case.cfg["f_src"], case.grid.dt, steps)
+    irs = {}
+    for prec in ("fp64", precision):
+        with fdtd.Plan(case.grid, prec) as plan:
+            if prec == "fp64":
+                short, _ = plan.run(taps, case.signal, case.receivers, mode=fdtd.MODE_IR)
+            irs[prec], _ = plan.run(taps, sig_long, case.receivers, mode=fdtd.MODE_IR)
+    out["timed_run"] = {"steps": len(case.signal), **errs(ir32, short)}
+    out["long_run"] = {"steps": steps, **errs(irs[precision], irs["fp64"])}
+    return out
 
 
 def main():
@@ -436,6 +501,11 @@ def main():
     ap.add_argument("--variant", type=int, default=-1)
     ap.add_argument("--steps-per-pass", type=int, default=0, help="1, 2 or 3; 0 = the plan's default for the grid")
     ap.add_argument("--pair-zchunk", type=int, default=0, help="z-chunk of multi-step passes; 0 = the default")
+    ap.add_argument("--oneshot", action="store_true", help="room_mix: also time one wv_fdtd_run call per solve")
+    ap.add_argument("--no-mix", action="store_true",
+                    help="hall line: leave out the room-mix batch and the serving trace")
+    ap.add_argument("--precision-steps", type=int, default=1500,
+                    help="hall line: steps of the FP32-against-FP64 run (0 = none)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-traffic", action="store_true", help="skip the HBM counter passes")
     args = ap.parse_args()
@@ -445,9 +515,11 @@ def main():
         sys.exit(2)
 
     if args.workload == "room_mix":
-        return bench_room_mix(args)
+        print(json.dumps(bench_room_mix(args)), flush=True)
+        return
     if args.workload == "serving":
-        return bench_serving(args)
+        print(json.dumps(bench_serving(args)), flush=True)
+        return
 
     from wavey_amd import configs, fdtd, workload
 
@@ -458,7 +530,8 @@ def main():
     else:
         cfg = configs.shoebox(fdtd.fit_real_impedance)
     if args.error_steps:
-        return bench_error(args, cfg)
+        print(json.dumps(bench_error(args, cfg)), flush=True)
+        return
     total = args.warmup + args.steps
     t0 = time.perf_counter()
     case = workload.build_case(cfg, steps=max(total, 1), builder=args.grid_builder)
@@ -526,12 +599,22 @@ def main():
         "setup_s": {"grid_build": round(t_build, 2), "grid_builder": args.grid_builder,
                     "upload_and_layout": round(t_setup, 2)},
     }
+    if args.precision_steps and args.precision != "fp64":
+        t0 = time.perf_counter()
+        line["fp32_vs_fp64"] = precision_check(case, taps, ir, args.precision_steps, args.precision)
+        print(f"bench: precision check {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     if not args.no_traffic:
         traffic, detail = measure_traffic(args, copy_bytes)
         line["roofline"]["traffic"] = traffic
         line["roofline"]["traffic_detail"] = detail
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(cfg, case)
+    if not args.no_mix and args.workload == "hall":
+        # The batched form on the uneven room mix, then the serving trace loaded
+        # at args.load of the batch's measured end-to-end throughput.
+        mix = bench_room_mix(args)
+        line["room_mix"] = mix
+        line["serving"] = bench_serving(args, throughput=mix["end_to_end_gcell_per_s"])
     print(json.dumps(line), flush=True)
 
 
