@@ -292,10 +292,12 @@ void wv_serve_free(wv_server* server);
 /* -------------------------------------------------------------- wave job
  *
  * The wave solver's part of a wave job (run.rs:593-661): for each source, its
- * runs on one grid (one component for an omni source's taps, four for a
- * first-order directional source: fdtd.rs:730-743), each run's spectra turned
- * into a transfer function (to_transfer, fdtd.rs:819-840) and summed with the
- * source's per-frequency complex component weights (fdtd.rs:812-815).
+ * runs on one grid (one component for an omni or calibrated source's taps,
+ * four for a first-order directional source: fdtd.rs:730-743; see
+ * wv_wave_source_taps), each run's spectra turned into a transfer function
+ * (to_transfer, fdtd.rs:819-840), divided by the cabinet calibration's
+ * on-axis response where the source has one, and summed with the source's
+ * per-frequency complex component weights (fdtd.rs:812-815).
  */
 typedef struct {
     const wv_tap* taps;        /* every component's taps, back to back */
@@ -304,6 +306,8 @@ typedef struct {
     const double* weights;     /* [nf][n_comps] complex (re, im); NULL for a single component (weight 1) */
     const double* h0;          /* [nf] complex source spectrum; NULL for 1 */
     double r_ref;              /* reference distance of the source's sensitivity, m */
+    const double* on_axis;     /* [nf] complex: the cabinet calibration's on-axis response, which every
+                                  run's transfer function is divided by (run.rs:651-656); NULL for none */
 } wv_wave_source;
 /* transfer_out: [n_sources][n_recv][nf] complex (re, im). batched = 0 runs
  * the components one after another on one plan (the reference's order),
@@ -438,6 +442,24 @@ wv_status wv_nearest_air(const wv_grid* g, const double p[3], uint64_t* cell);
 /* fdtd.rs:497-512: where a driver on a cabinet's baffle meets the air, from
  * p (inside the cabinet) along axis; p itself when no air is met within 2 m. */
 void wv_baffle_point(const wv_grid* g, const double p[3], const double axis[3], double out[3]);
+/* fdtd.rs:516-533 (FdtdGrid::air_along): the first air node along axis from
+ * p (steps of dx/2, up to 2 m), else the nearest air node; WV_ERR_INVALID
+ * when neither exists (None). */
+wv_status wv_air_along(const wv_grid* g, const double p[3], const double axis[3], uint64_t* cell);
+/* run.rs:601-624: a wave-job source's runs. The source node: with a cabinet
+ * calibration (calibrated = 1) the first air node along the axis, else the
+ * nearest air node, else the first along the axis. A calibrated source, or
+ * one of order 0, is one component: the trilinear taps (taps_at) of the
+ * driver point (calibrated: baffle_point in front of the acoustic centre;
+ * else the centre), or the node alone. Order 1: source_components of the
+ * node, or the node alone when it is too close to a surface
+ * (*omni_fallback = 1, the reference's note). *n_comps = 1: comp_len[0]
+ * (<= 8) taps from comps[0]; *n_comps = 4: component c at comps[2c] ..
+ * comps[2c + comp_len[c]) as wv_source_components. WV_ERR_INVALID when the
+ * source is not inside the grid (SolverError::Invalid). */
+wv_status wv_wave_source_taps(const wv_grid* g, const double center[3], const double axis[3], int calibrated,
+                              uint32_t order, wv_tap comps[8], uint32_t comp_len[4], uint32_t* n_comps,
+                              int32_t* omni_fallback);
 /* fdtd.rs:467-492; *n_taps <= 8; WV_ERR_INVALID when None. */
 wv_status wv_taps_at(const wv_grid* g, const double p[3], wv_tap taps[8], size_t* n_taps);
 /* fdtd.rs:730-743: comps[4][2], comp_len[4], *n_comps = 1 or 4. None -> WV_ERR_INVALID. */

@@ -1002,6 +1002,63 @@ void ob_baffle_point(const ob_grid* g, const double p[3], const double axis[3], 
     for (int i = 0; i < 3; ++i) out[i] = p[i];
 }
 
+/* fdtd.rs:516-533 (FdtdGrid::air_along): from p along axis in steps of
+ * dx/2 (up to 2 m) the first point whose rounded node is air, else the
+ * nearest air node; -1 for None. */
+int64_t ob_air_along(const ob_grid* g, const double p[3], const double axis[3]) {
+    const double step = g->dx * 0.5;
+    const int64_t kmax = (int64_t)(2.0 / step);
+    for (int64_t k = 0; k < kmax; ++k) {
+        double r[3];
+        int neg = 0;
+        for (int i = 0; i < 3; ++i) {
+            double q = p[i] + axis[i] * ((double)k * step);
+            r[i] = round((q - g->origin[i]) / g->dx);
+            neg |= r[i] < 0.0;
+        }
+        if (neg) continue;
+        int64_t x = (int64_t)r[0], y = (int64_t)r[1], z = (int64_t)r[2];
+        if (x < g->n[0] && y < g->n[1] && z < g->n[2] && (g->flags[gidx(g, x, y, z)] & AIR)) return gidx(g, x, y, z);
+    }
+    return ob_nearest_air(g, p);
+}
+
+/* run.rs:601-624: the source node (calibrated: air_along first), the driver
+ * point (calibrated: baffle_point, else the centre), and the component taps:
+ * a calibrated or order-0 source is taps_at(driver) (or the node alone);
+ * otherwise source_components(node, order), or the node alone (*fallback =
+ * 1). cells/w are [8]; len[4]. Returns the number of components, 0 when no
+ * air node is found. */
+int ob_wave_source_taps(const ob_grid* g, const double center[3], const double axis[3], int calibrated, int order,
+                        int64_t* cells, double* w, int* len, int* fallback) {
+    int64_t node = calibrated ? ob_air_along(g, center, axis) : -1;
+    if (node < 0) node = ob_nearest_air(g, center);
+    if (node < 0) node = ob_air_along(g, center, axis);
+    if (node < 0) return 0;
+    *fallback = 0;
+    if (calibrated || order == 0) {
+        double driver[3] = {center[0], center[1], center[2]};
+        if (calibrated) ob_baffle_point(g, center, axis, driver);
+        int nt = ob_taps_at(g, driver, cells, w);
+        if (nt == 0) {
+            cells[0] = node;
+            w[0] = 1.0;
+            nt = 1;
+        }
+        len[0] = nt;
+        return 1;
+    }
+    int nc = ob_source_components(g, node, order, cells, w, len);
+    if (nc == 0) {
+        cells[0] = node;
+        w[0] = 1.0;
+        len[0] = 1;
+        *fallback = 1;
+        nc = 1;
+    }
+    return nc;
+}
+
 /* fdtd.rs:847-862: power-complementary crossfade of two complex fields. */
 void ob_blend(const double ism[2], const double wave[2], double w, double out[2]) {
     if (w <= 0.0) { out[0] = ism[0]; out[1] = ism[1]; return; }

@@ -79,6 +79,12 @@ void ob_source_signal(double f_max, double dt, int64_t steps, double* out);
 int64_t ob_nearest_air(const ob_grid* g, const double p[3]);
 /* fdtd.rs:497-512 (FdtdGrid::baffle_point). */
 void ob_baffle_point(const ob_grid* g, const double p[3], const double axis[3], double out[3]);
+/* fdtd.rs:516-533 (FdtdGrid::air_along); -1 for None. */
+int64_t ob_air_along(const ob_grid* g, const double p[3], const double axis[3]);
+/* run.rs:601-624: a wave-job source's component taps (cells/w [8], len[4]);
+ * returns the number of components, 0 when the source is outside the grid. */
+int ob_wave_source_taps(const ob_grid* g, const double center[3], const double axis[3], int calibrated, int order,
+                        int64_t* cells, double* w, int* len, int* fallback);
 /* fdtd.rs:847-862 (blend); complex values as (re, im). */
 void ob_blend(const double ism[2], const double wave[2], double w, double out[2]);
 /* fdtd.rs:467-492; returns the number of taps (0 = None). */

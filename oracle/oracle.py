@@ -133,6 +133,11 @@ def lib():
         L.ob_build_room.restype = C.c_int
         L.ob_baffle_point.argtypes = [P(ObGrid), P(C.c_double), P(C.c_double), P(C.c_double)]
         L.ob_baffle_point.restype = None
+        L.ob_air_along.argtypes = [P(ObGrid), P(C.c_double), P(C.c_double)]
+        L.ob_air_along.restype = C.c_int64
+        L.ob_wave_source_taps.argtypes = [P(ObGrid), P(C.c_double), P(C.c_double), C.c_int, C.c_int, P(C.c_int64),
+                                          P(C.c_double), P(C.c_int), P(C.c_int)]
+        L.ob_wave_source_taps.restype = C.c_int
         L.ob_blend.argtypes = [P(C.c_double), P(C.c_double), C.c_double, P(C.c_double)]
         L.ob_blend.restype = None
         L.ob_calibrate_cabinet.argtypes = [P(C.c_double), C.c_int32, P(C.c_int32), C.c_int32, P(C.c_double), C.c_double,
@@ -416,6 +421,34 @@ def source_components(grid, c, order):
     if nc == 0:
         return None
     return [[(int(cc[2 * k + t]), float(cw[2 * k + t])) for t in range(ln[k])] for k in range(nc)]
+
+
+def air_along(grid, p, axis):
+    """fdtd.rs:516-533; None when no air node is found."""
+    g = grid._obgrid()
+    pp = np.ascontiguousarray(p, dtype=np.float64)
+    ax = np.ascontiguousarray(axis, dtype=np.float64)
+    i = lib().ob_air_along(C.byref(g), _ptr(pp, C.c_double), _ptr(ax, C.c_double))
+    return None if i < 0 else int(i)
+
+
+def wave_source_taps(grid, center, axis, calibrated, order):
+    """run.rs:601-624: (component taps, omni fallback) or None."""
+    g = grid._obgrid()
+    pp = np.ascontiguousarray(center, dtype=np.float64)
+    ax = np.ascontiguousarray(axis, dtype=np.float64)
+    cc = np.zeros(8, dtype=np.int64)
+    cw = np.zeros(8)
+    ln = np.zeros(4, dtype=np.int32)
+    fb = np.zeros(1, dtype=np.int32)
+    nc = lib().ob_wave_source_taps(C.byref(g), _ptr(pp, C.c_double), _ptr(ax, C.c_double), int(calibrated),
+                                   int(order), _ptr(cc, C.c_int64), _ptr(cw, C.c_double), _ptr(ln, C.c_int),
+                                   _ptr(fb, C.c_int))
+    if nc == 0:
+        return None
+    if nc == 1:
+        return [[(int(cc[t]), float(cw[t])) for t in range(ln[0])]], bool(fb[0])
+    return [[(int(cc[2 * k + t]), float(cw[2 * k + t])) for t in range(ln[k])] for k in range(nc)], bool(fb[0])
 
 
 def component_weights(a0, a, k, dx):

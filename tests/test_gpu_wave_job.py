@@ -5,6 +5,12 @@ component weights (fdtd.rs:812-815), summed per source.
 
 FP64 device against the oracle: 1e-9 relative L2 over the transfer
 functions; FP32: 1e-4. The batched and one-plan forms are bit-identical.
+
+A calibrated cabinet source (run.rs:601-656): the driver's taps at the
+baffle point in front of the acoustic centre (source node from air_along,
+fdtd.rs:516-533) in a room holding the cabinet as a solid, each transfer
+function divided by the calibration's on-axis response; against the same
+steps composed with the oracle (its calibration, taps, run, to_transfer).
 """
 import numpy as np
 import pytest
@@ -55,3 +61,53 @@ def test_wave_job_matches_oracle_composition(job, prec, tol):
     assert np.array_equal(one, bat)
     for s in range(len(sources)):
         assert rel_l2(one[s], ref[s]) < tol
+
+
+CAL_FREQS = [20.0, 40.0, 60.0, 80.0]
+
+
+@pytest.fixture(scope="module")
+def cabinet_job():
+    """A room with a speaker cabinet as a rigid solid (run.rs:442-450), the
+    cabinet's calibration (fdtd.rs:890-955) and the source set-up of a
+    calibrated driver (run.rs:601-624)."""
+    from wavey_amd import meshes
+
+    v, t = meshes.cuboid((0.4, 0.3, 0.5))
+    ac = np.array([0.15, 0.0, 0.05])
+    pos = np.array([1.63, 1.37, 1.02])
+    room = {"L": (4.1, 3.3, 2.7), "pillars": [], "fits": [(0.4, [])], "patches": [(1, 1, [0])] * 6}
+    mesh = meshes.box_room(room)
+    solid = fdtd.solid_from_mesh(v, t, None, pos)
+    og = oracle.build_room(mesh, 343.0, 0.1, [solid])
+    g = fdtd.FdtdGrid.build_room(mesh, 343.0, 0.1, [solid])
+    center, axis = pos + ac, (1.0, 0.0, 0.0)
+    comps, fb = oracle.wave_source_taps(og, center, axis, 1, 1)
+    assert len(comps) == 1 and not fb
+    assert fdtd.wave_source_taps(g, center, axis, 1, 1) == (comps, fb)
+    cal_ref = oracle.calibrate_cabinet(v, t, ac, 343.0, 100.0, 10.0, 3.0, CAL_FREQS, threads=8)
+    rng = np.random.default_rng(3)
+    rx = [oracle.nearest_air(og, rng.uniform((0.5, 0.5, 0.5), (3.6, 2.8, 2.2))) for _ in range(6)]
+    sig = oracle.source_signal(100.0, og.dt, 1200)
+    h0 = np.exp(1j * np.linspace(0.2, 0.6, len(CAL_FREQS)))
+    _, spec, _ = oracle.run(og, comps[0], sig, rx, freqs=CAL_FREQS, ir=False, threads=8)
+    tc = np.asarray(oracle.to_transfer(spec, sig, og, CAL_FREQS, 343.0, 1.0, h0)).reshape(len(rx), len(CAL_FREQS))
+    ref = tc / cal_ref["on_axis"][None, :]
+    return (v, t, ac, g, comps, rx, sig, h0), ref
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-9), ("fp32", 1e-4)])
+def test_calibrated_cabinet_source_matches_oracle(cabinet_job, prec, tol):
+    (v, t, ac, g, comps, rx, sig, h0), ref = cabinet_job
+    print(f"case: wave job with a calibrated cabinet source, {prec}", flush=True)
+    cal = fdtd.calibrate_cabinet(v, t, ac, 343.0, 100.0, 10.0, 3.0, CAL_FREQS, precision=prec)
+    src = {"comps": comps, "weights": None, "h0": h0, "r_ref": 1.0, "on_axis": cal["on_axis"]}
+    one = fdtd.wave_job(g, [src], sig, rx, CAL_FREQS, 343.0, prec, batched=False)
+    bat = fdtd.wave_job(g, [src], sig, rx, CAL_FREQS, 343.0, prec, batched=True)
+    assert np.array_equal(one, bat)
+    err = rel_l2(one[0], ref)
+    print(f"calibrated source: transfer vs oracle composition {err:.3e}", flush=True)
+    assert err < tol
+    # Without the calibration the transfer differs by exactly the on-axis factor.
+    raw = fdtd.wave_job(g, [dict(src, on_axis=None)], sig, rx, CAL_FREQS, 343.0, prec, batched=False)
+    assert rel_l2(raw[0] / cal["on_axis"][None, :], one[0]) < 1e-12

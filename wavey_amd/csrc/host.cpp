@@ -363,6 +363,67 @@ extern "C" void wv_baffle_point(const wv_grid* g, const double p[3], const doubl
     for (int i = 0; i < 3; ++i) out[i] = p[i];
 }
 
+extern "C" wv_status wv_air_along(const wv_grid* g, const double p[3], const double axis[3], uint64_t* cell) {
+    // fdtd.rs:516-533: from p along the axis in steps of dx/2 (for up to
+    // 2 m), the first point whose rounded node is air; else the nearest air
+    // node to p.
+    const double step = g->dx * 0.5;
+    const int64_t kmax = (int64_t)(2.0 / step);
+    for (int64_t k = 0; k < kmax; ++k) {
+        double r[3];
+        bool neg = false;
+        for (int i = 0; i < 3; ++i) {
+            const double q = p[i] + axis[i] * ((double)k * step);
+            r[i] = std::round((q - g->origin[i]) / g->dx);
+            neg = neg || r[i] < 0.0;
+        }
+        if (neg) continue;
+        const int64_t x = (int64_t)r[0], y = (int64_t)r[1], z = (int64_t)r[2];
+        if (in_grid(g, x, y, z) && (g->flags[cell_of(g, x, y, z)] & 64)) {
+            *cell = cell_of(g, x, y, z);
+            return WV_OK;
+        }
+    }
+    return wv_nearest_air(g, p, cell);
+}
+
+extern "C" wv_status wv_wave_source_taps(const wv_grid* g, const double center[3], const double axis[3],
+                                         int calibrated, uint32_t order, wv_tap comps[8], uint32_t comp_len[4],
+                                         uint32_t* n_comps, int32_t* omni_fallback) {
+    // run.rs:601-624. The source node: for a calibrated cabinet the first air
+    // node along the axis, else the nearest air node, else the first air node
+    // along the axis. The driver: on the baffle in front of the acoustic
+    // centre with a calibration, else the centre itself. A calibrated or
+    // order-0 source is the driver's trilinear taps (or the source node);
+    // otherwise the source node's components, or the node alone when it is
+    // too close to a surface for the dipoles.
+    uint64_t node = 0;
+    bool have = calibrated && wv_air_along(g, center, axis, &node) == WV_OK;
+    if (!have) have = wv_nearest_air(g, center, &node) == WV_OK;
+    if (!have) have = wv_air_along(g, center, axis, &node) == WV_OK;
+    if (!have) return WV_ERR_INVALID;
+    if (omni_fallback) *omni_fallback = 0;
+    if (calibrated || order == 0) {
+        double driver[3] = {center[0], center[1], center[2]};
+        if (calibrated) wv_baffle_point(g, center, axis, driver);
+        size_t nt = 0;
+        if (wv_taps_at(g, driver, comps, &nt) != WV_OK) {
+            comps[0] = wv_tap{node, 1.0};
+            nt = 1;
+        }
+        comp_len[0] = (uint32_t)nt;
+        *n_comps = 1;
+        return WV_OK;
+    }
+    if (wv_source_components(g, node, order, comps, comp_len, n_comps) != WV_OK) {
+        comps[0] = wv_tap{node, 1.0};
+        comp_len[0] = 1;
+        *n_comps = 1;
+        if (omni_fallback) *omni_fallback = 1;
+    }
+    return WV_OK;
+}
+
 extern "C" void wv_blend(const double ism[2], const double wave[2], double w, double out[2]) {
     // fdtd.rs:847-862: the phase of the complex crossfade with the energy
     // (1 - w)|ism|^2 + w|wave|^2; where the two nearly cancel, the larger

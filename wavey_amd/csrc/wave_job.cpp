@@ -4,8 +4,10 @@
 // A source is one or more runs (its components: an omni source's taps, or a
 // directional source's monopole and three axis dipoles,
 // fdtd.rs:730-743); each run's receiver spectra go through to_transfer
-// (fdtd.rs:819-840) and are summed with the source's per-frequency complex
-// component weights (fdtd.rs:812-815, run.rs:625-661). The reference runs
+// (fdtd.rs:819-840), is divided by the cabinet calibration's on-axis
+// response when the source has one (run.rs:651-656), and is summed with the
+// source's per-frequency complex component weights (fdtd.rs:812-815,
+// run.rs:625-661). The reference runs
 // the components one after another; here they are either run the same way
 // on one plan (the grid uploaded once) or handed to one batch (every run in
 // one launch per step), which gives the same results bit for bit.
@@ -53,6 +55,16 @@ void accumulate(const wv_wave_source& src, const Run& r, const std::vector<doubl
                    tc.data());
     for (size_t k = 0; k < n_recv * nf; ++k) {
         const size_t f = k % nf;
+        if (src.on_axis) {
+            // run.rs:651-656: calibrated so the cabinet's free-field on-axis
+            // response is the measured one; the quotient in the textbook form
+            // (a c + b d, b c - a d) / |z|^2.
+            const double cr = src.on_axis[2 * f], ci = src.on_axis[2 * f + 1];
+            const double n = cr * cr + ci * ci;
+            const double re = tc[2 * k] * cr + tc[2 * k + 1] * ci, im = tc[2 * k + 1] * cr - tc[2 * k] * ci;
+            tc[2 * k] = re / n;
+            tc[2 * k + 1] = im / n;
+        }
         std::complex<double> w(1.0, 0.0);
         if (src.weights) {
             const double* wp = src.weights + 2 * (f * src.n_comps + r.comp);

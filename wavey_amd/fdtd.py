@@ -106,13 +106,15 @@ PROGRESS_FN = C.CFUNCTYPE(None, C.c_double, C.c_void_p)
 
 class WvWaveSource(C.Structure):
     _fields_ = [("taps", C.POINTER(WvTap)), ("comp_off", C.POINTER(C.c_uint32)), ("n_comps", C.c_uint32),
-                ("weights", C.POINTER(C.c_double)), ("h0", C.POINTER(C.c_double)), ("r_ref", C.c_double)]
+                ("weights", C.POINTER(C.c_double)), ("h0", C.POINTER(C.c_double)), ("r_ref", C.c_double),
+                ("on_axis", C.POINTER(C.c_double))]
 
 
 EXPORTS = [
     "wv_fdtd_run", "wv_fdtd_run_noalloc", "wv_plan_create", "wv_plan_run", "wv_run_begin", "wv_run_advance", "wv_run_finish",
     "wv_plan_last_timing", "wv_plan_stats", "wv_plan_set_tile", "wv_plan_set_pass", "wv_plan_pass_info", "wv_plan_copy_probe", "wv_plan_free", "wv_grid_build_box",
-    "wv_built_grid_view", "wv_grid_free", "wv_source_signal", "wv_nearest_air", "wv_taps_at",
+    "wv_built_grid_view", "wv_grid_free", "wv_source_signal", "wv_nearest_air", "wv_air_along",
+    "wv_wave_source_taps", "wv_taps_at",
     "wv_source_components", "wv_component_weights", "wv_to_transfer", "wv_admittance_eval", "wv_paris",
     "wv_fit_real_impedance", "wv_version", "wv_footprint_estimate", "wv_batch_create", "wv_batch_add",
     "wv_batch_autotune", "wv_batch_set_tile", "wv_batch_run", "wv_batch_result", "wv_batch_info",
@@ -186,6 +188,11 @@ def lib():
             L.wv_source_signal.restype = None
             L.wv_nearest_air.argtypes = [P(WvGrid), dp, P(C.c_uint64)]
             L.wv_nearest_air.restype = st
+            L.wv_air_along.argtypes = [P(WvGrid), dp, dp, P(C.c_uint64)]
+            L.wv_air_along.restype = st
+            L.wv_wave_source_taps.argtypes = [P(WvGrid), dp, dp, C.c_int, C.c_uint32, P(WvTap), P(C.c_uint32),
+                                              P(C.c_uint32), P(C.c_int32)]
+            L.wv_wave_source_taps.restype = st
             L.wv_taps_at.argtypes = [P(WvGrid), dp, P(WvTap), P(C.c_size_t)]
             L.wv_taps_at.restype = st
             L.wv_source_components.argtypes = [P(WvGrid), C.c_uint64, C.c_uint32, P(WvTap), P(C.c_uint32),
@@ -491,6 +498,15 @@ class FdtdGrid:
         lib().wv_baffle_point(C.byref(self.view()), _dp(pp), _dp(ax), _dp(out))
         return out
 
+    def air_along(self, p, axis):
+        """fdtd.rs:516-533: the first air node along axis from p, else the
+        nearest; None when there is none."""
+        out = C.c_uint64()
+        pp = np.ascontiguousarray(p, dtype=np.float64)
+        ax = np.ascontiguousarray(axis, dtype=np.float64)
+        st = lib().wv_air_along(C.byref(self.view()), _dp(pp), _dp(ax), C.byref(out))
+        return None if st != WV_OK else int(out.value)
+
     def taps_at(self, p):
         taps = (WvTap * 8)()
         n = C.c_size_t()
@@ -517,6 +533,26 @@ def source_components(grid, c, order):
     if st != WV_OK:
         return None
     return [[(int(comps[2 * k + t].cell), float(comps[2 * k + t].w)) for t in range(ln[k])] for k in range(nc.value)]
+
+
+def wave_source_taps(grid, center, axis, calibrated, order):
+    """run.rs:601-624 (wv_wave_source_taps): a wave-job source's component
+    taps and whether a directional source fell back to one node; None when
+    the source is not inside the grid."""
+    comps = (WvTap * 8)()
+    ln = (C.c_uint32 * 4)()
+    nc = C.c_uint32()
+    fb = C.c_int32()
+    pp = np.ascontiguousarray(center, dtype=np.float64)
+    ax = np.ascontiguousarray(axis, dtype=np.float64)
+    st = lib().wv_wave_source_taps(C.byref(grid.view()), _dp(pp), _dp(ax), int(calibrated), int(order), comps, ln,
+                                   C.byref(nc), C.byref(fb))
+    if st != WV_OK:
+        return None
+    if nc.value == 1:
+        return [[(int(comps[t].cell), float(comps[t].w)) for t in range(ln[0])]], bool(fb.value)
+    return [[(int(comps[2 * c + t].cell), float(comps[2 * c + t].w)) for t in range(ln[c])]
+            for c in range(nc.value)], bool(fb.value)
 
 
 def component_weights(a0, a, k, dx):
@@ -915,7 +951,8 @@ def wave_job(grid: FdtdGrid, sources, signal, receivers, freqs, c, precision="fp
     """The wave solver's part of a wave job (run.rs:593-661, wv_wave_job).
 
     sources: [{"comps": [taps, ...], "weights": [nf][n_comps] complex or None
-    (one component), "h0": [nf] complex or None, "r_ref": m}]. Returns the
+    (one component), "h0": [nf] complex or None, "r_ref": m, "on_axis": [nf]
+    complex calibration or None}]. Returns the
     transfer functions [n_sources][n_recv][nf] complex, or None when cancelled."""
     sig = np.ascontiguousarray(signal, dtype=np.float64)
     rc = np.ascontiguousarray(receivers, dtype=np.uint64)
@@ -944,6 +981,10 @@ def wave_job(grid: FdtdGrid, sources, signal, receivers, freqs, c, precision="fp
             keep.append(h)
             e.h0 = _dp(h)
         e.r_ref = float(src.get("r_ref", 1.0))
+        if src.get("on_axis") is not None:
+            oa = np.ascontiguousarray(np.asarray(src["on_axis"], dtype=np.complex128).view(np.float64))
+            keep.append(oa)
+            e.on_axis = _dp(oa)
     out = np.zeros(2 * len(sources) * R * F)
     err = C.create_string_buffer(1024)
     prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)

@@ -4,7 +4,8 @@ The wave job of the reference does the same per source
 (core/wavey-solver/src/run.rs:593-661): receivers snap to the nearest air
 node, an omni source spreads over trilinear taps (taps_at, falling back to
 the nearest node), a directional source runs as its monopole and three
-dipole components (source_components) combined in frequency.
+dipole components (source_components) combined in frequency; the library's
+wv_wave_source_taps makes that choice (run.rs:601-624).
 """
 from __future__ import annotations
 
@@ -45,18 +46,15 @@ def build_case(cfg, threads=0, steps=None, builder="box"):
         receivers.append(r)
     runs = []
     for si, s in enumerate(cfg["sources"]):
-        if s["kind"] == "omni":
-            taps = g.taps_at(s["pos"])
-            if taps is None:
-                raise fdtd.FdtdError(-1, f"source {si} is not inside the grid")
-            runs.append((si, 0, taps))
-        else:
-            node = g.nearest_air(s["pos"])
-            comps = fdtd.source_components(g, node, 1)
-            if comps is None:
-                comps = [[(node, 1.0)]]
-            for ci, taps in enumerate(comps):
-                runs.append((si, ci, taps))
+        # run.rs:601-624: an omni source is its trilinear taps (order 0), a
+        # cardioid the nearest node's monopole and dipoles (order 1); no
+        # cabinet calibrations here.
+        order = 0 if s["kind"] == "omni" else 1
+        got = fdtd.wave_source_taps(g, s["pos"], s.get("axis", (1.0, 0.0, 0.0)), 0, order)
+        if got is None:
+            raise fdtd.FdtdError(-1, f"source {si} is not inside the grid")
+        for ci, taps in enumerate(got[0]):
+            runs.append((si, ci, taps))
     return Case(cfg=cfg, grid=g, signal=signal, receivers=receivers, runs=runs)
 
 
