@@ -339,7 +339,11 @@ wv_status wv_calibrate_cabinet(const double* verts, uint32_t n_verts, const uint
 
 /* ------------------------------------------------------------ grid build */
 
-/* impedance_fit.rs:27-44 (Branch, AdmittanceFit). */
+/* impedance_fit.rs:27-44 (Branch, AdmittanceFit). Faces of a node share
+ * slots only where their whole fits are equal, as the reference's derived
+ * PartialEq compares them (fdtd.rs:542): beta0, the branches, band,
+ * rms_error and max_error, each with ==, so a fit holding a NaN equals no
+ * fit, itself included. Callers without a fit's metadata pass zeros. */
 typedef struct {
     double r, m, k;
 } wv_branch;
@@ -347,6 +351,8 @@ typedef struct {
     double beta0;
     uint32_t n_branches;
     const wv_branch* branches;
+    double band[2];
+    double rms_error, max_error;
 } wv_admittance;
 
 /*

@@ -288,10 +288,13 @@ int ob_fit_admittance(const ob_material* mat, double band_lo, double band_hi, in
     band_out[0] = lo;
     band_out[1] = hi;
     memset(out, 0, sizeof(*out));
+    out->band[0] = lo;
+    out->band[1] = hi;
     if (all_small) {
         double m = 0.0;
         for (int i = 0; i < NF; ++i) m = fmax(m, target[i]);
         err_out[0] = err_out[1] = m;
+        out->rms_error = out->max_error = m;
         return 0;
     }
     double mean = -0.0;
@@ -356,8 +359,12 @@ int ob_fit_admittance(const ob_material* mat, double band_lo, double band_hi, in
         if (any) kept.br[kept.nb++] = best.br[b];
     }
     if (kept.beta0 < 1e-7) kept.beta0 = 0.0;
-    *out = kept;
     fit_errors(&kept, freqs, target, NF, &err_out[0], &err_out[1]);
+    kept.band[0] = lo;
+    kept.band[1] = hi;
+    kept.rms_error = err_out[0];
+    kept.max_error = err_out[1];
+    *out = kept;
     return kept.nb;
 }
 
@@ -371,11 +378,14 @@ double ob_phase_error_deg_per_m(double f, double c, double dx) {
 
 /* ------------------------------------------------------------ boundary */
 
+/* impedance_fit.rs:34-44: #[derive(PartialEq)] over beta0, branches, band,
+ * rms_error and max_error. */
 static int fit_eq(const ob_fit* a, const ob_fit* b) {
     if (a->beta0 != b->beta0 || a->nb != b->nb) return 0;
     for (int i = 0; i < a->nb; ++i)
         if (a->br[i].r != b->br[i].r || a->br[i].m != b->br[i].m || a->br[i].k != b->br[i].k) return 0;
-    return 1;
+    return a->band[0] == b->band[0] && a->band[1] == b->band[1] && a->rms_error == b->rms_error &&
+           a->max_error == b->max_error;
 }
 
 /* fdtd.rs:539-571: faces with equal fits share slots (impedance / count). */

@@ -25,7 +25,15 @@ extern "C" {
 
 /* core/wavey-materials/src/impedance_fit.rs:27-44 (Branch, AdmittanceFit). */
 typedef struct { double r, m, k; } ob_branch;
-typedef struct { double beta0; int32_t nb; ob_branch br[OB_MAX_BRANCHES]; } ob_fit;
+/* impedance_fit.rs:34-44 (AdmittanceFit) with its band and errors: fits
+ * compare as the derived PartialEq does, every field with ==. */
+typedef struct {
+    double beta0;
+    int32_t nb;
+    ob_branch br[OB_MAX_BRANCHES];
+    double band[2];
+    double rms_error, max_error;
+} ob_fit;
 
 /* core/wavey-solver/src/fdtd.rs:178-187 (Slot). */
 typedef struct { double inv_d, cv, k, w; } ob_slot;

@@ -29,7 +29,8 @@ class ObBranch(C.Structure):
 
 
 class ObFit(C.Structure):
-    _fields_ = [("beta0", C.c_double), ("nb", C.c_int32), ("br", ObBranch * MAX_BRANCHES)]
+    _fields_ = [("beta0", C.c_double), ("nb", C.c_int32), ("br", ObBranch * MAX_BRANCHES),
+                ("band", C.c_double * 2), ("rms_error", C.c_double), ("max_error", C.c_double)]
 
 
 class ObSlot(C.Structure):
@@ -163,7 +164,9 @@ def _ptr(a, ctype):
     return a.ctypes.data_as(C.POINTER(ctype))
 
 
-def make_fit(beta0, branches):
+def make_fit(beta0, branches, meta=None):
+    """An AdmittanceFit; meta = {"band": (lo, hi), "rms_error", "max_error"}
+    (zeros when absent) takes part in the fit's equality."""
     f = ObFit()
     f.beta0 = float(beta0)
     f.nb = len(branches)
@@ -171,6 +174,10 @@ def make_fit(beta0, branches):
         raise ValueError("too many branches")
     for i, (r, m, k) in enumerate(branches):
         f.br[i].r, f.br[i].m, f.br[i].k = float(r), float(m), float(k)
+    if meta:
+        f.band[0], f.band[1] = (float(v) for v in meta.get("band", (0.0, 0.0)))
+        f.rms_error = float(meta.get("rms_error", 0.0))
+        f.max_error = float(meta.get("max_error", 0.0))
     return f
 
 
@@ -228,7 +235,7 @@ def _room_struct(room):
     for k in range(3):
         r.L[k] = float(room["L"][k])
     pil = np.ascontiguousarray(np.array(room.get("pillars", []), dtype=np.float64).reshape(-1, 4))
-    fits = (ObFit * len(room["fits"]))(*[make_fit(b0, br) for b0, br in room["fits"]])
+    fits = (ObFit * len(room["fits"]))(*[make_fit(*f) for f in room["fits"]])
     patches = room["patches"]
     n_surf = 6 + 4 * len(pil)
     if len(patches) != n_surf:
@@ -294,7 +301,7 @@ def _mesh_struct(mesh):
     pos = np.ascontiguousarray(np.asarray(mesh["positions"], dtype=np.float64).reshape(-1, 3))
     tri = np.ascontiguousarray(np.asarray(mesh["triangles"], dtype=np.int32).reshape(-1, 3))
     ts = np.ascontiguousarray(np.asarray(mesh["tri_surface"], dtype=np.int32))
-    fits = (ObFit * max(1, len(mesh["fits"])))(*[make_fit(b0, br) for b0, br in mesh["fits"]])
+    fits = (ObFit * max(1, len(mesh["fits"])))(*[make_fit(*f) for f in mesh["fits"]])
     regions = mesh.get("regions", []) if mesh.get("prism") else []
     rc, off, xy = [0], [0], []
     for reg in regions:
@@ -493,7 +500,7 @@ def fit_real_impedance(alpha):
 
 def boundary_slots(face_fits, fits, dt):
     ff = np.asarray(face_fits, dtype=np.int32)
-    fa = (ObFit * len(fits))(*[make_fit(b0, br) for b0, br in fits])
+    fa = (ObFit * len(fits))(*[make_fit(*f) for f in fits])
     out = (ObSlot * 64)()
     ns = lib().ob_boundary_slots(_ptr(ff, C.c_int32), len(ff), fa, dt, out, 64)
     return np.array([[out[j].inv_d, out[j].cv, out[j].k, out[j].w] for j in range(ns)]).reshape(-1, 4)

@@ -213,7 +213,7 @@ def library_room():
     ids = ["audience-standing", "mineral-wool-50", "plaster-rough", "carpet-heavy", "wood-panel",
            "ceiling-industrial"]
     fits = fdtd.fit_surfaces([lib[i] for i in ids], (20.0, 300.0))
-    assert any(k > 0 for _, br in fits for (_, _, k) in br)
+    assert any(k > 0 for f in fits for (_, _, k) in f[1])
     c = 343.0
     dx = c / (300.0 * 12.0)
     room = {"L": (5.0, 4.0, 3.0), "pillars": [], "fits": fits, "patches": [(1, 1, [i]) for i in range(6)]}

@@ -175,3 +175,51 @@ def test_settings_functions():
     # The scheme's axial dispersion is second order in dx.
     e1, e2 = fdtd.phase_error_deg_per_m(100.0, 343.0, 0.04), fdtd.phase_error_deg_per_m(100.0, 343.0, 0.02)
     assert 3.9 < e1 / e2 < 4.1
+
+
+def _corner_slots(fits, builder):
+    """Slots of the node at cell (1, 3, 1) of a 1.0 x 0.8 x 0.6 m room at dx
+    0.1 m, whose missing faces are the wall x = 0 (fit 0) and the floor (fit
+    1); the other surfaces take fit 2."""
+    from wavey_amd import meshes
+
+    room = {"L": (1.0, 0.8, 0.6), "pillars": [], "fits": fits,
+            "patches": [(1, 1, [0]), (1, 1, [2]), (1, 1, [2]), (1, 1, [2]), (1, 1, [1]), (1, 1, [2])]}
+    if builder == "box":
+        g = fdtd.FdtdGrid.build_box(room, 343.0, 0.1)
+        og = O.build_box(room, 343.0, 0.1)
+    else:
+        mesh = meshes.box_room(room)
+        g = fdtd.FdtdGrid.build_room(mesh, 343.0, 0.1)
+        og = O.build_room(mesh, 343.0, 0.1)
+    for name in ("flags", "b_cell", "b_start", "slots"):
+        assert np.array_equal(getattr(g, name), getattr(og, name)), name
+    b = int(np.flatnonzero(g.b_cell == g.index(1, 3, 1))[0])
+    return g.slots[g.b_start[b]:g.b_start[b + 1]]
+
+
+@pytest.mark.parametrize("builder", ["box", "mesh"])
+def test_faces_merge_only_where_whole_fits_are_equal(builder):
+    """fdtd.rs:542 groups a node's faces by AdmittanceFit's derived PartialEq
+    (impedance_fit.rs:34-44), which compares the band and both errors too:
+    equal fits merge into one conductance slot of admittance x 2 (inv_d = 2
+    beta0, fdtd.rs:552-553, 561-562), value-equal fits with different
+    metadata stay two slots of inv_d = beta0, and a fit holding a NaN equals
+    no fit, itself included."""
+    beta = 0.3
+    other = (0.05, [])
+    meta = {"band": (20.0, 300.0), "rms_error": 0.01, "max_error": 0.02}
+    merged = _corner_slots([(beta, [], meta), (beta, [], dict(meta)), other], builder)
+    assert merged.shape == (1, 4) and merged[0, 0] == 2 * beta
+    apart = _corner_slots([(beta, [], meta), (beta, [], dict(meta, max_error=0.03)), other], builder)
+    assert apart.shape == (2, 4) and np.all(apart[:, 0] == beta)
+    band = _corner_slots([(beta, [], meta), (beta, [], dict(meta, band=(20.0, 250.0))), other], builder)
+    assert band.shape == (2, 4)
+    # No metadata on either: equal (zeros), merged.
+    assert _corner_slots([(beta, []), (beta, []), other], builder).shape == (1, 4)
+    # A NaN in the metadata: never equal, even the same fit on both faces.
+    nan = dict(meta, rms_error=float("nan"))
+    assert _corner_slots([(beta, [], nan), (beta, [], nan), other], builder).shape == (2, 4)
+    fits = [(beta, [], nan), (0.7, []), other]
+    room_one_fit = _corner_slots([fits[0], fits[0], other], builder)
+    assert room_one_fit.shape == (2, 4) and np.all(room_one_fit[:, 0] == beta)

@@ -126,5 +126,6 @@ def test_fit_surfaces_per_material():
     """fdtd.rs:213-222: one fit per distinct id, returned per surface."""
     mats = [LIBRARY[0], LIBRARY[2], LIBRARY[0]]
     fits = fdtd.fit_surfaces(mats, (20.0, 424.0))
-    assert fits[0] == fits[2] == oracle.fit_admittance(LIBRARY[0], (20.0, 424.0))[0]
-    assert fits[1] == oracle.fit_admittance(LIBRARY[2], (20.0, 424.0))[0]
+    for k, m in ((0, LIBRARY[0]), (1, LIBRARY[2]), (2, LIBRARY[0])):
+        fit, meta = oracle.fit_admittance(m, (20.0, 424.0))
+        assert fits[k][:2] == fit and fits[k][2] == meta

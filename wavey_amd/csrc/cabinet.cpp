@@ -97,7 +97,7 @@ extern "C" wv_status wv_calibrate_cabinet(const double* verts, uint32_t n_verts,
     uint32_t tri_surface[12];
     for (int i = 0; i < 12; ++i) tri_surface[i] = (uint32_t)(i / 2);
     wv_admittance fits[6];
-    for (auto& f : fits) f = {1.0, 0, nullptr};
+    for (auto& f : fits) f = {1.0, 0, nullptr, {0.0, 0.0}, 0.0, 0.0};
     const double plan_xy[8] = {0.0, 0.0, side, 0.0, side, side, 0.0, side};
     const uint32_t region_cycles[2] = {0, 1}, cycle_off[2] = {0, 4};
     wv_room_mesh room{1, side, 8, &pos[0][0], 12, &box_tris[0][0], tri_surface, 6, fits, 1,

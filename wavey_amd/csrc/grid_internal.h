@@ -22,13 +22,19 @@ namespace wv {
 wv_slot slot_of(double r, double m, double k, double count, double dt);
 
 // The slot list of one node: faces in direction order, each given as its
-// canonical fit (equal fits share one index), fdtd.rs:539-571.
+// canonical fit (equal fits share one index; see canonical_fits),
+// fdtd.rs:539-571.
 void node_slots(const int* canon_fits, int n_faces, const wv_admittance* fits, double dt,
                 std::vector<wv_slot>& out);
 
-// AdmittanceFit equality on the values the slots depend on (beta0 and the
-// branches, impedance_fit.rs:34-44).
+// AdmittanceFit equality as the reference's derived PartialEq
+// (impedance_fit.rs:34-44): beta0, the branches, band and both errors, each
+// with == (NaN equals nothing).
 bool fits_equal(const wv_admittance& a, const wv_admittance& b);
+
+// Canonical index of each fit: the first equal fit, or ~i for a fit that
+// equals no fit (not even itself), whose faces never share slots.
+std::vector<int32_t> canonical_fits(const wv_admittance* fits, uint32_t n);
 
 // Runs f(lo, hi, t) over [0, n) split into contiguous chunks, one per thread.
 template <class F>

@@ -17,6 +17,7 @@
 #include <memory>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "wavey_fdtd.h"
@@ -69,25 +70,54 @@ bool fits_equal(const wv_admittance& a, const wv_admittance& b) {
         if (a.branches[i].r != b.branches[i].r || a.branches[i].m != b.branches[i].m ||
             a.branches[i].k != b.branches[i].k)
             return false;
-    return true;
+    return a.band[0] == b.band[0] && a.band[1] == b.band[1] && a.rms_error == b.rms_error &&
+           a.max_error == b.max_error;
+}
+
+std::vector<int32_t> canonical_fits(const wv_admittance* fits, uint32_t n) {
+    // Candidates are bucketed by beta0 and the branch count (equal fits have
+    // equal ones; -0 and +0 compare equal, so both hash as +0).
+    std::vector<int32_t> canon(n);
+    std::unordered_map<uint64_t, std::vector<int32_t>> by_head;
+    for (uint32_t i = 0; i < n; ++i) {
+        const wv_admittance& a = fits[i];
+        if (!fits_equal(a, a)) {
+            canon[i] = ~(int32_t)i;
+            continue;
+        }
+        const double b0 = a.beta0 == 0.0 ? 0.0 : a.beta0;
+        uint64_t h;
+        std::memcpy(&h, &b0, 8);
+        h ^= (uint64_t)a.n_branches * 0x9E3779B97F4A7C15ull;
+        auto& bucket = by_head[h];
+        canon[i] = (int32_t)i;
+        for (int32_t j : bucket)
+            if (fits_equal(fits[j], a)) { canon[i] = j; break; }
+        if (canon[i] == (int32_t)i) bucket.push_back((int32_t)i);
+    }
+    return canon;
 }
 
 // fdtd.rs:539-571: faces grouped by fit in order of first appearance; each
 // group gives its conductance slot (beta0 > 0) and one slot per branch, with
-// the impedances divided by the group's face count.
+// the impedances divided by the group's face count. A negative canonical
+// index ~i is fit i that equals no fit: each of its faces is a group.
 void node_slots(const int* canon_fits, int n_faces, const wv_admittance* fits, double dt,
                 std::vector<wv_slot>& v) {
     int grp[6], cnt[6], ng = 0;
     for (int f = 0; f < n_faces; ++f) {
         int fit = canon_fits[f];
         int j = 0;
-        while (j < ng && grp[j] != fit) ++j;
+        if (fit >= 0)
+            while (j < ng && grp[j] != fit) ++j;
+        else
+            j = ng;
         if (j < ng) cnt[j]++;
         else { grp[ng] = fit; cnt[ng] = 1; ++ng; }
     }
     v.clear();
     for (int j = 0; j < ng; ++j) {
-        const wv_admittance& a = fits[grp[j]];
+        const wv_admittance& a = fits[grp[j] >= 0 ? grp[j] : ~grp[j]];
         double n = (double)cnt[j];
         if (a.beta0 > 0.0) v.push_back(slot_of(1.0 / a.beta0, 0.0, 0.0, n, dt));
         for (uint32_t b = 0; b < a.n_branches; ++b)
@@ -167,10 +197,13 @@ struct BoxGrid {
 
 // A node's missing-face fits in direction order, 8 bits per face (fit + 1),
 // zero-terminated: the key of its slot list.
+// A node's faces as their canonical fits, 9 bits a face: c + 1 for a fit
+// index c (0..254), 257 + i for ~i (fit i equals no fit).
 struct FaceSig {
     uint64_t key = 0;
     int n = 0;
-    void push(int fit) { key |= (uint64_t)(fit + 1) << (8 * n); ++n; }
+    void push(int fit) { key |= (uint64_t)(fit >= 0 ? fit + 1 : 257 + ~fit) << (9 * n); ++n; }
+    static int fit_of(uint64_t v) { return v <= 256 ? (int)v - 1 : ~(int)(v - 257); }
 };
 
 }  // namespace
@@ -199,12 +232,7 @@ extern "C" wv_status wv_grid_build_box(const wv_box_room* room, double c, double
                                   " M cells; lower the crossover frequency");
         return WV_ERR_LIMIT;
     }
-    B.canon.resize(room->n_fits);
-    for (uint32_t i = 0; i < room->n_fits; ++i) {
-        B.canon[i] = (int)i;
-        for (uint32_t j = 0; j < i; ++j)
-            if (fits_equal(room->fits[i], room->fits[j])) { B.canon[i] = (int)j; break; }
-    }
+    B.canon = canonical_fits(room->fits, room->n_fits);
     for (int k = 0; k < 3; ++k) {
         auto& a = k == 0 ? B.ax : k == 1 ? B.ay : B.az;
         a.resize(B.n[k]);
@@ -259,8 +287,8 @@ extern "C" wv_status wv_grid_build_box(const wv_box_room* room, double c, double
         auto it = cache.find(key);
         if (it != cache.end()) return it->second;
         int faces[6], nf = 0;
-        while (nf < 6 && ((key >> (8 * nf)) & 0xff)) {
-            faces[nf] = (int)((key >> (8 * nf)) & 0xff) - 1;
+        while (nf < 6 && ((key >> (9 * nf)) & 0x1ff)) {
+            faces[nf] = FaceSig::fit_of((key >> (9 * nf)) & 0x1ff);
             ++nf;
         }
         std::vector<wv_slot> v;

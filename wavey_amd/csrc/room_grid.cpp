@@ -464,22 +464,7 @@ extern "C" wv_status wv_grid_build_room(const wv_room_mesh* room, const wv_solid
     }
 
     // Canonical fit of each surface: the first surface with an equal fit.
-    std::vector<int32_t> canon(room->n_fits);
-    {
-        std::unordered_map<uint64_t, std::vector<int32_t>> by_head;
-        for (uint32_t i = 0; i < room->n_fits; ++i) {
-            const wv_admittance& a = room->fits[i];
-            double b0 = a.beta0 == 0.0 ? 0.0 : a.beta0;  // -0 and +0 compare equal
-            uint64_t h;
-            std::memcpy(&h, &b0, 8);
-            h ^= (uint64_t)a.n_branches * 0x9E3779B97F4A7C15ull;
-            auto& bucket = by_head[h];
-            canon[i] = (int32_t)i;
-            for (int32_t j : bucket)
-                if (fits_equal(room->fits[j], a)) { canon[i] = j; break; }
-            if (canon[i] == (int32_t)i) bucket.push_back((int32_t)i);
-        }
-    }
+    const std::vector<int32_t> canon = canonical_fits(room->fits, room->n_fits);
 
     // Face rays need the 3D tables only when there are triangles.
     const int64_t FB = 8;

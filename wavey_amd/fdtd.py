@@ -64,7 +64,8 @@ class WvBranch(C.Structure):
 
 
 class WvAdmittance(C.Structure):
-    _fields_ = [("beta0", C.c_double), ("n_branches", C.c_uint32), ("branches", C.POINTER(WvBranch))]
+    _fields_ = [("beta0", C.c_double), ("n_branches", C.c_uint32), ("branches", C.POINTER(WvBranch)),
+                ("band", C.c_double * 2), ("rms_error", C.c_double), ("max_error", C.c_double)]
 
 
 class WvBoxRoom(C.Structure):
@@ -292,9 +293,13 @@ def _check(status, err):
 # ------------------------------------------------------------------- grid
 
 def _admittances(fits):
+    """Fits as (beta0, [(r, m, k), ...]) or (beta0, branches, meta) with meta
+    = {"band": (lo, hi), "rms_error", "max_error"} (zeros when absent)."""
     keep = []
     arr = (WvAdmittance * len(fits))()
-    for i, (beta0, branches) in enumerate(fits):
+    for i, fit in enumerate(fits):
+        beta0, branches = fit[0], fit[1]
+        meta = fit[2] if len(fit) > 2 and fit[2] else {}
         br = (WvBranch * max(1, len(branches)))()
         for j, (r, m, k) in enumerate(branches):
             br[j].r, br[j].m, br[j].k = float(r), float(m), float(k)
@@ -302,6 +307,9 @@ def _admittances(fits):
         arr[i].beta0 = float(beta0)
         arr[i].n_branches = len(branches)
         arr[i].branches = C.cast(br, C.POINTER(WvBranch))
+        arr[i].band[0], arr[i].band[1] = (float(v) for v in meta.get("band", (0.0, 0.0)))
+        arr[i].rms_error = float(meta.get("rms_error", 0.0))
+        arr[i].max_error = float(meta.get("max_error", 0.0))
     return arr, keep
 
 
@@ -1084,14 +1092,19 @@ def fit_admittance(mat, band, max_branches=2):
 
 def fit_surfaces(materials, band, threads=8):
     """fit_surfaces (fdtd.rs:213-222): one fit per distinct material id, fitted
-    in parallel, returned per surface."""
+    in parallel, returned per surface as (beta0, branches, {"band",
+    "rms_error", "max_error"}), the whole AdmittanceFit."""
     from concurrent.futures import ThreadPoolExecutor
 
     unique = {}
     for m in materials:
         unique.setdefault(m["id"], m)
     with ThreadPoolExecutor(max_workers=threads) as ex:
-        fits = dict(zip(unique, ex.map(lambda m: fit_admittance(m, band, 2)[0], unique.values())))
+        def whole(m):
+            fit, meta = fit_admittance(m, band, 2)
+            return (fit[0], fit[1], meta)
+
+        fits = dict(zip(unique, ex.map(whole, unique.values())))
     return [fits[m["id"]] for m in materials]
 
 
