@@ -337,6 +337,52 @@ wv_status wv_calibrate_cabinet(const double* verts, uint32_t n_verts, const uint
                                const volatile int32_t* cancel, double* on_axis_out, double* angles_out,
                                double* directivity_out, char* err, size_t err_len);
 
+/* ---------------------------------------------------------- result files
+ *
+ * SimulationResult::save (core/wavey-solver/src/result.rs:444-461), as
+ * `wavey simulate` writes it (core/wavey-cli/src/main.rs:513-516), for the
+ * wave solver's part of a result: results.json (serde's metadata, field
+ * order of result.rs:16-90 and fdtd.rs:85-114, no cut planes, has_late
+ * false) and results.bin (every microphone's transfer functions
+ * [speaker][freq] as complex f32, microphones in order).
+ */
+enum { WV_RESULT_BANDS = 8 }; /* raytrace.rs:29-30: octave bands 63 Hz .. 8 kHz */
+typedef struct {
+    uint64_t id;
+    const char* name;      /* UTF-8 */
+    double position[3];
+} wv_microphone;           /* receivers.rs:5-10 */
+typedef struct {
+    double crossover_hz, duration_s, points_per_wavelength;
+    uint32_t source_order;
+    int32_t cabinets;
+    int32_t accuracy;      /* WV_ACCURACY_*; High is not written (fdtd.rs:112-113) */
+} wv_fdtd_settings;        /* fdtd.rs:85-114 */
+typedef struct {
+    size_t nf;
+    const double* freqs;
+    size_t n_speakers;
+    const uint64_t* speaker_ids;
+    size_t n_mics;
+    const wv_microphone* mics;
+    const double* transfer;  /* [n_mics][n_speakers][nf] complex (re, im), stored as f32 */
+    const char* backend;     /* RunStats (result.rs:80-90) */
+    double seconds;
+    uint64_t images, paths, diffracted_paths, receivers;
+    size_t n_notes;
+    const char* const* notes;
+    uint64_t fingerprint;
+    const double* rt_sabine; /* [WV_RESULT_BANDS] or NULL for zeros */
+    const double* rt_eyring;
+    const wv_fdtd_settings* fdtd; /* NULL: no wave stage (null) */
+} wv_results;
+/* Writes dir/results.json and dir/results.bin, creating dir. */
+wv_status wv_results_save(const char* dir, const wv_results* r, char* err, size_t err_len);
+/* result.rs:463-476: the transfers back from dir/results.bin, out
+ * [n_mics][n_speakers][nf] complex f32 (re, im). */
+wv_status wv_results_read_transfer(const char* dir, size_t n_mics, size_t n_speakers, size_t nf, float* out,
+                                   char* err, size_t err_len);
+
 /* ------------------------------------------------------------ grid build */
 
 /* impedance_fit.rs:27-44 (Branch, AdmittanceFit). Faces of a node share

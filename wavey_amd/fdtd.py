@@ -105,6 +105,25 @@ class WvSolid(C.Structure):
 
 PROGRESS_FN = C.CFUNCTYPE(None, C.c_double, C.c_void_p)
 
+class WvMicrophone(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("name", C.c_char_p), ("position", C.c_double * 3)]
+
+
+class WvFdtdSettings(C.Structure):
+    _fields_ = [("crossover_hz", C.c_double), ("duration_s", C.c_double), ("points_per_wavelength", C.c_double),
+                ("source_order", C.c_uint32), ("cabinets", C.c_int32), ("accuracy", C.c_int32)]
+
+
+class WvResults(C.Structure):
+    _fields_ = [("nf", C.c_size_t), ("freqs", C.POINTER(C.c_double)), ("n_speakers", C.c_size_t),
+                ("speaker_ids", C.POINTER(C.c_uint64)), ("n_mics", C.c_size_t), ("mics", C.POINTER(WvMicrophone)),
+                ("transfer", C.POINTER(C.c_double)), ("backend", C.c_char_p), ("seconds", C.c_double),
+                ("images", C.c_uint64), ("paths", C.c_uint64), ("diffracted_paths", C.c_uint64),
+                ("receivers", C.c_uint64), ("n_notes", C.c_size_t), ("notes", C.POINTER(C.c_char_p)),
+                ("fingerprint", C.c_uint64), ("rt_sabine", C.POINTER(C.c_double)),
+                ("rt_eyring", C.POINTER(C.c_double)), ("fdtd", C.POINTER(WvFdtdSettings))]
+
+
 class WvWaveSource(C.Structure):
     _fields_ = [("taps", C.POINTER(WvTap)), ("comp_off", C.POINTER(C.c_uint32)), ("n_comps", C.c_uint32),
                 ("weights", C.POINTER(C.c_double)), ("h0", C.POINTER(C.c_double)), ("r_ref", C.c_double),
@@ -120,7 +139,7 @@ EXPORTS = [
     "wv_fit_real_impedance", "wv_version", "wv_footprint_estimate", "wv_batch_create", "wv_batch_add",
     "wv_batch_autotune", "wv_batch_set_tile", "wv_batch_run", "wv_batch_result", "wv_batch_info",
     "wv_batch_timing", "wv_batch_free", "wv_serve_create", "wv_serve_submit", "wv_serve_wait", "wv_serve_stats", "wv_serve_set_policy",
-    "wv_serve_free", "wv_wave_job", "wv_grid_build_room", "wv_solid_from_mesh", "wv_grid_spacing",
+    "wv_serve_free", "wv_wave_job", "wv_results_save", "wv_results_read_transfer", "wv_grid_build_room", "wv_solid_from_mesh", "wv_grid_spacing",
     "wv_baffle_point", "wv_blend", "wv_calibrate_cabinet", "wv_material_absorption", "wv_fit_admittance",
     "wv_first_order_directions", "wv_fit_first_order", "wv_accuracy_headroom", "wv_accuracy_cabinet_cells",
     "wv_settings_f_max", "wv_grid_fmax", "wv_crossover_weight", "wv_phase_error_deg_per_m",
@@ -246,6 +265,10 @@ def lib():
             L.wv_serve_stats.restype = st
             L.wv_serve_free.argtypes = [C.c_void_p]
             L.wv_serve_free.restype = None
+            L.wv_results_save.argtypes = [C.c_char_p, P(WvResults)] + errbuf
+            L.wv_results_save.restype = st
+            L.wv_results_read_transfer.argtypes = [C.c_char_p, sz, sz, sz, P(C.c_float)] + errbuf
+            L.wv_results_read_transfer.restype = st
             L.wv_wave_job.argtypes = [P(WvGrid), P(WvWaveSource), sz, dp, sz, P(C.c_uint64), sz, dp, sz, C.c_double,
                                       C.c_int, C.c_int, P(C.c_int32), dp] + errbuf
             L.wv_wave_job.restype = st
@@ -1005,6 +1028,69 @@ def wave_job(grid: FdtdGrid, sources, signal, receivers, freqs, c, precision="fp
     if st == WV_CANCELLED:
         return None
     return (out[0::2] + 1j * out[1::2]).reshape(len(sources), R, F)
+
+
+ACCURACIES = {"draft": 0, "standard": 1, "high": 2}
+
+
+def save_results(directory, freqs, speaker_ids, mics, transfer, stats=None, fingerprint=0, rt_sabine=None,
+                 rt_eyring=None, fdtd_settings=None):
+    """SimulationResult::save (result.rs:444-461) of a wave-solver result:
+    mics = [{"id", "name", "position"}], transfer [mic][speaker][freq]
+    complex, stats = {"backend", "seconds", "images", "paths",
+    "diffracted_paths", "receivers", "notes"}, fdtd_settings =
+    {"crossover_hz", "duration_s", "points_per_wavelength", "source_order",
+    "cabinets", "accuracy"} or None."""
+    fr = np.ascontiguousarray(freqs, dtype=np.float64)
+    ids = np.ascontiguousarray(speaker_ids, dtype=np.uint64)
+    tr = np.ascontiguousarray(np.asarray(transfer, dtype=np.complex128).reshape(len(mics), len(ids), len(fr)))
+    trv = np.ascontiguousarray(tr.view(np.float64))
+    st = dict(stats or {})
+    r = WvResults()
+    r.nf, r.freqs = len(fr), _dp(fr)
+    r.n_speakers, r.speaker_ids = len(ids), ids.ctypes.data_as(C.POINTER(C.c_uint64))
+    marr = (WvMicrophone * max(1, len(mics)))()
+    names = [str(m["name"]).encode() for m in mics]
+    for k, m in enumerate(mics):
+        marr[k].id = int(m["id"])
+        marr[k].name = names[k]
+        for i in range(3):
+            marr[k].position[i] = float(m["position"][i])
+    r.n_mics, r.mics = len(mics), C.cast(marr, C.POINTER(WvMicrophone))
+    r.transfer = _dp(trv)
+    r.backend = str(st.get("backend", "")).encode()
+    r.seconds = float(st.get("seconds", 0.0))
+    for k in ("images", "paths", "diffracted_paths", "receivers"):
+        setattr(r, k, int(st.get(k, 0)))
+    notes = [str(n).encode() for n in st.get("notes", [])]
+    narr = (C.c_char_p * max(1, len(notes)))(*notes)
+    r.n_notes, r.notes = len(notes), C.cast(narr, C.POINTER(C.c_char_p))
+    r.fingerprint = int(fingerprint)
+    keep = []
+    for name, v in (("rt_sabine", rt_sabine), ("rt_eyring", rt_eyring)):
+        if v is not None:
+            a = np.ascontiguousarray(v, dtype=np.float64)
+            keep.append(a)
+            setattr(r, name, _dp(a))
+    if fdtd_settings is not None:
+        fs = WvFdtdSettings(float(fdtd_settings.get("crossover_hz", 250.0)),
+                            float(fdtd_settings.get("duration_s", 0.0)),
+                            float(fdtd_settings.get("points_per_wavelength", 15.0)),
+                            int(fdtd_settings.get("source_order", 1)), int(bool(fdtd_settings.get("cabinets", True))),
+                            ACCURACIES[fdtd_settings.get("accuracy", "high")])
+        keep.append(fs)
+        r.fdtd = C.pointer(fs)
+    err = C.create_string_buffer(1024)
+    _check(lib().wv_results_save(str(directory).encode(), C.byref(r), err, 1024), err)
+
+
+def read_results_transfer(directory, n_mics, n_speakers, nf):
+    """result.rs:463-476: [mic][speaker][freq] complex64 from results.bin."""
+    out = np.zeros((n_mics, n_speakers, nf, 2), dtype=np.float32)
+    err = C.create_string_buffer(1024)
+    _check(lib().wv_results_read_transfer(str(directory).encode(), n_mics, n_speakers, nf,
+                                          out.ctypes.data_as(C.POINTER(C.c_float)), err, 1024), err)
+    return out.view(np.complex64).reshape(n_mics, n_speakers, nf)
 
 
 def blend(ism, wave, w):
