@@ -52,7 +52,6 @@ namespace {
 
 constexpr uint32_t kGraphSteps = kStepsPerBlock;  // steps per captured block
 constexpr uint64_t kDefaultFillCells = 16ull << 20;  // the server's fill: air cells per block
-constexpr uint64_t kDefaultLaneCells = 8ull << 20;   // air cells from which a request takes the large lane
 constexpr uint32_t kTuneZChunks[] = {4, 8};
 constexpr int kTuneVariants[] = {0, 1, 2};
 
@@ -637,7 +636,7 @@ struct wv_server {
     uint64_t cap = 0;
     std::atomic<int> policy{WV_SERVE_LANES};
     std::atomic<uint64_t> fill{kDefaultFillCells};
-    std::atomic<uint64_t> lane_cells{kDefaultLaneCells};
+    std::atomic<uint64_t> lane_cells{0};  // 0: three-step plans take the large lane
     std::atomic<uint64_t> submitted{0};  // submissions so far (wakes the worker's wait)
     int device = 0;
     std::mutex mu;
@@ -691,9 +690,19 @@ bool lane_finished(const Lane& l) {
 
 uint64_t work_left(const Instance* in) { return in->plan->air * (uint64_t)(in->steps - in->n0); }
 
+// Whether a request belongs to the large-request lane (policy
+// WV_SERVE_LANES): with lane_cells 0 (the default), the plans that take
+// three steps per pass (the plain run's pass mode: FP32 grids of 8 Mi cells
+// and more, FP64 7 Mi), whose extra field buffers the lane's multi-step
+// kernels use; else multi-step plans of at least lane_cells air cells.
+bool in_large_lane(const wv_server* sv, const wv_plan* pl) {
+    const uint64_t lane = sv->lane_cells.load();
+    return lane ? pl->run_steps_per_pass >= 2 && pl->air >= lane : pl->run_steps_per_pass == 3;
+}
+
 // The instance for the large-request lane's next block (policy
-// WV_SERVE_LANES): of the instances with two-step passes and at least
-// lane_cells air cells, the one the lane has started, else the earliest
+// WV_SERVE_LANES): of the instances in the lane (in_large_lane), the one the
+// lane has started, else the earliest
 // admitted (so the lane serves large requests first come, first served, as
 // a queue of plain runs would), or null.
 Instance* pick_lane_instance(wv_server* sv) {
@@ -701,7 +710,7 @@ Instance* pick_lane_instance(wv_server* sv) {
     for (auto& up : sv->batch->inst) {
         Instance* in = up.get();
         if (in->done || in->n0 >= in->steps || in->lane >= 0) continue;
-        if (in->plan->run_steps_per_pass < 2 || in->plan->air < sv->lane_cells.load()) continue;
+        if (!in_large_lane(sv, in->plan.get())) continue;
         if (in->n0 > 0) return in;
         if (!first) first = in;
     }
@@ -734,7 +743,7 @@ std::vector<Instance*> select_batch(wv_server* sv) {
         std::vector<size_t> order;
         for (size_t k = 0; k < live.size(); ++k) {
             const wv_plan* pl = live[k]->plan.get();
-            if (policy == WV_SERVE_LANES && pl->run_steps_per_pass >= 2 && pl->air >= sv->lane_cells.load()) run[k] = 0;
+            if (policy == WV_SERVE_LANES && in_large_lane(sv, pl)) run[k] = 0;
             else order.push_back(k);
         }
         std::stable_sort(order.begin(), order.end(),
@@ -1070,7 +1079,7 @@ extern "C" wv_status wv_serve_set_policy(wv_server* sv, int policy, uint64_t fil
         return WV_ERR_INVALID;
     sv->policy.store(policy);
     sv->fill.store(fill_cells ? fill_cells : kDefaultFillCells);
-    sv->lane_cells.store(lane_cells ? lane_cells : kDefaultLaneCells);
+    sv->lane_cells.store(lane_cells);
     return WV_OK;
 }
 

@@ -283,6 +283,56 @@ __global__ __launch_bounds__(1024) void k_scan(const uint32_t* __restrict__ coun
     if (threadIdx.x == 1023) *total = part[1023];
 }
 
+// Exclusive scan of many counts (one per compaction block) in three
+// launches: the sum of each chunk of kScanChunk counts (coalesced reads), a
+// one-group scan of those sums, then each chunk's scan from its base, eight
+// consecutive counts per thread. The offsets are those of a serial scan.
+constexpr uint32_t kScanThreads = 1024, kScanItems = 8, kScanChunk = kScanThreads * kScanItems;
+
+__device__ __forceinline__ uint32_t group_exclusive(uint32_t v, uint32_t* part) {
+    part[threadIdx.x] = v;
+    __syncthreads();
+    for (uint32_t off = 1; off < kScanThreads; off <<= 1) {
+        const uint32_t w = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += w;
+        __syncthreads();
+    }
+    return part[threadIdx.x] - v;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_sums(const uint32_t* __restrict__ counts, uint32_t n,
+                                                            uint32_t* __restrict__ sums) {
+    __shared__ uint32_t part[kScanThreads];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanChunk;
+    uint32_t s = 0;
+    for (uint32_t k = 0; k < kScanItems; ++k) {
+        const uint64_t i = base + (uint64_t)k * kScanThreads + threadIdx.x;
+        if (i < n) s += counts[i];
+    }
+    const uint32_t ex = group_exclusive(s, part);
+    if (threadIdx.x == kScanThreads - 1) sums[blockIdx.x] = ex + s;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_chunks(const uint32_t* __restrict__ counts, uint32_t n,
+                                                              const uint32_t* __restrict__ bases,
+                                                              uint32_t* __restrict__ offs) {
+    __shared__ uint32_t part[kScanThreads];
+    const uint64_t first = (uint64_t)blockIdx.x * kScanChunk + (uint64_t)threadIdx.x * kScanItems;
+    uint32_t c[kScanItems];
+    uint32_t s = 0;
+    for (uint32_t k = 0; k < kScanItems; ++k) {
+        c[k] = first + k < n ? counts[first + k] : 0u;
+        s += c[k];
+    }
+    uint32_t o = bases[blockIdx.x] + group_exclusive(s, part);
+    for (uint32_t k = 0; k < kScanItems; ++k)
+        if (first + k < n) {
+            offs[first + k] = o;
+            o += c[k];
+        }
+}
+
 __device__ __forceinline__ int64_t find_sorted(const uint32_t* __restrict__ a, uint32_t n, uint32_t key) {
     uint32_t lo = 0, hi = n;
     while (lo < hi) {
@@ -529,6 +579,18 @@ void fill_async(wv_plan* pl, DBuf& d, size_t n) {
     WV_HIP(hipGetLastError());
 }
 
+// offs[0 .. n) = exclusive scan of counts[0 .. n), offs[n] = the total.
+void scan_counts(wv_plan* pl, const uint32_t* counts, uint32_t* offs, uint32_t n) {
+    const uint32_t chunks = std::max<uint32_t>(1, div_up(n, kScanChunk));
+    pl->scan_tmp.alloc((size_t)2 * chunks * 4);
+    uint32_t* sums = pl->scan_tmp.as<uint32_t>();
+    uint32_t* bases = sums + chunks;
+    hipLaunchKernelGGL(k_scan_sums, dim3(chunks), dim3(kScanThreads), 0, pl->stream, counts, n, sums);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, pl->stream, sums, bases, chunks, offs + n);
+    hipLaunchKernelGGL(k_scan_chunks, dim3(chunks), dim3(kScanThreads), 0, pl->stream, counts, n, bases, offs);
+    WV_HIP(hipGetLastError());
+}
+
 // Compacts the cells of planes 1 .. nz-2 that `at` selects into `cells`, in
 // pitched order (counts and offsets per 256 cells in `counts`, `offs`);
 // returns their number.
@@ -541,9 +603,7 @@ uint32_t compact_cells(wv_plan* pl, const At& at, const Geo& geo, DBuf& counts, 
     offs.alloc((rblocks + 1) * 4);
     hipLaunchKernelGGL((k_ring_count<At>), dim3((uint32_t)rblocks), dim3(kBlock), 0, pl->stream, at,
                        counts.as<uint32_t>(), geo);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, pl->stream, counts.as<uint32_t>(), offs.as<uint32_t>(),
-                       (uint32_t)rblocks, offs.as<uint32_t>() + rblocks);
-    WV_HIP(hipGetLastError());
+    scan_counts(pl, counts.as<uint32_t>(), offs.as<uint32_t>(), (uint32_t)rblocks);
     uint32_t n = 0;
     WV_HIP(hipMemcpyAsync(&n, offs.as<uint32_t>() + rblocks, 4, hipMemcpyDeviceToHost, pl->stream));
     WV_HIP(hipStreamSynchronize(pl->stream));
@@ -595,9 +655,7 @@ void run_begin_impl(wv_plan* pl, const wv_tap* taps, size_t n_taps, const double
     pl->seg_stride = L.seg / kBlock;
     hipLaunchKernelGGL(k_sp_count, dim3((uint32_t)nblocks), dim3(kBlock), 0, pl->stream, pl->flags.as<uint8_t>(),
                        pl->force.as<uint32_t>(), pl->counts.as<uint32_t>(), geo, wo);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, pl->stream, pl->counts.as<uint32_t>(), pl->offs.as<uint32_t>(),
-                       (uint32_t)nblocks, pl->offs.as<uint32_t>() + nblocks);
-    WV_HIP(hipGetLastError());
+    scan_counts(pl, pl->counts.as<uint32_t>(), pl->offs.as<uint32_t>(), (uint32_t)nblocks);
     uint32_t n_special = 0;
     WV_HIP(hipMemcpyAsync(&n_special, pl->offs.as<uint32_t>() + nblocks, 4, hipMemcpyDeviceToHost, pl->stream));
     WV_HIP(hipStreamSynchronize(pl->stream));
@@ -941,7 +999,7 @@ uint64_t wv::plan_bytes(const wv_plan* pl) {
                           &pl->offs, &pl->b_cell_sorted, &pl->b_set_sorted, &pl->set_n, &pl->set_off, &pl->set_c,
                           &pl->st_u, &pl->st_g, &pl->fa, &pl->fb, &pl->fc, &pl->fd, &pl->fe, &pl->sp_rec,
                           &pl->sp_tapw, &pl->sp_cell, &pl->ring_cell, &pl->ring_counts, &pl->ring_offs,
-                          &pl->ring2_cell, &pl->ring2_counts, &pl->ring2_offs, &pl->tap_cell, &pl->tap_w, &pl->recv,
+                          &pl->ring2_cell, &pl->ring2_counts, &pl->ring2_offs, &pl->scan_tmp, &pl->tap_cell, &pl->tap_w, &pl->recv,
                           &pl->ir, &pl->freqs, &pl->spec})
         t += d->bytes;
     return t;

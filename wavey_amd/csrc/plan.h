@@ -109,6 +109,7 @@ struct wv_plan {
     DBuf sp_cell, ring_cell, ring_counts, ring_offs;
     uint32_t n_ring = 0;
     DBuf fe, d2bits, ring2_cell, ring2_counts, ring2_offs;  // fe: the fifth field buffer
+    DBuf scan_tmp;  // chunk sums and bases of the compaction scans
     uint32_t n_ring2 = 0;
     // Which field buffer (0..4: fa, fb, fc, fd, fe) holds p and which prev.
     int buf_p = 0, buf_q = 1;
