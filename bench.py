@@ -449,7 +449,10 @@ def precision_check(case, taps, ir32, steps, precision):
     from wavey_amd import fdtd
 
     def errs(a, ref):
-        per = [rel_l2(a[r], ref[r]) for r in range(len(ref)) if np.linalg.norm(ref[r]) > 0]
+        # Per receiver, those the pulse has reached: series above 1e-3 of the
+        # largest one (the others hold only the stencil's precursor).
+        norms = np.linalg.norm(ref, axis=1)
+        per = [rel_l2(a[r], ref[r]) for r in range(len(ref)) if norms[r] > 1e-3 * norms.max()]
         return {"rel_l2_all": rel_l2(a, ref), "rel_l2_max_receiver": max(per) if per else None,
                 "receivers_reached": len(per)}
 

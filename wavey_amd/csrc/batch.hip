@@ -293,6 +293,9 @@ template <typename T, typename S>
 void enqueue_block(wv_batch* b, bool use_graph) {
     const uint32_t n = (uint32_t)b->table.size();
     if (n == 0) return;
+    // Re-centring of the instances whose block starts at one of its steps
+    // (their p and prev are in the plan's buf_p, buf_q at a block start).
+    for (Instance* in : b->table) recentre(in->plan.get(), in->n0, b->stream);
     auto* e = b->entries.as<BatchEntry<T, S>>();
     const uint32_t* be = b->block_entry.as<uint32_t>();
     auto issue = [&]() {

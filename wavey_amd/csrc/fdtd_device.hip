@@ -397,7 +397,7 @@ __global__ __launch_bounds__(kBlock) void k_sp_write(const uint8_t* __restrict__
 // in n order (fdtd.rs:679-682).
 template <typename T>
 __global__ void k_dft(const T* __restrict__ ir, uint32_t steps, uint32_t n_recv, const double* __restrict__ freqs,
-                      uint32_t nf, double dt, double* __restrict__ out) {
+                      uint32_t nf, double dt, const double* __restrict__ d0h, double* __restrict__ out) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n_recv * nf) return;
     const uint32_t r = t / nf, fi = t % nf;
@@ -408,7 +408,8 @@ __global__ void k_dft(const T* __restrict__ ir, uint32_t steps, uint32_t n_recv,
         const double ph = -2.0 * 3.141592653589793 * (x - trunc(x));
         double sv, cv;
         sincos(ph, &sv, &cv);
-        const double v = (double)ir[(size_t)n * n_recv + r];
+        double v = (double)ir[(size_t)n * n_recv + r];
+        if (d0h) v += d0h[offset_index(n)];  // FP32: the pressure is the stored value plus the offset
         re += v * cv;
         im += v * sv;
     }
@@ -427,6 +428,62 @@ template <typename T>
 __global__ void k_fill(T* __restrict__ p, size_t n, T v) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
+}
+
+// Re-centring of FP32 fields: the mean of p^n over the air cells in a fixed
+// order (block b sums rows b, b + grid, ... thread by thread in x, then a
+// tree in LDS), the final sum in one block, then every cell of p^n and
+// p^{n-1} less the mean and the run's offset plus it.
+constexpr uint32_t kMeanBlocks = 2048;
+constexpr uint32_t kOffsets = 32;  // offset_index of any step below 2^31 is under 26
+
+__global__ __launch_bounds__(256) void k_mean_part(const uint8_t* __restrict__ flags, const float* __restrict__ p,
+                                                   uint32_t nx, uint32_t pitch, uint32_t rows,
+                                                   double* __restrict__ part) {
+    __shared__ double s[256];
+    double acc = 0.0;
+    for (uint32_t r = blockIdx.x; r < rows; r += gridDim.x)
+        for (uint32_t x = threadIdx.x; x < nx; x += 256)
+            if (flags[(uint64_t)r * nx + x] & 64u) acc += (double)p[(uint64_t)r * pitch + x];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (uint32_t h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) s[threadIdx.x] += s[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+__global__ __launch_bounds__(256) void k_mean_final(const double* __restrict__ part, uint32_t n, double air,
+                                                    float* __restrict__ shift, double* __restrict__ d0,
+                                                    double* __restrict__ d0h, uint32_t j) {
+    __shared__ double s[256];
+    double acc = 0.0;
+    for (uint32_t i = threadIdx.x; i < n; i += 256) acc += part[i];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (uint32_t h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) s[threadIdx.x] += s[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const float m = (float)(s[0] / air);
+        *shift = m;
+        *d0 += (double)m;
+        d0h[j] = *d0;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_shift(float4* __restrict__ a, float4* __restrict__ b, uint64_t n4,
+                                               const float* __restrict__ shift) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n4) return;
+    const float m = *shift;
+    float4 u = a[i], v = b[i];
+    u.x -= m; u.y -= m; u.z -= m; u.w -= m;
+    v.x -= m; v.y -= m; v.z -= m; v.w -= m;
+    a[i] = u;
+    b[i] = v;
 }
 
 }  // namespace
@@ -579,6 +636,27 @@ void fill_async(wv_plan* pl, DBuf& d, size_t n) {
     WV_HIP(hipGetLastError());
 }
 
+}  // namespace
+
+void wv::recentre(wv_plan* pl, size_t n, hipStream_t stream) {
+    if (!recentres_at(pl, n) || pl->recentred_at == n || pl->air == 0) return;
+    pl->recentred_at = n;
+    DBuf* fields[5] = {&pl->fa, &pl->fb, &pl->fc, &pl->fd, &pl->fe};
+    const uint32_t rows = pl->ny * pl->nz;
+    const uint32_t grid = std::min(rows, kMeanBlocks);
+    hipLaunchKernelGGL(k_mean_part, dim3(grid), dim3(256), 0, stream, pl->flags.as<uint8_t>(),
+                       fields[pl->buf_p]->as<float>(), pl->nx, pl->pitch, rows, pl->mean_part.as<double>());
+    hipLaunchKernelGGL(k_mean_final, dim3(1), dim3(256), 0, stream, pl->mean_part.as<double>(), grid,
+                       (double)pl->air, pl->shift.as<float>(), pl->d0.as<double>(), pl->d0h.as<double>(),
+                       offset_index((uint32_t)n));
+    const uint64_t n4 = pl->npitched / 4;  // the pitch is a multiple of 64 cells
+    hipLaunchKernelGGL(k_shift, dim3(div_up(n4, 256)), dim3(256), 0, stream, fields[pl->buf_p]->as<float4>(),
+                       fields[pl->buf_q]->as<float4>(), n4, pl->shift.as<float>());
+    WV_HIP(hipGetLastError());
+}
+
+namespace {
+
 // offs[0 .. n) = exclusive scan of counts[0 .. n), offs[n] = the total.
 void scan_counts(wv_plan* pl, const uint32_t* counts, uint32_t* offs, uint32_t n) {
     const uint32_t chunks = std::max<uint32_t>(1, div_up(n, kScanChunk));
@@ -721,6 +799,13 @@ void run_begin_impl(wv_plan* pl, const wv_tap* taps, size_t n_taps, const double
     else fill_async<float>(pl, pl->ir, steps * n_recv);
     fill_async<S>(pl, pl->st_u, nstate);
     fill_async<S>(pl, pl->st_g, nstate);
+    pl->d0.alloc(8);
+    pl->d0h.alloc(kOffsets * 8);
+    pl->mean_part.alloc(kMeanBlocks * 8);
+    pl->shift.alloc(4);
+    fill_async<double>(pl, pl->d0, 1);
+    fill_async<double>(pl, pl->d0h, kOffsets);
+    pl->recentred_at = 0;
     WV_HIP(hipStreamSynchronize(pl->stream));
     pl->signal.assign(signal, signal + steps);
     pl->steps = steps;
@@ -813,9 +898,16 @@ uint64_t enqueue_range(wv_plan* pl, size_t n, size_t stop, hipStream_t stream) {
     T* ir = pl->ir.as<T>();
     uint64_t launches = 0;
     while (n < stop) {
+        // Passes end at re-centring steps; the re-centring runs before the
+        // step's update.
+        if (recentres_at(pl, n) && pl->recentred_at != n) {
+            recentre(pl, n, stream);
+            launches += 3;
+        }
+        const size_t cap = std::min(stop, next_recentre(pl, n));
         T* P = bufs[pl->buf_p];
         T* Q = bufs[pl->buf_q];
-        if (triples && n + 3 <= stop) {
+        if (triples && n + 3 <= cap) {
             // Steps n, n+1, n+2 in three launches; outputs go to three free
             // buffers, which then hold p^{n+1} (read by passes 2 and 3 only),
             // p^{n+2} (prev) and p^{n+3} (p).
@@ -848,7 +940,7 @@ uint64_t enqueue_range(wv_plan* pl, size_t n, size_t stop, hipStream_t stream) {
             pl->buf_q = o2;
             launches += 3;
             n += 3;
-        } else if (pairs && n + 2 <= stop) {
+        } else if (pairs && n + 2 <= cap) {
             // Steps n, n+1 in two launches; outputs go to two free buffers,
             // which then hold p (p^{n+2}) and prev (p^{n+1}).
             const int o1 = free_buf(0), o2 = free_buf(1);
@@ -931,16 +1023,23 @@ void finish_impl(wv_plan* pl, const double* freqs, size_t nf, int mode, double* 
         pl->spec.alloc(R * nf * 16);
         hipLaunchKernelGGL(k_dft<T>, dim3(div_up(R * nf, 64)), dim3(64), 0, pl->stream, pl->ir.as<T>(),
                            (uint32_t)Tn, (uint32_t)R, pl->freqs.as<double>(), (uint32_t)nf, pl->dt,
-                           pl->spec.as<double>());
+                           pl->prec == WV_FP64 ? nullptr : pl->d0h.as<double>(), pl->spec.as<double>());
         WV_HIP(hipGetLastError());
         WV_HIP(hipMemcpyAsync(spectra_out, pl->spec.p, R * nf * 16, hipMemcpyDeviceToHost, pl->stream));
     }
     if ((mode & WV_MODE_IR) && ir_out && R) {
         std::vector<T> h(Tn * R);
+        std::vector<double> off(kOffsets, 0.0);
         if (Tn) WV_HIP(hipMemcpyAsync(h.data(), pl->ir.p, Tn * R * sizeof(T), hipMemcpyDeviceToHost, pl->stream));
+        if (pl->prec != WV_FP64)
+            WV_HIP(hipMemcpyAsync(off.data(), pl->d0h.p, kOffsets * 8, hipMemcpyDeviceToHost, pl->stream));
         WV_HIP(hipStreamSynchronize(pl->stream));
         for (size_t r = 0; r < R; ++r)
-            for (size_t n = 0; n < Tn; ++n) ir_out[r * pl->steps + n] = (double)h[n * R + r];
+            for (size_t n = 0; n < Tn; ++n) {
+                const double v = (double)h[n * R + r];
+                // FP32: the stored value plus the offset in force at step n.
+                ir_out[r * pl->steps + n] = pl->prec == WV_FP64 ? v : v + off[offset_index((uint32_t)n)];
+            }
     }
     WV_HIP(hipStreamSynchronize(pl->stream));
 }
@@ -999,7 +1098,8 @@ uint64_t wv::plan_bytes(const wv_plan* pl) {
                           &pl->offs, &pl->b_cell_sorted, &pl->b_set_sorted, &pl->set_n, &pl->set_off, &pl->set_c,
                           &pl->st_u, &pl->st_g, &pl->fa, &pl->fb, &pl->fc, &pl->fd, &pl->fe, &pl->sp_rec,
                           &pl->sp_tapw, &pl->sp_cell, &pl->ring_cell, &pl->ring_counts, &pl->ring_offs,
-                          &pl->ring2_cell, &pl->ring2_counts, &pl->ring2_offs, &pl->scan_tmp, &pl->tap_cell, &pl->tap_w, &pl->recv,
+                          &pl->ring2_cell, &pl->ring2_counts, &pl->ring2_offs, &pl->scan_tmp, &pl->d0, &pl->d0h, &pl->mean_part,
+                          &pl->shift, &pl->tap_cell, &pl->tap_w, &pl->recv,
                           &pl->ir, &pl->freqs, &pl->spec})
         t += d->bytes;
     return t;

@@ -110,6 +110,12 @@ struct wv_plan {
     uint32_t n_ring = 0;
     DBuf fe, d2bits, ring2_cell, ring2_counts, ring2_offs;  // fe: the fifth field buffer
     DBuf scan_tmp;  // chunk sums and bases of the compaction scans
+    // FP32 re-centring: the run's offset (double), the offsets in force
+    // from each re-centring on (d0h[j], j = offset_index(n), d0h[0] = 0), the
+    // mean's block partial sums and the last shift (float), and the step
+    // last re-centred at.
+    DBuf d0, d0h, mean_part, shift;
+    size_t recentred_at = 0;
     uint32_t n_ring2 = 0;
     // Which field buffer (0..4: fa, fb, fc, fd, fe) holds p and which prev.
     int buf_p = 0, buf_q = 1;
@@ -186,6 +192,7 @@ inline StepParams<T, S> make_params(wv_plan* pl, uint32_t tiles_per_block) {
     a.max_slots = pl->max_slots;
     a.n_recv = pl->n_recv;
     a.recv = pl->recv.as<uint32_t>();
+    a.d0 = pl->d0.as<double>();
     const double l2 = kCourant * kCourant;
     if (sizeof(T) == 4) {
         // FP32 lambda^2: 1/3 rounded toward zero, so 6 l2 <= 2 (the scheme stays
@@ -249,6 +256,32 @@ void run_begin(wv_plan* pl, const wv_tap* taps, size_t n_taps, const double* sig
 // then name the buffers of p^stop and p^(stop-1). Returns the launches.
 // Throws DevErr.
 uint64_t enqueue_steps(wv_plan* pl, size_t n, size_t stop, hipStream_t stream);
+// FP32 re-centring. A room that keeps a mean pressure after the source
+// pulse (walls that hardly absorb at DC) holds every FP32 cell near that
+// mean, so the stored field resolves the waves on top of it, and the
+// velocity p^n - p^{n-1}, only in steps of the mean's ulp, and the rounding
+// that accumulates in the room's mean-pressure mode grows with the run.
+// At steps 128, 256, 512, ... (powers of two) the mean of p^n over the air
+// cells (a fixed-order FP64 sum, so every pass mode and batch gets the same
+// value) moves from both stored fields (p^n and p^{n-1} less the mean, in
+// FP32, every cell) into the run's offset d0. The interior update does not
+// depend on a uniform shift (it is formed from differences); the boundary
+// slots and the records add d0 back. FP64 runs are never re-centred.
+inline bool recentres_at(const wv_plan* pl, size_t n) {
+    return pl->prec != WV_FP64 && n >= 128 && (n & (n - 1)) == 0;
+}
+// The first re-centring step after n (SIZE_MAX for FP64): multi-step passes
+// end there.
+inline size_t next_recentre(const wv_plan* pl, size_t n) {
+    if (pl->prec == WV_FP64) return ~(size_t)0;
+    size_t m = 128;
+    while (m <= n) m <<= 1;
+    return m;
+}
+// Enqueues the re-centring of step n on `stream` (p^n and p^{n-1} in the
+// plan's buffers buf_p, buf_q), once per step of a run. Throws DevErr.
+void recentre(wv_plan* pl, size_t n, hipStream_t stream);
+
 // Spectra (FP64 DFT of the record) and/or the receiver series of the run's
 // completed steps into caller buffers. Throws DevErr.
 void run_finish(wv_plan* pl, const double* freqs, size_t nf, int mode, double* spectra_out, double* ir_out);

@@ -52,9 +52,20 @@ struct StepParams {
     uint32_t n_recv;
     const uint32_t* __restrict__ recv;
     T* __restrict__ ir;  // this step's receiver row
+    // FP32 fields hold the pressure less the run's offset *d0, which
+    // re-centring moves out of them: the boundary slots add it back, and the
+    // record (the stored value) gets it in FP64 when it is read out. FP64
+    // fields hold the pressure.
+    const double* __restrict__ d0;
     T l2, c2;            // interior: p+ = c2 p + l2 sum - p-
     S l2s, lam, half_dt, s_n;
 };
+
+// Index of the offset in force at step n: the number of re-centrings at
+// steps 128, 256, 512, ... up to n.
+__host__ __device__ __forceinline__ uint32_t offset_index(uint32_t n) {
+    return n < 128 ? 0u : 31u - (uint32_t)__builtin_clz(n) - 6u;
+}
 
 template <typename T>
 __device__ __forceinline__ T interior_update(T c, T q, T xm, T xp, T ym, T yp, T zm, T zp, T l2, T c2) {
@@ -306,26 +317,39 @@ __device__ __forceinline__ S special_update(const StepParams<T, S>& a, uint32_t 
             // stay in registers (one pass for nodes with up to G slots);
             // further groups are read again for the second pass.
             constexpr uint32_t G = kSlotGroup<S>;
+            // The slots see the pressure itself: FP32 fields add the run's
+            // offset (the Laplacian and the update's own terms do not
+            // depend on it).
+            S off = S(0), pa = pc, ppa = pp;
+            if constexpr (sizeof(T) == 4) {
+                off = (S)*a.d0;
+                pa = pc + off;
+                ppa = pp + off;
+            }
             SlotGroup<S, G> g0;
             g0.load(a, c0, rank, 0, ns, u0);
             S s1 = 0, s2 = 0;
-            g0.sums(0, ns, pc, pp, s1, s2);
+            g0.sums(0, ns, pa, ppa, s1, s2);
             for (uint32_t jb = G; jb < ns; jb += G) {
                 SlotGroup<S, G> gj;
                 gj.load(a, c0, rank, jb, ns);
-                gj.sums(jb, ns, pc, pp, s1, s2);
+                gj.sums(jb, ns, pa, ppa, s1, s2);
             }
+            S nabs;
             if constexpr (sizeof(S) == 8) {
-                next = (S(2) * pc - pp + lap - a.lam * s2) / (S(1) + a.lam * s1);
+                nabs = (S(2) * pa - ppa + lap - a.lam * s2) / (S(1) + a.lam * s1);
+                next = nabs;
+                if constexpr (sizeof(T) == 4) next = nabs - off;
             } else {
                 // The same solve written as pc + (small terms).
-                next = pc + ((pc - pp) + lap - a.lam * (s2 + s1 * pc)) / (S(1) + a.lam * s1);
+                next = pc + ((pc - pp) + lap - a.lam * (s2 + s1 * pa)) / (S(1) + a.lam * s1);
+                nabs = next + off;
             }
-            g0.advance(a, rank, 0, ns, pc, pp, next);
+            g0.advance(a, rank, 0, ns, pa, ppa, nabs);
             for (uint32_t jb = G; jb < ns; jb += G) {
                 SlotGroup<S, G> gj;
                 gj.load(a, c0, rank, jb, ns);
-                gj.advance(a, rank, jb, ns, pc, pp, next);
+                gj.advance(a, rank, jb, ns, pa, ppa, nabs);
             }
         }
     }
