@@ -432,8 +432,8 @@ __global__ void k_fill(T* __restrict__ p, size_t n, T v) {
 
 // Re-centring of FP32 fields: the mean of p^n over the air cells in a fixed
 // order (block b sums rows b, b + grid, ... thread by thread in x, then a
-// tree in LDS), the final sum in one block, then every cell of p^n and
-// p^{n-1} less the mean and the run's offset plus it.
+// tree in LDS), the final sum and the decision in one block, then every
+// cell of p^n and p^{n-1} less the shift and the run's offset plus it.
 constexpr uint32_t kMeanBlocks = 2048;
 constexpr uint32_t kOffsets = 32;  // offset_index of any step below 2^31 is under 26
 
@@ -456,7 +456,8 @@ __global__ __launch_bounds__(256) void k_mean_part(const uint8_t* __restrict__ f
 
 __global__ __launch_bounds__(256) void k_mean_final(const double* __restrict__ part, uint32_t n, double air,
                                                     float* __restrict__ shift, double* __restrict__ d0,
-                                                    double* __restrict__ d0h, uint32_t j) {
+                                                    double* __restrict__ d0h, double* __restrict__ prev_mean,
+                                                    uint32_t j) {
     __shared__ double s[256];
     double acc = 0.0;
     for (uint32_t i = threadIdx.x; i < n; i += 256) acc += part[i];
@@ -467,7 +468,14 @@ __global__ __launch_bounds__(256) void k_mean_final(const double* __restrict__ p
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        const float m = (float)(s[0] / air);
+        // The stored mean and the pressure's (plus the offset). The first
+        // check only records the pressure's mean; later ones follow a mean
+        // that persists (at least half of the last check's) and, where it
+        // decays, move the offset back into the fields.
+        const double stored = s[0] / air, mean = stored + *d0;
+        float m = 0.0f;
+        if (j > 1) m = fabs(mean) >= 0.5 * fabs(*prev_mean) ? (float)stored : (float)(-*d0);
+        *prev_mean = mean;
         *shift = m;
         *d0 += (double)m;
         d0h[j] = *d0;
@@ -477,8 +485,8 @@ __global__ __launch_bounds__(256) void k_mean_final(const double* __restrict__ p
 __global__ __launch_bounds__(256) void k_shift(float4* __restrict__ a, float4* __restrict__ b, uint64_t n4,
                                                const float* __restrict__ shift) {
     const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n4) return;
     const float m = *shift;
+    if (i >= n4 || m == 0.0f) return;
     float4 u = a[i], v = b[i];
     u.x -= m; u.y -= m; u.z -= m; u.w -= m;
     v.x -= m; v.y -= m; v.z -= m; v.w -= m;
@@ -648,7 +656,7 @@ void wv::recentre(wv_plan* pl, size_t n, hipStream_t stream) {
                        fields[pl->buf_p]->as<float>(), pl->nx, pl->pitch, rows, pl->mean_part.as<double>());
     hipLaunchKernelGGL(k_mean_final, dim3(1), dim3(256), 0, stream, pl->mean_part.as<double>(), grid,
                        (double)pl->air, pl->shift.as<float>(), pl->d0.as<double>(), pl->d0h.as<double>(),
-                       offset_index((uint32_t)n));
+                       pl->prev_mean.as<double>(), offset_index((uint32_t)n));
     const uint64_t n4 = pl->npitched / 4;  // the pitch is a multiple of 64 cells
     hipLaunchKernelGGL(k_shift, dim3(div_up(n4, 256)), dim3(256), 0, stream, fields[pl->buf_p]->as<float4>(),
                        fields[pl->buf_q]->as<float4>(), n4, pl->shift.as<float>());
@@ -800,10 +808,12 @@ void run_begin_impl(wv_plan* pl, const wv_tap* taps, size_t n_taps, const double
     fill_async<S>(pl, pl->st_u, nstate);
     fill_async<S>(pl, pl->st_g, nstate);
     pl->d0.alloc(8);
+    pl->prev_mean.alloc(8);
     pl->d0h.alloc(kOffsets * 8);
     pl->mean_part.alloc(kMeanBlocks * 8);
     pl->shift.alloc(4);
     fill_async<double>(pl, pl->d0, 1);
+    fill_async<double>(pl, pl->prev_mean, 1);
     fill_async<double>(pl, pl->d0h, kOffsets);
     pl->recentred_at = 0;
     WV_HIP(hipStreamSynchronize(pl->stream));
@@ -1098,7 +1108,7 @@ uint64_t wv::plan_bytes(const wv_plan* pl) {
                           &pl->offs, &pl->b_cell_sorted, &pl->b_set_sorted, &pl->set_n, &pl->set_off, &pl->set_c,
                           &pl->st_u, &pl->st_g, &pl->fa, &pl->fb, &pl->fc, &pl->fd, &pl->fe, &pl->sp_rec,
                           &pl->sp_tapw, &pl->sp_cell, &pl->ring_cell, &pl->ring_counts, &pl->ring_offs,
-                          &pl->ring2_cell, &pl->ring2_counts, &pl->ring2_offs, &pl->scan_tmp, &pl->d0, &pl->d0h, &pl->mean_part,
+                          &pl->ring2_cell, &pl->ring2_counts, &pl->ring2_offs, &pl->scan_tmp, &pl->d0, &pl->prev_mean, &pl->d0h, &pl->mean_part,
                           &pl->shift, &pl->tap_cell, &pl->tap_w, &pl->recv,
                           &pl->ir, &pl->freqs, &pl->spec})
         t += d->bytes;

@@ -110,11 +110,11 @@ struct wv_plan {
     uint32_t n_ring = 0;
     DBuf fe, d2bits, ring2_cell, ring2_counts, ring2_offs;  // fe: the fifth field buffer
     DBuf scan_tmp;  // chunk sums and bases of the compaction scans
-    // FP32 re-centring: the run's offset (double), the offsets in force
-    // from each re-centring on (d0h[j], j = offset_index(n), d0h[0] = 0), the
-    // mean's block partial sums and the last shift (float), and the step
-    // last re-centred at.
-    DBuf d0, d0h, mean_part, shift;
+    // FP32 re-centring: the run's offset (double), the pressure's mean at
+    // the last check, the offsets in force from each check on (d0h[j],
+    // j = offset_index(n), d0h[0] = 0), the mean's block partial sums and the
+    // last shift (float), and the step last checked at.
+    DBuf d0, prev_mean, d0h, mean_part, shift;
     size_t recentred_at = 0;
     uint32_t n_ring2 = 0;
     // Which field buffer (0..4: fa, fb, fc, fd, fe) holds p and which prev.
@@ -262,11 +262,15 @@ uint64_t enqueue_steps(wv_plan* pl, size_t n, size_t stop, hipStream_t stream);
 // velocity p^n - p^{n-1}, only in steps of the mean's ulp, and the rounding
 // that accumulates in the room's mean-pressure mode grows with the run.
 // At steps 128, 256, 512, ... (powers of two) the mean of p^n over the air
-// cells (a fixed-order FP64 sum, so every pass mode and batch gets the same
-// value) moves from both stored fields (p^n and p^{n-1} less the mean, in
-// FP32, every cell) into the run's offset d0. The interior update does not
-// depend on a uniform shift (it is formed from differences); the boundary
-// slots and the records add d0 back. FP64 runs are never re-centred.
+// cells is taken (a fixed-order FP64 sum, so every pass mode and batch gets
+// the same value). The first check records the pressure's mean; from the
+// second on, while that mean keeps at least half of its value at the last
+// check (a room that holds it), the stored mean moves from both fields
+// (p^n and p^{n-1} less it, in FP32, every cell) into the run's offset d0,
+// and once it decays (a room that absorbs it) the offset moves back into
+// the fields. The interior update does not depend on a uniform shift (it
+// is formed from differences); the boundary slots and the records add d0
+// back. FP64 runs are never re-centred.
 inline bool recentres_at(const wv_plan* pl, size_t n) {
     return pl->prec != WV_FP64 && n >= 128 && (n & (n - 1)) == 0;
 }
