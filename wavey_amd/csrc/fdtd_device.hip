@@ -3,8 +3,9 @@
 //
 // Layout (the step kernel itself is in step_kernels.h):
 //
-//  * Fields p, prev live in a pitched grid: row pitch = nx rounded up to 64
-//    cells, so a wave's 16-byte-per-lane accesses cover aligned row segments.
+//  * Fields p, prev live in a pitched grid: row pitch = nx rounded up to 32
+//    cells, so rows start on 128-byte lines and a wave's 16-byte-per-lane
+//    accesses cover aligned row segments.
 //  * An interior bitmap, packed along z (one 32-bit word per column per 32
 //    planes), marks plain interior air: six air neighbours, no boundary
 //    slots, no source tap.
@@ -529,7 +530,7 @@ void wv::plan_setup(wv_plan* pl, const wv_grid* g) {
     pl->ny = (uint32_t)g->n[1];
     pl->nz = (uint32_t)g->n[2];
     pl->cells = g->n[0] * g->n[1] * g->n[2];
-    pl->pitch = (pl->nx + 63u) & ~63u;
+    pl->pitch = (pl->nx + 31u) & ~31u;  // rows start on 128-byte lines (FP32; 256 bytes FP64)
     pl->sz = pl->pitch * pl->ny;
     pl->npitched = (uint64_t)pl->sz * pl->nz;
     pl->dx = g->dx;
@@ -657,7 +658,7 @@ void wv::recentre(wv_plan* pl, size_t n, hipStream_t stream) {
     hipLaunchKernelGGL(k_mean_final, dim3(1), dim3(256), 0, stream, pl->mean_part.as<double>(), grid,
                        (double)pl->air, pl->shift.as<float>(), pl->d0.as<double>(), pl->d0h.as<double>(),
                        pl->prev_mean.as<double>(), offset_index((uint32_t)n));
-    const uint64_t n4 = pl->npitched / 4;  // the pitch is a multiple of 64 cells
+    const uint64_t n4 = pl->npitched / 4;  // the pitch is a multiple of 32 cells
     hipLaunchKernelGGL(k_shift, dim3(div_up(n4, 256)), dim3(256), 0, stream, fields[pl->buf_p]->as<float4>(),
                        fields[pl->buf_q]->as<float4>(), n4, pl->shift.as<float>());
     WV_HIP(hipGetLastError());
@@ -1118,7 +1119,7 @@ uint64_t wv::plan_bytes(const wv_plan* pl) {
 uint64_t wv::estimate_bytes(const wv_grid* g, int prec, size_t steps, size_t n_recv, size_t n_taps) {
     const uint64_t tsz = prec == WV_FP64 ? 8 : 4, ssz = prec == WV_FP32 ? 4 : 8;
     const uint64_t cells = g->n[0] * g->n[1] * g->n[2];
-    const uint64_t pitch = (g->n[0] + 63) & ~63ull;
+    const uint64_t pitch = (g->n[0] + 31) & ~31ull;
     const uint64_t npitched = pitch * g->n[1] * g->n[2];
     uint32_t max_slots = 0;
     for (uint64_t b = 0; b < g->n_boundary; ++b) max_slots = std::max(max_slots, g->b_start[b + 1] - g->b_start[b]);
@@ -1151,7 +1152,7 @@ wv_status wv::check_grid(const wv_grid* g, char* err, size_t err_len) {
             return WV_ERR_INVALID;
         }
     const uint64_t cells = g->n[0] * g->n[1] * g->n[2];
-    const uint64_t pitched = ((g->n[0] + 63) & ~63ull) * g->n[1] * g->n[2];
+    const uint64_t pitched = ((g->n[0] + 31) & ~31ull) * g->n[1] * g->n[2];
     if (cells > kMaxCells || pitched >= (1ull << 32)) {
         set_err(err, err_len, "grid of " + std::to_string(cells / 1000000) + " M cells exceeds the GPU layout limit");
         return WV_ERR_LIMIT;
