@@ -255,6 +255,20 @@ def bench_room_mix(args):
     identical = sum(int(np.array_equal(x, y)) for x, y in zip(batched, plain))
     max_diff = max(float(np.max(np.abs(x - y))) for x, y in zip(batched, plain))
 
+    # The other launch form, with the tuned shapes: one persistent launch per
+    # 128-step block (or a launch per step, if the autotuner kept that one).
+    other = fdtd.BATCH_STEPS if bt["launch"] == fdtd.BATCH_PERSISTENT else fdtd.BATCH_PERSISTENT
+    with fdtd.Batch(args.precision) as b2:
+        ids2 = b2.add_many([(c.grid, taps, c.signal, c.receivers) for _, c, taps in inst], threads=8)
+        b2.set_launch(other)
+        t0 = time.perf_counter()
+        b2.run(graphs=not args.no_graphs)
+        t_other = time.perf_counter() - t0
+        same_other = sum(int(np.array_equal(b2.result(i)[0], plain[k])) for k, i in enumerate(ids2))
+    forms = {fdtd.BATCH_STEPS: "launch_per_step", fdtd.BATCH_PERSISTENT: "persistent"}
+    print(f"bench: batch run {t_run:.2f} s ({forms[bt['launch']]}), {forms[other]} {t_other:.2f} s",
+          file=sys.stderr, flush=True)
+
     # FP32 against an FP64 run of the same scheme, on a sample of instances.
     errs = []
     if args.precision != "fp64":
@@ -286,7 +300,10 @@ def bench_room_mix(args):
                     "device_run_s": round(bt["device_ms"] / 1e3, 3),
                     "run_gcell_per_s": round(work / max(t_run, 1e-9) / 1e9, 1),
                     "graph_launches": bt["graph_launches"], "graph_captures": bt["graph_captures"],
-                    "variant": bt["variant"], "zchunks": zch},
+                    "variant": bt["variant"], "zchunks": zch, "launch": forms[bt["launch"]]},
+        "other_launch_form": {"launch": forms[other], "run_s": round(t_other, 3),
+                              "run_gcell_per_s": round(work / max(t_other, 1e-9) / 1e9, 1),
+                              "bit_identical_to_plain": f"{same_other}/{n}"},
         "speedup_vs_sequential_plain": round(t_plain / t_batch, 3),
         "speedup_step_loop_only": round(t_plain / max(t_run, 1e-9), 3),
         "parity": {"bit_identical_to_plain": f"{identical}/{n}", "max_abs_diff": max_diff,

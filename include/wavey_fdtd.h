@@ -204,10 +204,12 @@ wv_status wv_footprint_estimate(const wv_grid* grid, int precision, size_t steps
 /* ----------------------------------------------------------------- batch
  *
  * Many runs (rooms, the per-source and per-component runs of a wave job,
- * run.rs:645-661) of different grid sizes advanced together: one launch per
- * step covers every instance's tiles and 128-step blocks are replayed as HIP
- * graphs. Each instance runs the same arithmetic as wv_plan_run, so its
- * results equal that run's. The reference runs them one after another.
+ * run.rs:645-661) of different grid sizes advanced together, in 128-step
+ * blocks replayed as HIP graphs. A block is one launch per step over every
+ * instance's tiles (WV_BATCH_STEPS) or one persistent launch (WV_BATCH_PERSISTENT:
+ * each XCD runs a share of the instances through the whole block, an
+ * instance's step waiting only on its own previous step). Each instance runs
+ * the same arithmetic as wv_plan_run, so its results equal that run's. The reference runs them one after another.
  * wv_batch_add may be called from several threads at once (each instance is
  * set up on its own stream); every other call on a batch is made by one
  * thread at a time, with no wv_batch_add in progress. Inputs are copied at
@@ -221,13 +223,19 @@ wv_status wv_batch_add(wv_batch* batch, const wv_grid* grid, const wv_tap* taps,
                        int32_t* id, char* err, size_t err_len);
 /* Times the whole batch over trial_steps steps per candidate: the block variant
  * for the launch, then the z-chunk of each size class (floor(log2(air
- * cells)), largest first). The choices are kept for later batches and servers
- * of the process; every instance restarts from rest. */
+ * cells)), largest first), then the launch form. The choices are kept for
+ * later batches and servers of the process (servers use WV_BATCH_STEPS);
+ * every instance restarts from rest. */
 wv_status wv_batch_autotune(wv_batch* batch, uint32_t trial_steps, char* err, size_t err_len);
 /* Sets one shape for the whole batch (variant as wv_plan_set_tile; zchunk 0 =
  * default); instances restart from rest. */
 wv_status wv_batch_set_tile(wv_batch* batch, int32_t variant, uint32_t zchunk, char* err,
                             size_t err_len);
+enum { WV_BATCH_STEPS = 0, WV_BATCH_PERSISTENT = 1 };
+/* Sets the launch form of the batch's blocks; instances keep their state.
+ * WV_BATCH_PERSISTENT where the device's XCC_ID register does not name 8
+ * XCDs returns WV_ERR_LIMIT. */
+wv_status wv_batch_set_launch(wv_batch* batch, int form, char* err, size_t err_len);
 /* Runs every instance to its last step. use_graphs = 0 issues the same
  * launches one by one. Cancel is polled between step blocks. An instance
  * left alone on a grid of at least 4 Mi cells runs its remaining steps with
@@ -242,8 +250,8 @@ wv_status wv_batch_result(wv_batch* batch, int32_t id, const double* freqs, size
 wv_status wv_batch_info(const wv_batch* batch, int32_t id, uint64_t out[6]);
 /* Last wv_batch_run: device ms (HIP events on the batch stream); out: [step
  * launches per instance block sequence, cell updates, graph launches, graph
- * captures, block variant, blocks run alone]. */
-wv_status wv_batch_timing(const wv_batch* batch, double* ms, uint64_t out[6]);
+ * captures, block variant, blocks run alone, launch form]. */
+wv_status wv_batch_timing(const wv_batch* batch, double* ms, uint64_t out[7]);
 void wv_batch_free(wv_batch* batch);
 
 /* ----------------------------------------------------------------- serving

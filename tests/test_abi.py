@@ -150,3 +150,12 @@ def test_header_builds_and_links_as_c(tmp_path):
     assert r.returncode == 0, r.stdout + r.stderr
     ref = fdtd.source_signal(250.0, 1.6832e-4, 16)
     assert float(r.stdout.split()[0]) == ref[5]
+
+
+def test_batch_launch_form_argument_errors():
+    # A null batch or an unknown launch form is refused before any HIP call.
+    L = fdtd.lib()
+    err = C.create_string_buffer(256)
+    for form in (fdtd.BATCH_STEPS, fdtd.BATCH_PERSISTENT, 5):
+        assert L.wv_batch_set_launch(None, form, err, 256) == -1
+        assert b"invalid argument" in err.value

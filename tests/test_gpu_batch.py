@@ -62,6 +62,53 @@ def test_batch_equals_plain_runs(rooms, graphs):
         assert t["graph_launches"] == 0
 
 
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+@pytest.mark.parametrize("graphs", [True, False])
+def test_persistent_launch_equals_plain_runs(rooms, prec, graphs):
+    """One persistent launch per 128-step block: each of the 8 XCDs runs its
+    share of the instances, an instance's step waiting only on its own
+    previous step. All speaker runs of the five rooms (every XCD gets a
+    share), step counts that end inside a block (the table and the shares
+    rebuilt as instances finish): each instance bit-identical to the same
+    instance in a launch per step, and the first run of each room to its
+    plain run."""
+    cases, plain = rooms
+    inst = [(ci, c, taps) for ci, c in enumerate(cases) for (_, _, taps) in c.runs]
+    assert len(inst) >= 8
+    print(f"case: persistent batch of {len(inst)} instances, {prec}, graphs={graphs}", flush=True)
+    out = {}
+    for form in (fdtd.BATCH_STEPS, fdtd.BATCH_PERSISTENT):
+        with fdtd.Batch(prec) as b:
+            ids = [b.add(c.grid, taps, c.signal, c.receivers) for _, c, taps in inst]
+            b.set_launch(form)
+            assert b.run(graphs=graphs) == fdtd.WV_OK
+            assert b.timing()["launch"] == form
+            out[form] = [b.result(i)[0] for i in ids]
+    first = {}
+    for k, (ci, _, _) in enumerate(inst):
+        first.setdefault(ci, k)
+        assert np.array_equal(out[fdtd.BATCH_PERSISTENT][k], out[fdtd.BATCH_STEPS][k]), f"instance {k}"
+    for ci, k in first.items():
+        assert np.array_equal(out[fdtd.BATCH_PERSISTENT][k], plain[(prec, ci)]), f"room {ci}"
+
+
+def test_persistent_launch_after_autotune(rooms):
+    """The autotuner times both launch forms; the form it keeps, and the
+    persistent form set afterwards, keep every result."""
+    cases, plain = rooms
+    with fdtd.Batch("fp32") as b:
+        ids = [b.add(c.grid, c.runs[0][2], c.signal, c.receivers) for c in cases]
+        b.autotune(trial_steps=4)
+        tuned = b.timing()["launch"]
+        assert tuned in (fdtd.BATCH_STEPS, fdtd.BATCH_PERSISTENT)
+        b.set_launch(fdtd.BATCH_PERSISTENT)
+        assert b.run() == fdtd.WV_OK
+        for i, iid in enumerate(ids):
+            assert np.array_equal(b.result(iid)[0], plain[("fp32", i)]), f"instance {i}"
+        with pytest.raises(fdtd.FdtdError):
+            b.set_launch(7)
+
+
 def test_batch_autotuned_shapes_keep_results(rooms):
     cases, plain = rooms
     print("case: batch autotune, fp32", flush=True)

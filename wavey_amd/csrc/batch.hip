@@ -60,20 +60,16 @@ struct BatchEntry {
     StepParams<T, S> a;  // even-step buffers (p = fa, prev = fb) and the record base
     const S* sig;        // the signal in the slot arithmetic's type
     uint32_t steps;
-    uint32_t n0;      // first step of the current 128-step block
-    uint32_t block0;  // first launch block of this instance
+    uint32_t n0;       // first step of the current 128-step block
+    uint32_t block0;   // first launch block of this instance (a launch per step)
+    uint32_t blocks;   // launch blocks per step: tile blocks, then receiver blocks
+    uint32_t block0x;  // first block in its XCD share's list (persistent launch)
     uint32_t pad;
 };
 
-// One time step of every instance in the table: block b runs local block
-// b - block0 of instance block_entry[b] (its tiles, then its receivers) at
-// step n0 + k. Instances past their last step return at once.
+// Local block lb of an instance at step n: tile blocks, then receiver blocks.
 template <typename T, typename S, int TY>
-__global__ __launch_bounds__(kTileX*(TY + 1)) void k_batch_step(const BatchEntry<T, S>* __restrict__ entries,
-                                                           const uint32_t* __restrict__ block_entry, uint32_t k) {
-    const BatchEntry<T, S>& be = entries[block_entry[blockIdx.x]];
-    const uint32_t n = be.n0 + k;
-    if (n >= be.steps) return;
+__device__ __forceinline__ void batch_item(const BatchEntry<T, S>& be, uint32_t n, uint32_t lb) {
     StepParams<T, S> a = be.a;
     if (n & 1u) {
         // Odd steps swap the two buffers (gpu_fdtd.rs:242-243).
@@ -82,7 +78,6 @@ __global__ __launch_bounds__(kTileX*(TY + 1)) void k_batch_step(const BatchEntry
     }
     a.s_n = be.sig[n];
     a.ir = be.a.ir + (size_t)n * a.n_recv;
-    const uint32_t lb = blockIdx.x - be.block0;
     if (lb < a.main_blocks) {
         block_part<T, S, TY>(a, lb);  // the whole block takes this branch
         return;
@@ -90,6 +85,146 @@ __global__ __launch_bounds__(kTileX*(TY + 1)) void k_batch_step(const BatchEntry
     // p^n before its update (fdtd.rs:673-683).
     const uint32_t r = (lb - a.main_blocks) * blockDim.x + threadIdx.x;
     if (r < a.n_recv) a.ir[r] = a.p[a.recv[r]];
+}
+
+// One time step of every instance in the table: block b runs local block
+// b - block0 of instance block_entry[b] at step n0 + k. Instances past their
+// last step return at once.
+template <typename T, typename S, int TY>
+__global__ __launch_bounds__(kTileX*(TY + 1)) void k_batch_step(const BatchEntry<T, S>* __restrict__ entries,
+                                                           const uint32_t* __restrict__ block_entry, uint32_t k) {
+    const BatchEntry<T, S>& be = entries[block_entry[blockIdx.x]];
+    const uint32_t n = be.n0 + k;
+    if (n >= be.steps) return;
+    batch_item<T, S, TY>(be, n, blockIdx.x - be.block0);
+}
+
+// The persistent form: one launch runs steps kbase .. kbase+K-1 of a block
+// for every instance. Each of the 8 XCDs, each with its own L2, takes a share
+// of the instances (balanced by air cells on the host); the XCD's workgroups
+// hand out its items, (step, instance, local block) in step-major order,
+// kClaimRun at a time from a counter of their own, and an item of step k
+// first waits until every item of its instance's step k-1 has been counted
+// done. A workgroup reads its XCD from the XCC_ID hardware register, so an
+// instance's fields are written and read through one L2 for the whole
+// launch, which keeps them coherent without L2 write-backs or invalidations
+// (what an agent-scope fence costs here, per item): a workgroup's stores are
+// acknowledged by the L2 before it counts its items, and after a wait it
+// invalidates its CU's L1 before reading. The counters are device-scope
+// atomics. A workgroup counts its finished items at the end of each run, or
+// before it waits, and skips the wait when a count it read before already
+// covers the item. Items are claimed in order by running workgroups and wait
+// only on items claimed before them, so the launch drains whatever the
+// residency. A wait is still bounded: past the bound (or once another wait
+// has failed) it sets the error word, which the host checks, and the launch
+// runs on to its end.
+constexpr uint32_t kXcds = 8;
+constexpr uint32_t kLineWords = 32;  // 128 bytes: one cache line per counter
+constexpr uint32_t kSpinLimit = 1u << 21;
+constexpr uint32_t kClaimRun = 4;
+
+struct XcdShare {
+    uint32_t first[kXcds];   // the share's first block in the block list
+    uint32_t blocks[kXcds];  // the share's blocks per step
+};
+
+// Counter words: the XCDs' claim counters (a line each), the error word's
+// line, then one line per table entry (its items done in this launch).
+inline size_t persist_ctl_words(size_t entries) { return (kXcds + 1 + entries) * kLineWords; }
+
+__global__ void k_persist_reset(uint32_t* __restrict__ ctl, uint32_t n_entries) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < kXcds) ctl[i * kLineWords] = 0;
+    if (i < n_entries) ctl[(kXcds + 1 + i) * kLineWords] = 0;
+}
+
+__device__ __forceinline__ uint32_t xcc_id() {
+    uint32_t x;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(x));
+    return x;
+}
+
+template <typename T, typename S, int TY>
+__global__ __launch_bounds__(kTileX*(TY + 1)) void k_batch_persist(const BatchEntry<T, S>* __restrict__ entries,
+                                                              const uint32_t* __restrict__ pblock,
+                                                              uint32_t* __restrict__ ctl, XcdShare xs,
+                                                              uint32_t kbase, uint32_t K) {
+    __shared__ uint32_t s_run, s_seen;
+    const uint32_t x = xcc_id() & (kXcds - 1);
+    const uint32_t nb = xs.blocks[x], first = xs.first[x];
+    const uint32_t total = K * nb;  // below 2^31 (checked on the host)
+    uint32_t* claim = ctl + x * kLineWords;
+    uint32_t* err = ctl + kXcds * kLineWords;
+    uint32_t* done = ctl + (kXcds + 1) * kLineWords;
+    // Items finished but not yet counted (all of one entry), and the last
+    // count read (of entry seen_e).
+    uint32_t pend_e = 0, pend_n = 0, seen_e = ~0u, seen_v = 0;
+    auto flush = [&]() {
+        if (pend_n == 0) return;
+        __builtin_amdgcn_s_waitcnt(0);  // this wave's stores are in the L2
+        __syncthreads();
+        if (threadIdx.x == 0)
+            __hip_atomic_fetch_add(done + pend_e * kLineWords, pend_n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        pend_n = 0;
+    };
+    if (threadIdx.x == 0) s_run = __hip_atomic_fetch_add(claim, kClaimRun, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    uint32_t run = __builtin_amdgcn_readfirstlane(s_run);
+    while (run < total) {
+        __syncthreads();  // every wave has read s_run
+        if (threadIdx.x == 0)  // the next run, claimed ahead
+            s_run = __hip_atomic_fetch_add(claim, kClaimRun, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t run_end = min(run + kClaimRun, total);
+        for (uint32_t item = run; item < run_end; ++item) {
+            const uint32_t k = item / nb, j = item - k * nb;
+            const uint32_t ei = pblock[first + j];
+            const BatchEntry<T, S>& be = entries[ei];
+            if (pend_n && pend_e != ei) flush();
+            const uint32_t need = k * be.blocks;
+            if (k > 0 && !(seen_e == ei && seen_v >= need)) {
+                flush();
+                if (threadIdx.x == 0) {
+                    uint32_t* cnt = done + ei * kLineWords;
+                    uint32_t v, spins = 0;
+                    while ((v = __hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < need) {
+                        if ((spins & 255u) == 0 &&
+                            __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                            break;
+                        if (++spins >= kSpinLimit) {
+                            __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(2);
+                    }
+                    s_seen = v;
+                    // This CU's L1 may hold lines of the instance's fields
+                    // from before the wait: an agent-scope acquire
+                    // invalidates it (and the L2's unmodified lines; the
+                    // other items' writes are in this XCD's L2).
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                    __builtin_amdgcn_s_waitcnt(0);
+                }
+                __syncthreads();
+                seen_e = ei;
+                seen_v = __builtin_amdgcn_readfirstlane(s_seen);
+            }
+            const uint32_t n = be.n0 + kbase + k;
+            if (n < be.steps) batch_item<T, S, TY>(be, n, j - be.block0x);
+            // Every wave is done with the item's shared memory (the tile
+            // block's hand-out counter) before the next item resets it.
+            __syncthreads();
+            pend_e = ei;
+            ++pend_n;
+        }
+        flush();
+        run = __builtin_amdgcn_readfirstlane(s_run);
+    }
+}
+
+// The XCC_ID of each of 64 blocks (the persistent form's check that the
+// register names the 8 XCDs).
+__global__ void k_xcc_probe(uint32_t* __restrict__ out) {
+    if (threadIdx.x == 0) out[blockIdx.x] = xcc_id();
 }
 
 template <typename T, typename S>
@@ -110,6 +245,65 @@ void launch_batch_step(int variant, uint32_t grid, hipStream_t st, const BatchEn
     }
 }
 
+template <typename T, typename S>
+void launch_batch_persist(int variant, uint32_t grid, hipStream_t st, const BatchEntry<T, S>* e, const uint32_t* pb,
+                          uint32_t* ctl, const XcdShare& xs, uint32_t kbase, uint32_t K) {
+    const dim3 blk(variant_threads(variant));
+    switch (variant) {
+        case 0: hipLaunchKernelGGL((k_batch_persist<T, S, 7>), dim3(grid), blk, 0, st, e, pb, ctl, xs, kbase, K); break;
+        case 1: hipLaunchKernelGGL((k_batch_persist<T, S, 3>), dim3(grid), blk, 0, st, e, pb, ctl, xs, kbase, K); break;
+        case 2: hipLaunchKernelGGL((k_batch_persist<T, S, 15>), dim3(grid), blk, 0, st, e, pb, ctl, xs, kbase, K); break;
+        default: hipLaunchKernelGGL((k_batch_persist<T, S, 1>), dim3(grid), blk, 0, st, e, pb, ctl, xs, kbase, K); break;
+    }
+}
+
+// Workgroups of a persistent launch: as many as the CUs hold at once (the
+// occupancy of the variant's kernel), a multiple of the 8 XCDs.
+template <typename T, typename S>
+uint32_t persist_grid(int variant) {
+    int per_cu = 0;
+    const int threads = (int)variant_threads(variant);
+    switch (variant) {
+        case 0: WV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_persist<T, S, 7>, threads, 0)); break;
+        case 1: WV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_persist<T, S, 3>, threads, 0)); break;
+        case 2: WV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_persist<T, S, 15>, threads, 0)); break;
+        default: WV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_persist<T, S, 1>, threads, 0)); break;
+    }
+    int dev = 0, cus = 0;
+    WV_HIP(hipGetDevice(&dev));
+    WV_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const uint32_t g = (uint32_t)std::max(1, per_cu) * (uint32_t)std::max(1, cus);
+    return std::max(kXcds, g / kXcds * kXcds);
+}
+
+// Whether the XCC_ID register names 8 XCDs on this device (each of 0..7
+// seen, nothing else), checked once per device: the persistent form needs
+// every share's workgroups.
+bool xcds_ok() {
+    static std::mutex mu;
+    static std::map<int, bool> known;
+    int dev = 0;
+    WV_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = known.find(dev);
+    if (it != known.end()) return it->second;
+    DBuf d;
+    d.alloc(64 * sizeof(uint32_t));
+    hipLaunchKernelGGL(k_xcc_probe, dim3(64), dim3(64), 0, 0, d.as<uint32_t>());
+    WV_HIP(hipGetLastError());
+    uint32_t h[64];
+    WV_HIP(hipMemcpy(h, d.p, sizeof h, hipMemcpyDeviceToHost));
+    uint32_t seen = 0;
+    bool ok = true;
+    for (uint32_t v : h) {
+        if (v >= kXcds) ok = false;
+        else seen |= 1u << v;
+    }
+    ok = ok && seen == (1u << kXcds) - 1;
+    known[dev] = ok;
+    return ok;
+}
+
 // Size class of an instance: floor(log2(air cells)).
 int size_class_of(uint64_t air) {
     int c = 0;
@@ -125,6 +319,7 @@ int size_class_of(uint64_t air) {
 std::mutex g_tune_mu;
 std::map<std::pair<int, int>, uint32_t> g_tune_zchunk;
 std::map<int, int> g_tune_variant;
+std::map<int, int> g_tune_launch;
 
 uint32_t tuned_zchunk(int prec, int cls) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -136,6 +331,12 @@ int tuned_variant(int prec) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tune_variant.find(prec);
     return it == g_tune_variant.end() ? kDefaultVariant : it->second;
+}
+
+int tuned_launch(int prec) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune_launch.find(prec);
+    return it == g_tune_launch.end() ? WV_BATCH_STEPS : it->second;
 }
 
 struct Instance {
@@ -166,10 +367,17 @@ struct wv_batch {
     hipStream_t stream = nullptr;
     hipEvent_t t0 = nullptr, t1 = nullptr;
     int variant = kDefaultVariant;
+    int launch = WV_BATCH_STEPS;  // the launch form of a block
     std::vector<std::unique_ptr<Instance>> inst;
     DBuf entries, block_entry;
     std::vector<Instance*> table;
     uint32_t total_blocks = 0;
+    // The persistent form: the XCD shares' block lists and counters.
+    DBuf pblock, ctl;
+    XcdShare share{};
+    std::vector<uint32_t> entry_blocks;  // blocks per step of each table entry
+    uint32_t max_share_blocks = 0;
+    bool last_persist = false;  // the last enqueued block was a persistent launch
     bool dirty = true;
     bool graph_valid = false;
     hipGraph_t graph = nullptr;
@@ -192,6 +400,8 @@ struct wv_batch {
         inst.clear();
         entries.release();
         block_entry.release();
+        pblock.release();
+        ctl.release();
         for (auto e : {t0, t1})
             if (e) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
@@ -207,6 +417,7 @@ void batch_init(wv_batch* b, int prec) {
     WV_HIP(hipEventCreate(&b->t0));
     WV_HIP(hipEventCreate(&b->t1));
     b->variant = tuned_variant(prec);
+    b->launch = tuned_launch(prec);
 }
 
 // Starts (or restarts from rest) an instance's run with its z-chunk.
@@ -255,6 +466,7 @@ void build_table(wv_batch* b) {
     std::vector<BatchEntry<T, S>> es;
     std::vector<uint32_t> be;
     b->table.clear();
+    b->entry_blocks.clear();
     for (auto& up : b->inst) {
         Instance* in = up.get();
         if (in->done || in->paused || in->n0 >= in->steps) continue;
@@ -271,19 +483,84 @@ void build_table(wv_batch* b) {
         e.block0 = (uint32_t)be.size();
         const uint32_t blocks = e.a.main_blocks + div_up(pl->n_recv, variant_threads(b->variant));
         if ((uint64_t)be.size() + blocks >= (1ull << 31)) throw DevErr{"batch launch too large", WV_ERR_LIMIT};
+        e.blocks = blocks;
         be.insert(be.end(), blocks, (uint32_t)es.size());
         es.push_back(e);
         b->table.push_back(in);
+        b->entry_blocks.push_back(blocks);
     }
     b->total_blocks = (uint32_t)be.size();
+    // The persistent form's XCD shares: instances largest first, each to the
+    // share with the fewest air cells so far; each share's blocks listed
+    // instance by instance.
+    std::vector<uint32_t> order(es.size());
+    for (uint32_t i = 0; i < (uint32_t)order.size(); ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t x, uint32_t y) { return b->table[x]->plan->air > b->table[y]->plan->air; });
+    std::vector<uint32_t> xcd_of(es.size(), 0);
+    uint64_t load[kXcds] = {};
+    for (uint32_t i : order) {
+        const uint32_t x = (uint32_t)(std::min_element(load, load + kXcds) - load);
+        xcd_of[i] = x;
+        load[x] += b->table[i]->plan->air;
+    }
+    std::vector<uint32_t> pb;
+    pb.reserve(be.size());
+    b->max_share_blocks = 0;
+    for (uint32_t x = 0; x < kXcds; ++x) {
+        b->share.first[x] = (uint32_t)pb.size();
+        for (uint32_t i = 0; i < (uint32_t)es.size(); ++i)
+            if (xcd_of[i] == x) {
+                es[i].block0x = (uint32_t)pb.size() - b->share.first[x];
+                pb.insert(pb.end(), es[i].blocks, i);
+            }
+        b->share.blocks[x] = (uint32_t)pb.size() - b->share.first[x];
+        b->max_share_blocks = std::max(b->max_share_blocks, b->share.blocks[x]);
+    }
     if (!es.empty()) {
         b->entries.alloc(es.size() * sizeof(es[0]));
         b->block_entry.alloc(be.size() * 4);
+        b->pblock.alloc(pb.size() * 4);
+        b->ctl.alloc(persist_ctl_words(es.size()) * 4);
         WV_HIP(hipMemcpy(b->entries.p, es.data(), es.size() * sizeof(es[0]), hipMemcpyHostToDevice));
         WV_HIP(hipMemcpy(b->block_entry.p, be.data(), be.size() * 4, hipMemcpyHostToDevice));
+        WV_HIP(hipMemcpy(b->pblock.p, pb.data(), pb.size() * 4, hipMemcpyHostToDevice));
+        // The error word starts clear; the claim and done counters are
+        // cleared by every launch.
+        const uint32_t zero = 0;
+        WV_HIP(hipMemcpy(b->ctl.as<uint32_t>() + kXcds * kLineWords, &zero, 4, hipMemcpyHostToDevice));
     }
     b->drop_graph();
     b->dirty = false;
+}
+
+// Whether a block of K steps of the current table may take the persistent
+// form: the 8 XCDs found, and every share's items counted in 31 bits.
+bool persist_usable(const wv_batch* b, uint32_t K) {
+    return b->launch == WV_BATCH_PERSISTENT && (uint64_t)K * b->max_share_blocks < (1ull << 31) && xcds_ok();
+}
+
+// Steps kbase .. kbase+K-1 of the current table (each instance from its n0)
+// in the persistent form: the counters cleared, then the one launch.
+template <typename T, typename S>
+void enqueue_persist(wv_batch* b, uint32_t grid, uint32_t kbase, uint32_t K) {
+    const uint32_t n = (uint32_t)b->table.size();
+    hipLaunchKernelGGL(k_persist_reset, dim3(div_up(std::max(n, kXcds), 256)), dim3(256), 0, b->stream,
+                       b->ctl.as<uint32_t>(), n);
+    launch_batch_persist<T, S>(b->variant, grid, b->stream, b->entries.as<BatchEntry<T, S>>(), b->pblock.as<uint32_t>(),
+                               b->ctl.as<uint32_t>(), b->share, kbase, K);
+}
+
+// After the stream has drained a persistent launch of K steps: every
+// instance's items done, no wait past its bound.
+void check_persist(wv_batch* b, uint32_t K) {
+    const size_t n = b->table.size();
+    std::vector<uint32_t> h(persist_ctl_words(n));
+    WV_HIP(hipMemcpy(h.data(), b->ctl.p, h.size() * 4, hipMemcpyDeviceToHost));
+    if (h[kXcds * kLineWords]) throw DevErr{"persistent batch launch: a step wait passed its bound", WV_ERR_DEVICE};
+    for (size_t i = 0; i < n; ++i)
+        if (h[(kXcds + 1 + i) * kLineWords] != K * b->entry_blocks[i])
+            throw DevErr{"persistent batch launch: an instance's items did not all run", WV_ERR_DEVICE};
 }
 
 // Enqueues one block of kGraphSteps steps of the current table: the captured
@@ -298,8 +575,13 @@ void enqueue_block(wv_batch* b, bool use_graph) {
     for (Instance* in : b->table) recentre(in->plan.get(), in->n0, b->stream);
     auto* e = b->entries.as<BatchEntry<T, S>>();
     const uint32_t* be = b->block_entry.as<uint32_t>();
+    const bool persist = persist_usable(b, kGraphSteps);
+    const uint32_t pgrid = persist ? persist_grid<T, S>(b->variant) : 0;
     auto issue = [&]() {
-        for (uint32_t k = 0; k < kGraphSteps; ++k) launch_batch_step<T, S>(b->variant, b->total_blocks, b->stream, e, be, k);
+        if (persist) enqueue_persist<T, S>(b, pgrid, 0, kGraphSteps);
+        else
+            for (uint32_t k = 0; k < kGraphSteps; ++k)
+                launch_batch_step<T, S>(b->variant, b->total_blocks, b->stream, e, be, k);
         hipLaunchKernelGGL((k_batch_tick<T, S>), dim3(div_up(n, 256)), dim3(256), 0, b->stream, e, n, kGraphSteps);
     };
     if (use_graph) {
@@ -321,6 +603,7 @@ void enqueue_block(wv_batch* b, bool use_graph) {
         WV_HIP(hipGetLastError());
     }
     for (Instance* in : b->table) in->n0 += kGraphSteps;
+    b->last_persist = persist;
 }
 
 // Enqueues the next block (up to kGraphSteps steps) of one instance alone,
@@ -381,6 +664,7 @@ wv_status batch_run(wv_batch* b, bool use_graph, const volatile int32_t* cancel,
         }
         max_steps += (uint64_t)issued * kGraphSteps;
         WV_HIP(hipStreamSynchronize(b->stream));
+        if (!solo && b->last_persist) check_persist(b, kGraphSteps);
         done_steps += (uint64_t)b->table.size() * issued * kGraphSteps;
         collect_done(b);
         if (progress && total) progress(std::min(1.0, (double)done_steps / (double)total), ctx);
@@ -402,13 +686,19 @@ double time_table(wv_batch* b, uint32_t warmup, uint32_t steps) {
     if (b->table.empty()) return 0.0;
     auto* e = b->entries.as<BatchEntry<T, S>>();
     const uint32_t* be = b->block_entry.as<uint32_t>();
-    for (uint32_t k = 0; k < warmup; ++k) launch_batch_step<T, S>(b->variant, b->total_blocks, b->stream, e, be, k);
+    const bool persist = persist_usable(b, warmup + steps);
+    const uint32_t pgrid = persist ? persist_grid<T, S>(b->variant) : 0;
+    if (persist && warmup) enqueue_persist<T, S>(b, pgrid, 0, warmup);
+    for (uint32_t k = 0; !persist && k < warmup; ++k)
+        launch_batch_step<T, S>(b->variant, b->total_blocks, b->stream, e, be, k);
     WV_HIP(hipEventRecord(b->t0, b->stream));
-    for (uint32_t k = 0; k < steps; ++k)
+    if (persist) enqueue_persist<T, S>(b, pgrid, warmup, steps);
+    for (uint32_t k = 0; !persist && k < steps; ++k)
         launch_batch_step<T, S>(b->variant, b->total_blocks, b->stream, e, be, warmup + k);
     WV_HIP(hipGetLastError());
     WV_HIP(hipEventRecord(b->t1, b->stream));
     WV_HIP(hipEventSynchronize(b->t1));
+    if (persist) check_persist(b, steps);
     float ms = 0.f;
     WV_HIP(hipEventElapsedTime(&ms, b->t0, b->t1));
     return ms / std::max(steps, 1u);
@@ -452,9 +742,22 @@ void batch_autotune(wv_batch* b, uint32_t trial_steps) {
         for (auto& in : b->inst)
             if (in->size_class == cls) instance_begin(in.get(), best_z);
     }
+    // The launch form, with the shapes chosen.
+    int best_l = WV_BATCH_STEPS;
+    if (xcds_ok()) {
+        double tl[2];
+        for (int l : {WV_BATCH_STEPS, WV_BATCH_PERSISTENT}) {
+            b->launch = l;
+            for (auto& in : b->inst) instance_begin(in.get(), chosen[in->size_class]);
+            tl[l] = time_table<T, S>(b, 2, trial_steps);
+        }
+        best_l = tl[WV_BATCH_PERSISTENT] < tl[WV_BATCH_STEPS] ? WV_BATCH_PERSISTENT : WV_BATCH_STEPS;
+    }
+    b->launch = best_l;
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         g_tune_variant[b->prec] = best_v;
+        g_tune_launch[b->prec] = best_l;
         for (auto& [cls, zc] : chosen) g_tune_zchunk[{b->prec, cls}] = zc;
     }
     // Every instance restarts from rest with its class's z-chunk.
@@ -555,6 +858,22 @@ extern "C" wv_status wv_batch_set_tile(wv_batch* b, int32_t variant, uint32_t zc
     });
 }
 
+extern "C" wv_status wv_batch_set_launch(wv_batch* b, int form, char* err, size_t err_len) {
+    if (!b || (form != WV_BATCH_STEPS && form != WV_BATCH_PERSISTENT)) {
+        set_err(err, err_len, "wv_batch_set_launch: invalid argument");
+        return WV_ERR_INVALID;
+    }
+    return guarded(err, err_len, [&]() -> wv_status {
+        WV_HIP(hipSetDevice(b->device));
+        if (form == WV_BATCH_PERSISTENT && !xcds_ok())
+            throw DevErr{"wv_batch_set_launch: the device's XCC_ID register does not name 8 XCDs", WV_ERR_LIMIT};
+        WV_HIP(hipStreamSynchronize(b->stream));
+        b->launch = form;
+        b->drop_graph();
+        return WV_OK;
+    });
+}
+
 extern "C" wv_status wv_batch_run(wv_batch* b, int use_graphs, const volatile int32_t* cancel,
                                   wv_progress_fn progress, void* ctx, char* err, size_t err_len) {
     if (!b) {
@@ -599,7 +918,7 @@ extern "C" wv_status wv_batch_info(const wv_batch* b, int32_t id, uint64_t out[6
     return WV_OK;
 }
 
-extern "C" wv_status wv_batch_timing(const wv_batch* b, double* ms, uint64_t out[6]) {
+extern "C" wv_status wv_batch_timing(const wv_batch* b, double* ms, uint64_t out[7]) {
     if (!b || !ms || !out) return WV_ERR_INVALID;
     *ms = b->last_ms;
     out[0] = b->last_steps;
@@ -608,6 +927,7 @@ extern "C" wv_status wv_batch_timing(const wv_batch* b, double* ms, uint64_t out
     out[3] = b->captures;
     out[4] = (uint64_t)b->variant;
     out[5] = b->solo_blocks;
+    out[6] = (uint64_t)b->launch;
     return WV_OK;
 }
 
@@ -996,6 +1316,10 @@ extern "C" wv_status wv_serve_create(int precision, uint64_t hbm_cap_bytes, wv_s
         WV_HIP(hipGetDevice(&sv->device));
         sv->batch = std::make_unique<wv_batch>();
         batch_init(sv->batch.get(), precision);
+        // A launch per step: the blocks share the device with the large-request
+        // lane, which a persistent launch holding every CU's slots would keep
+        // waiting until the block ends.
+        sv->batch->launch = WV_BATCH_STEPS;
         WV_HIP(hipStreamCreateWithFlags(&sv->lane_stream, hipStreamNonBlocking));
         for (auto& e : sv->lane_ev) WV_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         wv_server* p = sv.get();

@@ -33,6 +33,7 @@ MODE_SPECTRA, MODE_IR, MODE_BOTH = 1, 2, 3
 FP32, FP64, FP32_STATE64 = 0, 1, 2
 PRECISIONS = {"fp32": FP32, "fp64": FP64, "fp32_state64": FP32_STATE64}
 SERVE_ALL, SERVE_SHORTEST, SERVE_LANES = 0, 1, 2
+BATCH_STEPS, BATCH_PERSISTENT = 0, 1
 COURANT = 0.5773502691896257
 AIR = 1 << 6
 
@@ -137,7 +138,7 @@ EXPORTS = [
     "wv_wave_source_taps", "wv_taps_at",
     "wv_source_components", "wv_component_weights", "wv_to_transfer", "wv_admittance_eval", "wv_paris",
     "wv_fit_real_impedance", "wv_version", "wv_footprint_estimate", "wv_batch_create", "wv_batch_add",
-    "wv_batch_autotune", "wv_batch_set_tile", "wv_batch_run", "wv_batch_result", "wv_batch_info",
+    "wv_batch_autotune", "wv_batch_set_tile", "wv_batch_set_launch", "wv_batch_run", "wv_batch_result", "wv_batch_info",
     "wv_batch_timing", "wv_batch_free", "wv_serve_create", "wv_serve_submit", "wv_serve_wait", "wv_serve_stats", "wv_serve_set_policy",
     "wv_serve_free", "wv_wave_job", "wv_results_save", "wv_results_read_transfer", "wv_grid_build_room", "wv_solid_from_mesh", "wv_grid_spacing",
     "wv_baffle_point", "wv_blend", "wv_calibrate_cabinet", "wv_material_absorption", "wv_fit_admittance",
@@ -242,6 +243,8 @@ def lib():
             L.wv_batch_autotune.restype = st
             L.wv_batch_set_tile.argtypes = [C.c_void_p, C.c_int32, C.c_uint32] + errbuf
             L.wv_batch_set_tile.restype = st
+            L.wv_batch_set_launch.argtypes = [C.c_void_p, C.c_int] + errbuf
+            L.wv_batch_set_launch.restype = st
             L.wv_batch_run.argtypes = [C.c_void_p, C.c_int, P(C.c_int32), PROGRESS_FN, C.c_void_p] + errbuf
             L.wv_batch_run.restype = st
             L.wv_batch_result.argtypes = [C.c_void_p, C.c_int32, dp, sz, C.c_int, dp, dp] + errbuf
@@ -889,11 +892,17 @@ class Batch:
         keys = ["cells", "air_cells", "steps", "zchunk", "size_class", "device_bytes"]
         return dict(zip(keys, [int(v) for v in out]))
 
+    def set_launch(self, form):
+        """BATCH_STEPS (a launch per step) or BATCH_PERSISTENT (one launch per
+        128-step block, each XCD running its share of the instances)."""
+        err = C.create_string_buffer(1024)
+        _check(lib().wv_batch_set_launch(self._h, int(form), err, 1024), err)
+
     def timing(self):
         ms = C.c_double()
-        out = (C.c_uint64 * 6)()
+        out = (C.c_uint64 * 7)()
         lib().wv_batch_timing(self._h, C.byref(ms), out)
-        keys = ["steps", "cell_updates", "graph_launches", "graph_captures", "variant", "solo_blocks"]
+        keys = ["steps", "cell_updates", "graph_launches", "graph_captures", "variant", "solo_blocks", "launch"]
         d = dict(zip(keys, [int(v) for v in out]))
         d["device_ms"] = ms.value
         return d
