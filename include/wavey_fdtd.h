@@ -206,15 +206,10 @@ wv_status wv_footprint_estimate(const wv_grid* grid, int precision, size_t steps
  * Many runs (rooms, the per-source and per-component runs of a wave job,
  * run.rs:645-661) of different grid sizes advanced together, in 128-step
  * blocks replayed as HIP graphs. A block is one launch per step over every
- * instance's tiles (WV_BATCH_STEPS), one persistent launch (WV_BATCH_PERSISTENT:
+ * instance's tiles (WV_BATCH_STEPS) or one persistent launch (WV_BATCH_PERSISTENT:
  * each XCD runs a share of the instances through the whole block, an
- * instance's step waiting only on its own previous step), or fused passes
- * (WV_BATCH_FUSED: one launch per three steps for the FP32 instances whose
- * rows are at most 768 cells and whose receivers lie on air cells of planes
- * 1 .. nz-2, boundary cells updated inside the same z-march; a launch per
- * step for the others). Each instance runs the same arithmetic as
- * wv_plan_run, so its results equal that run's. The reference runs them one
- * after another.
+ * instance's step waiting only on its own previous step). Each instance runs
+ * the same arithmetic as wv_plan_run, so its results equal that run's. The reference runs them one after another.
  * wv_batch_add may be called from several threads at once (each instance is
  * set up on its own stream); every other call on a batch is made by one
  * thread at a time, with no wv_batch_add in progress. Inputs are copied at
@@ -228,19 +223,16 @@ wv_status wv_batch_add(wv_batch* batch, const wv_grid* grid, const wv_tap* taps,
                        int32_t* id, char* err, size_t err_len);
 /* Times the whole batch over trial_steps steps per candidate: the block variant
  * for the launch, then the z-chunk of each size class (floor(log2(air
- * cells)), largest first), then the launch form (fused passes with z-chunks
- * of 16, 32 and 64 planes among the forms). The choices are kept for
+ * cells)), largest first), then the launch form. The choices are kept for
  * later batches and servers of the process (servers use WV_BATCH_STEPS);
  * every instance restarts from rest. */
 wv_status wv_batch_autotune(wv_batch* batch, uint32_t trial_steps, char* err, size_t err_len);
 /* Sets one shape for the whole batch (variant as wv_plan_set_tile; zchunk 0 =
- * default, also the planes per block of the fused passes); instances restart
- * from rest. */
+ * default); instances restart from rest. */
 wv_status wv_batch_set_tile(wv_batch* batch, int32_t variant, uint32_t zchunk, char* err,
                             size_t err_len);
-enum { WV_BATCH_STEPS = 0, WV_BATCH_PERSISTENT = 1, WV_BATCH_FUSED = 2 };
-/* Sets the launch form of the batch's blocks; instances keep their state
- * (an instance takes or leaves the fused form only when its run starts).
+enum { WV_BATCH_STEPS = 0, WV_BATCH_PERSISTENT = 1 };
+/* Sets the launch form of the batch's blocks; instances keep their state.
  * WV_BATCH_PERSISTENT where the device's XCC_ID register does not name 8
  * XCDs returns WV_ERR_LIMIT. */
 wv_status wv_batch_set_launch(wv_batch* batch, int form, char* err, size_t err_len);

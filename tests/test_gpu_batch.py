@@ -100,7 +100,7 @@ def test_persistent_launch_after_autotune(rooms):
         ids = [b.add(c.grid, c.runs[0][2], c.signal, c.receivers) for c in cases]
         b.autotune(trial_steps=4)
         tuned = b.timing()["launch"]
-        assert tuned in (fdtd.BATCH_STEPS, fdtd.BATCH_PERSISTENT, fdtd.BATCH_FUSED)
+        assert tuned in (fdtd.BATCH_STEPS, fdtd.BATCH_PERSISTENT)
         b.set_launch(fdtd.BATCH_PERSISTENT)
         assert b.run() == fdtd.WV_OK
         for i, iid in enumerate(ids):
@@ -116,10 +116,7 @@ def test_batch_autotuned_shapes_keep_results(rooms):
         ids = [b.add(c.grid, c.runs[0][2], c.signal, c.receivers) for c in cases]
         b.autotune(trial_steps=4)
         zchunks = {b.info(i)["zchunk"] for i in ids}
-        if b.timing()["launch"] == fdtd.BATCH_FUSED:
-            assert zchunks <= {16, 32, 64}  # the fused passes' planes per block
-        else:
-            assert zchunks <= {2, 4, 8}
+        assert zchunks <= {2, 4, 8}
         b.run()
         for i, iid in enumerate(ids):
             assert np.array_equal(b.result(iid)[0], plain[("fp32", i)]), f"instance {i}"

@@ -255,6 +255,12 @@ def test_library_fits_with_springs_match_oracle(library_room, prec):
     assert err < SPRING_IR_TOL[prec]
     assert worst < (1e-9 if prec == "fp64" else 1e-3)
     assert np.array_equal(out[1], out[2]) and np.array_equal(out[1], out[3])
+    # The batched kernel inlines the same boundary solve (spring branches:
+    # state g live) in another context: the same bits.
+    with fdtd.Batch(prec) as b:
+        i = b.add(g, [(src, 1.0)], sig, rx)
+        b.run()
+        assert np.array_equal(b.result(i)[0], out[1])
 
 
 def test_two_step_spectra_and_shoebox(shoebox):

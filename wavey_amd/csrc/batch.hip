@@ -44,7 +44,6 @@
 #include <unordered_map>
 #include <vector>
 
-#include "fuse_kernels.h"
 #include "plan.h"
 
 using namespace wv;
@@ -334,17 +333,6 @@ int tuned_variant(int prec) {
     return it == g_tune_variant.end() ? kDefaultVariant : it->second;
 }
 
-// Planes per block of the fused passes, per (precision, size class).
-constexpr uint32_t kFuseDefaultZChunk = 32;
-constexpr uint32_t kTuneFuseZChunks[] = {16, 32, 64};
-std::map<std::pair<int, int>, uint32_t> g_tune_fzchunk;
-
-uint32_t tuned_fzchunk(int prec, int cls) {
-    std::lock_guard<std::mutex> lk(g_tune_mu);
-    auto it = g_tune_fzchunk.find({prec, cls});
-    return it == g_tune_fzchunk.end() ? kFuseDefaultZChunk : it->second;
-}
-
 int tuned_launch(int prec) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tune_launch.find(prec);
@@ -365,16 +353,6 @@ struct Instance {
     int lane = -1;        // the server lane with a block of this instance in flight, or -1
     int size_class = 0;
     uint64_t footprint = 0;
-    // The fused form (fuse_kernels.h): whether the instance may take it
-    // (FP32 fields, rows of at most kFuseMaxPitch cells, receivers on air
-    // cells of planes 1 .. nz - 2) and its device tables while its run uses it.
-    bool fusable = false;
-    struct Fused {
-        bool on = false;      // the current run takes fused passes
-        uint32_t zchunk = 0;  // planes per block of its passes
-        uint32_t span_step = 0, nblk = 0, nzc = 0;
-        DBuf su1, sg1, cu0, cu1, cg0, cg1, cls, sq_first, rx_first, rx, flag, p1s;
-    } fz;
 };
 
 double now_s() {
@@ -394,12 +372,6 @@ struct wv_batch {
     DBuf entries, block_entry;
     std::vector<Instance*> table;
     uint32_t total_blocks = 0;
-    // Instances in the fused form: their entries (FuseEntry) and the launch
-    // block -> entry map of a fused pass.
-    std::vector<Instance*> ftable;
-    DBuf fentries, fblock_entry, fpre_entry;
-    uint32_t fuse_blocks = 0, fpre_blocks = 0;
-    uint32_t fzchunk = 0;  // the fused passes' z-chunk set by wv_batch_set_tile (0: tuned or default)
     // The persistent form: the XCD shares' block lists and counters.
     DBuf pblock, ctl;
     XcdShare share{};
@@ -428,9 +400,6 @@ struct wv_batch {
         inst.clear();
         entries.release();
         block_entry.release();
-        fentries.release();
-        fblock_entry.release();
-        fpre_entry.release();
         pblock.release();
         ctl.release();
         for (auto e : {t0, t1})
@@ -454,120 +423,11 @@ void batch_init(wv_batch* b, int prec) {
 // Starts (or restarts from rest) an instance's run with its z-chunk.
 void instance_begin(Instance* in, uint32_t zchunk) {
     wv_plan* pl = in->plan.get();
-    in->fz.on = false;
     pl->zchunk = zchunk;
     run_begin(pl, in->taps.data(), in->taps.size(), in->signal.data(), in->signal.size(), in->recv.data(),
               in->recv.size());
     in->n0 = 0;
     in->done = false;
-}
-
-// The fused passes' receiver tables for z-chunk fz: each receiver goes to
-// the block that owns its cell (receiver, plane, span position). Returns
-// false when a block would own more receivers than it has threads.
-bool fuse_receivers(Instance* in, uint32_t fz) {
-    wv_plan* pl = in->plan.get();
-    auto& f = in->fz;
-    const uint32_t P = pl->pitch;
-    const uint32_t nblk = div_up(pl->sz, f.span_step), nzc = div_up(pl->nz - 2, fz);
-    const uint32_t blocks = nblk * nzc;
-    const size_t nr = in->recv.size();
-    std::vector<uint32_t> first(blocks + 1, 0), lb(nr), rz(nr), rs(nr);
-    for (size_t r = 0; r < nr; ++r) {
-        const uint64_t c = in->recv[r];
-        const uint32_t idx = (uint32_t)((c / pl->nx) * pl->pitch + c % pl->nx);
-        const uint32_t z = idx / pl->sz, off = idx % pl->sz;
-        const uint32_t kb = off / f.span_step;
-        lb[r] = ((z - 1) / fz) * nblk + kb;
-        rz[r] = z;
-        rs[r] = off + 2 * P - kb * f.span_step;
-        ++first[lb[r] + 1];
-    }
-    for (uint32_t k = 0; k < blocks; ++k)
-        if (first[k + 1] > kFuseThreads) return false;
-    for (uint32_t k = 0; k < blocks; ++k) first[k + 1] += first[k];
-    std::vector<uint32_t> rx(std::max<size_t>(3, 3 * nr)), pos(first.begin(), first.end() - 1);
-    for (size_t r = 0; r < nr; ++r) {
-        const uint32_t k = pos[lb[r]]++;
-        rx[3 * k] = (uint32_t)r;
-        rx[3 * k + 1] = rz[r];
-        rx[3 * k + 2] = rs[r];
-    }
-    f.rx_first.alloc(first.size() * 4);
-    f.rx.alloc(rx.size() * 4);
-    WV_HIP(hipMemcpyAsync(f.rx_first.p, first.data(), first.size() * 4, hipMemcpyHostToDevice, pl->stream));
-    WV_HIP(hipMemcpyAsync(f.rx.p, rx.data(), rx.size() * 4, hipMemcpyHostToDevice, pl->stream));
-    WV_HIP(hipStreamSynchronize(pl->stream));
-    f.zchunk = fz;
-    f.nblk = nblk;
-    f.nzc = nzc;
-    return true;
-}
-
-// Starts (or restarts from rest) an instance's run in the fused form: the
-// plain layout with one-plane chunks (special cells compacted in pitched
-// order), the second pair of field buffers, the state arrays of the passes,
-// the class bytes, the special list's bounds per (plane, span) and the
-// receiver tables. Returns false (the caller restarts it in the single-step
-// form) when a block would own more receivers than it has threads.
-template <typename S>
-bool instance_begin_fused(Instance* in, uint32_t fz) {
-    instance_begin(in, 1);
-    wv_plan* pl = in->plan.get();
-    auto& f = in->fz;
-    f.span_step = kFuseSpan - 4 * pl->pitch;
-    if (!fuse_receivers(in, fz)) return false;
-    hipStream_t st = pl->stream;
-    const size_t fbytes = pl->npitched * sizeof(float);
-    pl->fc.alloc(fbytes);
-    pl->fd.alloc(fbytes);
-    WV_HIP(hipMemsetAsync(pl->fc.p, 0, fbytes, st));
-    WV_HIP(hipMemsetAsync(pl->fd.p, 0, fbytes, st));
-    const size_t sbytes = std::max<size_t>(1, (size_t)pl->max_slots * pl->sp_pad) * sizeof(S);
-    std::vector<DBuf*> states = {&f.su1, &f.cu0, &f.cu1};  // (st_u is the pair's first)
-    if (pl->ns_k) states.insert(states.end(), {&f.sg1, &f.cg0, &f.cg1});
-    for (DBuf* d : states) {
-        d->alloc(sbytes);
-        WV_HIP(hipMemsetAsync(d->p, 0, sbytes, st));
-    }
-    f.p1s.alloc(std::max<size_t>(1, pl->n_special) * sizeof(float2));
-    const uint64_t words = pl->npitched / 4;  // the pitch is a multiple of 32 cells
-    f.cls.alloc(words * 4);
-    hipLaunchKernelGGL(k_cls, dim3((uint32_t)div_up(words, kBlock)), dim3(kBlock), 0, st, pl->zbits.as<uint32_t>(),
-                       f.cls.as<uint32_t>(), pl->sz, pl->nz, words);
-    const uint32_t nt = pl->nz * f.nblk * kFuseFirst;
-    f.sq_first.alloc((size_t)nt * 4);
-    f.flag.alloc(4);
-    WV_HIP(hipMemsetAsync(f.flag.p, 0, 4, st));
-    if (pl->n_special) {
-        const uint32_t g = div_up(pl->n_special, kBlock);
-        hipLaunchKernelGGL(k_cls_special, dim3(g), dim3(kBlock), 0, st, pl->sp_cell.as<uint32_t>(), pl->n_special,
-                           f.cls.as<uint32_t>());
-        hipLaunchKernelGGL(k_sq_check, dim3(g), dim3(kBlock), 0, st, pl->sp_cell.as<uint32_t>(), pl->n_special,
-                           f.flag.as<uint32_t>());
-    }
-    hipLaunchKernelGGL(k_sq_first, dim3(div_up(nt, kBlock)), dim3(kBlock), 0, st, pl->sp_cell.as<uint32_t>(),
-                       pl->n_special, f.sq_first.as<uint32_t>(), pl->sz, pl->nz, pl->pitch, f.span_step, f.nblk);
-    WV_HIP(hipGetLastError());
-    uint32_t bad = 0;
-    WV_HIP(hipMemcpyAsync(&bad, f.flag.p, 4, hipMemcpyDeviceToHost, st));
-    WV_HIP(hipStreamSynchronize(st));
-    if (bad) throw DevErr{"fused pass: the special cells are not in pitched order", WV_ERR_DEVICE};
-    f.on = true;
-    return true;
-}
-
-// Whether an instance may take the fused form: FP32 fields, rows that fit a
-// span, and receivers on air cells of planes 1 .. nz - 2 (the cells the
-// passes update).
-bool fusable_run(int prec, const wv_plan* pl, const wv_grid* g, const uint64_t* recv, size_t n_recv) {
-    if (prec == WV_FP64 || pl->pitch > kFuseMaxPitch || pl->nz < 3) return false;
-    const uint64_t plane = (uint64_t)pl->nx * pl->ny;
-    for (size_t r = 0; r < n_recv; ++r) {
-        const uint64_t c = recv[r], z = c / plane;
-        if (c >= pl->cells || !((g->flags[c] >> 6) & 1u) || z < 1 || z + 2 > pl->nz) return false;
-    }
-    return true;
 }
 
 template <typename S>
@@ -590,7 +450,6 @@ std::unique_ptr<Instance> make_instance(int prec, const wv_grid* g, const wv_tap
     for (size_t n = 0; n < steps; ++n) sig[n] = (S)signal[n];
     in->sig.alloc(sig.size() * sizeof(S));
     WV_HIP(hipMemcpy(in->sig.p, sig.data(), sig.size() * sizeof(S), hipMemcpyHostToDevice));
-    in->fusable = fusable_run(prec, in->plan.get(), g, recv, n_recv);
     instance_begin(in.get(), tuned_zchunk(prec, in->size_class));
     in->footprint = plan_bytes(in->plan.get()) + in->sig.bytes;
     return in;
@@ -607,71 +466,11 @@ void build_table(wv_batch* b) {
     std::vector<BatchEntry<T, S>> es;
     std::vector<uint32_t> be;
     b->table.clear();
-    b->ftable.clear();
     b->entry_blocks.clear();
-    std::vector<FuseEntry<S>> fes;
-    std::vector<uint32_t> fbe, fpe;
     for (auto& up : b->inst) {
         Instance* in = up.get();
         if (in->done || in->paused || in->n0 >= in->steps) continue;
         wv_plan* pl = in->plan.get();
-        if constexpr (sizeof(T) == 4) {
-            // The form of a run is chosen when it starts: fused passes when
-            // the batch takes them and the instance may.
-            if (in->n0 == 0) {
-                const bool want = b->launch == WV_BATCH_FUSED && in->fusable;
-                if (want && !in->fz.on) {
-                    const uint32_t fz = b->fzchunk ? b->fzchunk : tuned_fzchunk(b->prec, in->size_class);
-                    if (!instance_begin_fused<S>(in, fz)) {
-                        in->fusable = false;
-                        instance_begin(in, tuned_zchunk(b->prec, in->size_class));
-                    }
-                } else if (!want && in->fz.on) {
-                    instance_begin(in, tuned_zchunk(b->prec, in->size_class));
-                }
-            }
-            if (in->fz.on) {
-                auto& f = in->fz;
-                FuseEntry<S> fe{};
-                fe.a = make_params<float, S>(pl, 1);
-                fe.a.ir = pl->ir.as<float>();
-                const DBuf* fields[4] = {&pl->fa, &pl->fb, &pl->fc, &pl->fd};
-                for (int k = 0; k < 4; ++k) fe.f[k] = fields[k]->as<float>();
-                S* gdef = pl->st_g.as<S>();
-                fe.su[0] = pl->st_u.as<S>();
-                fe.su[1] = f.su1.as<S>();
-                fe.sg[0] = gdef;
-                fe.sg[1] = pl->ns_k ? f.sg1.as<S>() : gdef;
-                fe.cu[0] = f.cu0.as<S>();
-                fe.cu[1] = f.cu1.as<S>();
-                fe.cg[0] = pl->ns_k ? f.cg0.as<S>() : gdef;
-                fe.cg[1] = pl->ns_k ? f.cg1.as<S>() : gdef;
-                fe.cls = f.cls.as<uint32_t>();
-                fe.sq_cell = pl->sp_cell.as<uint32_t>();
-                fe.sq_first = f.sq_first.as<uint32_t>();
-                fe.rx_first = f.rx_first.as<uint32_t>();
-                fe.rx = f.rx.as<uint32_t>();
-                fe.sig = in->sig.as<S>();
-                fe.p1s = f.p1s.as<float2>();
-                fe.n_special = pl->n_special;
-                fe.n_setc = pl->n_setc;
-                fe.pre0 = (uint32_t)fpe.size();
-                fpe.insert(fpe.end(), div_up(pl->n_special, kBlock), (uint32_t)fes.size());
-                fe.steps = in->steps;
-                fe.n0 = in->n0;
-                fe.span_step = f.span_step;
-                fe.nblk = f.nblk;
-                fe.nzc = f.nzc;
-                fe.zchunk = f.zchunk;
-                fe.block0 = (uint32_t)fbe.size();
-                fe.blocks = f.nblk * f.nzc;
-                if ((uint64_t)fbe.size() + fe.blocks >= (1ull << 31)) throw DevErr{"batch launch too large", WV_ERR_LIMIT};
-                fbe.insert(fbe.end(), fe.blocks, (uint32_t)fes.size());
-                fes.push_back(fe);
-                b->ftable.push_back(in);
-                continue;
-            }
-        }
         const DBuf* fields[5] = {&pl->fa, &pl->fb, &pl->fc, &pl->fd, &pl->fe};
         BatchEntry<T, S> e{};
         e.a = make_params<T, S>(pl, ty);
@@ -718,16 +517,6 @@ void build_table(wv_batch* b) {
         b->share.blocks[x] = (uint32_t)pb.size() - b->share.first[x];
         b->max_share_blocks = std::max(b->max_share_blocks, b->share.blocks[x]);
     }
-    b->fuse_blocks = (uint32_t)fbe.size();
-    b->fpre_blocks = (uint32_t)fpe.size();
-    if (!fes.empty()) {
-        b->fentries.alloc(fes.size() * sizeof(fes[0]));
-        b->fblock_entry.alloc(fbe.size() * 4);
-        b->fpre_entry.alloc(std::max<size_t>(1, fpe.size()) * 4);
-        WV_HIP(hipMemcpy(b->fentries.p, fes.data(), fes.size() * sizeof(fes[0]), hipMemcpyHostToDevice));
-        WV_HIP(hipMemcpy(b->fblock_entry.p, fbe.data(), fbe.size() * 4, hipMemcpyHostToDevice));
-        if (!fpe.empty()) WV_HIP(hipMemcpy(b->fpre_entry.p, fpe.data(), fpe.size() * 4, hipMemcpyHostToDevice));
-    }
     if (!es.empty()) {
         b->entries.alloc(es.size() * sizeof(es[0]));
         b->block_entry.alloc(be.size() * 4);
@@ -748,8 +537,7 @@ void build_table(wv_batch* b) {
 // Whether a block of K steps of the current table may take the persistent
 // form: the 8 XCDs found, and every share's items counted in 31 bits.
 bool persist_usable(const wv_batch* b, uint32_t K) {
-    return b->launch == WV_BATCH_PERSISTENT && !b->table.empty() &&
-           (uint64_t)K * b->max_share_blocks < (1ull << 31) && xcds_ok();
+    return b->launch == WV_BATCH_PERSISTENT && (uint64_t)K * b->max_share_blocks < (1ull << 31) && xcds_ok();
 }
 
 // Steps kbase .. kbase+K-1 of the current table (each instance from its n0)
@@ -778,58 +566,23 @@ void check_persist(wv_batch* b, uint32_t K) {
 // Enqueues one block of kGraphSteps steps of the current table: the captured
 // graph (captured on first use after a table change), or the same launches
 // issued one by one.
-// The fused passes kp0 .. kp1 - 1 of a 128-step block of the fused table.
-template <typename T, typename S>
-void launch_fused(wv_batch* b, uint32_t kp0, uint32_t kp1) {
-    if constexpr (sizeof(T) == 4) {
-        for (uint32_t k = kp0; k < kp1; ++k) {
-            if (b->fpre_blocks)
-                hipLaunchKernelGGL(k_fuse_pre<S>, dim3(b->fpre_blocks), dim3(kBlock), 0, b->stream,
-                                   b->fentries.as<FuseEntry<S>>(), b->fpre_entry.as<uint32_t>(), k);
-            hipLaunchKernelGGL(k_fuse<S>, dim3(b->fuse_blocks), dim3(kFuseThreads), 0, b->stream,
-                               b->fentries.as<FuseEntry<S>>(), b->fblock_entry.as<uint32_t>(), k);
-        }
-    } else {
-        (void)b, (void)kp0, (void)kp1;
-    }
-}
-
 template <typename T, typename S>
 void enqueue_block(wv_batch* b, bool use_graph) {
     const uint32_t n = (uint32_t)b->table.size();
-    const uint32_t nf = (uint32_t)b->ftable.size();
-    if (n == 0 && nf == 0) return;
+    if (n == 0) return;
     // Re-centring of the instances whose block starts at one of its steps
-    // (their p and prev are in the plan's buf_p, buf_q at a block start; a
-    // fused run's in the pair of the block's parity).
+    // (their p and prev are in the plan's buf_p, buf_q at a block start).
     for (Instance* in : b->table) recentre(in->plan.get(), in->n0, b->stream);
-    for (Instance* in : b->ftable) {
-        wv_plan* pl = in->plan.get();
-        const int par = (int)((in->n0 / kGraphSteps) & 1u);
-        pl->buf_p = 2 * par;
-        pl->buf_q = 2 * par + 1;
-        recentre(pl, in->n0, b->stream);
-    }
     auto* e = b->entries.as<BatchEntry<T, S>>();
     const uint32_t* be = b->block_entry.as<uint32_t>();
     const bool persist = persist_usable(b, kGraphSteps);
     const uint32_t pgrid = persist ? persist_grid<T, S>(b->variant) : 0;
     auto issue = [&]() {
-        if (n) {
-            if (persist) enqueue_persist<T, S>(b, pgrid, 0, kGraphSteps);
-            else
-                for (uint32_t k = 0; k < kGraphSteps; ++k)
-                    launch_batch_step<T, S>(b->variant, b->total_blocks, b->stream, e, be, k);
-            hipLaunchKernelGGL((k_batch_tick<T, S>), dim3(div_up(n, 256)), dim3(256), 0, b->stream, e, n,
-                               kGraphSteps);
-        }
-        if constexpr (sizeof(T) == 4) {
-            if (nf) {
-                launch_fused<T, S>(b, 0, kFusePasses);
-                hipLaunchKernelGGL(k_fuse_tick<S>, dim3(div_up(nf, 256)), dim3(256), 0, b->stream,
-                                   b->fentries.as<FuseEntry<S>>(), nf, kGraphSteps);
-            }
-        }
+        if (persist) enqueue_persist<T, S>(b, pgrid, 0, kGraphSteps);
+        else
+            for (uint32_t k = 0; k < kGraphSteps; ++k)
+                launch_batch_step<T, S>(b->variant, b->total_blocks, b->stream, e, be, k);
+        hipLaunchKernelGGL((k_batch_tick<T, S>), dim3(div_up(n, 256)), dim3(256), 0, b->stream, e, n, kGraphSteps);
     };
     if (use_graph) {
         if (!b->graph_valid) {
@@ -850,7 +603,6 @@ void enqueue_block(wv_batch* b, bool use_graph) {
         WV_HIP(hipGetLastError());
     }
     for (Instance* in : b->table) in->n0 += kGraphSteps;
-    for (Instance* in : b->ftable) in->n0 += kGraphSteps;
     b->last_persist = persist;
 }
 
@@ -871,15 +623,6 @@ uint32_t collect_done(wv_batch* b) {
     for (auto& up : b->inst) {
         Instance* in = up.get();
         if (!in->done && in->n0 >= in->steps) {
-            if (in->fz.on && in->steps) {
-                // The pair the last pass wrote: the last block's parity plus
-                // its passes.
-                const uint32_t blk = (in->steps - 1) / kGraphSteps;
-                const uint32_t passes = div_up(in->steps - blk * kGraphSteps, 3);
-                const int par = (int)((blk + passes) & 1u);
-                in->plan->buf_p = 2 * par;
-                in->plan->buf_q = 2 * par + 1;
-            }
             in->done = true;
             in->plan->done = in->steps;
             in->plan->steps = in->steps;
@@ -906,15 +649,12 @@ wv_status batch_run(wv_batch* b, bool use_graph, const volatile int32_t* cancel,
     while (true) {
         if (cancel && *cancel) return WV_CANCELLED;
         if (b->dirty) build_table<T, S>(b);
-        if (b->table.empty() && b->ftable.empty()) break;
+        if (b->table.empty()) break;
         // Blocks until the first instance of the table finishes: the table
         // does not change before then, so they are enqueued back to back.
         uint32_t blocks = ~0u;
         for (Instance* in : b->table) blocks = std::min(blocks, div_up(in->steps - in->n0, kGraphSteps));
-        for (Instance* in : b->ftable) blocks = std::min(blocks, div_up(in->steps - in->n0, kGraphSteps));
-        Instance* solo = b->table.size() == 1 && b->ftable.empty() && b->table[0]->plan->run_steps_per_pass >= 2
-                             ? b->table[0]
-                             : nullptr;
+        Instance* solo = b->table.size() == 1 && b->table[0]->plan->run_steps_per_pass >= 2 ? b->table[0] : nullptr;
         uint32_t issued = 0;
         while (issued < blocks) {
             if (solo) enqueue_solo(b, solo);
@@ -925,7 +665,7 @@ wv_status batch_run(wv_batch* b, bool use_graph, const volatile int32_t* cancel,
         max_steps += (uint64_t)issued * kGraphSteps;
         WV_HIP(hipStreamSynchronize(b->stream));
         if (!solo && b->last_persist) check_persist(b, kGraphSteps);
-        done_steps += (uint64_t)(b->table.size() + b->ftable.size()) * issued * kGraphSteps;
+        done_steps += (uint64_t)b->table.size() * issued * kGraphSteps;
         collect_done(b);
         if (progress && total) progress(std::min(1.0, (double)done_steps / (double)total), ctx);
     }
@@ -943,25 +683,18 @@ wv_status batch_run(wv_batch* b, bool use_graph, const volatile int32_t* cancel,
 template <typename T, typename S>
 double time_table(wv_batch* b, uint32_t warmup, uint32_t steps) {
     build_table<T, S>(b);
-    if (b->table.empty() && b->ftable.empty()) return 0.0;
-    // Whole fused passes (three steps) for both counts, within one block.
-    warmup = 3 * div_up(warmup, 3);
-    steps = 3 * div_up(std::max(steps, 1u), 3);
-    if (warmup + steps > 3 * (kFusePasses - 1)) throw DevErr{"time_table: too many trial steps", WV_ERR_INVALID};
-    const bool st = !b->table.empty();
+    if (b->table.empty()) return 0.0;
     auto* e = b->entries.as<BatchEntry<T, S>>();
     const uint32_t* be = b->block_entry.as<uint32_t>();
     const bool persist = persist_usable(b, warmup + steps);
     const uint32_t pgrid = persist ? persist_grid<T, S>(b->variant) : 0;
     if (persist && warmup) enqueue_persist<T, S>(b, pgrid, 0, warmup);
-    for (uint32_t k = 0; st && !persist && k < warmup; ++k)
+    for (uint32_t k = 0; !persist && k < warmup; ++k)
         launch_batch_step<T, S>(b->variant, b->total_blocks, b->stream, e, be, k);
-    if (!b->ftable.empty()) launch_fused<T, S>(b, 0, warmup / 3);
     WV_HIP(hipEventRecord(b->t0, b->stream));
     if (persist) enqueue_persist<T, S>(b, pgrid, warmup, steps);
-    for (uint32_t k = 0; st && !persist && k < steps; ++k)
+    for (uint32_t k = 0; !persist && k < steps; ++k)
         launch_batch_step<T, S>(b->variant, b->total_blocks, b->stream, e, be, warmup + k);
-    if (!b->ftable.empty()) launch_fused<T, S>(b, warmup / 3, (warmup + steps) / 3);
     WV_HIP(hipGetLastError());
     WV_HIP(hipEventRecord(b->t1, b->stream));
     WV_HIP(hipEventSynchronize(b->t1));
@@ -1009,53 +742,16 @@ void batch_autotune(wv_batch* b, uint32_t trial_steps) {
         for (auto& in : b->inst)
             if (in->size_class == cls) instance_begin(in.get(), best_z);
     }
-    // The launch form, with the shapes chosen: a launch per step, the
-    // persistent launch (8 XCDs found), and fused passes (some instance may
-    // take them) with the z-chunk of those passes tried in turn.
+    // The launch form, with the shapes chosen.
     int best_l = WV_BATCH_STEPS;
-    double best_lt = 1e300;
-    std::vector<int> forms = {WV_BATCH_STEPS};
-    if (xcds_ok()) forms.push_back(WV_BATCH_PERSISTENT);
-    bool any_fusable = false;
-    for (auto& in : b->inst) any_fusable = any_fusable || in->fusable;
-    for (int l : forms) {
-        b->launch = l;
-        for (auto& in : b->inst) instance_begin(in.get(), chosen[in->size_class]);
-        const double t = time_table<T, S>(b, 2, trial_steps);
-        if (t < best_lt) {
-            best_lt = t;
-            best_l = l;
+    if (xcds_ok()) {
+        double tl[2];
+        for (int l : {WV_BATCH_STEPS, WV_BATCH_PERSISTENT}) {
+            b->launch = l;
+            for (auto& in : b->inst) instance_begin(in.get(), chosen[in->size_class]);
+            tl[l] = time_table<T, S>(b, 2, trial_steps);
         }
-    }
-    uint32_t best_fz = kFuseDefaultZChunk;
-    b->fzchunk = 0;
-    if (any_fusable && sizeof(T) == 4) {
-        b->launch = WV_BATCH_FUSED;
-        for (auto& in : b->inst) instance_begin(in.get(), chosen[in->size_class]);
-        bool first = true;
-        for (uint32_t fz : kTuneFuseZChunks) {
-            // The first trial starts the fused runs (build_table); the others
-            // only change their receiver tables and block counts.
-            if (!first) {
-                for (auto& in : b->inst)
-                    if (in->fz.on && !fuse_receivers(in.get(), fz)) instance_begin(in.get(), chosen[in->size_class]);
-                b->dirty = true;
-            }
-            {
-                std::lock_guard<std::mutex> lk(g_tune_mu);
-                for (auto& [cls, air] : classes) {
-                    (void)air;
-                    g_tune_fzchunk[{b->prec, cls}] = fz;
-                }
-            }
-            first = false;
-            const double t = time_table<T, S>(b, 2, trial_steps);
-            if (t < best_lt) {
-                best_lt = t;
-                best_l = WV_BATCH_FUSED;
-                best_fz = fz;
-            }
-        }
+        best_l = tl[WV_BATCH_PERSISTENT] < tl[WV_BATCH_STEPS] ? WV_BATCH_PERSISTENT : WV_BATCH_STEPS;
     }
     b->launch = best_l;
     {
@@ -1063,10 +759,6 @@ void batch_autotune(wv_batch* b, uint32_t trial_steps) {
         g_tune_variant[b->prec] = best_v;
         g_tune_launch[b->prec] = best_l;
         for (auto& [cls, zc] : chosen) g_tune_zchunk[{b->prec, cls}] = zc;
-        for (auto& [cls, air] : classes) {
-            (void)air;
-            g_tune_fzchunk[{b->prec, cls}] = best_fz;
-        }
     }
     // Every instance restarts from rest with its class's z-chunk.
     for (auto& in : b->inst) instance_begin(in.get(), chosen[in->size_class]);
@@ -1160,7 +852,6 @@ extern "C" wv_status wv_batch_set_tile(wv_batch* b, int32_t variant, uint32_t zc
         WV_HIP(hipSetDevice(b->device));
         WV_HIP(hipStreamSynchronize(b->stream));
         b->variant = variant < 0 ? kDefaultVariant : variant;
-        b->fzchunk = zchunk;
         for (auto& in : b->inst) instance_begin(in.get(), zchunk);
         b->dirty = true;
         return WV_OK;
@@ -1168,7 +859,7 @@ extern "C" wv_status wv_batch_set_tile(wv_batch* b, int32_t variant, uint32_t zc
 }
 
 extern "C" wv_status wv_batch_set_launch(wv_batch* b, int form, char* err, size_t err_len) {
-    if (!b || (form != WV_BATCH_STEPS && form != WV_BATCH_PERSISTENT && form != WV_BATCH_FUSED)) {
+    if (!b || (form != WV_BATCH_STEPS && form != WV_BATCH_PERSISTENT)) {
         set_err(err, err_len, "wv_batch_set_launch: invalid argument");
         return WV_ERR_INVALID;
     }
@@ -1179,7 +870,6 @@ extern "C" wv_status wv_batch_set_launch(wv_batch* b, int form, char* err, size_
         WV_HIP(hipStreamSynchronize(b->stream));
         b->launch = form;
         b->drop_graph();
-        b->dirty = true;  // runs that have not started take the form at the next table
         return WV_OK;
     });
 }
@@ -1222,7 +912,7 @@ extern "C" wv_status wv_batch_info(const wv_batch* b, int32_t id, uint64_t out[6
     out[0] = in->plan->cells;
     out[1] = in->plan->air;
     out[2] = in->steps;
-    out[3] = in->fz.on ? in->fz.zchunk : in->plan->run_zchunk;
+    out[3] = in->plan->run_zchunk;
     out[4] = (uint64_t)in->size_class;
     out[5] = in->footprint;
     return WV_OK;

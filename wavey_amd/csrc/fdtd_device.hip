@@ -591,7 +591,6 @@ void wv::plan_setup(wv_plan* pl, const wv_grid* g) {
     pl->ns = nb ? g->b_start[nb] : 0;
     pl->ns_k = ns_k;
     pl->n_sets = (uint32_t)set_n.size();
-    pl->n_setc = (uint32_t)inv_d.size();
     upload(pl, pl->b_cell_sorted, bcell);
     upload(pl, pl->b_set_sorted, bset);
     upload(pl, pl->set_n, set_n);

@@ -91,7 +91,7 @@ struct wv_plan {
     uint32_t run_zchunk = 0;  // the current run's layout
     uint32_t nzb = 0;         // 32-plane blocks of the z-packed bitmaps
     int variant = wv::kDefaultVariant;
-    uint32_t max_slots = 0, n_sets = 0, n_setc = 0;  // n_setc: slot-constant records in set_c
+    uint32_t max_slots = 0, n_sets = 0;
 
     DBuf flags, force_base, force, zbits, dbits, counts, offs;
     DBuf b_cell_sorted, b_set_sorted;

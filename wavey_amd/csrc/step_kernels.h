@@ -216,19 +216,6 @@ __device__ __forceinline__ uint32_t plane_bits(const uint32_t (&w)[W], uint32_t 
 template <typename S>
 constexpr uint32_t kSlotGroup = sizeof(S) == 4 ? 3 : 2;
 
-// Where a boundary solve reads its slot state (u, and g when a slot has a
-// spring) and writes the advanced state. A single step reads and writes the
-// same arrays; a fused multi-step pass reads the state of one step and
-// writes that of the next to other arrays.
-template <typename S>
-struct SlotIo {
-    const S* u_in;
-    const S* g_in;
-    S* u_out;
-    S* g_out;
-    const S* set_c = nullptr;  // a copy of the slot constants (in LDS), else StepParams::set_c
-};
-
 // Constants and state of slots jb .. jb+G-1 of a node (indices clamped to
 // the last slot; the caller masks the extras). `upre`, when given, holds the
 // states of rows jb .. jb+G-1 read earlier.
@@ -236,12 +223,12 @@ template <typename S, uint32_t G>
 struct SlotGroup {
     S inv_d[G], w[G], cv[G], u[G], kk[G], g[G];
     template <typename T>
-    __device__ __forceinline__ void load(const StepParams<T, S>& a, const SlotIo<S>& io, uint32_t c0, uint32_t rank,
-                                         uint32_t jb, uint32_t ns, const S* upre = nullptr) {
+    __device__ __forceinline__ void load(const StepParams<T, S>& a, uint32_t c0, uint32_t rank, uint32_t jb,
+                                         uint32_t ns, const S* upre = nullptr) {
 #pragma unroll
         for (uint32_t k = 0; k < G; ++k) {
             const uint32_t j = min(jb + k, ns - 1);
-            const S* c = (io.set_c ? io.set_c : a.set_c) + 4 * (c0 + j);
+            const S* c = a.set_c + 4 * (c0 + j);
             if constexpr (sizeof(S) == 4) {
                 const float4 q = *reinterpret_cast<const float4*>(c);
                 inv_d[k] = q.x;
@@ -254,12 +241,12 @@ struct SlotGroup {
                 cv[k] = q0.y;
                 w[k] = q1.y;
             }
-            u[k] = upre ? upre[k] : io.u_in[j * a.sp_pad + rank];
+            u[k] = upre ? upre[k] : a.st_u[j * a.sp_pad + rank];
             kk[k] = 0;
             g[k] = 0;
             if (a.has_k) {
                 kk[k] = c[2];
-                g[k] = io.g_in[j * a.sp_pad + rank];
+                g[k] = a.st_g[j * a.sp_pad + rank];
             }
         }
     }
@@ -278,16 +265,16 @@ struct SlotGroup {
             }
     }
     template <typename T>
-    __device__ __forceinline__ void advance(const StepParams<T, S>& a, const SlotIo<S>& io, uint32_t rank, uint32_t jb,
-                                            uint32_t ns, S pc, S pp, S next) const {
+    __device__ __forceinline__ void advance(const StepParams<T, S>& a, uint32_t rank, uint32_t jb, uint32_t ns, S pc,
+                                            S pp, S next) const {
 #pragma clang fp contract(off)
 #pragma unroll
         for (uint32_t k = 0; k < G; ++k)
             if (jb + k < ns) {
                 const uint32_t j = jb + k;
                 const S un = (S(0.5) - w[k]) * inv_d[k] * next + rest(k, pc, pp);
-                if (a.has_k) io.g_out[j * a.sp_pad + rank] = g[k] + a.half_dt * (un + u[k]);
-                io.u_out[j * a.sp_pad + rank] = un;
+                if (a.has_k) a.st_g[j * a.sp_pad + rank] = g[k] + a.half_dt * (un + u[k]);
+                a.st_u[j * a.sp_pad + rank] = un;
             }
     }
 };
@@ -296,17 +283,17 @@ struct SlotGroup {
 // fdtd.rs:581-598 when it has slots, and its tap (injection after the
 // update, fdtd.rs:718-720). nb = -x, +x, -y, +y, -z, +z; c0 is the first
 // constant of the node's slot set, rank its index in the state arrays and u0
-// the states of its first PF rows, read ahead by the caller (G slots are
-// held at a time; PF is a multiple of G).
-template <typename T, typename S, uint32_t G = kSlotGroup<S>, uint32_t PF = G>
-__device__ __forceinline__ S special_update_io(const StepParams<T, S>& a, const SlotIo<S>& io, uint32_t meta,
-                                               uint32_t c0, uint32_t rank, S pc, S pp, const S (&nb)[6],
-                                               const S (&u0)[PF], S s_n) {
+// the states of its first kSlotGroup rows, read ahead by the caller.
+template <typename T, typename S>
+__device__ __forceinline__ S special_update(const StepParams<T, S>& a, uint32_t meta, uint32_t c0, uint32_t rank,
+                                            S pc, S pp, const S (&nb)[6], const S (&u0)[kSlotGroup<S>]) {
     // Every product and sum of the boundary solve is rounded on its own (no
-    // fused multiply-adds), so each kernel that calls it, whatever its
-    // inlining and register pressure, gives the same bits.
+    // fused multiply-adds): the compiler would otherwise fuse them
+    // differently in the kernels that inline this (measured: FP32 series
+    // with spring branches differed in the last bits between a single-step
+    // and a fused multi-step kernel), and the pass modes, batch and server
+    // promise the same bits.
 #pragma clang fp contract(off)
-    static_assert(PF % G == 0, "prefetched rows are whole slot groups");
     S next;
     if (meta & kMetaSolid) {
         next = pp;  // a tap on a solid cell: the reference only injects there
@@ -338,8 +325,8 @@ __device__ __forceinline__ S special_update_io(const StepParams<T, S>& a, const 
             // Each slot's u+ is linear in p+, so p+ is solved first from the
             // summed face flux, then the states advance. The first G slots
             // stay in registers (one pass for nodes with up to G slots);
-            // further groups are read again for the second pass. The sums
-            // and updates run in slot order whatever G is.
+            // further groups are read again for the second pass.
+            constexpr uint32_t G = kSlotGroup<S>;
             // The slots see the pressure itself: FP32 fields add the run's
             // offset (the Laplacian and the update's own terms do not
             // depend on it).
@@ -350,20 +337,12 @@ __device__ __forceinline__ S special_update_io(const StepParams<T, S>& a, const 
                 ppa = pp + off;
             }
             SlotGroup<S, G> g0;
-            g0.load(a, io, c0, rank, 0, ns, u0);
+            g0.load(a, c0, rank, 0, ns, u0);
             S s1 = 0, s2 = 0;
             g0.sums(0, ns, pa, ppa, s1, s2);
-#pragma unroll
-            for (uint32_t t = 1; t < PF / G; ++t) {
-                if (t * G < ns) {
-                    SlotGroup<S, G> gj;
-                    gj.load(a, io, c0, rank, t * G, ns, u0 + t * G);
-                    gj.sums(t * G, ns, pa, ppa, s1, s2);
-                }
-            }
-            for (uint32_t jb = PF; jb < ns; jb += G) {
+            for (uint32_t jb = G; jb < ns; jb += G) {
                 SlotGroup<S, G> gj;
-                gj.load(a, io, c0, rank, jb, ns);
+                gj.load(a, c0, rank, jb, ns);
                 gj.sums(jb, ns, pa, ppa, s1, s2);
             }
             S nabs;
@@ -376,32 +355,16 @@ __device__ __forceinline__ S special_update_io(const StepParams<T, S>& a, const 
                 next = pc + ((pc - pp) + lap - a.lam * (s2 + s1 * pa)) / (S(1) + a.lam * s1);
                 nabs = next + off;
             }
-            g0.advance(a, io, rank, 0, ns, pa, ppa, nabs);
-#pragma unroll
-            for (uint32_t t = 1; t < PF / G; ++t) {
-                if (t * G < ns) {
-                    SlotGroup<S, G> gj;
-                    gj.load(a, io, c0, rank, t * G, ns, u0 + t * G);
-                    gj.advance(a, io, rank, t * G, ns, pa, ppa, nabs);
-                }
-            }
-            for (uint32_t jb = PF; jb < ns; jb += G) {
+            g0.advance(a, rank, 0, ns, pa, ppa, nabs);
+            for (uint32_t jb = G; jb < ns; jb += G) {
                 SlotGroup<S, G> gj;
-                gj.load(a, io, c0, rank, jb, ns);
-                gj.advance(a, io, rank, jb, ns, pa, ppa, nabs);
+                gj.load(a, c0, rank, jb, ns);
+                gj.advance(a, rank, jb, ns, pa, ppa, nabs);
             }
         }
     }
-    if (meta & kMetaTap) next += a.sp_tapw[rank] * s_n;
+    if (meta & kMetaTap) next += a.sp_tapw[rank] * a.s_n;
     return next;
-}
-
-// special_update_io of a single step: the state advances in place.
-template <typename T, typename S>
-__device__ __forceinline__ S special_update(const StepParams<T, S>& a, uint32_t meta, uint32_t c0, uint32_t rank,
-                                            S pc, S pp, const S (&nb)[6], const S (&u0)[kSlotGroup<S>]) {
-    const SlotIo<S> io{a.st_u, a.st_g, a.st_u, a.st_g};
-    return special_update_io(a, io, meta, c0, rank, pc, pp, nb, u0, a.s_n);
 }
 
 // A tile is a span of 64*W consecutive cells of the pitched xy plane, one

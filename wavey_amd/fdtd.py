@@ -33,7 +33,7 @@ MODE_SPECTRA, MODE_IR, MODE_BOTH = 1, 2, 3
 FP32, FP64, FP32_STATE64 = 0, 1, 2
 PRECISIONS = {"fp32": FP32, "fp64": FP64, "fp32_state64": FP32_STATE64}
 SERVE_ALL, SERVE_SHORTEST, SERVE_LANES = 0, 1, 2
-BATCH_STEPS, BATCH_PERSISTENT, BATCH_FUSED = 0, 1, 2
+BATCH_STEPS, BATCH_PERSISTENT = 0, 1
 COURANT = 0.5773502691896257
 AIR = 1 << 6
 
@@ -893,11 +893,8 @@ class Batch:
         return dict(zip(keys, [int(v) for v in out]))
 
     def set_launch(self, form):
-        """BATCH_STEPS (a launch per step), BATCH_PERSISTENT (one launch per
-        128-step block, each XCD running its share of the instances) or
-        BATCH_FUSED (one launch per three steps, boundary cells in the same
-        march, for the FP32 instances with rows of at most 768 cells; a launch
-        per step for the others). Runs that have not started take the form."""
+        """BATCH_STEPS (a launch per step) or BATCH_PERSISTENT (one launch per
+        128-step block, each XCD running its share of the instances)."""
         err = C.create_string_buffer(1024)
         _check(lib().wv_batch_set_launch(self._h, int(form), err, 1024), err)
 
