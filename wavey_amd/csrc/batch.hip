@@ -430,6 +430,21 @@ void instance_begin(Instance* in, uint32_t zchunk) {
     in->done = false;
 }
 
+// Returns an instance's run to step 0 with the layout it has (no
+// reclassification or compaction).
+void instance_rewind(Instance* in) {
+    run_rewind(in->plan.get());
+    in->n0 = 0;
+    in->done = false;
+}
+
+// Restarts an instance from rest with z-chunk zchunk: a rewind when its
+// layout already has it.
+void instance_restart(Instance* in, uint32_t zchunk) {
+    if (in->plan->run_zchunk == (zchunk ? zchunk : kDefaultZChunk)) instance_rewind(in);
+    else instance_begin(in, zchunk);
+}
+
 template <typename S>
 std::unique_ptr<Instance> make_instance(int prec, const wv_grid* g, const wv_tap* taps, size_t n_taps,
                                         const double* signal, size_t steps, const uint64_t* recv, size_t n_recv) {
@@ -710,7 +725,7 @@ double time_table(wv_batch* b, uint32_t warmup, uint32_t steps) {
 template <typename T, typename S>
 void batch_autotune(wv_batch* b, uint32_t trial_steps) {
     WV_HIP(hipStreamSynchronize(b->stream));
-    for (auto& in : b->inst) instance_begin(in.get(), in->plan->run_zchunk);
+    for (auto& in : b->inst) instance_rewind(in.get());
     int best_v = b->variant;
     double best = 1e300;
     for (int v : kTuneVariants) {
@@ -730,8 +745,9 @@ void batch_autotune(wv_batch* b, uint32_t trial_steps) {
         uint32_t best_z = 0;
         double best_t = 1e300;
         for (uint32_t zc : kTuneZChunks) {
+            // Timing needs the layout, not a run from rest.
             for (auto& in : b->inst)
-                if (in->size_class == cls) instance_begin(in.get(), zc);
+                if (in->size_class == cls && in->plan->run_zchunk != zc) instance_begin(in.get(), zc);
             const double t = time_table<T, S>(b, 2, trial_steps);
             if (t < best_t) {
                 best_t = t;
@@ -740,7 +756,7 @@ void batch_autotune(wv_batch* b, uint32_t trial_steps) {
         }
         chosen[cls] = best_z;
         for (auto& in : b->inst)
-            if (in->size_class == cls) instance_begin(in.get(), best_z);
+            if (in->size_class == cls && in->plan->run_zchunk != best_z) instance_begin(in.get(), best_z);
     }
     // The launch form, with the shapes chosen.
     int best_l = WV_BATCH_STEPS;
@@ -748,7 +764,6 @@ void batch_autotune(wv_batch* b, uint32_t trial_steps) {
         double tl[2];
         for (int l : {WV_BATCH_STEPS, WV_BATCH_PERSISTENT}) {
             b->launch = l;
-            for (auto& in : b->inst) instance_begin(in.get(), chosen[in->size_class]);
             tl[l] = time_table<T, S>(b, 2, trial_steps);
         }
         best_l = tl[WV_BATCH_PERSISTENT] < tl[WV_BATCH_STEPS] ? WV_BATCH_PERSISTENT : WV_BATCH_STEPS;
@@ -761,7 +776,7 @@ void batch_autotune(wv_batch* b, uint32_t trial_steps) {
         for (auto& [cls, zc] : chosen) g_tune_zchunk[{b->prec, cls}] = zc;
     }
     // Every instance restarts from rest with its class's z-chunk.
-    for (auto& in : b->inst) instance_begin(in.get(), chosen[in->size_class]);
+    for (auto& in : b->inst) instance_restart(in.get(), chosen[in->size_class]);
     b->dirty = true;
 }
 

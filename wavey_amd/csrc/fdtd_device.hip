@@ -701,6 +701,32 @@ uint32_t compact_cells(wv_plan* pl, const At& at, const Geo& geo, DBuf& counts, 
     return n;
 }
 
+// The state of step 0 of the plan's run (its layout already built): zeroed
+// fields, records, slot state and re-centring offsets, p in the first field
+// buffer and prev in the second. Waits for the fills.
+template <class S>
+void run_zero(wv_plan* pl, size_t steps, size_t n_recv) {
+    const size_t nfields = pl->run_steps_per_pass == 3 ? 5 : pl->run_steps_per_pass == 2 ? 4 : 2;
+    DBuf* fields[5] = {&pl->fa, &pl->fb, &pl->fc, &pl->fd, &pl->fe};
+    for (size_t f = 0; f < nfields; ++f) {
+        if (pl->prec == WV_FP64) fill_async<double>(pl, *fields[f], pl->npitched);
+        else fill_async<float>(pl, *fields[f], pl->npitched);
+    }
+    if (pl->prec == WV_FP64) fill_async<double>(pl, pl->ir, steps * n_recv);
+    else fill_async<float>(pl, pl->ir, steps * n_recv);
+    const size_t nstate = (size_t)pl->max_slots * pl->sp_pad;
+    fill_async<S>(pl, pl->st_u, nstate);
+    fill_async<S>(pl, pl->st_g, nstate);
+    fill_async<double>(pl, pl->d0, 1);
+    fill_async<double>(pl, pl->prev_mean, 1);
+    fill_async<double>(pl, pl->d0h, kOffsets);
+    pl->recentred_at = 0;
+    pl->buf_p = 0;
+    pl->buf_q = 1;
+    pl->done = 0;
+    WV_HIP(hipStreamSynchronize(pl->stream));
+}
+
 template <class S>
 void run_begin_impl(wv_plan* pl, const wv_tap* taps, size_t n_taps, const double* signal, size_t steps,
                     const uint64_t* receivers, size_t n_recv) {
@@ -797,27 +823,12 @@ void run_begin_impl(wv_plan* pl, const wv_tap* taps, size_t n_taps, const double
                                     geo, pl->ring2_counts, pl->ring2_offs, pl->ring2_cell);
     }
     pl->ir.alloc(std::max<size_t>(1, steps * n_recv) * pl->tsize());
-    // Zeroed fields, state and records.
-    const size_t nfields = pl->run_steps_per_pass == 3 ? 5 : pl->run_steps_per_pass == 2 ? 4 : 2;
-    DBuf* fields[5] = {&pl->fa, &pl->fb, &pl->fc, &pl->fd, &pl->fe};
-    for (size_t f = 0; f < nfields; ++f) {
-        if (pl->prec == WV_FP64) fill_async<double>(pl, *fields[f], pl->npitched);
-        else fill_async<float>(pl, *fields[f], pl->npitched);
-    }
-    if (pl->prec == WV_FP64) fill_async<double>(pl, pl->ir, steps * n_recv);
-    else fill_async<float>(pl, pl->ir, steps * n_recv);
-    fill_async<S>(pl, pl->st_u, nstate);
-    fill_async<S>(pl, pl->st_g, nstate);
     pl->d0.alloc(8);
     pl->prev_mean.alloc(8);
     pl->d0h.alloc(kOffsets * 8);
     pl->mean_part.alloc(kMeanBlocks * 8);
     pl->shift.alloc(4);
-    fill_async<double>(pl, pl->d0, 1);
-    fill_async<double>(pl, pl->prev_mean, 1);
-    fill_async<double>(pl, pl->d0h, kOffsets);
-    pl->recentred_at = 0;
-    WV_HIP(hipStreamSynchronize(pl->stream));
+    run_zero<S>(pl, steps, n_recv);
     pl->signal.assign(signal, signal + steps);
     pl->steps = steps;
     pl->done = 0;
@@ -1061,6 +1072,12 @@ void wv::run_begin(wv_plan* pl, const wv_tap* taps, size_t n_taps, const double*
                    const uint64_t* receivers, size_t n_recv) {
     if (pl->prec == WV_FP32) run_begin_impl<float>(pl, taps, n_taps, signal, steps, receivers, n_recv);
     else run_begin_impl<double>(pl, taps, n_taps, signal, steps, receivers, n_recv);
+}
+
+void wv::run_rewind(wv_plan* pl) {
+    if (!pl->active) throw DevErr{"run_rewind: no run begun", WV_ERR_INVALID};
+    if (pl->prec == WV_FP32) run_zero<float>(pl, pl->steps, pl->n_recv);
+    else run_zero<double>(pl, pl->steps, pl->n_recv);
 }
 
 uint64_t wv::enqueue_steps(wv_plan* pl, size_t n, size_t stop, hipStream_t stream) {

@@ -251,6 +251,9 @@ void plan_setup(wv_plan* pl, const wv_grid* g);
 // cell records, zeroed fields and state. Throws DevErr.
 void run_begin(wv_plan* pl, const wv_tap* taps, size_t n_taps, const double* signal, size_t steps,
                const uint64_t* receivers, size_t n_recv);
+// Returns the plan's begun run to step 0 (fields, records, slot state and
+// re-centring offsets zeroed) without rebuilding its layout. Throws DevErr.
+void run_rewind(wv_plan* pl);
 // Enqueues steps [n, stop) of the plan's current run on `stream` (two-step
 // passes where the run has them) without waiting; the plan's buf_p / buf_q
 // then name the buffers of p^stop and p^(stop-1). Returns the launches.
