@@ -203,12 +203,21 @@ def bench_room_mix(args):
 
     rooms = configs.room_mix(n_rooms=args.rooms)
     t0 = time.perf_counter()
-    cases = [workload.build_case(cfg) for cfg in rooms]
+    cases = [workload.build_case(cfg, steps=args.mix_steps or None) for cfg in rooms]
     t_build = time.perf_counter() - t0
     inst = _instances(cases)
     work = sum(c.grid.air_cells() * len(c.signal) for _, c, _ in inst)
     print(f"bench: room mix {len(cases)} rooms, {len(inst)} instances, {work / 1e12:.2f} T cell-updates, "
           f"built in {t_build:.1f} s", file=sys.stderr, flush=True)
+
+    # A plain copy of known size in the same process (the calibration of
+    # HBM counter profiles of short runs, --mix-steps).
+    copy = None
+    if args.mix_steps:
+        big = max(cases, key=lambda c: c.grid.air_cells())
+        with fdtd.Plan(big.grid, args.precision) as p:
+            nbytes = (big.grid.air_cells() * 4) // 4096 * 4096
+            copy = {"bytes": int(nbytes), "ms": round(p.copy_probe(nbytes, reps=5), 4)}
 
     # Sequential plain runs.
     plain = []
@@ -287,7 +296,8 @@ def bench_room_mix(args):
         "dtype": {"fp32": "f32", "fp64": "f64", "fp32_state64": "f32+f64state"}[args.precision],
         "data": "synthetic (seeded rooms, materials, speakers, receivers)",
         "config": {"workload": f"room mix: {len(cases)} rooms, dx 0.05 m, {n} per-speaker instances, "
-                               f"{len(cases[0].signal)} steps (0.5 s), 16 receivers each",
+                               f"{len(cases[0].signal)} steps{' (0.5 s)' if not args.mix_steps else ''}, "
+                               f"16 receivers each",
                    "instances": n, "cell_updates": int(work), "precision": args.precision,
                    "graphs": not args.no_graphs, "autotune": not args.no_autotune},
         "sequential_plain": {"solves_per_s": round(n / t_plain, 3), "seconds": round(t_plain, 3),
@@ -311,6 +321,8 @@ def bench_room_mix(args):
         "end_to_end_gcell_per_s": round(work / t_batch / 1e9, 2),
         "setup_s": {"grid_build": round(t_build, 2)},
     }
+    if copy:
+        line["copy_probe"] = copy
     return line
 
 
@@ -495,6 +507,8 @@ def main():
     ap.add_argument("--workload", default="hall",
                     choices=["hall", "pillared_room", "shoebox", "room_mix", "serving"])
     ap.add_argument("--rooms", type=int, default=48, help="room_mix: rooms")
+    ap.add_argument("--mix-steps", type=int, default=0,
+                    help="room_mix: steps per instance (0 = the workload's 0.5 s; fewer for counter profiles)")
     ap.add_argument("--requests", type=int, default=40, help="serving: requests")
     ap.add_argument("--load", type=float, default=0.8, help="serving: offered fraction of --throughput")
     ap.add_argument("--burst", action="store_true", help="serving: every request arrives at once")
