@@ -132,6 +132,43 @@ def test_hall_full_size_pass_modes_and_oracle(hall):
     assert e32 < 1e-4 and w32 < 1e-4
 
 
+def test_hall_every_component_run(hall):
+    """All eight component runs of the hall's wave job (two cardioid sources,
+    a monopole and three dipoles each; run.rs:601-661), not only the first:
+    each run's series at cells near its source, the plain FP32 run (three
+    steps per pass) against the FP64 run of the same scheme (1e-4 relative
+    L2), and the same eight runs as one batch, bit-identical to the plain
+    FP32 runs."""
+    case, _, _ = hall
+    g = case.grid
+    sig = case.signal
+    nx, ny, nz = g.n
+    assert len(case.runs) == 8 and {si for si, _, _ in case.runs} == {0, 1}
+    rxs = []
+    for _, _, taps in case.runs:
+        c0 = taps[0][0]
+        x, y, z = c0 % nx, (c0 // nx) % ny, c0 // (nx * ny)
+        pts = [(x + 3, y, z), (x, y - 5, z + 2), (x + 8, y + 8, z + 8), (x - 12, y + 4, z - 6)]
+        rxs.append([air_cell(g, *p) for p in pts])
+    plain32, plain64 = [], []
+    for k, (si, ci, taps) in enumerate(case.runs):
+        print(f"case: hall source {si} component {ci} ({len(taps)} taps), {HALL_STEPS} steps, fp32 and fp64",
+              flush=True)
+        for prec, out in (("fp32", plain32), ("fp64", plain64)):
+            with fdtd.Plan(g, prec) as plan:
+                out.append(plan.run(taps, sig, rxs[k], mode=fdtd.MODE_IR)[0])
+        err = rel_l2(plain32[k], plain64[k])
+        print(f"  FP32 vs FP64 relative L2 {err:.3e}", flush=True)
+        assert np.count_nonzero(plain64[k], axis=1).min() >= 100
+        assert err < 1e-4
+    print("case: hall, the eight runs as one batch, fp32", flush=True)
+    with fdtd.Batch("fp32") as b:
+        ids = [b.add(g, taps, sig, rxs[k]) for k, (_, _, taps) in enumerate(case.runs)]
+        assert b.run() == fdtd.WV_OK
+        for k, i in enumerate(ids):
+            assert np.array_equal(b.result(i)[0], plain32[k]), f"run {k}"
+
+
 def tall_room(seed=configs.MASTER_SEED):
     """12 x 7.5 x 6 m at dx 0.04 m: 302 x 190 x 152 cells (8.7 Mi, three steps
     per pass by default), two 0.6 m pillars, ten three-section materials on
