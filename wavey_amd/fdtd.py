@@ -26,7 +26,9 @@ from dataclasses import dataclass, field
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libwavey_fdtd.so")
+# WAVEY_FDTD_LIB names another build of the same library (for example the
+# host-sanitizer build of wavey_amd/csrc/Makefile's `asan` target).
+LIB_PATH = os.environ.get("WAVEY_FDTD_LIB") or os.path.join(_HERE, "libwavey_fdtd.so")
 
 WV_OK, WV_CANCELLED = 0, 1
 MODE_SPECTRA, MODE_IR, MODE_BOTH = 1, 2, 3
