@@ -1,22 +1,37 @@
 #!/bin/bash
-# Runs the CPU tests against the host code built with AddressSanitizer and
-# UBSan (make asan). python is not linked with the sanitizer runtime, so it
-# is preloaded, with libstdc++ for its exception interceptor; any report
-# fails the run. Usage: wavey_amd/csrc/check_host.sh [pytest args]
+# Runs the tests against the host code built with AddressSanitizer and UBSan.
+#   check_host.sh [pytest args]        CPU tests; host sources built with g++
+#                                      (make asan)
+#   check_host.sh --gpu [pytest args]  GPU tests; every source's host half
+#                                      built with clang's sanitizers, device
+#                                      code unchanged (make asan-hip)
+# python is not linked with a sanitizer runtime, so it is preloaded, with
+# libstdc++ for its exception interceptor; any report fails the run.
 set -e
 here="$(cd "$(dirname "$0")" && pwd)"
 repo="$(cd "$here/../.." && pwd)"
-make -C "$here" -j8 asan
-logs="$here/_build/asan/logs"
+if [ "$1" = "--gpu" ]; then
+    shift
+    target=asan-hip
+    lib="$here/_build/asan_hip/libwavey_fdtd.so"
+    san_rt="$(ls /opt/rocm/llvm/lib/clang/*/lib/linux/libclang_rt.asan-x86_64.so | head -1)"
+    marker=gpu
+else
+    target=asan
+    lib="$here/_build/asan/libwavey_fdtd.so"
+    san_rt="$(g++ -print-file-name=libasan.so)"
+    marker="not gpu"
+fi
+make -C "$here" -j8 "$target"
+logs="$(dirname "$lib")/logs"
 rm -rf "$logs" && mkdir -p "$logs"
-asan_rt="$(g++ -print-file-name=libasan.so)"
 cxx_rt="$(g++ -print-file-name=libstdc++.so.6)"
 set +e
-WAVEY_FDTD_LIB="$here/_build/asan/libwavey_fdtd.so" \
+WAVEY_FDTD_LIB="$lib" \
 ASAN_OPTIONS="detect_leaks=0:verify_asan_link_order=0:log_path=$logs/asan" \
 UBSAN_OPTIONS="print_stacktrace=1:log_path=$logs/ubsan" \
-LD_PRELOAD="$asan_rt:$cxx_rt${LD_PRELOAD:+:$LD_PRELOAD}" \
-    python3 -m pytest "$repo/tests" -q -m "not gpu" -p no:cacheprovider "$@"
+LD_PRELOAD="$san_rt:$cxx_rt${LD_PRELOAD:+:$LD_PRELOAD}" \
+    python3 -m pytest "$repo/tests" -q -m "$marker" -p no:cacheprovider "$@"
 rc=$?
 set -e
 if ls "$logs"/* > /dev/null 2>&1; then
