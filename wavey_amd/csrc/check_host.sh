@@ -3,7 +3,7 @@
 #   check_host.sh [pytest args]        CPU tests; host sources built with g++
 #                                      (make asan)
 #   check_host.sh --gpu [pytest args]  GPU tests; every source's host half
-#                                      built with clang's sanitizers, device
+#                                      built with the sanitizers, device
 #                                      code unchanged (make asan-hip)
 # python is not linked with a sanitizer runtime, so it is preloaded, with
 # libstdc++ for its exception interceptor; any report fails the run.
@@ -14,7 +14,7 @@ if [ "$1" = "--gpu" ]; then
     shift
     target=asan-hip
     lib="$here/_build/asan_hip/libwavey_fdtd.so"
-    san_rt="$(ls /opt/rocm/llvm/lib/clang/*/lib/linux/libclang_rt.asan-x86_64.so | head -1)"
+    san_rt="$(g++ -print-file-name=libasan.so)"
     marker=gpu
 else
     target=asan
