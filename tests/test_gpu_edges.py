@@ -42,6 +42,50 @@ def test_no_receivers(shoebox):
     assert ir.shape == (0, 300) and sp.shape == (0, 1)
 
 
+def test_zero_steps_every_entry_point(shoebox):
+    """An empty signal runs no step (gpu_fdtd.rs:200-203): all-zero spectra,
+    an empty receiver series and no progress call, through the plan, both
+    one-call entry points, a batch beside a run of 200 steps and the server;
+    the longer run beside it is unchanged."""
+    case, _ = shoebox
+    taps, rx = case.runs[0][2], case.receivers
+    freqs = [62.5, 125.0]
+    empty, sig = case.signal[:0], case.signal[:200]
+    R = len(rx)
+    print("case: shoebox, zero steps, plan", flush=True)
+    calls = []
+    with fdtd.Plan(case.grid, "fp32") as plan:
+        ir, sp = plan.run(taps, empty, rx, freqs, mode=fdtd.MODE_BOTH, progress=calls.append)
+        assert ir.shape == (R, 0) and sp.shape == (R, 2) and np.all(sp == 0)
+        ref, _ = plan.run(taps, sig, rx, mode=fdtd.MODE_IR)
+    assert calls == []
+    assert np.count_nonzero(ref) > 0
+    for fn in (fdtd.fdtd_run, fdtd.fdtd_run_noalloc):
+        print(f"case: shoebox, zero steps, {fn.__name__}", flush=True)
+        ir, sp = fn(case.grid, taps, empty, rx, freqs, fdtd.MODE_BOTH, "fp64", progress=calls.append)
+        assert ir.shape == (R, 0) and sp.shape == (R, 2) and np.all(sp == 0)
+    assert calls == []
+    print("case: shoebox, zero steps, batch beside 200 steps", flush=True)
+    with fdtd.Batch("fp32") as b:
+        i0 = b.add(case.grid, taps, empty, rx)
+        i1 = b.add(case.grid, taps, sig, rx)
+        assert b.run() == fdtd.WV_OK
+        ir0, sp0 = b.result(i0, freqs, mode=fdtd.MODE_BOTH)
+        ir1, _ = b.result(i1)
+    assert ir0.shape == (R, 0) and np.all(sp0 == 0)
+    assert np.array_equal(ir1, ref)
+    print("case: shoebox, zero steps, server beside 200 steps", flush=True)
+    with fdtd.Server("fp32") as sv:
+        r0 = sv.submit(case.grid, taps, empty, rx)
+        r1 = sv.submit(case.grid, taps, sig, rx)
+        ir0, _, _ = sv.wait(r0)
+        ir1, _, _ = sv.wait(r1)
+        st = sv.stats()
+    assert ir0.shape == (R, 0)
+    assert np.array_equal(ir1, ref)
+    assert st["completed"] == 2 and st["failed"] == 0
+
+
 @pytest.mark.parametrize("steps", [1, 2, 3, 130])
 def test_short_runs_both_pass_modes(shoebox, steps):
     case, og = shoebox

@@ -679,7 +679,8 @@ wv_status batch_run(wv_batch* b, bool use_graph, const volatile int32_t* cancel,
         }
         max_steps += (uint64_t)issued * kGraphSteps;
         WV_HIP(hipStreamSynchronize(b->stream));
-        if (!solo && b->last_persist) check_persist(b, kGraphSteps);
+        // (No block is issued while an instance of no steps heads the table.)
+        if (!solo && issued && b->last_persist) check_persist(b, kGraphSteps);
         done_steps += (uint64_t)b->table.size() * issued * kGraphSteps;
         collect_done(b);
         if (progress && total) progress(std::min(1.0, (double)done_steps / (double)total), ctx);

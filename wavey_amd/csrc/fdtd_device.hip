@@ -1027,7 +1027,9 @@ wv_status advance_impl(wv_plan* pl, size_t k, const volatile int32_t* cancel, wv
     }
     WV_HIP(hipEventRecord(pl->t1, pl->stream));
     WV_HIP(hipEventSynchronize(pl->t1));
-    if (st == WV_OK && progress) progress((double)pl->done / (double)pl->steps, ctx);
+    // An empty signal reports no progress, as the reference's step loop
+    // (gpu_fdtd.rs:200-260) makes no call then.
+    if (st == WV_OK && progress && pl->steps) progress((double)pl->done / (double)pl->steps, ctx);
     float ms = 0.f;
     WV_HIP(hipEventElapsedTime(&ms, pl->t0, pl->t1));
     pl->last_ms = ms;
