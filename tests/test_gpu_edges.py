@@ -9,8 +9,10 @@
   with a receiver on the solid cell;
 * air up to the grid's x ends (a caller's grid without the builder's padding
   columns), so that cells at x = 1 and x = nx - 2 are interior: every pass
-  mode bit-identical and FP64 against the oracle.
+  mode bit-identical and FP64 against the oracle;
+* an empty signal, and bad run arguments, through every entry point.
 """
+import ctypes as C
 import dataclasses
 
 import numpy as np
@@ -84,6 +86,50 @@ def test_zero_steps_every_entry_point(shoebox):
     assert ir0.shape == (R, 0)
     assert np.array_equal(ir1, ref)
     assert st["completed"] == 2 and st["failed"] == 0
+
+
+def test_run_argument_errors(shoebox):
+    """Bad run arguments come back as status codes with a message (where the
+    reference would panic or return Err), from the plan, the one-call entry
+    point, the batch and the server, and the plan runs normally afterwards: a
+    tap on the grid's first plane or beyond its last cell, a receiver beyond
+    the last cell (WV_ERR_INVALID), and a receiver record of 2^31 steps
+    (WV_ERR_LIMIT, refused before the signal is read)."""
+    case, _ = shoebox
+    g = case.grid
+    taps, rx, sig = case.runs[0][2], case.receivers, case.signal[:50]
+    cells = g.cells
+    bad = [("tap on plane 0", [(g.n[0] + 1, 1.0)], rx, "source tap"),
+           ("tap beyond the grid", [(cells, 1.0)], rx, "source tap"),
+           ("receiver beyond the grid", taps, [cells], "receiver outside")]
+    with fdtd.Plan(g, "fp32") as plan:
+        ref, _ = plan.run(taps, sig, rx, mode=fdtd.MODE_IR)
+        for name, t, r, msg in bad:
+            print(f"case: shoebox, {name}", flush=True)
+            for call in (lambda: plan.run(t, sig, r, mode=fdtd.MODE_IR),
+                         lambda: fdtd.fdtd_run(g, t, sig, r, (), fdtd.MODE_IR)):
+                with pytest.raises(fdtd.FdtdError) as e:
+                    call()
+                assert e.value.status == fdtd.WV_ERR_INVALID and msg in str(e.value)
+            with fdtd.Batch("fp32") as b:
+                with pytest.raises(fdtd.FdtdError) as e:
+                    b.add(g, t, sig, r)
+                assert e.value.status == fdtd.WV_ERR_INVALID
+            with fdtd.Server("fp32") as sv:
+                with pytest.raises(fdtd.FdtdError) as e:
+                    sv.submit(g, t, sig, r)
+                assert e.value.status == fdtd.WV_ERR_INVALID
+        again, _ = plan.run(taps, sig, rx, mode=fdtd.MODE_IR)
+    assert np.array_equal(again, ref) and np.count_nonzero(ref) > 0
+    print("case: shoebox, 2^31 steps", flush=True)
+    one = np.zeros(1)
+    rc = np.ascontiguousarray(rx, dtype=np.uint64)
+    err = C.create_string_buffer(256)
+    st = fdtd.lib().wv_fdtd_run(C.byref(g.view()), fdtd._taps_array(taps), len(taps),
+                                one.ctypes.data_as(C.POINTER(C.c_double)), 1 << 31,
+                                rc.ctypes.data_as(C.POINTER(C.c_uint64)), len(rc), None, 0, fdtd.MODE_IR, 0,
+                                None, fdtd.PROGRESS_FN(), None, None, None, err, 256)
+    assert st == fdtd.WV_ERR_LIMIT and b"too large" in err.value
 
 
 @pytest.mark.parametrize("steps", [1, 2, 3, 130])

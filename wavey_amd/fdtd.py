@@ -31,6 +31,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("WAVEY_FDTD_LIB") or os.path.join(_HERE, "libwavey_fdtd.so")
 
 WV_OK, WV_CANCELLED = 0, 1
+WV_ERR_INVALID, WV_ERR_DEVICE, WV_ERR_LIMIT, WV_ERR_NOMEM = -1, -2, -3, -4
 MODE_SPECTRA, MODE_IR, MODE_BOTH = 1, 2, 3
 FP32, FP64, FP32_STATE64 = 0, 1, 2
 PRECISIONS = {"fp32": FP32, "fp64": FP64, "fp32_state64": FP32_STATE64}
@@ -158,7 +159,7 @@ def lib():
     with _lock:
         if _lib is None:
             if not os.path.exists(LIB_PATH):
-                raise FdtdError(-2, f"libwavey_fdtd.so is not built ({LIB_PATH}); run __graft_entry__.build()")
+                raise FdtdError(WV_ERR_DEVICE, f"libwavey_fdtd.so is not built ({LIB_PATH}); run __graft_entry__.build()")
             L = C.CDLL(LIB_PATH)
             P = C.POINTER
             st, sz, dp = C.c_int, C.c_size_t, P(C.c_double)
@@ -430,7 +431,7 @@ class FdtdGrid:
         fits, keep = _admittances(room["fits"])
         patches = room["patches"]
         if len(patches) != 6 + 4 * len(pil):
-            raise FdtdError(-1, "one patch entry per surface is needed")
+            raise FdtdError(WV_ERR_INVALID, "one patch entry per surface is needed")
         gu = np.array([p[0] for p in patches], dtype=np.uint32)
         gv = np.array([p[1] for p in patches], dtype=np.uint32)
         off = np.zeros(len(patches), dtype=np.uint32)
@@ -890,7 +891,7 @@ class Batch:
     def info(self, iid):
         out = (C.c_uint64 * 6)()
         if lib().wv_batch_info(self._h, int(iid), out) != WV_OK:
-            raise FdtdError(-1, "unknown instance")
+            raise FdtdError(WV_ERR_INVALID, "unknown instance")
         keys = ["cells", "air_cells", "steps", "zchunk", "size_class", "device_bytes"]
         return dict(zip(keys, [int(v) for v in out]))
 
@@ -933,7 +934,7 @@ class Server:
         (fill_cells 0 = the library's 16 Mi)."""
         code = {"all": SERVE_ALL, "shortest": SERVE_SHORTEST, "lanes": SERVE_LANES}[policy]
         if lib().wv_serve_set_policy(self._h, code, int(fill_cells), int(lane_cells)) != 0:
-            raise FdtdError(-1, "wv_serve_set_policy: invalid argument")
+            raise FdtdError(WV_ERR_INVALID, "wv_serve_set_policy: invalid argument")
 
     def close(self):
         if self._h:
