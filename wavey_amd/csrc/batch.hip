@@ -1283,7 +1283,12 @@ void serve_loop(wv_server* sv) {
     }
 }
 
+// The worker thread. Whatever ends it early (a device error, a failed host
+// allocation, any other exception) fails every running and queued request,
+// so no wv_serve_wait is left waiting, and later submissions are refused.
 void serve_main(wv_server* sv) {
+    wv_status code = WV_OK;
+    std::string msg;
     try {
         WV_HIP(hipSetDevice(sv->device));
         auto run = [&](auto t, auto s) -> wv_status {
@@ -1292,15 +1297,22 @@ void serve_main(wv_server* sv) {
         };
         dispatch_prec(sv->prec, run);
     } catch (const DevErr& e) {
+        code = e.code;
+        msg = e.msg;
+    } catch (const std::bad_alloc&) {
+        code = WV_ERR_NOMEM;
+        msg = "host allocation failed";
+    } catch (const std::exception& e) {
+        code = WV_ERR_DEVICE;
+        msg = std::string("server worker: ") + e.what();
+    }
+    if (code != WV_OK) {
         std::lock_guard<std::mutex> lk(sv->mu);
-        sv->fatal = e.msg;
-        for (auto& r : sv->running) fail_request(sv, r, e.code, e.msg);
-        for (auto& r : sv->pending) fail_request(sv, r, e.code, e.msg);
+        sv->fatal = msg;
+        for (auto& r : sv->running) fail_request(sv, r, code, msg);
+        for (auto& r : sv->pending) fail_request(sv, r, code, msg);
         sv->running.clear();
         sv->pending.clear();
-    } catch (const std::bad_alloc&) {
-        std::lock_guard<std::mutex> lk(sv->mu);
-        sv->fatal = "host allocation failed";
     }
     sv->cv_done.notify_all();
 }

@@ -53,11 +53,11 @@ wv_status run_plain(const wv_grid* g, const std::vector<wv_tap>& taps, const std
 
 }  // namespace
 
-extern "C" wv_status wv_calibrate_cabinet(const double* verts, uint32_t n_verts, const uint32_t* tris,
-                                          uint32_t n_tris, const double ac[3], double c, double f_max, double ppw,
-                                          double cells_across, const double* freqs, size_t nf, int precision,
-                                          int batched, const volatile int32_t* cancel, double* on_axis_out,
-                                          double* angles_out, double* directivity_out, char* err, size_t err_len) {
+static wv_status calibrate_cabinet_impl(const double* verts, uint32_t n_verts, const uint32_t* tris,
+                                        uint32_t n_tris, const double ac[3], double c, double f_max, double ppw,
+                                        double cells_across, const double* freqs, size_t nf, int precision,
+                                        int batched, const volatile int32_t* cancel, double* on_axis_out,
+                                        double* angles_out, double* directivity_out, char* err, size_t err_len) {
     if (!verts || !n_verts || !tris || !n_tris || !ac || (nf && !freqs) || !on_axis_out || !angles_out ||
         !directivity_out || !(c > 0.0) || !(f_max > 0.0) || !(ppw > 0.0) || !(cells_across > 0.0)) {
         set_err(err, err_len, "wv_calibrate_cabinet: invalid argument");
@@ -201,4 +201,16 @@ extern "C" wv_status wv_calibrate_cabinet(const double* verts, uint32_t n_verts,
         }
     }
     return WV_OK;
+}
+
+extern "C" wv_status wv_calibrate_cabinet(const double* verts, uint32_t n_verts, const uint32_t* tris,
+                                          uint32_t n_tris, const double ac[3], double c, double f_max, double ppw,
+                                          double cells_across, const double* freqs, size_t nf, int precision,
+                                          int batched, const volatile int32_t* cancel, double* on_axis_out,
+                                          double* angles_out, double* directivity_out, char* err, size_t err_len) {
+    return host_guarded(err, err_len, [&]() {
+        return calibrate_cabinet_impl(verts, n_verts, tris, n_tris, ac, c, f_max, ppw, cells_across, freqs, nf,
+                                      precision, batched, cancel, on_axis_out, angles_out, directivity_out, err,
+                                      err_len);
+    });
 }

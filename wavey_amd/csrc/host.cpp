@@ -208,8 +208,8 @@ struct FaceSig {
 
 }  // namespace
 
-extern "C" wv_status wv_grid_build_box(const wv_box_room* room, double c, double dx, int threads,
-                                       wv_built_grid** out, char* err, size_t err_len) {
+static wv_status grid_build_box_impl(const wv_box_room* room, double c, double dx, int threads,
+                                     wv_built_grid** out, char* err, size_t err_len) {
     if (!room || !out || !(dx > 0.0) || !(c > 0.0)) {
         set_err(err, err_len, "wv_grid_build_box: invalid argument");
         return WV_ERR_INVALID;
@@ -312,6 +312,11 @@ extern "C" wv_status wv_grid_build_box(const wv_box_room* room, double c, double
     finish_view(*g, B.n, org, dx, dt);
     *out = g.release();
     return WV_OK;
+}
+
+extern "C" wv_status wv_grid_build_box(const wv_box_room* room, double c, double dx, int threads,
+                                       wv_built_grid** out, char* err, size_t err_len) {
+    return host_guarded(err, err_len, [&]() { return grid_build_box_impl(room, c, dx, threads, out, err, err_len); });
 }
 
 extern "C" const wv_grid* wv_built_grid_view(const wv_built_grid* g) { return g ? &g->view : nullptr; }

@@ -241,6 +241,10 @@ inline wv_status guarded(char* err, size_t err_len, F f) {
     } catch (const std::bad_alloc&) {
         set_err(err, err_len, "host allocation failed");
         return WV_ERR_NOMEM;
+    } catch (const std::exception& e) {
+        // A host-side failure (a thread that cannot start): the caller falls back.
+        set_err(err, err_len, std::string("internal error: ") + e.what());
+        return WV_ERR_DEVICE;
     }
 }
 

@@ -96,7 +96,7 @@ bool write_file(const std::filesystem::path& p, const void* data, size_t n, std:
 
 }  // namespace
 
-extern "C" wv_status wv_results_save(const char* dir, const wv_results* r, char* err, size_t err_len) {
+static wv_status results_save_impl(const char* dir, const wv_results* r, char* err, size_t err_len) {
     if (!dir || !r || (r->nf && !r->freqs) || (r->n_speakers && !r->speaker_ids) || (r->n_mics && !r->mics) ||
         (r->n_mics && r->n_speakers && r->nf && !r->transfer) || (r->n_notes && !r->notes)) {
         set_err(err, err_len, "wv_results_save: invalid argument");
@@ -226,8 +226,12 @@ extern "C" wv_status wv_results_save(const char* dir, const wv_results* r, char*
     return WV_OK;
 }
 
-extern "C" wv_status wv_results_read_transfer(const char* dir, size_t n_mics, size_t n_speakers, size_t nf,
-                                              float* out, char* err, size_t err_len) {
+extern "C" wv_status wv_results_save(const char* dir, const wv_results* r, char* err, size_t err_len) {
+    return host_guarded(err, err_len, [&]() { return results_save_impl(dir, r, err, err_len); });
+}
+
+static wv_status results_read_transfer_impl(const char* dir, size_t n_mics, size_t n_speakers, size_t nf,
+                                            float* out, char* err, size_t err_len) {
     if (!dir || (n_mics * n_speakers * nf != 0 && !out)) {
         set_err(err, err_len, "wv_results_read_transfer: invalid argument");
         return WV_ERR_INVALID;
@@ -248,4 +252,11 @@ extern "C" wv_status wv_results_read_transfer(const char* dir, size_t n_mics, si
         return WV_ERR_INVALID;
     }
     return WV_OK;
+}
+
+extern "C" wv_status wv_results_read_transfer(const char* dir, size_t n_mics, size_t n_speakers, size_t nf,
+                                              float* out, char* err, size_t err_len) {
+    return host_guarded(err, err_len, [&]() {
+        return results_read_transfer_impl(dir, n_mics, n_speakers, nf, out, err, err_len);
+    });
 }

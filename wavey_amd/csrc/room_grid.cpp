@@ -332,9 +332,9 @@ extern "C" double wv_grid_spacing(double c, double f_max, double ppw, const wv_s
     return std::fmin(c / (f_max * ppw), cab);
 }
 
-extern "C" wv_status wv_grid_build_room(const wv_room_mesh* room, const wv_solid* solids, size_t n_solids,
-                                        double c, double dx, int threads, wv_built_grid** out, char* err,
-                                        size_t err_len) {
+static wv_status grid_build_room_impl(const wv_room_mesh* room, const wv_solid* solids, size_t n_solids,
+                                      double c, double dx, int threads, wv_built_grid** out, char* err,
+                                      size_t err_len) {
     if (!room || !out || !(dx > 0.0) || !(c > 0.0)) {
         set_err(err, err_len, "wv_grid_build_room: invalid argument");
         return WV_ERR_INVALID;
@@ -559,4 +559,12 @@ extern "C" wv_status wv_grid_build_room(const wv_room_mesh* room, const wv_solid
     finish_view(*g, G.n, G.origin, dx, dt);
     *out = g.release();
     return WV_OK;
+}
+
+extern "C" wv_status wv_grid_build_room(const wv_room_mesh* room, const wv_solid* solids, size_t n_solids,
+                                        double c, double dx, int threads, wv_built_grid** out, char* err,
+                                        size_t err_len) {
+    return host_guarded(err, err_len, [&]() {
+        return grid_build_room_impl(room, solids, n_solids, c, dx, threads, out, err, err_len);
+    });
 }

@@ -78,10 +78,10 @@ void accumulate(const wv_wave_source& src, const Run& r, const std::vector<doubl
 
 }  // namespace
 
-extern "C" wv_status wv_wave_job(const wv_grid* grid, const wv_wave_source* sources, size_t n_sources,
-                                 const double* signal, size_t steps, const uint64_t* receivers, size_t n_recv,
-                                 const double* freqs, size_t nf, double c, int precision, int batched,
-                                 const volatile int32_t* cancel, double* transfer_out, char* err, size_t err_len) {
+static wv_status wave_job_impl(const wv_grid* grid, const wv_wave_source* sources, size_t n_sources,
+                               const double* signal, size_t steps, const uint64_t* receivers, size_t n_recv,
+                               const double* freqs, size_t nf, double c, int precision, int batched,
+                               const volatile int32_t* cancel, double* transfer_out, char* err, size_t err_len) {
     if (!grid || (n_sources && !sources) || !signal || !steps || (n_recv && !receivers) || (nf && !freqs) ||
         !transfer_out) {
         set_err(err, err_len, "wv_wave_job: invalid argument");
@@ -132,4 +132,14 @@ extern "C" wv_status wv_wave_job(const wv_grid* grid, const wv_wave_source* sour
                    transfer_out + 2 * per * runs[i].source);
     }
     return WV_OK;
+}
+
+extern "C" wv_status wv_wave_job(const wv_grid* grid, const wv_wave_source* sources, size_t n_sources,
+                                 const double* signal, size_t steps, const uint64_t* receivers, size_t n_recv,
+                                 const double* freqs, size_t nf, double c, int precision, int batched,
+                                 const volatile int32_t* cancel, double* transfer_out, char* err, size_t err_len) {
+    return host_guarded(err, err_len, [&]() {
+        return wave_job_impl(grid, sources, n_sources, signal, steps, receivers, n_recv, freqs, nf, c, precision,
+                             batched, cancel, transfer_out, err, err_len);
+    });
 }
