@@ -290,9 +290,9 @@ __device__ __forceinline__ S special_update(const StepParams<T, S>& a, uint32_t 
     // Every product and sum of the boundary solve is rounded on its own (no
     // fused multiply-adds): the compiler would otherwise fuse them
     // differently in the kernels that inline this (measured: FP32 series
-    // with spring branches differed in the last bits between a single-step
-    // and a fused multi-step kernel), and the pass modes, batch and server
-    // promise the same bits.
+    // with spring branches differed in the last bits between the single-step
+    // kernel and a multi-step kernel that inlined the solve in its z-march),
+    // and the pass modes, batch and server promise the same bits.
 #pragma clang fp contract(off)
     S next;
     if (meta & kMetaSolid) {
