@@ -21,6 +21,9 @@ receivers, against the FP64 CPU oracle.
 * The pillared room at its full configuration (302 x 227 x 102, 6000 steps,
   the cardioid's four component runs, 64 receivers): every component in FP64
   (1e-9) and FP32 (1e-4) against the oracle.
+* The largest grid the library takes (589.8 M cells, 597.7 M pitched): taps
+  and receivers at the far corner, default passes against single steps and
+  both precisions against the oracle.
 
 Reference: core/wavey-solver/tests/fdtd.rs:171-204 (the room with a pillar,
 GPU against CPU).
@@ -243,5 +246,43 @@ def test_pillared_room_full_config_against_oracle(pillared_full, comp):
         err = rel_l2(ir, ref)
         worst = max(per_receiver(ir, ref))
         print(f"pillared room component {comp} {prec}: vs oracle {err:.3e} (worst receiver {worst:.3e})",
+              flush=True)
+        assert err < tol and worst < tol
+
+
+def test_largest_grid_the_library_takes():
+    """The hall's box at dx 0.0295 m: 1358 x 850 x 511 = 589.8 M cells, just
+    under MAX_CELLS (600 M), 597.7 M pitched cells (32-bit cell offsets with
+    the least headroom any accepted grid leaves). Taps and receivers next to
+    the far corner (the last air cell, the largest offsets) and the near one;
+    a 20-step pulse over 60 steps. The default three-step passes equal single
+    steps bit for bit; FP32 within 1e-4 and FP64 within 1e-9 of the oracle."""
+    cfg = configs.hall(dx=0.0295, steps=60)
+    g = fdtd.FdtdGrid.build_box(cfg["room"], cfg["c"], cfg["dx"], 0)
+    nx, ny, nz = g.n
+    assert (nx, ny, nz) == (1358, 850, 511) and g.cells <= 600_000_000
+    ex, ey, ez = nx - 3, ny - 3, nz - 3  # the last air cell on each axis
+    assert not g.flags[g.index(ex + 1, ey + 1, ez + 1)] & fdtd.AIR
+    taps = [(air_cell(g, ex, ey, ez), 1.0), (air_cell(g, ex - 4, ey - 3, ez - 2), -0.6),
+            (air_cell(g, 3, 4, 5), 0.8)]
+    pts = [(ex, ey, ez), (ex - 1, ey, ez), (ex - 12, ey - 3, ez), (ex, ey - 15, ez - 2), (ex - 6, ey - 6, ez - 20),
+           (ex - 20, ey - 10, ez - 5), (1, 1, 1), (12, 4, 5), (3, 20, 9), (5, 5, 25)]
+    receivers = [air_cell(g, *p) for p in pts]
+    signal = np.zeros(60)
+    signal[:20] = np.sin(np.pi * np.arange(1, 21) / 21.0)
+    print(f"case: largest grid {g.n} ({g.cells} cells), 60 steps, FP64 oracle", flush=True)
+    ref = oracle.run(oracle_grid(g), taps, signal, receivers, threads=0)[0]
+    assert np.count_nonzero(ref) > 0.5 * ref.size
+    for prec, tol in (("fp32", 1e-4), ("fp64", 1e-9)):
+        with fdtd.Plan(g, prec) as plan:
+            print(f"case: largest grid, {prec}, pass mode {plan.pass_info()['steps_per_pass']}", flush=True)
+            ir, _ = plan.run(taps, signal, receivers, mode=fdtd.MODE_IR)
+            plan.set_pass(1)
+            print(f"case: largest grid, {prec}, single steps", flush=True)
+            one, _ = plan.run(taps, signal, receivers, mode=fdtd.MODE_IR)
+        assert np.array_equal(ir, one)
+        err = rel_l2(ir, ref)
+        worst = max(per_receiver(ir, ref))
+        print(f"largest grid {prec}: pass modes identical; vs oracle {err:.3e} (worst receiver {worst:.3e})",
               flush=True)
         assert err < tol and worst < tol
