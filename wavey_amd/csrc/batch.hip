@@ -420,10 +420,11 @@ void batch_init(wv_batch* b, int prec) {
     b->launch = tuned_launch(prec);
 }
 
-// Starts (or restarts from rest) an instance's run with its z-chunk.
+// Starts (or restarts from rest) an instance's run with its z-chunk (0: the
+// batch's default, not the plain run's by size).
 void instance_begin(Instance* in, uint32_t zchunk) {
     wv_plan* pl = in->plan.get();
-    pl->zchunk = zchunk;
+    pl->zchunk = zchunk ? zchunk : kDefaultZChunk;
     run_begin(pl, in->taps.data(), in->taps.size(), in->signal.data(), in->signal.size(), in->recv.data(),
               in->recv.size());
     in->n0 = 0;

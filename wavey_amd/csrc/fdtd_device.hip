@@ -757,7 +757,8 @@ void run_begin_impl(wv_plan* pl, const wv_tap* taps, size_t n_taps, const double
                        pl->stream, pl->flags.as<uint8_t>(), pl->force.as<uint32_t>(), pl->zbits.as<uint32_t>(),
                        pl->dbits.as<uint32_t>(), geo, pl->nzb);
     // Special cells in wave order.
-    pl->run_zchunk = pl->zchunk ? pl->zchunk : kDefaultZChunk;
+    pl->run_zchunk = pl->zchunk ? pl->zchunk : default_step_zchunk(pl->cells, pl->prec == WV_FP64);
+    pl->run_variant = pl->variant >= 0 ? pl->variant : default_step_variant(pl->cells, pl->prec == WV_FP64);
     const Layout L = layout_of(pl);
     const WaveOrder wo{L.bx, L.zchunk, L.seg, L.n_tiles};
     const uint64_t nblocks = L.positions / kBlock;
@@ -891,8 +892,8 @@ TriParams<T, S> make_tri_params(wv_plan* pl, const StepParams<T, S>& base) {
 // buf_q follow the field buffers. Returns the number of launches.
 template <class T, class S>
 uint64_t enqueue_range(wv_plan* pl, size_t n, size_t stop, hipStream_t stream) {
-    StepParams<T, S> a = make_params<T, S>(pl, kVariantTiles[pl->variant]);
-    const uint32_t grid = a.main_blocks + div_up(pl->n_recv, variant_threads(pl->variant));
+    StepParams<T, S> a = make_params<T, S>(pl, kVariantTiles[pl->run_variant]);
+    const uint32_t grid = a.main_blocks + div_up(pl->n_recv, variant_threads(pl->run_variant));
     const bool triples = pl->run_steps_per_pass == 3;
     const bool pairs = pl->run_steps_per_pass >= 2;
     PairParams<T, S> pp{};
@@ -992,7 +993,7 @@ uint64_t enqueue_range(wv_plan* pl, size_t n, size_t stop, hipStream_t stream) {
             a.prev = Q;
             a.s_n = sig(n);
             a.ir = ir + n * pl->n_recv;
-            if (grid) launch_step<T, S>(pl->variant, grid, stream, a);
+            if (grid) launch_step<T, S>(pl->run_variant, grid, stream, a);
             std::swap(pl->buf_p, pl->buf_q);
             ++launches;
             ++n;
@@ -1213,7 +1214,7 @@ extern "C" wv_status wv_plan_set_tile(wv_plan* plan, uint32_t zchunk, int32_t va
     // z-chunks must divide the 32-plane blocks of the interior bitmap.
     if (!plan || zchunk > 32 || (zchunk & (zchunk - 1)) || variant >= kVariants) return WV_ERR_INVALID;
     plan->zchunk = zchunk;
-    plan->variant = variant < 0 ? kDefaultVariant : variant;
+    plan->variant = variant < 0 ? -1 : variant;
     return WV_OK;
 }
 

@@ -90,7 +90,8 @@ struct wv_plan {
     uint32_t zchunk = 0;      // requested (0 = default)
     uint32_t run_zchunk = 0;  // the current run's layout
     uint32_t nzb = 0;         // 32-plane blocks of the z-packed bitmaps
-    int variant = wv::kDefaultVariant;
+    int variant = -1;                          // requested (-1 = default by size)
+    int run_variant = wv::kDefaultVariant;     // the current run's
     uint32_t max_slots = 0, n_sets = 0;
 
     DBuf flags, force_base, force, zbits, dbits, counts, offs;

@@ -561,6 +561,27 @@ inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) /
 constexpr int kVariants = 4;
 constexpr int kDefaultVariant = 0;
 constexpr uint32_t kDefaultZChunk = 4;
+// A plain run's default shape by grid size. A small grid runs few blocks, and
+// the special cells of a block with a floor or wall in it are the longest
+// path of its launch, so small grids take the smallest blocks (one tile and
+// the special-cell wave) and z-chunks. Per step, FP32: 36 K cells 7.8 to 4.2
+// us with 1-plane chunks; 0.4, 0.8 and 1.5 M cells 11.1, 11.7 and 13.9 to
+// 7.6, 9.8 and 11.2 us with 2-plane chunks; from 1.9 M cells within 3 % of
+// the 7-tile blocks with 4-plane chunks. FP64 (tiles half as wide): 36 K
+// cells 6.4 to 4.2 us, but 0.4-1.9 M cells 0.70-1.10 times as fast, so only
+// the smallest grids change. Batches keep kDefaultVariant and kDefaultZChunk
+// unless tuned: with the whole GPU busy the larger blocks win there.
+constexpr uint64_t kSmallStepCells = 2ull << 20;
+constexpr uint64_t kTinyStepCells = 128ull << 10;
+inline bool small_step_grid(uint64_t cells, bool fp64) {
+    return cells < (fp64 ? kTinyStepCells : kSmallStepCells);
+}
+inline int default_step_variant(uint64_t cells, bool fp64) {
+    return small_step_grid(cells, fp64) ? 3 : kDefaultVariant;
+}
+inline uint32_t default_step_zchunk(uint64_t cells, bool fp64) {
+    return cells < kTinyStepCells ? 1u : small_step_grid(cells, fp64) ? 2u : kDefaultZChunk;
+}
 // Default steps per pass and pass z-chunk by grid size (measured on the
 // hall and on hall-shaped grids of 58, 29, 15 and 7.6 Mi cells, and smaller
 // rooms): small grids are launch- and latency-bound, where one step per
