@@ -11,6 +11,16 @@
  * nodes are updated in parallel (fdtd.rs:710-713), the scatter, injection
  * and recording are serial. Every cell and node update is independent, so
  * results do not depend on the thread count.
+ *
+ * Pinning: every function is checked by tests/ against what fixes it without
+ * the reference (closed forms, brute force, the reference's own tests and
+ * data). Three parts are partly "parity unpinned", as only running the
+ * reference could pin them exactly: the triangle a face ray takes where two
+ * are hit at exactly the same distance (ob_build_room takes the lower index;
+ * the reference's BVH takes its tree order), the exact Nelder-Mead
+ * trajectory of ob_fit_admittance (the last digits of a fit), and the
+ * composed outputs of ob_calibrate_cabinet (pinned through its pieces and
+ * its exact limits).
  */
 #include "fdtd_oracle.h"
 

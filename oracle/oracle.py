@@ -9,6 +9,11 @@ Room specs are plain data, the same dicts the product's grid builder takes:
 
 with one ``patches`` entry per surface: walls x=0, x=L0, y=0, y=L1, floor,
 ceiling, then the four sides (facing -x, +x, -y, +y) of each pillar.
+
+Pinning (the C file's header has the detail): every function is checked by
+the tests; three parts are partly "parity unpinned": the exact-tie triangle
+choice of the general-room builder, the exact Nelder-Mead trajectory of the
+admittance fit and the composed outputs of the cabinet calibration.
 """
 from __future__ import annotations
 
