@@ -165,7 +165,7 @@ wv_status wv_plan_stats(const wv_plan* plan, uint64_t out[7]);
 /* Single-step kernel shape: z planes per block (0 = default, else a power of
  * two up to 32) and the block variant (-1 = default; 0..3 = blocks of 7, 3,
  * 15 and 1 plane tiles of 256 FP32 or 128 FP64 cells, plus one special-cell
- * wave). Defaults by grid size: below 128 Ki cells variant 3 with 1-plane
+ * wave, two with one tile). Defaults by grid size: below 128 Ki cells variant 3 with 1-plane
  * chunks, FP32 grids below 2 Mi cells variant 3 with 2-plane chunks, else
  * variant 0 with 4-plane chunks. Takes effect at the next wv_run_begin;
  * results do not depend on the shape. No reference counterpart: the

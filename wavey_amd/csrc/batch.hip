@@ -91,7 +91,7 @@ __device__ __forceinline__ void batch_item(const BatchEntry<T, S>& be, uint32_t 
 // b - block0 of instance block_entry[b] at step n0 + k. Instances past their
 // last step return at once.
 template <typename T, typename S, int TY>
-__global__ __launch_bounds__(kTileX*(TY + 1)) void k_batch_step(const BatchEntry<T, S>* __restrict__ entries,
+__global__ __launch_bounds__(kTileX*(TY + special_waves(TY))) void k_batch_step(const BatchEntry<T, S>* __restrict__ entries,
                                                            const uint32_t* __restrict__ block_entry, uint32_t k) {
     const BatchEntry<T, S>& be = entries[block_entry[blockIdx.x]];
     const uint32_t n = be.n0 + k;
@@ -145,7 +145,7 @@ __device__ __forceinline__ uint32_t xcc_id() {
 }
 
 template <typename T, typename S, int TY>
-__global__ __launch_bounds__(kTileX*(TY + 1)) void k_batch_persist(const BatchEntry<T, S>* __restrict__ entries,
+__global__ __launch_bounds__(kTileX*(TY + special_waves(TY))) void k_batch_persist(const BatchEntry<T, S>* __restrict__ entries,
                                                               const uint32_t* __restrict__ pblock,
                                                               uint32_t* __restrict__ ctl, XcdShare xs,
                                                               uint32_t kbase, uint32_t K) {

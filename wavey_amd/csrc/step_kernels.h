@@ -516,11 +516,18 @@ __device__ __forceinline__ void special_share(const StepParams<T, S>& a, const u
     }
 }
 
-// One block: TY tiles of the plane over one z-chunk, a wave each, and one
-// more wave that starts on the block's special cells at once; the tile
-// waves join it when their z-march ends, so special cells neither wait for
-// the z-march nor hold it up, and blocks with many of them (floors, walls)
-// share them among all their waves.
+// Waves of a block that start on its special cells at once: one, and two in
+// blocks of a single tile, the small grids' blocks, where the special cells
+// of a floor or wall are the longest path of a launch (0.8-1.9 M cells 3-7 %
+// faster per step; with three or seven tiles a block gains nothing from a
+// second one).
+constexpr int special_waves(int ty) { return ty == 1 ? 2 : 1; }
+
+// One block: TY tiles of the plane over one z-chunk, a wave each, and the
+// special-cell waves, which start on the block's special cells at once; the
+// tile waves join them when their z-march ends, so special cells neither
+// wait for the z-march nor hold it up, and blocks with many of them (floors,
+// walls) share them among all their waves.
 template <typename T, typename S, int TY>
 __device__ __forceinline__ void block_part(const StepParams<T, S>& a, uint32_t t) {
     __shared__ uint32_t handed;  // special records handed out so far
@@ -543,7 +550,7 @@ __device__ __forceinline__ void block_part(const StepParams<T, S>& a, uint32_t t
 // fdtd.rs:673-683). Consecutive blocks go to the 8 XCDs round-robin, so
 // neighbouring tiles run at the same time and spread over all HBM stacks.
 template <typename T, typename S, int TY>
-__global__ __launch_bounds__(kTileX*(TY + 1)) void k_step(StepParams<T, S> a) {
+__global__ __launch_bounds__(kTileX*(TY + special_waves(TY))) void k_step(StepParams<T, S> a) {
     if (blockIdx.x < a.main_blocks) {
         block_part<T, S, TY>(a, blockIdx.x);
         return;
@@ -554,10 +561,10 @@ __global__ __launch_bounds__(kTileX*(TY + 1)) void k_step(StepParams<T, S> a) {
 
 inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
-// Step-kernel variants: TY tiles per block plus the special-cell wave, so
-// blocks of 8, 4, 16 and 2 waves, which divide the 32 wave slots of a CU at
-// the kernel's register use (at most 64 VGPRs: 8 waves per SIMD). The
-// default variant and z-chunk.
+// Step-kernel variants: TY tiles per block plus the special-cell waves, so
+// blocks of 8, 4, 16 and 3 waves (the first three divide the 32 wave slots
+// of a CU at the kernel's register use, at most 64 VGPRs: 8 waves per SIMD;
+// ten blocks of three fill 30). The default variant and z-chunk.
 constexpr int kVariants = 4;
 constexpr int kDefaultVariant = 0;
 constexpr uint32_t kDefaultZChunk = 4;
@@ -599,7 +606,9 @@ inline uint32_t default_steps_per_pass(uint64_t cells, bool fp64) {
 inline uint32_t default_pair_zchunk(uint64_t cells) { return cells >= kPairLongChunkCells ? 32u : 16u; }
 constexpr uint32_t kVariantTiles[] = {7, 3, 15, 1};
 
-inline uint32_t variant_threads(int variant) { return kTileX * (kVariantTiles[variant] + 1); }
+inline uint32_t variant_threads(int variant) {
+    return kTileX * (kVariantTiles[variant] + special_waves((int)kVariantTiles[variant]));
+}
 
 template <class T, class S>
 void launch_step(int variant, uint32_t grid, hipStream_t st, const StepParams<T, S>& a) {
