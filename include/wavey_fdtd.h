@@ -165,11 +165,12 @@ wv_status wv_plan_stats(const wv_plan* plan, uint64_t out[7]);
 /* Single-step kernel shape: z planes per block (0 = default, else a power of
  * two up to 32) and the block variant (-1 = default; 0..3 = blocks of 7, 3,
  * 15 and 1 plane tiles of 256 FP32 or 128 FP64 cells, plus one special-cell
- * wave, two with one tile). Defaults by grid size: below 128 Ki cells variant 3 with 1-plane
- * chunks, FP32 grids below 2 Mi cells variant 3 with 2-plane chunks, else
- * variant 0 with 4-plane chunks. Takes effect at the next wv_run_begin;
- * results do not depend on the shape. No reference counterpart: the
- * autotuner's knobs. */
+ * wave, four with one tile). Defaults by grid size: FP32 variant 3 with
+ * 1-plane chunks below 128 Ki cells, 2-plane below 1 Mi and 4-plane below
+ * 2.5 Mi; FP64 variant 3 with 2-plane chunks below 128 Ki cells and 4-plane
+ * below 1 Mi; else variant 0 with 4-plane chunks. Takes effect at the next
+ * wv_run_begin; results do not depend on the shape. No reference
+ * counterpart: the autotuner's knobs. */
 wv_status wv_plan_set_tile(wv_plan* plan, uint32_t zchunk, int32_t variant);
 /* Steps per pass over the fields: 1; 2 (two steps per z-march: deep
  * interior cells advance twice in one pass, the cells next to boundaries in

@@ -516,12 +516,15 @@ __device__ __forceinline__ void special_share(const StepParams<T, S>& a, const u
     }
 }
 
-// Waves of a block that start on its special cells at once: one, and two in
+// Waves of a block that start on its special cells at once: one, and four in
 // blocks of a single tile, the small grids' blocks, where the special cells
-// of a floor or wall are the longest path of a launch (0.8-1.9 M cells 3-7 %
-// faster per step; with three or seven tiles a block gains nothing from a
-// second one).
-constexpr int special_waves(int ty) { return ty == 1 ? 2 : 1; }
+// of a floor or wall are the longest path of a launch. Per step against one
+// special wave (FP32, 2-plane chunks): 36 K cells 4.2 to 3.3 us, 0.38 M 7.6
+// to 6.0, 0.83 M 9.5 to 8.2-8.8, 1.53 M 11.4 to 10.6-10.9, 1.93 M 13.8 to
+// 13.0-13.7 (two waves: 12.6-12.9 there but 4.2, 7.7, 9.2 and 11.1 below;
+// three, five and seven: slower than four below 1.5 M cells). With three or
+// seven tiles a block gains nothing from a second special wave.
+constexpr int special_waves(int ty) { return ty == 1 ? 4 : 1; }
 
 // One block: TY tiles of the plane over one z-chunk, a wave each, and the
 // special-cell waves, which start on the block's special cells at once; the
@@ -562,32 +565,34 @@ __global__ __launch_bounds__(kTileX*(TY + special_waves(TY))) void k_step(StepPa
 inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
 // Step-kernel variants: TY tiles per block plus the special-cell waves, so
-// blocks of 8, 4, 16 and 3 waves (the first three divide the 32 wave slots
+// blocks of 8, 4, 16 and 5 waves (the first three divide the 32 wave slots
 // of a CU at the kernel's register use, at most 64 VGPRs: 8 waves per SIMD;
-// ten blocks of three fill 30). The default variant and z-chunk.
+// six blocks of five fill 30). The default variant and z-chunk.
 constexpr int kVariants = 4;
 constexpr int kDefaultVariant = 0;
 constexpr uint32_t kDefaultZChunk = 4;
 // A plain run's default shape by grid size. A small grid runs few blocks, and
 // the special cells of a block with a floor or wall in it are the longest
-// path of its launch, so small grids take the smallest blocks (one tile and
-// the special-cell wave) and z-chunks. Per step, FP32: 36 K cells 7.8 to 4.2
-// us with 1-plane chunks; 0.4, 0.8 and 1.5 M cells 11.1, 11.7 and 13.9 to
-// 7.6, 9.8 and 11.2 us with 2-plane chunks; from 1.9 M cells within 3 % of
-// the 7-tile blocks with 4-plane chunks. FP64 (tiles half as wide): 36 K
-// cells 6.4 to 4.2 us, but 0.4-1.9 M cells 0.70-1.10 times as fast, so only
-// the smallest grids change. Batches keep kDefaultVariant and kDefaultZChunk
-// unless tuned: with the whole GPU busy the larger blocks win there.
-constexpr uint64_t kSmallStepCells = 2ull << 20;
+// path of its launch, so small grids take blocks of one tile (and four
+// special-cell waves, special_waves) and short z-chunks. Measured per step
+// against 7-tile blocks with 4-plane chunks, FP32: 36 K cells 7.7 to 3.3 us
+// (1-plane chunks, under 128 Ki cells); 0.38 and 0.83 M cells 11.0 and 11.9
+// to 6.0 and 8.0 us (2-plane chunks, under 1 Mi); 1.2-2.5 M cells, 24 rooms,
+// 1.01-1.59 times as fast (4-plane chunks, under 2.5 Mi); from 2.5 M cells
+// mixed (0.89-1.19), above 4 M slower. FP64 (tiles half as wide): 36 K cells
+// 6.4 to 3.4 us (2-plane chunks), 0.38 and 0.83 M 10.1 and 14.4 to 9.9 and
+// 12.5 us (4-plane chunks), slower from 1.2 M cells. Batches keep
+// kDefaultVariant and kDefaultZChunk unless tuned: with the whole GPU busy
+// the larger blocks win there.
 constexpr uint64_t kTinyStepCells = 128ull << 10;
-inline bool small_step_grid(uint64_t cells, bool fp64) {
-    return cells < (fp64 ? kTinyStepCells : kSmallStepCells);
-}
+constexpr uint64_t kSmallStepCells = 1ull << 20;
+constexpr uint64_t kMidStepCells = 5ull << 19;  // 2.5 Mi
 inline int default_step_variant(uint64_t cells, bool fp64) {
-    return small_step_grid(cells, fp64) ? 3 : kDefaultVariant;
+    return cells < (fp64 ? kSmallStepCells : kMidStepCells) ? 3 : kDefaultVariant;
 }
 inline uint32_t default_step_zchunk(uint64_t cells, bool fp64) {
-    return cells < kTinyStepCells ? 1u : small_step_grid(cells, fp64) ? 2u : kDefaultZChunk;
+    if (fp64) return cells < kTinyStepCells ? 2u : kDefaultZChunk;
+    return cells < kTinyStepCells ? 1u : cells < kSmallStepCells ? 2u : kDefaultZChunk;
 }
 // Default steps per pass and pass z-chunk by grid size (measured on the
 // hall and on hall-shaped grids of 58, 29, 15 and 7.6 Mi cells, and smaller
