@@ -367,6 +367,11 @@ struct wv_batch {
     hipStream_t stream = nullptr;
     hipEvent_t t0 = nullptr, t1 = nullptr;
     int variant = kDefaultVariant;
+    // The current table's variant: variant, or blocks of one tile for a
+    // small table (small_table) unless the variant was set or is being tuned.
+    int table_variant = kDefaultVariant;
+    bool variant_set = false;
+    bool tuning = false;
     int launch = WV_BATCH_STEPS;  // the launch form of a block
     std::vector<std::unique_ptr<Instance>> inst;
     DBuf entries, block_entry;
@@ -476,9 +481,25 @@ std::unique_ptr<Instance> make_instance(int prec, const wv_grid* g, const wv_tap
 // even step (a multiple of 128), where p is in the plan's buffer buf_p and
 // prev in buf_q: the batched step updates prev in place, so after a block p
 // is back in buf_p.
+// A table of few air cells launches blocks of one tile, as a plain run of a
+// grid that size does (default_step_variant): the server's small-request
+// lane often holds one or two small rooms, where the larger blocks leave the
+// device waiting on their special cells (measured: one 0.38 M-cell room 6.9
+// against 9.3-11.0 us per step).
+bool small_table(const wv_batch* b) {
+    if (b->variant_set || b->tuning || b->launch != WV_BATCH_STEPS) return false;
+    uint64_t air = 0;
+    for (auto& up : b->inst) {
+        const Instance* in = up.get();
+        if (!in->done && !in->paused && in->n0 < in->steps) air += in->plan->air;
+    }
+    return default_step_variant(air, b->prec == WV_FP64) == 3;
+}
+
 template <typename T, typename S>
 void build_table(wv_batch* b) {
-    const uint32_t ty = kVariantTiles[b->variant];
+    b->table_variant = small_table(b) ? 3 : b->variant;
+    const uint32_t ty = kVariantTiles[b->table_variant];
     std::vector<BatchEntry<T, S>> es;
     std::vector<uint32_t> be;
     b->table.clear();
@@ -497,7 +518,7 @@ void build_table(wv_batch* b) {
         e.steps = in->steps;
         e.n0 = in->n0;
         e.block0 = (uint32_t)be.size();
-        const uint32_t blocks = e.a.main_blocks + div_up(pl->n_recv, variant_threads(b->variant));
+        const uint32_t blocks = e.a.main_blocks + div_up(pl->n_recv, variant_threads(b->table_variant));
         if ((uint64_t)be.size() + blocks >= (1ull << 31)) throw DevErr{"batch launch too large", WV_ERR_LIMIT};
         e.blocks = blocks;
         be.insert(be.end(), blocks, (uint32_t)es.size());
@@ -563,7 +584,7 @@ void enqueue_persist(wv_batch* b, uint32_t grid, uint32_t kbase, uint32_t K) {
     const uint32_t n = (uint32_t)b->table.size();
     hipLaunchKernelGGL(k_persist_reset, dim3(div_up(std::max(n, kXcds), 256)), dim3(256), 0, b->stream,
                        b->ctl.as<uint32_t>(), n);
-    launch_batch_persist<T, S>(b->variant, grid, b->stream, b->entries.as<BatchEntry<T, S>>(), b->pblock.as<uint32_t>(),
+    launch_batch_persist<T, S>(b->table_variant, grid, b->stream, b->entries.as<BatchEntry<T, S>>(), b->pblock.as<uint32_t>(),
                                b->ctl.as<uint32_t>(), b->share, kbase, K);
 }
 
@@ -592,12 +613,12 @@ void enqueue_block(wv_batch* b, bool use_graph) {
     auto* e = b->entries.as<BatchEntry<T, S>>();
     const uint32_t* be = b->block_entry.as<uint32_t>();
     const bool persist = persist_usable(b, kGraphSteps);
-    const uint32_t pgrid = persist ? persist_grid<T, S>(b->variant) : 0;
+    const uint32_t pgrid = persist ? persist_grid<T, S>(b->table_variant) : 0;
     auto issue = [&]() {
         if (persist) enqueue_persist<T, S>(b, pgrid, 0, kGraphSteps);
         else
             for (uint32_t k = 0; k < kGraphSteps; ++k)
-                launch_batch_step<T, S>(b->variant, b->total_blocks, b->stream, e, be, k);
+                launch_batch_step<T, S>(b->table_variant, b->total_blocks, b->stream, e, be, k);
         hipLaunchKernelGGL((k_batch_tick<T, S>), dim3(div_up(n, 256)), dim3(256), 0, b->stream, e, n, kGraphSteps);
     };
     if (use_graph) {
@@ -704,14 +725,14 @@ double time_table(wv_batch* b, uint32_t warmup, uint32_t steps) {
     auto* e = b->entries.as<BatchEntry<T, S>>();
     const uint32_t* be = b->block_entry.as<uint32_t>();
     const bool persist = persist_usable(b, warmup + steps);
-    const uint32_t pgrid = persist ? persist_grid<T, S>(b->variant) : 0;
+    const uint32_t pgrid = persist ? persist_grid<T, S>(b->table_variant) : 0;
     if (persist && warmup) enqueue_persist<T, S>(b, pgrid, 0, warmup);
     for (uint32_t k = 0; !persist && k < warmup; ++k)
-        launch_batch_step<T, S>(b->variant, b->total_blocks, b->stream, e, be, k);
+        launch_batch_step<T, S>(b->table_variant, b->total_blocks, b->stream, e, be, k);
     WV_HIP(hipEventRecord(b->t0, b->stream));
     if (persist) enqueue_persist<T, S>(b, pgrid, warmup, steps);
     for (uint32_t k = 0; !persist && k < steps; ++k)
-        launch_batch_step<T, S>(b->variant, b->total_blocks, b->stream, e, be, warmup + k);
+        launch_batch_step<T, S>(b->table_variant, b->total_blocks, b->stream, e, be, warmup + k);
     WV_HIP(hipGetLastError());
     WV_HIP(hipEventRecord(b->t1, b->stream));
     WV_HIP(hipEventSynchronize(b->t1));
@@ -727,6 +748,11 @@ double time_table(wv_batch* b, uint32_t warmup, uint32_t steps) {
 template <typename T, typename S>
 void batch_autotune(wv_batch* b, uint32_t trial_steps) {
     WV_HIP(hipStreamSynchronize(b->stream));
+    struct Tuning {
+        wv_batch* b;
+        explicit Tuning(wv_batch* bb) : b(bb) { b->tuning = true; }
+        ~Tuning() { b->tuning = false; }
+    } tuning(b);
     for (auto& in : b->inst) instance_rewind(in.get());
     int best_v = b->variant;
     double best = 1e300;
@@ -869,6 +895,7 @@ extern "C" wv_status wv_batch_set_tile(wv_batch* b, int32_t variant, uint32_t zc
         WV_HIP(hipSetDevice(b->device));
         WV_HIP(hipStreamSynchronize(b->stream));
         b->variant = variant < 0 ? kDefaultVariant : variant;
+        b->variant_set = variant >= 0;
         for (auto& in : b->inst) instance_begin(in.get(), zchunk);
         b->dirty = true;
         return WV_OK;
@@ -887,6 +914,7 @@ extern "C" wv_status wv_batch_set_launch(wv_batch* b, int form, char* err, size_
         WV_HIP(hipStreamSynchronize(b->stream));
         b->launch = form;
         b->drop_graph();
+        b->dirty = true;  // the table's variant depends on the launch form
         return WV_OK;
     });
 }
