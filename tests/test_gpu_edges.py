@@ -10,7 +10,8 @@
 * air up to the grid's x ends (a caller's grid without the builder's padding
   columns), so that cells at x = 1 and x = nx - 2 are interior: every pass
   mode bit-identical and FP64 against the oracle;
-* an empty signal, and bad run arguments, through every entry point.
+* an empty signal, and bad run arguments, through every entry point;
+* every single-step block shape and z-chunk bit-identical to the default.
 """
 import ctypes as C
 import dataclasses
@@ -130,6 +131,33 @@ def test_run_argument_errors(shoebox):
                                 rc.ctypes.data_as(C.POINTER(C.c_uint64)), len(rc), None, 0, fdtd.MODE_IR, 0,
                                 None, fdtd.PROGRESS_FN(), None, None, None, err, 256)
     assert st == fdtd.WV_ERR_LIMIT and b"too large" in err.value
+
+
+def test_every_single_step_shape_bit_identical(shoebox):
+    """The single-step kernel's block shapes (7, 3, 15 and 1 tiles per block,
+    one or four special-cell waves) and z-chunks (1-32 planes) split the
+    same cell updates differently over blocks and waves: every shape gives
+    the default shape's series bit for bit, FP32 and FP64, on the shoebox
+    (the small-grid default: one-tile blocks) and on a room of 0.4 M cells.
+    The default is checked against the oracle in the other tests."""
+    case, _ = shoebox
+    rng = np.random.default_rng(3)
+    cfg = configs.mix_room(rng, configs.material_library(), dx=0.05, dims=(3.4, 3.8, 3.3))
+    cfg["steps"] = 160
+    room = workload.build_case(cfg)
+    for name, c, steps in (("shoebox", case, 200), ("room", room, 160)):
+        taps, sig, rx = c.runs[0][2], c.signal[:steps], c.receivers
+        for prec in ("fp32", "fp64"):
+            with fdtd.Plan(c.grid, prec) as plan:
+                plan.set_pass(1)
+                ref, _ = plan.run(taps, sig, rx, mode=fdtd.MODE_IR)
+                assert np.count_nonzero(ref) > 0.3 * ref.size
+                for variant in range(4):
+                    for zc in (1, 2, 4, 8, 16, 32):
+                        print(f"case: {name} {c.grid.n}, {prec}, variant {variant}, z-chunk {zc}", flush=True)
+                        plan.set_tile(zc, variant)
+                        ir, _ = plan.run(taps, sig, rx, mode=fdtd.MODE_IR)
+                        assert np.array_equal(ir, ref), (name, prec, variant, zc)
 
 
 @pytest.mark.parametrize("steps", [1, 2, 3, 130])
