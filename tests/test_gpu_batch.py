@@ -171,6 +171,36 @@ def test_server_results_same_alone_and_in_mix(rooms):
     assert 0 < st["p50_s"] <= st["p95_s"] <= st["p99_s"]
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32_state64"])
+def test_server_and_batch_in_other_precisions(rooms, prec):
+    """The server (lanes policy) and the batch in FP64 and in FP32 with FP64
+    boundary state: every result equals the plain run of the same precision
+    bit for bit (the server mixes the requests as they arrive; the batch's
+    small table takes one-tile blocks)."""
+    cases, plain = rooms
+    ref = []
+    for i, c in enumerate(cases):
+        if (prec, i) in plain:
+            ref.append(plain[(prec, i)])
+        else:
+            with fdtd.Plan(c.grid, prec) as p:
+                ref.append(p.run(c.runs[0][2], c.signal, c.receivers, mode=fdtd.MODE_IR)[0])
+    print(f"case: server, {len(cases)} requests, {prec}", flush=True)
+    with fdtd.Server(prec) as sv:
+        ids = [sv.submit(c.grid, c.runs[0][2], c.signal, c.receivers) for c in cases]
+        out = [sv.wait(r) for r in ids]
+        st = sv.stats()
+    for i, (ir, _, _) in enumerate(out):
+        assert np.array_equal(ir, ref[i]), f"request {i}"
+    assert st["completed"] == len(cases) and st["failed"] == 0
+    print(f"case: batch, {len(cases)} rooms, {prec}", flush=True)
+    with fdtd.Batch(prec) as b:
+        bids = [b.add(c.grid, c.runs[0][2], c.signal, c.receivers) for c in cases]
+        assert b.run() == fdtd.WV_OK
+        for i, iid in enumerate(bids):
+            assert np.array_equal(b.result(iid)[0], ref[i]), f"instance {i}"
+
+
 def test_server_admission_cap_serializes(rooms):
     """A cap that holds one request at a time: requests run one after another
     and still give the same results."""
