@@ -201,6 +201,32 @@ def test_server_and_batch_in_other_precisions(rooms, prec):
             assert np.array_equal(b.result(iid)[0], ref[i]), f"instance {i}"
 
 
+def test_batch_cancel_and_resume(rooms):
+    """A batch cancelled from its progress callback returns WV_CANCELLED
+    between step blocks, as the reference's run does (gpu_fdtd.rs:202-205);
+    run again, it resumes where it stopped, its progress rises to 1, and
+    every instance still equals its plain run."""
+    cases, plain = rooms
+    flag = np.zeros(1, dtype=np.int32)
+    seen = []
+
+    def stop_at_first(f):
+        seen.append(f)
+        flag[0] = 1
+
+    print(f"case: batch of {len(cases)} rooms cancelled at its first progress report, then resumed", flush=True)
+    with fdtd.Batch("fp32") as b:
+        ids = [b.add(c.grid, c.runs[0][2], c.signal, c.receivers) for c in cases]
+        assert b.run(cancel=flag, progress=stop_at_first) == fdtd.WV_CANCELLED
+        assert len(seen) == 1 and 0.0 < seen[0] < 1.0
+        flag[0] = 0
+        again = []
+        assert b.run(cancel=flag, progress=again.append) == fdtd.WV_OK
+        assert again and again[-1] == 1.0 and all(x <= y for x, y in zip(again, again[1:]))
+        for i, iid in enumerate(ids):
+            assert np.array_equal(b.result(iid)[0], plain[("fp32", i)]), f"instance {i}"
+
+
 def test_server_admission_cap_serializes(rooms):
     """A cap that holds one request at a time: requests run one after another
     and still give the same results."""
