@@ -768,8 +768,17 @@ void batch_autotune(wv_batch* b, uint32_t trial_steps) {
     std::map<int, uint64_t, std::greater<int>> classes;
     for (auto& in : b->inst) classes[in->size_class] += in->plan->air;
     std::map<int, uint32_t> chosen;
+    uint64_t total_air = 0;
+    for (auto& [cls, air] : classes) total_air += air;
     for (auto& [cls, air] : classes) {
-        (void)air;
+        // A class with under 5 % of the air cells keeps its z-chunk: its
+        // trials (each sets its instances up again) cost more than its
+        // choice can win (room mix: autotune 0.19 to 0.16 s, run unchanged).
+        if (air * 20 < total_air) {
+            const uint32_t zc = tuned_zchunk(b->prec, cls);
+            chosen[cls] = zc ? zc : kDefaultZChunk;
+            continue;
+        }
         uint32_t best_z = 0;
         double best_t = 1e300;
         for (uint32_t zc : kTuneZChunks) {
