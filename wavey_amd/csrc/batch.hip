@@ -476,11 +476,6 @@ std::unique_ptr<Instance> make_instance(int prec, const wv_grid* g, const wv_tap
     return in;
 }
 
-// Rebuilds the launch table from the instances that still have steps to
-// run and are not paused. The stream must be idle. Every block starts at an
-// even step (a multiple of 128), where p is in the plan's buffer buf_p and
-// prev in buf_q: the batched step updates prev in place, so after a block p
-// is back in buf_p.
 // A table of few air cells launches blocks of one tile, as a plain run of a
 // grid that size does (default_step_variant): the server's small-request
 // lane often holds one or two small rooms, where the larger blocks leave the
@@ -496,6 +491,11 @@ bool small_table(const wv_batch* b) {
     return default_step_variant(air, b->prec == WV_FP64) == 3;
 }
 
+// Rebuilds the launch table from the instances that still have steps to
+// run and are not paused. The stream must be idle. Every block starts at an
+// even step (a multiple of 128), where p is in the plan's buffer buf_p and
+// prev in buf_q: the batched step updates prev in place, so after a block p
+// is back in buf_p.
 template <typename T, typename S>
 void build_table(wv_batch* b) {
     b->table_variant = small_table(b) ? 3 : b->variant;
