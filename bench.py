@@ -5,8 +5,8 @@ A step is one time step of the FDTD run over the whole grid. FP32 grids of
 k_tri1 (cells three or more from a wall three times in one z-march, special
 cells' first step, receivers) and two launches of k_pair2 (special cells and the cells near the
 walls, second and third steps, receivers), so per-step figures are a third
-of a pass; grids of 4 Mi cells and more take two steps per pass (k_pair1 and
-k_pair2), smaller grids one launch of k_step per step.
+of a pass; grids of 2.25 Mi cells and more (FP64: 0.75 Mi) take two steps per
+pass (k_pair1 and k_pair2), smaller grids one launch of k_step per step.
 Workload (default): the 40 x 25 x 15 m hall at dx = 0.04 m (1002 x 627 x 377
 cells, 232.5M air cells, 8 pillars, 10 three-section admittances on surface
 patches, 256 receivers), FP32 fields, one cardioid source component. Inputs

@@ -186,11 +186,11 @@ wv_status wv_plan_set_pass(wv_plan* plan, uint32_t steps_per_pass, uint32_t pair
  * [steps per pass, pass z-chunk, ring cells (interior, next to a
  * non-interior cell) and cells within two of a non-interior cell (three
  * steps per pass only) of the current run]. By default FP32 grids of 8 Mi
- * cells and more (FP64: 7 Mi) take three steps per pass, from 4 Mi cells
- * two, smaller ones one. Default z-chunk: FP32 three-step passes the
- * longest of 64, 32, 16 that gives 640 tile blocks; FP64 three-step and
- * every two-step pass 16 below 64 Mi cells and 32 from 64 Mi. No reference
- * counterpart. */
+ * cells and more (FP64: 7 Mi) take three steps per pass, from 2.25 Mi cells
+ * (FP64: 0.75 Mi) two, smaller ones one. Default z-chunk: FP32 three-step
+ * passes the longest of 64, 32, 16 that gives 640 tile blocks; FP64
+ * three-step and every two-step pass 32 from 64 Mi cells; else 16, or 8
+ * where 16 give fewer than 150 tile blocks. No reference counterpart. */
 wv_status wv_plan_pass_info(const wv_plan* plan, uint32_t out[4]);
 /* Copies `bytes` (at most one field buffer) between the plan's field
  * buffers `reps` times with a plain streaming kernel and returns the mean
@@ -242,8 +242,8 @@ enum { WV_BATCH_STEPS = 0, WV_BATCH_PERSISTENT = 1 };
 wv_status wv_batch_set_launch(wv_batch* batch, int form, char* err, size_t err_len);
 /* Runs every instance to its last step. use_graphs = 0 issues the same
  * launches one by one. Cancel is polled between step blocks. An instance
- * left alone on a grid of at least 4 Mi cells runs its remaining steps with
- * the plain run's multi-step kernels (two or three steps per pass, as its
+ * left alone on a grid of at least 2.25 Mi cells (FP64: 0.75 Mi) runs its
+ * remaining steps with the plain run's multi-step kernels (two or three steps per pass, as its
  * plain run; same results). */
 wv_status wv_batch_run(wv_batch* batch, int use_graphs, const volatile int32_t* cancel,
                        wv_progress_fn progress, void* ctx, char* err, size_t err_len);
@@ -291,8 +291,8 @@ wv_status wv_serve_wait(wv_server* server, uint64_t id, double* spectra_out, dou
  * least lane_cells air cells) run one at a time, in admission order and each
  * to its end, on a second stream beside the batch, which the device runs at
  * the same time.
- * A request that runs alone on a grid of at least 4 Mi cells runs the plain
- * run's multi-step kernels. Results do not depend on the policy. */
+ * A request that runs alone on a grid of at least 2.25 Mi cells (FP64:
+ * 0.75 Mi) runs the plain run's multi-step kernels. Results do not depend on the policy. */
 enum { WV_SERVE_ALL = 0, WV_SERVE_SHORTEST = 1, WV_SERVE_LANES = 2 };
 wv_status wv_serve_set_policy(wv_server* server, int policy, uint64_t fill_cells, uint64_t lane_cells);
 /* out: [completed, p50, p95, p99 latency s, first submission to last

@@ -120,7 +120,7 @@ def test_two_step_passes_bit_identical(pillared_mini, prec):
     with fdtd.Plan(case.grid, prec) as plan:
         plan.set_pass(1)
         one, _ = plan.run(taps, sig, case.receivers, mode=fdtd.MODE_IR)
-        for zc in (8, 16, 32):
+        for zc in (1, 2, 8, 16, 32):
             plan.set_pass(2, zc)
             two, _ = plan.run(taps, sig, case.receivers, mode=fdtd.MODE_IR)
             assert np.array_equal(one, two), f"pair z-chunk {zc}"
@@ -142,7 +142,7 @@ def test_three_step_passes_bit_identical(pillared_mini, prec):
     with fdtd.Plan(case.grid, prec) as plan:
         plan.set_pass(1)
         one, _ = plan.run(taps, case.signal[:1499], case.receivers, mode=fdtd.MODE_IR)
-        for zc in (4, 8, 16, 32, 64):
+        for zc in (1, 2, 4, 8, 16, 32, 64):
             plan.set_pass(3, zc)
             three, _ = plan.run(taps, case.signal[:1499], case.receivers, mode=fdtd.MODE_IR)
             assert np.array_equal(one, three), f"z-chunk {zc}"

@@ -4,7 +4,8 @@ wv_fdtd_run, bit for bit.
 Two calls in a row on the same device array, the second on a smaller grid
 (the pillared room at dx 0.08 m, then the shoebox), in FP32 and FP64, with
 receiver series and spectra; then a run too large for the array is refused
-with WV_ERR_LIMIT and the array still serves a run afterwards.
+with WV_ERR_LIMIT and the array still serves a run afterwards; and a run
+whose default pass mode does not fit the array takes single steps.
 """
 import numpy as np
 import pytest
@@ -56,3 +57,21 @@ def test_noalloc_refuses_a_run_beyond_its_array():
     ir, _ = fdtd.fdtd_run_noalloc(g2, taps, sig2, rx, (), fdtd.MODE_IR, "fp32")
     ref, _ = fdtd.fdtd_run(g2, taps, sig2, rx, (), fdtd.MODE_IR, "fp32")
     assert np.array_equal(ir, ref)
+
+
+def test_noalloc_takes_single_steps_where_passes_do_not_fit():
+    # An FP64 grid of 1.37 M cells defaults to two steps per pass (four field
+    # buffers), whose footprint exceeds the array; the allocation-free call
+    # runs it with single steps (two buffers) and gives the same results.
+    case = workload.build_case(dict(configs.pillared_room(), dx=0.07, steps=400))
+    g, taps = case.grid, case.runs[1][2]
+    with fdtd.Plan(g, "fp64") as plan:
+        assert plan.pass_info()["steps_per_pass"] == 2
+    need = fdtd.footprint_estimate(g, "fp64", len(case.signal), len(case.receivers), len(taps))
+    assert need > 64 << 20
+    print(f"case: wv_fdtd_run_noalloc {g.n} fp64, default pass mode beyond the array", flush=True)
+    ir, sp = fdtd.fdtd_run_noalloc(g, taps, case.signal, case.receivers, FREQS, fdtd.MODE_BOTH, "fp64")
+    ref_ir, ref_sp = fdtd.fdtd_run(g, taps, case.signal, case.receivers, FREQS, fdtd.MODE_BOTH, "fp64")
+    assert np.count_nonzero(ref_ir) > 0.3 * ref_ir.size
+    assert np.array_equal(ir, ref_ir)
+    assert np.array_equal(sp, ref_sp)

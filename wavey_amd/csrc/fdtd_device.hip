@@ -1136,7 +1136,8 @@ uint64_t wv::plan_bytes(const wv_plan* pl) {
     return t;
 }
 
-uint64_t wv::estimate_bytes(const wv_grid* g, int prec, size_t steps, size_t n_recv, size_t n_taps) {
+uint64_t wv::estimate_bytes(const wv_grid* g, int prec, size_t steps, size_t n_recv, size_t n_taps,
+                            uint32_t steps_per_pass) {
     const uint64_t tsz = prec == WV_FP64 ? 8 : 4, ssz = prec == WV_FP32 ? 4 : 8;
     const uint64_t cells = g->n[0] * g->n[1] * g->n[2];
     const uint64_t pitch = (g->n[0] + 31) & ~31ull;
@@ -1146,7 +1147,7 @@ uint64_t wv::estimate_bytes(const wv_grid* g, int prec, size_t steps, size_t n_r
     const uint64_t special = g->n_boundary + n_taps;
     const uint64_t sp_pad = (special + 63) & ~63ull;
     const uint64_t slots = g->n_boundary ? g->b_start[g->n_boundary] : 0;
-    const uint32_t spp = default_steps_per_pass(cells, prec == WV_FP64);
+    const uint32_t spp = steps_per_pass ? steps_per_pass : default_steps_per_pass(cells, prec == WV_FP64);
     const uint64_t fields = spp == 3 ? 5 : spp == 2 ? 4 : 2;
     uint64_t t = cells;                              // flags
     t += npitched / 4 + npitched / 4;                // boundary marks (two copies), interior and deep bitmaps
@@ -1371,7 +1372,18 @@ extern "C" wv_status wv_fdtd_run_noalloc(const wv_grid* grid, const wv_tap* taps
         set_err(err, err_len, "wv_fdtd_run_noalloc: invalid argument");
         return WV_ERR_INVALID;
     }
-    const uint64_t need = estimate_bytes(grid, precision, steps, n_recv, n_taps);
+    uint64_t need = estimate_bytes(grid, precision, steps, n_recv, n_taps);
+    uint32_t steps_per_pass = 0;
+    if (need > kNoAllocBytes) {
+        // Single steps hold two field buffers instead of four or five: the
+        // run takes them where the default pass mode does not fit (the
+        // results do not depend on the mode).
+        const uint64_t single = estimate_bytes(grid, precision, steps, n_recv, n_taps, 1);
+        if (single <= kNoAllocBytes) {
+            need = single;
+            steps_per_pass = 1;
+        }
+    }
     if (need > kNoAllocBytes) {
         set_err(err, err_len, "wv_fdtd_run_noalloc: the run needs about " + std::to_string((need + (1u << 20) - 1) >> 20) +
                                   " MiB of device memory; the allocation-free entry point holds " +
@@ -1387,6 +1399,7 @@ extern "C" wv_status wv_fdtd_run_noalloc(const wv_grid* grid, const wv_tap* taps
         ArenaScope scope(&arena);
         wv_plan pl;
         pl.prec = precision;
+        pl.steps_per_pass = steps_per_pass;
         pl.borrowed = true;
         pl.stream = d->stream;
         pl.ev[0] = d->ev[0];

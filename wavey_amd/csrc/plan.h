@@ -219,17 +219,28 @@ inline StepParams<T, S> make_params(wv_plan* pl, uint32_t tiles_per_block) {
 // bitmaps, except FP32 three-step passes, which hold two bitmap words per
 // lane and take up to 64 planes: by default the longest of 64, 32 and 16
 // that still gives 640 tile blocks (2.5 per CU; the hall takes 64, a 58 Mi
-// cell grid 32, a 29 Mi one 16).
+// cell grid 32, a 29 Mi one 16). Other passes take 32 planes from 64 Mi
+// cells. Below those, 16 planes, or 8 where 16 leave fewer than 150 tile
+// blocks (on rooms of 0.8-7.5 M cells: below 150 blocks 8 planes were
+// faster on all but one, by up to 31 %; from 150 blocks 16 were faster on
+// all but one, by up to 11 %; 4 planes lost to 8 even at 270 blocks
+// against 135).
+constexpr uint64_t kPassMinBlocks = 150;
 inline uint32_t pass_zchunk(const wv_plan* pl, uint32_t steps_per_pass) {
     const bool wide = steps_per_pass == 3 && pl->prec != WV_FP64;
     if (pl->pair_zchunk) return std::min(pl->pair_zchunk, wide ? 64u : 32u);
-    if (!wide) return default_pair_zchunk(pl->cells);
-    const uint64_t cols = std::max<uint64_t>(1, pl->nx > 4 ? (pl->nx - 4 + kTriStride<float> - 1) / kTriStride<float> : 1);
-    const uint64_t rows = (pl->ny + kTriRows - 1) / kTriRows;
-    for (uint32_t zc : {64u, 32u}) {
-        if (cols * rows * ((pl->nz - 2 + zc - 1) / zc) >= 640) return zc;
+    if (!wide && pl->cells >= kPairLongChunkCells) return 32u;
+    const uint64_t stride = pl->prec == WV_FP64 ? kTileStep<double> : kTileStep<float>;
+    const uint64_t cols = std::max<uint64_t>(1, pl->nx > 4 ? (pl->nx - 4 + stride - 1) / stride : 1);
+    const uint64_t rows = (pl->ny + (steps_per_pass == 3 ? kTriRows : kPairRows) - 1) /
+                          (steps_per_pass == 3 ? kTriRows : kPairRows);
+    auto blocks = [&](uint64_t zc) { return cols * rows * ((pl->nz - 2 + zc - 1) / zc); };
+    if (wide) {
+        for (uint32_t zc : {64u, 32u}) {
+            if (blocks(zc) >= 640) return zc;
+        }
     }
-    return 16u;
+    return blocks(16) >= kPassMinBlocks ? 16u : 8u;
 }
 
 template <class F>
@@ -302,7 +313,8 @@ wv_status check_grid(const wv_grid* g, char* err, size_t err_len);
 // allocation, of what a plan of this grid and run will hold (special cells
 // counted as boundary nodes plus taps).
 uint64_t plan_bytes(const wv_plan* pl);
-uint64_t estimate_bytes(const wv_grid* g, int prec, size_t steps, size_t n_recv, size_t n_taps);
+uint64_t estimate_bytes(const wv_grid* g, int prec, size_t steps, size_t n_recv, size_t n_taps,
+                        uint32_t steps_per_pass = 0);
 // Checks of wv_run_begin's arguments against a plan (tap planes, receivers,
 // record size); WV_OK or an error with its message.
 wv_status check_run_args(const wv_plan* pl, const wv_tap* taps, size_t n_taps, const double* signal, size_t steps,

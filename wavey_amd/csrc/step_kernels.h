@@ -594,21 +594,25 @@ inline uint32_t default_step_zchunk(uint64_t cells, bool fp64) {
     if (fp64) return cells < kTinyStepCells ? 2u : kDefaultZChunk;
     return cells < kTinyStepCells ? 1u : cells < kSmallStepCells ? 2u : kDefaultZChunk;
 }
-// Default steps per pass and pass z-chunk by grid size (measured on the
-// hall and on hall-shaped grids of 58, 29, 15 and 7.6 Mi cells, and smaller
-// rooms): small grids are launch- and latency-bound, where one step per
-// launch wins; from a few million cells two steps per pass win, and three
-// from 8 Mi cells in FP32 and from 7 Mi in FP64 (which gains more: 11-21 %
-// against 0-17 % in FP32 on those grids), with z-chunks that keep enough tile
-// blocks in flight (pass_zchunk).
-constexpr uint64_t kPairMinCells = 4ull << 20;
+// Default steps per pass by grid size (measured on the hall, on hall-shaped
+// grids of 58, 29, 15 and 7.6 Mi cells and on rooms of 0.04-7.5 M cells):
+// small grids are launch- and latency-bound, where one step per launch wins;
+// two steps per pass win from 2.25 Mi cells in FP32 (on 26 rooms of 1.9-3.2
+// M cells single steps were 4-31 % faster up to 2.3 M, two steps 2-23 %
+// faster from 2.5 M) and from 0.75 Mi in FP64 (2-44 % faster from 0.8 M,
+// slower below with 8-plane chunks), and three from 8 Mi cells in FP32 and
+// from 7 Mi in FP64 (which gains
+// more: 11-21 % against 0-17 % in FP32 on those grids), with z-chunks that
+// keep enough tile blocks in flight (pass_zchunk).
+constexpr uint64_t kPairMinCells = 9ull << 18;      // 2.25 Mi
+constexpr uint64_t kPairMinCellsFp64 = 3ull << 18;  // 0.75 Mi
 constexpr uint64_t kPairLongChunkCells = 64ull << 20;
 constexpr uint64_t kTriMinCells = 8ull << 20;
 constexpr uint64_t kTriMinCellsFp64 = 7ull << 20;
 inline uint32_t default_steps_per_pass(uint64_t cells, bool fp64) {
-    return cells >= (fp64 ? kTriMinCellsFp64 : kTriMinCells) ? 3u : cells >= kPairMinCells ? 2u : 1u;
+    if (cells >= (fp64 ? kTriMinCellsFp64 : kTriMinCells)) return 3u;
+    return cells >= (fp64 ? kPairMinCellsFp64 : kPairMinCells) ? 2u : 1u;
 }
-inline uint32_t default_pair_zchunk(uint64_t cells) { return cells >= kPairLongChunkCells ? 32u : 16u; }
 constexpr uint32_t kVariantTiles[] = {7, 3, 15, 1};
 
 inline uint32_t variant_threads(int variant) {
