@@ -423,7 +423,7 @@ def bench_serving(args, throughput=None):
                                   f"dx 0.05 m, Poisson rate {rate:.3f}/s = {args.load:.0%} of "
                                   f"{throughput:.1f} Gcell-updates/s"),
                    "policy": args.policy, "fill_cells": args.fill or 16 << 20,
-                   "lane_cells": args.lane_cells or "three-step plans",
+                   "lane_cells": args.lane_cells or "multi-step plans from 2.25 Mi cells",
                    "requests": n, "cell_updates": int(sum(works)), "cap_bytes": int(args.cap_gb * 1e9)},
         "latency_s": pct(lat),
         "sequential_plain": {"solves_per_s": round(n / span_plain, 3), "latency_s": pct(lat_plain),
@@ -520,8 +520,8 @@ def main():
                     help="serving: requests each block advances (least work first up to --fill, large "
                          "requests on their own stream; least work first only; or all)")
     ap.add_argument("--lane-cells", type=int, default=0, help="serving: air cells from which a multi-step request "
-                                                              "takes the large lane (0 = plans with three steps "
-                                                              "per pass)")
+                                                              "takes the large lane (0 = plans with two or "
+                                                              "three steps per pass from 2.25 Mi cells)")
     ap.add_argument("--fill", type=int, default=0, help="serving: air cells per block for --policy shortest "
                                                         "(0 = the library's 16 Mi)")
     ap.add_argument("--fp64-sample", type=int, default=6, help="instances rerun in FP64 for the error")

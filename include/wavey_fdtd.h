@@ -286,8 +286,8 @@ wv_status wv_serve_wait(wv_server* server, uint64_t id, double* spectra_out, dou
  * while the chosen air cells stay within fill_cells (0 = 16 Mi); the first
  * always runs, the others wait on the device.
  * WV_SERVE_LANES (the default): as WV_SERVE_SHORTEST for most requests; the
- * large ones (lane_cells = 0: those whose plan takes three steps per pass,
- * FP32 grids of 8 Mi cells and more, FP64 7 Mi; else multi-step plans of at
+ * large ones (lane_cells = 0: those whose plan takes two or three steps per
+ * pass on a grid of at least 2.25 Mi cells; else multi-step plans of at
  * least lane_cells air cells) run one at a time, in admission order and each
  * to its end, on a second stream beside the batch, which the device runs at
  * the same time.
