@@ -929,8 +929,8 @@ class Server:
         """Which admitted requests each 128-step block advances: "all";
         "shortest" remaining work first up to fill_cells air cells; "lanes":
         as "shortest", except the large requests (lane_cells 0: plans with
-        three steps per pass; else multi-step plans of lane_cells air cells
-        and more), one at a time in admission order on a second stream
+        two or three steps per pass on grids of 2.25 Mi cells and more; else
+        multi-step plans of lane_cells air cells and more), one at a time in admission order on a second stream
         (fill_cells 0 = the library's 16 Mi)."""
         code = {"all": SERVE_ALL, "shortest": SERVE_SHORTEST, "lanes": SERVE_LANES}[policy]
         if lib().wv_serve_set_policy(self._h, code, int(fill_cells), int(lane_cells)) != 0:
